@@ -1,0 +1,128 @@
+"""ctypes binding of libckd.so (the C ABI declared in include/ckd.h).
+
+The product path is the native library: if libckd.so is missing this module
+raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libckd.so"
+
+CKD_OK, CKD_EINVAL, CKD_ENOMEM, CKD_EDEVICE, CKD_ESTATE = range(5)
+CKD_SHADE_FP32, CKD_SHADE_FP16 = 0, 1
+CKD_FX_SDF, CKD_FX_VOXEL, CKD_FX_TUNNEL = 0, 1, 2
+
+
+class CkdError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"ckd status {status}: {msg}")
+        self.status = status
+
+
+class LandscapeCam(C.Structure):
+    _fields_ = [("X1", C.c_float), ("Y1", C.c_float), ("viewCos", C.c_float),
+                ("viewSin", C.c_float), ("mapTilt", C.c_int)]
+
+
+class SdfScene(C.Structure):
+    """Same layout as ckd_sdf_scene and the oracle's orc_sdf_scene."""
+    _fields_ = [("nSpheres", C.c_int), ("sphere", (C.c_float * 4) * 32),
+                ("albedo", (C.c_float * 3) * 33), ("groundY", C.c_float),
+                ("light", C.c_float * 3), ("fogColor", C.c_float * 3),
+                ("fogDensity", C.c_float), ("gamma", C.c_float), ("ambient", C.c_float),
+                ("eps", C.c_float), ("tMax", C.c_float), ("maxSteps", C.c_int),
+                ("camRadius", C.c_float), ("camHeight", C.c_float), ("camSpeed", C.c_float),
+                ("focal", C.c_float)]
+
+
+class Request(C.Structure):
+    _fields_ = [("effect", C.c_int), ("W", C.c_uint), ("H", C.c_uint),
+                ("dst", C.c_void_p), ("cam", C.c_float * 12), ("vcam", LandscapeCam),
+                ("grid", C.c_void_p), ("radius", C.c_int)]
+
+
+_u = C.c_uint
+_p = C.c_void_p
+_f = C.c_float
+_i = C.c_int
+
+# name -> argtypes (all return int status unless listed in _RESTYPES)
+SIGNATURES = {
+    "ckd_create": [_i, C.POINTER(_p)],
+    "ckd_destroy": [_p],
+    "ckd_last_error": [],
+    "ckd_set_stream": [_p, _p],
+    "ckd_synchronize": [_p],
+    "ckd_malloc": [_p, C.POINTER(_p), C.c_size_t],
+    "ckd_free": [_p, _p],
+    "ckd_memcpy_h2d": [_p, _p, _p, C.c_size_t],
+    "ckd_memcpy_d2h": [_p, _p, _p, C.c_size_t],
+    "ckd_memcpy_d2d": [_p, _p, _p, C.c_size_t],
+    "ckd_event_record": [_p, _i],
+    "ckd_event_elapsed_ms": [_p, _i, _i, C.POINTER(_f)],
+    "ckd_memset32": [_p, _p, C.c_uint32, C.c_size_t],
+    "ckd_fx_blit_2x2": [_p, _p, _p, _u, _u],
+    "ckd_add32": [_p, _p, _p, _u],
+    "ckd_blit_add32": [_p, _p, _p, _u, _u, _u],
+    "ckd_blit_add32a": [_p, _p, _p, _u, _u, _u, _f],
+    "ckd_blur07_h": [_p, _p, _p, _u, _u, _f],
+    "ckd_blur07_v": [_p, _p, _p, _u, _u, _f],
+    "ckd_blur07": [_p, _p, _p, _u, _u, _f],
+    "ckd_blur26_h": [_p, _p, _p, _u, _u, _f, _f, _u],
+    "ckd_blur26_v": [_p, _p, _p, _u, _u, _f, _f, _u],
+    "ckd_blur26": [_p, _p, _p, _u, _u, _f, _f, _u],
+    "ckd_polar_create": [_p, _u, _u],
+    "ckd_polar_blit": [_p, _p, _p, _u, _u, _i],
+    "ckd_grid_remap": [_p, _p, _u, _u, _p, _p, _u],
+    "ckd_landscape_create": [_p, _p, _p, _p],
+    "ckd_landscape_draw": [_p, _p, _u, _u, C.POINTER(LandscapeCam), _u],
+    "ckd_tunnelscape_create": [_p, _p, _p, _p],
+    "ckd_tunnelscape_rows": [_p, _p, _u, _u, _f, _f, _f, _f],
+    "ckd_sdf_render": [_p, _p, _u, _u, C.POINTER(SdfScene), C.POINTER(_f), _i, _p],
+    "ckd_tunnel_frame": [_p, _p, _p, _p, _u, _u, _p, _p, _u, _i],
+    "ckd_render_batch": [_p, C.POINTER(Request), _u, C.POINTER(SdfScene), _i, _p, _u, _p,
+                         C.c_size_t, _p],
+}
+_RESTYPES = {"ckd_destroy": None, "ckd_last_error": C.c_char_p}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libckd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                              "or python -m cookiedough_amd.build")
+        lib = C.CDLL(str(LIB_PATH))
+        for name, args in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = _RESTYPES.get(name, C.c_int)
+        _lib = lib
+    return _lib
+
+
+def last_error() -> str:
+    return load().ckd_last_error().decode()
+
+
+def check(status: int) -> None:
+    if status != CKD_OK:
+        raise CkdError(status, last_error())
+
+
+def ptr(a) -> int | None:
+    """Device pointer of a DevBuf, host pointer of a numpy array, or None."""
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return a
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return a.ptr

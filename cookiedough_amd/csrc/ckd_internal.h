@@ -1,0 +1,80 @@
+// ckd_internal.h -- context state and error plumbing shared by the HIP
+// translation units of libckd.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/ckd.h"
+
+#define CKD_NUM_EVENTS 16
+
+struct ckd_ctx {
+    int device = 0;
+    int cuCount = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t events[CKD_NUM_EVENTS] = {};
+
+    // landscape maps (Landscape_Create)
+    uint8_t *lsHeight = nullptr;
+    uint32_t *lsColor = nullptr;
+    uint32_t *lsFog = nullptr;
+    uint32_t lsFog0 = 0;
+
+    // tunnelscape maps (Tunnelscape_Create)
+    uint8_t *tsHeight = nullptr;
+    uint32_t *tsColor = nullptr;
+    uint32_t *tsFog = nullptr;
+    uint32_t tsFog0 = 0;
+
+    // polar maps (Polar_Create), interleaved (U,V) 24:8 per pixel
+    int32_t *polarMap = nullptr;
+    int32_t *polarInv = nullptr;
+    unsigned polarW = 0, polarH = 0;
+
+    // 2026 blur scratch (BoxBlur_Create): two images plus guard pixels
+    uint32_t *b26Scratch[2] = {nullptr, nullptr};
+    size_t b26Cap = 0;
+
+    // per-call parameter staging (device), grown on demand
+    void *staging = nullptr;
+    size_t stagingCap = 0;
+};
+
+namespace ckd {
+
+// Records a message for ckd_last_error() on this thread and returns s.
+ckd_status fail(ckd_status s, const char *fmt, ...)
+    __attribute__((format(printf, 2, 3)));
+
+// hipGetLastError after a launch: CKD_EDEVICE with the kernel name.
+ckd_status check_launch(const char *what);
+
+// Device buffer of at least `bytes` owned by the context for per-call
+// parameters; enqueues the host->device copy on the context stream.
+ckd_status stage(ckd_ctx *ctx, const void *host, size_t bytes, void **dev);
+
+} // namespace ckd
+
+#define CKD_REQUIRE(cond, ...)                                                 \
+    do {                                                                       \
+        if (!(cond)) return ckd::fail(CKD_EINVAL, __VA_ARGS__);               \
+    } while (0)
+
+#define CKD_REQUIRE_CTX(ctx)                                                   \
+    do {                                                                       \
+        if (!(ctx)) return ckd::fail(CKD_EINVAL, "%s: null context", __func__); \
+    } while (0)
+
+#define CKD_HIP(call)                                                          \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess)                                                  \
+            return ckd::fail(CKD_EDEVICE, "%s: %s: %s", __func__, #call,      \
+                             hipGetErrorString(e_));                           \
+    } while (0)
+
+static inline unsigned ckd_div_up(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }

@@ -1,0 +1,120 @@
+// ckd_pixel.h -- device-side pixel arithmetic shared by the kernels.
+//
+// Every helper reproduces an SSE lane sequence of the reference exactly,
+// written as plain 32-bit integer arithmetic for the GCN/CDNA VALU.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#define CKD_DEV __device__ __forceinline__
+
+// Two 8-bit channels held in the 16-bit fields of 0x00XX00YY.  The convex
+// combination a*(256-f) + b*f of two such fields never exceeds 255*256, so
+// the fields never carry into each other.
+CKD_DEV uint32_t ckd_lerp2(uint32_t a, uint32_t b, uint32_t f)
+{
+    return ((a * (256u - f) + b * f) >> 8) & 0x00ff00ffu;
+}
+
+// Bilinear blend of four ARGB texels with 8-bit fractions.  Equals
+// bsamp32_16 and bsamp32_32 (code/bilinear.h:58-125) channel for channel:
+// their 16-bit wrap-around intermediates equal S0*(256-f)+S1*f, which lies
+// in [0, 65280], so the mod-2^16 and 32-bit forms agree with this one.
+CKD_DEV uint32_t ckd_bilerp32(uint32_t p0, uint32_t p1, uint32_t p2, uint32_t p3,
+                              uint32_t fu, uint32_t fv)
+{
+    const uint32_t M = 0x00ff00ffu;
+    const uint32_t br = ckd_lerp2(ckd_lerp2(p0 & M, p1 & M, fu),
+                                  ckd_lerp2(p2 & M, p3 & M, fu), fv);
+    const uint32_t ga = ckd_lerp2(ckd_lerp2((p0 >> 8) & M, (p1 >> 8) & M, fu),
+                                  ckd_lerp2((p2 >> 8) & M, (p3 >> 8) & M, fu), fv);
+    return br | (ga << 8);
+}
+
+// bsamp8 (code/bilinear.h:34-53), one channel.
+CKD_DEV uint32_t ckd_bilerp8(uint32_t s0, uint32_t s1, uint32_t s2, uint32_t s3,
+                             uint32_t fu, uint32_t fv)
+{
+    const uint32_t a = (s0 * (256u - fu) + s1 * fu) >> 8;
+    const uint32_t b = (s2 * (256u - fu) + s3 * fu) >> 8;
+    return (a * (256u - fv) + b * fv) >> 8;
+}
+
+// _mm_subs_epu16 of two packed ARGB colours widened to 16-bit lanes.
+CKD_DEV uint32_t ckd_subs_u8x4(uint32_t c, uint32_t f)
+{
+    uint32_t r = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int v = (int)((c >> (8 * k)) & 0xff) - (int)((f >> (8 * k)) & 0xff);
+        r |= (uint32_t)(v > 0 ? v : 0) << (8 * k);
+    }
+    return r;
+}
+
+// Saturating per-byte add (_mm_add_epi16 of widened bytes + packus_epi16).
+CKD_DEV uint32_t ckd_adds_u8x4(uint32_t a, uint32_t b)
+{
+    uint32_t r = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t v = ((a >> (8 * k)) & 0xff) + ((b >> (8 * k)) & 0xff);
+        r |= (v > 255u ? 255u : v) << (8 * k);
+    }
+    return r;
+}
+
+// _mm_avg_epu8 of four bytes
+CKD_DEV uint32_t ckd_avg_u8x4(uint32_t a, uint32_t b)
+{
+    // (a+b+1)>>1 per byte without cross-byte carries
+    return (a | b) - (((a ^ b) >> 1) & 0x7f7f7f7fu);
+}
+
+// _mm_madd_epi16 of one 32-bit lane
+CKD_DEV int32_t ckd_madd16(uint32_t a, uint32_t b)
+{
+    const int32_t lo = (int32_t)(int16_t)(a & 0xffff) * (int32_t)(int16_t)(b & 0xffff);
+    const int32_t hi = (int32_t)(int16_t)(a >> 16) * (int32_t)(int16_t)(b >> 16);
+    return (int32_t)((uint32_t)lo + (uint32_t)hi);
+}
+
+// cspanISSE16 set-up (code/cspan.h:47-65): per-channel 16.16 start value
+// (A<<16)+preStep and step, with the lane quirks of _mm_mul_epi32 (only
+// lanes 0 and 2 receive the pre-step) and _mm_madd_epi16 (16-bit halves).
+struct CkdSpan {
+    uint32_t acc[4];
+    uint32_t step[4];
+};
+
+CKD_DEV void ckd_span_setup(CkdSpan &s, uint32_t A, uint32_t B, uint32_t length,
+                            uint32_t drawLength)
+{
+    const uint32_t divisor = 65536u / length;
+    const uint32_t pre = length - drawLength;
+    const int64_t m = (int64_t)(int32_t)divisor * (int64_t)(int32_t)pre;
+    const uint32_t mlo = (uint32_t)m, mhi = (uint32_t)((uint64_t)m >> 32);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t a = (A >> (8 * k)) & 0xff, b = (B >> (8 * k)) & 0xff;
+        const uint32_t delta = b - a;
+        const uint32_t M = (k & 1) ? mhi : mlo;
+        s.step[k] = (uint32_t)ckd_madd16(delta, divisor);
+        s.acc[k] = (a << 16) + (uint32_t)ckd_madd16(delta, M);
+    }
+}
+
+// one span pixel: v = acc>>16 (logical); packus_epi32 keeps v (<= 65535);
+// packus_epi16 maps v > 32767 to 0 and clamps to 255.
+CKD_DEV uint32_t ckd_span_pixel(const uint32_t acc[4])
+{
+    uint32_t px = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t v = acc[k] >> 16;
+        const uint32_t b = v > 32767u ? 0u : (v > 255u ? 255u : v);
+        px |= b << (8 * k);
+    }
+    return px;
+}

@@ -1,0 +1,364 @@
+// k_blur07.hip -- the 2007 box blur (code/deprecated/boxblur.cpp).
+//
+// Out of place the reference's sliding window is an exact window sum (its
+// u16 accumulator never saturates for spans <= 255), so each output pixel
+// is computed independently:
+//   odd span 2e+1:  acc = sum p[x-e .. x+e] (clamped to the line)
+//   even span 2e:   the same window with half weights at both ends,
+//                   ceil(p[x+e]/2) at the head and floor(p[x-e]/2) at the
+//                   tail (code/deprecated/boxblur.cpp:17-36 split each pixel
+//                   into p-(p>>1) now and p>>1 one step later)
+//   out = packus16(mulhi_u16(acc, div)) with div = (2^24/w16)>>4 truncated
+//   to 16 bits, w16 = 16*weight (+8 for even spans' half pixel).
+// Four 8-bit channels are summed at once in the 16-bit fields of a u64;
+// window sums stay below 2^16, so differences of u64 prefix sums are exact.
+//
+// In place (dst == src) the reference's subtract reads pixels it has
+// already overwritten, which makes every line a sequential recurrence with
+// u16 saturation; those kernels run one lane per line and replay it.
+#include "ckd_internal.h"
+#include "ckd_pixel.h"
+
+namespace {
+
+struct B07Params {
+    uint32_t span, edgeSpan, median, shift, subEdges;
+    uint32_t fullDiv;
+    uint16_t edgeDivs[256];
+};
+
+// code/deprecated/boxblur.cpp:11-14 (stored through _mm_set1_epi16)
+uint16_t weight_to_div(unsigned weight) { return (uint16_t)(((65536u * 256u) / weight) >> 4); }
+
+// code/deprecated/boxblur.cpp:52-76
+unsigned b07_span(float strength)
+{
+    const float f = strength * 255.f;
+    unsigned k = f >= 4294967040.f ? 0xffffffffu : (unsigned)f;
+    int ki = (int)k;
+    return (unsigned)(ki > 255 ? 255 : (ki < 1 ? 1 : ki));
+}
+
+void b07_setup(float strength, B07Params &p)
+{
+    p.span = b07_span(strength);
+    p.subEdges = (p.span & 1) == 0;
+    p.edgeSpan = p.span >> 1;
+    p.shift = 1 + (p.subEdges ? 0 : 7);
+    p.median = p.edgeSpan + (p.subEdges ? 0 : 1);
+    unsigned w = (p.median << 4) + (p.subEdges << 3);
+    for (unsigned i = 0; i < 256; ++i) p.edgeDivs[i] = 0;
+    for (unsigned i = 0; i < p.median; ++i, w += 16) p.edgeDivs[i] = weight_to_div(w);
+    p.fullDiv = weight_to_div(p.span << 4);
+}
+
+CKD_DEV uint64_t spread(uint32_t px)
+{
+    return (uint64_t)(px & 0xffu) | ((uint64_t)(px & 0xff00u) << 8) |
+           ((uint64_t)(px & 0xff0000u) << 16) | ((uint64_t)(px & 0xff000000u) << 24);
+}
+
+// per-channel floor(p/2) and ceil(p/2), spread
+CKD_DEV uint64_t floor_half(uint32_t px) { return spread((px >> 1) & 0x7f7f7f7fu); }
+CKD_DEV uint64_t ceil_half(uint32_t px) { return spread(px) - floor_half(px); }
+
+CKD_DEV uint32_t div_pack(uint64_t acc, uint32_t div)
+{
+    uint32_t px = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t a = (uint32_t)(acc >> (16 * k)) & 0xffffu;
+        const uint32_t q = (a * div) >> 16;
+        px |= (q > 32767u ? 0u : (q > 255u ? 255u : q)) << (8 * k);
+    }
+    return px;
+}
+
+CKD_DEV uint32_t divisor_at(const B07Params &p, int x, int len)
+{
+    if (x < (int)p.median) return p.edgeDivs[x];
+    if (x >= len - (int)p.edgeSpan) return p.edgeDivs[len - 1 - x];
+    return p.fullDiv;
+}
+
+// horizontal, out of place: one 256-thread block per row, LDS prefix sums
+constexpr int kHBlock = 256;
+constexpr int kHMaxW = 4096; // (W+1)*8 + W*4 bytes of LDS stay below 64 KiB
+
+__global__ __launch_bounds__(kHBlock) void k_blur07_h(uint32_t *__restrict__ dst,
+                                                      const uint32_t *__restrict__ src,
+                                                      int W, B07Params p)
+{
+    extern __shared__ uint64_t lds[];
+    uint64_t *P = lds;                                        // W+1 prefix sums
+    uint32_t *px = reinterpret_cast<uint32_t *>(lds + W + 1); // W pixels
+    __shared__ uint64_t waveTot[kHBlock / 64];
+
+    const int row = blockIdx.x;
+    const uint32_t *s = src + (size_t)row * W;
+    for (int i = threadIdx.x; i < W; i += kHBlock) px[i] = s[i];
+    __syncthreads();
+
+    const int chunk = (W + kHBlock - 1) / kHBlock;
+    const int c0 = threadIdx.x * chunk;
+    const int c1 = min(c0 + chunk, W);
+    uint64_t local = 0;
+    for (int i = c0; i < c1; ++i) local += spread(px[i]);
+
+    // block exclusive scan of the chunk totals
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint64_t incl = local;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint64_t y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) waveTot[wave] = incl;
+    __syncthreads();
+    uint64_t base = 0;
+    for (int w = 0; w < wave; ++w) base += waveTot[w];
+    uint64_t run = base + incl - local;
+    if (threadIdx.x == 0) P[0] = 0;
+    for (int i = c0; i < c1; ++i) {
+        run += spread(px[i]);
+        P[i + 1] = run;
+    }
+    __syncthreads();
+
+    const int e = (int)p.edgeSpan;
+    uint32_t *d = dst + (size_t)row * W;
+    for (int x = threadIdx.x; x < W; x += kHBlock) {
+        const int lo = x - e, hi = x + e;
+        const int loC = lo < 0 ? 0 : lo, hiC = hi > W - 1 ? W - 1 : hi;
+        uint64_t acc = P[hiC + 1] - P[loC];
+        if (p.subEdges) {
+            if (hi <= W - 1) acc -= floor_half(px[hi]);
+            if (lo >= 0) acc -= ceil_half(px[lo]);
+        }
+        d[x] = div_pack(acc, divisor_at(p, x, W));
+    }
+}
+
+// vertical, out of place: lane per column, each lane slides over a segment
+// of kVSeg rows (reads are coalesced across the wave)
+constexpr int kVSeg = 32;
+
+__global__ __launch_bounds__(256) void k_blur07_v(uint32_t *__restrict__ dst,
+                                                  const uint32_t *__restrict__ src, int W,
+                                                  int H, B07Params p)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y0 = blockIdx.y * kVSeg;
+    if (x >= W) return;
+    const int e = (int)p.edgeSpan;
+    const int y1 = min(y0 + kVSeg, H);
+    const uint32_t *col = src + x;
+    uint64_t F = 0;
+    for (int r = max(0, y0 - e); r <= min(H - 1, y0 + e); ++r) F += spread(col[(size_t)r * W]);
+    for (int y = y0; y < y1; ++y) {
+        const int lo = y - e, hi = y + e;
+        uint64_t acc = F;
+        if (p.subEdges) {
+            if (hi <= H - 1) acc -= floor_half(col[(size_t)hi * W]);
+            if (lo >= 0) acc -= ceil_half(col[(size_t)lo * W]);
+        }
+        dst[(size_t)y * W + x] = div_pack(acc, divisor_at(p, y, H));
+        if (hi + 1 <= H - 1) F += spread(col[(size_t)(hi + 1) * W]);
+        if (lo >= 0) F -= spread(col[(size_t)lo * W]);
+    }
+}
+
+// in place: the exact recurrence of one line (code/deprecated/boxblur.cpp:
+// 81-125 / 165-219 with pSrc == pDest).  Lane per line; the outputs the
+// subtract needs again `median` steps later live in an LDS ring of `ring`
+// entries per lane (ring > median, power of two).
+CKD_DEV void sat_add(uint32_t acc[4], uint32_t v, int k)
+{
+    const uint32_t s = acc[k] + v;
+    acc[k] = s > 65535u ? 65535u : s;
+}
+
+CKD_DEV void b07_add(uint32_t acc[4], uint32_t rem[4], uint32_t px, uint32_t shift)
+{
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t v = (px >> (8 * k)) & 0xff;
+        sat_add(acc, rem[k], k);
+        rem[k] = v >> shift;
+        sat_add(acc, v - rem[k], k);
+    }
+}
+
+CKD_DEV void b07_sub(uint32_t acc[4], uint32_t rem[4], uint32_t px, uint32_t shift)
+{
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t v = (px >> (8 * k)) & 0xff;
+        acc[k] = acc[k] > rem[k] ? acc[k] - rem[k] : 0;
+        rem[k] = v >> shift;
+        const uint32_t d = v - rem[k];
+        acc[k] = acc[k] > d ? acc[k] - d : 0;
+    }
+}
+
+CKD_DEV uint32_t b07_div(const uint32_t acc[4], uint32_t div)
+{
+    uint32_t px = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t q = (acc[k] * div) >> 16;
+        px |= (q > 32767u ? 0u : (q > 255u ? 255u : q)) << (8 * k);
+    }
+    return px;
+}
+
+__global__ __launch_bounds__(64) void k_blur07_inplace(uint32_t *__restrict__ img, int nLines,
+                                                       int len, size_t lineStride,
+                                                       size_t pixStride, int ring, B07Params p)
+{
+    extern __shared__ uint32_t ringBuf[];
+    const int line = blockIdx.x * blockDim.x + threadIdx.x;
+    if (line >= nLines) return;
+    uint32_t *r = ringBuf + (size_t)threadIdx.x * ring;
+    const int rmask = ring - 1;
+    uint32_t *base = img + (size_t)line * lineStride;
+    uint32_t acc[4] = {0, 0, 0, 0}, addRem[4] = {0, 0, 0, 0}, subRem[4] = {0, 0, 0, 0};
+    const int e = (int)p.edgeSpan, m = (int)p.median;
+    const int full = len - (m + e);
+    int addPos = 0, dest = 0, subPos = 0;
+
+    for (int i = 0; i < e; ++i) b07_add(acc, addRem, base[(size_t)(addPos++) * pixStride], p.shift);
+    for (int i = 0; i < m; ++i) {
+        b07_add(acc, addRem, base[(size_t)(addPos++) * pixStride], p.shift);
+        const uint32_t o = b07_div(acc, p.edgeDivs[i]);
+        r[dest & rmask] = o;
+        base[(size_t)(dest++) * pixStride] = o;
+    }
+    for (int i = 0; i < full; ++i) {
+        b07_add(acc, addRem, base[(size_t)(addPos++) * pixStride], p.shift);
+        b07_sub(acc, subRem, r[(subPos++) & rmask], p.shift);
+        const uint32_t o = b07_div(acc, p.fullDiv);
+        r[dest & rmask] = o;
+        base[(size_t)(dest++) * pixStride] = o;
+    }
+    if (p.subEdges)
+        for (int k = 0; k < 4; ++k) sat_add(acc, addRem[k], k);
+    for (int i = e; i > 0; --i) {
+        b07_sub(acc, subRem, r[(subPos++) & rmask], p.shift);
+        const uint32_t o = b07_div(acc, p.edgeDivs[i - 1]);
+        r[dest & rmask] = o;
+        base[(size_t)(dest++) * pixStride] = o;
+    }
+}
+
+ckd_status validate(const char *fn, const uint32_t *dst, const uint32_t *src, unsigned xRes,
+                    unsigned yRes, float strength, unsigned passLen, B07Params &p)
+{
+    CKD_REQUIRE(dst && src, "%s: null buffer", fn);
+    CKD_REQUIRE(xRes > 0 && yRes > 0 && xRes <= 16384 && yRes <= 16384,
+                "%s: bad size %ux%u", fn, xRes, yRes);
+    CKD_REQUIRE(strength >= 0.f && strength <= 1e30f, "%s: strength %g must be finite and >= 0",
+                fn, (double)strength);
+    b07_setup(strength, p);
+    CKD_REQUIRE(p.span <= passLen, "%s: kernel span %u exceeds the pass length %u", fn, p.span,
+                passLen);
+    return CKD_OK;
+}
+
+int ring_for(unsigned median)
+{
+    int r = 1;
+    while (r <= (int)median) r <<= 1;
+    return r;
+}
+
+} // namespace
+
+namespace {
+
+ckd_status launch_h(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, unsigned xRes,
+                    unsigned yRes, const B07Params &p)
+{
+    if (dst == src) {
+        const int ring = ring_for(p.median);
+        const size_t lds = (size_t)ring * 64 * sizeof(uint32_t);
+        k_blur07_inplace<<<ckd_div_up(yRes, 64), 64, lds, ctx->stream>>>(
+            dst, (int)yRes, (int)xRes, xRes, 1, ring, p);
+        return ckd::check_launch("blur07_h (in place)");
+    }
+    if (xRes > (unsigned)kHMaxW) return ckd::fail(CKD_EINVAL, "blur07_h: xRes %u > %d", xRes, kHMaxW);
+    const size_t lds = (size_t)(xRes + 1) * sizeof(uint64_t) + (size_t)xRes * sizeof(uint32_t);
+    k_blur07_h<<<yRes, kHBlock, lds, ctx->stream>>>(dst, src, (int)xRes, p);
+    return ckd::check_launch("blur07_h");
+}
+
+ckd_status launch_v(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, unsigned xRes,
+                    unsigned yRes, const B07Params &p)
+{
+    if (dst == src) {
+        const int ring = ring_for(p.median);
+        const size_t lds = (size_t)ring * 64 * sizeof(uint32_t);
+        k_blur07_inplace<<<ckd_div_up(xRes, 64), 64, lds, ctx->stream>>>(
+            dst, (int)xRes, (int)yRes, 1, xRes, ring, p);
+        return ckd::check_launch("blur07_v (in place)");
+    }
+    dim3 grid(ckd_div_up(xRes, 64), ckd_div_up(yRes, kVSeg));
+    k_blur07_v<<<grid, 64, 0, ctx->stream>>>(dst, src, (int)xRes, (int)yRes, p);
+    return ckd::check_launch("blur07_v");
+}
+
+} // namespace
+
+namespace ckd {
+
+// used by the tunnel pipeline (span 2r+1, out of place)
+ckd_status blur07_hv_oop(ckd_ctx *ctx, uint32_t *tmp, uint32_t *dst, const uint32_t *src,
+                         unsigned W, unsigned H, float strength)
+{
+    B07Params p;
+    b07_setup(strength, p);
+    if (p.span > W || p.span > H)
+        return fail(CKD_EINVAL, "tunnel blur: span %u exceeds %ux%u", p.span, W, H);
+    ckd_status s = launch_h(ctx, tmp, src, W, H, p);
+    if (s != CKD_OK) return s;
+    return launch_v(ctx, dst, tmp, W, H, p);
+}
+
+} // namespace ckd
+
+extern "C" {
+
+ckd_status ckd_blur07_h(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, unsigned xRes,
+                        unsigned yRes, float strength)
+{
+    B07Params p;
+    ckd_status s = validate("ckd_blur07_h", dst, src, xRes, yRes, strength, xRes, p);
+    if (s != CKD_OK) return s;
+    CKD_REQUIRE_CTX(ctx);
+    return launch_h(ctx, dst, src, xRes, yRes, p);
+}
+
+ckd_status ckd_blur07_v(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, unsigned xRes,
+                        unsigned yRes, float strength)
+{
+    B07Params p;
+    ckd_status s = validate("ckd_blur07_v", dst, src, xRes, yRes, strength, yRes, p);
+    if (s != CKD_OK) return s;
+    CKD_REQUIRE_CTX(ctx);
+    return launch_v(ctx, dst, src, xRes, yRes, p);
+}
+
+ckd_status ckd_blur07(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, unsigned xRes,
+                      unsigned yRes, float strength)
+{
+    B07Params p;
+    ckd_status s = validate("ckd_blur07", dst, src, xRes, yRes, strength,
+                            xRes < yRes ? xRes : yRes, p);
+    if (s != CKD_OK) return s;
+    CKD_REQUIRE_CTX(ctx);
+    s = launch_h(ctx, dst, src, xRes, yRes, p);
+    if (s != CKD_OK) return s;
+    return launch_v(ctx, dst, dst, xRes, yRes, p);
+}
+
+} // extern "C"

@@ -1,0 +1,202 @@
+// k_remap.hip -- polar UV-map remap (code/polar.cpp) and the per-8x8-block
+// UV grid remap of the tunnel benchmark.
+//
+// Both are one lane per output pixel: read the 24:8 (U,V), fetch four
+// texels, blend them with ckd_bilerp32 (exactly bsamp32_32) and store.
+#include "ckd_internal.h"
+#include "ckd_pixel.h"
+
+#include <cfloat>
+#include <cmath>
+#include <vector>
+
+namespace {
+
+// CalculateMaps (code/polar.cpp:18-59), host FP32 with libm, no contraction
+#pragma clang fp contract(off)
+void calculate_maps(int32_t *pDest, int32_t *pInvDest, unsigned srcResX, unsigned srcResY,
+                    unsigned destResX, unsigned destResY)
+{
+    const float kPI = 3.1415926535897932384626433832795f;
+    const float halfResX = destResX / 2.f;
+    const float halfResY = destResY / 2.f;
+    unsigned iPixel = 0;
+    const float maxDist = sqrtf(halfResX * halfResX + halfResY * halfResY);
+    for (float Y = -halfResY; Y < halfResY; Y += 1.f) {
+        for (float X = -halfResX + FLT_EPSILON; X < halfResX; X += 1.f) {
+            const float distance = sqrtf(X * X + Y * Y) / maxDist;
+            float theta = atan2f(Y, X);
+            theta += kPI;
+            theta /= kPI * 2.f;
+            const float U = distance * (srcResX - 1.f);
+            const float invU = (1.f - distance) * (srcResX - 1.f);
+            const float V = theta * (srcResY - 1.f);
+            pDest[iPixel] = U >= srcResX - 1.f ? (int32_t)(((srcResX - 2) << 8) | 0xff)
+                                               : (int32_t)(U * 256.f);
+            pInvDest[iPixel] = invU >= srcResX - 1.f ? (int32_t)(((srcResX - 2) << 8) | 0xff)
+                                                     : (int32_t)(invU * 256.f);
+            const int32_t v = V >= srcResY - 1.f ? (int32_t)(((srcResY - 2) << 8) | 0xff)
+                                                 : (int32_t)(V * 256.f);
+            pInvDest[iPixel + 1] = pDest[iPixel + 1] = v;
+            iPixel += 2;
+        }
+    }
+}
+#pragma clang fp contract(on)
+
+// Fetch32 + Polar_Blit_Tile (code/polar.cpp:82-133): no wrap, the map is
+// clamped so that (U0+1, V0+1) stays inside the W x H source.
+__global__ __launch_bounds__(256) void k_polar_blit(uint32_t *__restrict__ dst,
+                                                    const uint32_t *__restrict__ src,
+                                                    const int2 *__restrict__ map,
+                                                    unsigned W, unsigned H)
+{
+    const unsigned x = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned y = blockIdx.y;
+    if (x >= W) return;
+    const size_t i = (size_t)y * W + x;
+    const int2 uv = map[i];
+    const unsigned U0 = (unsigned)(uv.x >> 8);
+    const unsigned V0 = (unsigned)(uv.y >> 8) * W;
+    const uint32_t fu = (uint32_t)uv.x & 0xff, fv = (uint32_t)uv.y & 0xff;
+    const uint32_t p0 = src[U0 + V0], p1 = src[U0 + 1 + V0];
+    const uint32_t p2 = src[U0 + V0 + W], p3 = src[U0 + 1 + V0 + W];
+    dst[i] = ckd_bilerp32(p0, p1, p2, p3, fu, fv);
+}
+
+} // namespace
+
+namespace ckd {
+
+// Grid remap of one pixel (build-defined law):
+//   corners C00,C10,C01,C11 of block (x>>3, y>>3), local (i,j) = (x&7, y&7)
+//   L = C00*(8-j) + C01*j, R = C10*(8-j) + C11*j, UV = (L*(8-i) + R*i) >> 6
+// then bsamp_prepUVs (code/bilinear.h:10-31) with wrap mask 2^texLog2-1 and
+// bsamp32_32 (code/bilinear.h:86-125).
+CKD_DEV uint32_t grid_pixel(const int2 *__restrict__ grid, unsigned GW,
+                            const uint32_t *__restrict__ tex, unsigned texLog2, unsigned x,
+                            unsigned y)
+{
+    const unsigned bx = x >> 3, by = y >> 3;
+    const int i = (int)(x & 7), j = (int)(y & 7);
+    const int2 *g = grid + (size_t)by * GW + bx;
+    const int2 c00 = g[0], c10 = g[1], c01 = g[GW], c11 = g[GW + 1];
+    const int Lu = c00.x * (8 - j) + c01.x * j, Ru = c10.x * (8 - j) + c11.x * j;
+    const int Lv = c00.y * (8 - j) + c01.y * j, Rv = c10.y * (8 - j) + c11.y * j;
+    const int U = (Lu * (8 - i) + Ru * i) >> 6;
+    const int V = (Lv * (8 - i) + Rv * i) >> 6;
+    const unsigned mask = (1u << texLog2) - 1;
+    const unsigned u0 = ((unsigned)(U >> 8)) & mask, u1 = ((unsigned)(U >> 8) + 1) & mask;
+    const unsigned v0 = (((unsigned)(V >> 8)) & mask) << texLog2;
+    const unsigned v1 = (((unsigned)(V >> 8) + 1) & mask) << texLog2;
+    return ckd_bilerp32(tex[u0 + v0], tex[u1 + v0], tex[u0 + v1], tex[u1 + v1],
+                        (uint32_t)U & 0xff, (uint32_t)V & 0xff);
+}
+
+} // namespace ckd
+
+namespace {
+
+__global__ __launch_bounds__(256) void k_grid_remap(uint32_t *__restrict__ dst, unsigned W,
+                                                    unsigned H, const int2 *__restrict__ grid,
+                                                    const uint32_t *__restrict__ tex,
+                                                    unsigned texLog2)
+{
+    const unsigned x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const unsigned y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= W || y >= H) return;
+    const unsigned GW = (W + 7) / 8 + 1;
+    dst[(size_t)y * W + x] = ckd::grid_pixel(grid, GW, tex, texLog2, x, y);
+}
+
+} // namespace
+
+namespace ckd {
+ckd_status blur07_hv_oop(ckd_ctx *ctx, uint32_t *tmp, uint32_t *dst, const uint32_t *src,
+                         unsigned W, unsigned H, float strength);
+ckd_status launch_blit_add(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, unsigned destStride,
+                           unsigned srcStride, unsigned width, unsigned rows);
+
+ckd_status launch_grid_remap(ckd_ctx *ctx, uint32_t *dst, unsigned W, unsigned H,
+                             const int32_t *grid, const uint32_t *tex, unsigned texLog2)
+{
+    dim3 g(ckd_div_up(W, 64), ckd_div_up(H, 4));
+    k_grid_remap<<<g, 256, 0, ctx->stream>>>(dst, W, H, reinterpret_cast<const int2 *>(grid), tex,
+                                             texLog2);
+    return check_launch("grid_remap");
+}
+} // namespace ckd
+
+extern "C" {
+
+ckd_status ckd_polar_create(ckd_ctx *ctx, unsigned W, unsigned H)
+{
+    CKD_REQUIRE(W >= 4 && H >= 4 && W <= 8192 && H <= 8192 && (W % 2) == 0 && (H % 2) == 0,
+                "ckd_polar_create: bad size %ux%u (even, 4..8192)", W, H);
+    CKD_REQUIRE_CTX(ctx);
+    const size_t n = (size_t)W * H * 2;
+    std::vector<int32_t> map(n), inv(n);
+    calculate_maps(map.data(), inv.data(), W, H, W, H);
+    if (ctx->polarMap) {
+        CKD_HIP(hipStreamSynchronize(ctx->stream));
+        (void)hipFree(ctx->polarMap);
+        (void)hipFree(ctx->polarInv);
+        ctx->polarMap = ctx->polarInv = nullptr;
+    }
+    if (hipMalloc(&ctx->polarMap, n * 4) != hipSuccess ||
+        hipMalloc(&ctx->polarInv, n * 4) != hipSuccess) {
+        if (ctx->polarMap) (void)hipFree(ctx->polarMap);
+        ctx->polarMap = ctx->polarInv = nullptr;
+        return ckd::fail(CKD_ENOMEM, "ckd_polar_create: allocation failed");
+    }
+    CKD_HIP(hipMemcpy(ctx->polarMap, map.data(), n * 4, hipMemcpyHostToDevice));
+    CKD_HIP(hipMemcpy(ctx->polarInv, inv.data(), n * 4, hipMemcpyHostToDevice));
+    ctx->polarW = W;
+    ctx->polarH = H;
+    return CKD_OK;
+}
+
+ckd_status ckd_polar_blit(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, unsigned W,
+                          unsigned H, int inverse)
+{
+    CKD_REQUIRE(dst && src, "ckd_polar_blit: null buffer");
+    CKD_REQUIRE(dst != src, "ckd_polar_blit: dst must differ from src");
+    CKD_REQUIRE_CTX(ctx);
+    if (!ctx->polarMap || ctx->polarW != W || ctx->polarH != H)
+        return ckd::fail(CKD_ESTATE, "ckd_polar_blit: no polar maps for %ux%u (ckd_polar_create)",
+                         W, H);
+    dim3 g(ckd_div_up(W, 256), H);
+    k_polar_blit<<<g, 256, 0, ctx->stream>>>(
+        dst, src, reinterpret_cast<const int2 *>(inverse ? ctx->polarInv : ctx->polarMap), W, H);
+    return ckd::check_launch("polar_blit");
+}
+
+ckd_status ckd_grid_remap(ckd_ctx *ctx, uint32_t *dst, unsigned W, unsigned H,
+                          const int32_t *grid, const uint32_t *tex, unsigned texLog2)
+{
+    CKD_REQUIRE(dst && grid && tex, "ckd_grid_remap: null buffer");
+    CKD_REQUIRE(W > 0 && H > 0 && W <= 16384 && H <= 16384, "ckd_grid_remap: bad size %ux%u", W, H);
+    CKD_REQUIRE(texLog2 >= 1 && texLog2 <= 14, "ckd_grid_remap: texLog2 %u out of 1..14", texLog2);
+    CKD_REQUIRE_CTX(ctx);
+    return ckd::launch_grid_remap(ctx, dst, W, H, grid, tex, texLog2);
+}
+
+ckd_status ckd_tunnel_frame(ckd_ctx *ctx, uint32_t *dst, uint32_t *tmp0, uint32_t *tmp1,
+                            unsigned W, unsigned H, const int32_t *grid, const uint32_t *tex,
+                            unsigned texLog2, int radius)
+{
+    CKD_REQUIRE(dst && tmp0 && tmp1 && grid && tex, "ckd_tunnel_frame: null buffer");
+    CKD_REQUIRE(W > 0 && H > 0 && W <= 4096 && H <= 4096, "ckd_tunnel_frame: bad size %ux%u", W, H);
+    CKD_REQUIRE(texLog2 >= 1 && texLog2 <= 14, "ckd_tunnel_frame: texLog2 %u out of 1..14", texLog2);
+    CKD_REQUIRE(radius >= 1 && radius <= 127 && (unsigned)(2 * radius + 1) <= (W < H ? W : H),
+                "ckd_tunnel_frame: radius %d out of range", radius);
+    CKD_REQUIRE_CTX(ctx);
+    ckd_status s = ckd::launch_grid_remap(ctx, dst, W, H, grid, tex, texLog2);
+    if (s != CKD_OK) return s;
+    const float strength = (2.f * (float)radius + 1.5f) / 255.f;
+    s = ckd::blur07_hv_oop(ctx, tmp0, tmp1, dst, W, H, strength);
+    if (s != CKD_OK) return s;
+    return ckd::launch_blit_add(ctx, dst, tmp1, W, W, W, H);
+}
+
+} // extern "C"

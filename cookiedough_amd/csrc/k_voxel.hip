@@ -1,0 +1,325 @@
+// k_voxel.hip -- voxel landscape columns (code/landscape.cpp) and the
+// tunnelscape rows (code/tunnelscape.cpp).
+//
+// One lane per column (rows for tunnelscape) marches the reference's 512
+// front-to-back steps and draws the visible spans with the exact
+// cspanISSE16 arithmetic.  The memset32 of the fog colour is fused: after
+// the march every lane fills the part of its column no span reached, so
+// each pixel is written exactly once.
+#include "ckd_internal.h"
+#include "ckd_pixel.h"
+
+#include <cstring>
+#include <vector>
+
+namespace {
+
+constexpr unsigned kMapAnd = 1023, kMapShift = 10, kRayLength = 512;
+
+struct LsFrame {
+    float X1, Y1, viewCos, viewSin;
+    int mapTilt; // already scaled to H rows
+};
+
+constexpr int kLsFramesPerLaunch = 64;
+
+struct LsLaunch {
+    float rayY, colScale, halfW;
+    int mapScale;
+    LsFrame f[kLsFramesPerLaunch];
+};
+
+// exact floor(num/den) for num < 2^24, 0 < den < 2^24: a reciprocal
+// estimate is within one of the quotient; one correction each way fixes it
+CKD_DEV uint32_t udiv24(uint32_t num, uint32_t den)
+{
+    uint32_t q = (uint32_t)((float)num * __builtin_amdgcn_rcpf((float)den));
+    int32_t r = (int32_t)(num - q * den);
+    if (r < 0) {
+        --q;
+        r += (int32_t)den;
+    }
+    if (r >= (int32_t)den) ++q;
+    return q;
+}
+
+// Per-column ray set-up of vscape (code/landscape.cpp:164-190) and
+// voxel::vrot2D / vnorm2D (code/voxel-shared.h:8-24) in FP32 with no
+// contraction and correctly rounded division and square root, so the
+// integer ray parameters equal the host's bit for bit.
+#pragma clang fp contract(off)
+CKD_DEV void ls_setup(const LsLaunch &P, const LsFrame &F, unsigned iRay, int &dX, int &dY,
+                      int &fpFishMul)
+{
+    const float rayX = P.colScale * ((float)iRay - P.halfW);
+    const float rayY = P.rayY;
+    const float rotRayX = F.viewCos * rayX - F.viewSin * rayY;
+    const float rotRayY = F.viewSin * rayX + F.viewCos * rayY;
+    const float X2 = F.X1 + rotRayX;
+    const float Y2 = F.Y1 + rotRayY;
+    float fdX = X2 - F.X1;
+    float fdY = Y2 - F.Y1;
+    if (fabsf(fdX + fdY) > 1.1920928955078125e-07f) {
+        const float length = 1.f / sqrtf(fdX * fdX + fdY * fdY);
+        fdX *= length;
+        fdY *= length;
+    }
+    const float fishMul = rayY / sqrtf(rotRayX * rotRayX + rotRayY * rotRayY);
+    dX = (int)(fdX * 256.f);
+    dY = (int)(fdY * 256.f);
+    fpFishMul = (int)(fabsf(fishMul) * 256.f);
+}
+#pragma clang fp contract(on)
+
+// vscape_ray (code/landscape.cpp:56-107) for column iRay of frame blockIdx.y
+__global__ __launch_bounds__(64) void k_landscape(uint32_t *__restrict__ dst, unsigned W,
+                                                  unsigned H, const uint8_t *__restrict__ hmap,
+                                                  const uint32_t *__restrict__ cmap,
+                                                  const uint32_t *__restrict__ fog, LsLaunch P)
+{
+    const unsigned iRay = blockIdx.x * 64 + threadIdx.x;
+    if (iRay >= W) return;
+    const LsFrame F = P.f[blockIdx.y];
+    uint32_t *col = dst + (size_t)blockIdx.y * W * H + iRay;
+
+    int dX, dY, fpFishMul;
+    ls_setup(P, F, iRay, dX, dY, fpFishMul);
+    int curX = (int)(F.X1 * 256.f), curY = (int)(F.Y1 * 256.f);
+
+    int lastHeight = (int)H, lastDrawn = (int)H;
+    uint32_t lastColor =
+        cmap[(((unsigned)(curX >> 8)) & kMapAnd) | ((((unsigned)(curY >> 8)) & kMapAnd) << kMapShift)];
+
+    for (unsigned iStep = 0; iStep < kRayLength; ++iStep) {
+        curX += dX;
+        curY += dY;
+        const unsigned u0 = ((unsigned)(curX >> 8)) & kMapAnd;
+        const unsigned u1 = ((unsigned)(curX >> 8) + 1) & kMapAnd;
+        const unsigned v0 = (((unsigned)(curY >> 8)) & kMapAnd) << kMapShift;
+        const unsigned v1 = (((unsigned)(curY >> 8) + 1) & kMapAnd) << kMapShift;
+        const uint32_t fu = (uint32_t)curX & 0xff, fv = (uint32_t)curY & 0xff;
+        const uint32_t h = ckd_bilerp8(hmap[u0 + v0], hmap[u1 + v0], hmap[u0 + v1],
+                                       hmap[u1 + v1], fu, fv);
+        uint32_t color = ckd_bilerp32(cmap[u0 + v0], cmap[u1 + v0], cmap[u0 + v1],
+                                      cmap[u1 + v1], fu, fv);
+        color = ckd_subs_u8x4(color, fog[iStep >> 1]);
+
+        // ((255-h)<<16) / (fpFishMul*(iStep+1)) is an unsigned division
+        const uint32_t q = udiv24((255u - h) << 16, (uint32_t)fpFishMul * (iStep + 1));
+        const int hgt = (((int)q * P.mapScale) >> 8) + F.mapTilt;
+
+        if (hgt < lastDrawn) {
+            const uint32_t drawLength = (uint32_t)(lastDrawn - hgt);
+            CkdSpan s;
+            ckd_span_setup(s, color, lastColor, (uint32_t)(lastHeight - hgt), drawLength);
+            uint32_t *p = col + (size_t)hgt * W;
+            for (uint32_t k = 0; k < drawLength; ++k) {
+                *p = ckd_span_pixel(s.acc);
+                p += W;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) s.acc[c] += s.step[c];
+            }
+            lastDrawn = hgt;
+        }
+        lastHeight = hgt;
+        lastColor = color;
+    }
+    // the fused memset32(fog[0]) of Landscape_Draw for the rows never drawn
+    const uint32_t sky = fog[0];
+    for (int y = 0; y < lastDrawn; ++y) col[(size_t)y * W] = sky;
+}
+
+// ---- tunnelscape ---------------------------------------------------------
+
+constexpr unsigned kTsAnd = 2047, kTsShift = 11;
+
+// tscape_ray (code/tunnelscape.cpp:44-100), lane per output row.  The
+// caller guarantees map heights <= 159, so every height is positive.
+__global__ __launch_bounds__(64) void k_tunnelscape(uint32_t *__restrict__ dst, unsigned W,
+                                                    unsigned H, const uint8_t *__restrict__ hmap,
+                                                    const uint32_t *__restrict__ cmap,
+                                                    const uint32_t *__restrict__ fog,
+                                                    float mapStepX, float fromXAdd, int fpFromY,
+                                                    int dX, int dY, float kMapViewLenScale)
+{
+    const unsigned iRay = blockIdx.x * 64 + threadIdx.x;
+    if (iRay >= H) return;
+    uint32_t *row = dst + (size_t)iRay * W;
+    int curX, curY = fpFromY;
+    {
+#pragma clang fp contract(off)
+        const float mapX = (float)iRay * mapStepX;
+        const float fromX = mapX + fromXAdd;
+        curX = (int)(fromX * 256.f);
+    }
+    int lastHeight = (int)W, lastDrawn = (int)W;
+    uint32_t lastColor =
+        cmap[(((unsigned)(curX >> 8)) & kTsAnd) | ((((unsigned)(curY >> 8)) & kTsAnd) << kTsShift)];
+    unsigned written = 0;
+    for (unsigned iStep = 0; iStep < kRayLength; ++iStep) {
+        curX += dX;
+        curY += dY;
+        const unsigned u0 = ((unsigned)(curX >> 8)) & kTsAnd;
+        const unsigned u1 = ((unsigned)(curX >> 8) + 1) & kTsAnd;
+        const unsigned v0 = (((unsigned)(curY >> 8)) & kTsAnd) << kTsShift;
+        const unsigned v1 = (((unsigned)(curY >> 8) + 1) & kTsAnd) << kTsShift;
+        const uint32_t fu = (uint32_t)curX & 0xff, fv = (uint32_t)curY & 0xff;
+        const uint32_t h = ckd_bilerp8(hmap[u0 + v0], hmap[u1 + v0], hmap[u0 + v1],
+                                       hmap[u1 + v1], fu, fv);
+        uint32_t color = ckd_bilerp32(cmap[u0 + v0], cmap[u1 + v0], cmap[u0 + v1],
+                                      cmap[u1 + v1], fu, fv);
+        color = ckd_subs_u8x4(color, fog[iStep >> 1]);
+
+        int height = 255 - (int)h - 96;
+        height <<= 8;
+        height = (int)((float)height / kMapViewLenScale);
+        height = (int)((unsigned)height / (iStep + 1));
+        height *= 160;
+        height >>= 8;
+        height += 120;
+
+        if (height < lastDrawn) {
+            const uint32_t drawLength = (uint32_t)(lastDrawn - height);
+            CkdSpan s;
+            ckd_span_setup(s, color, lastColor, (uint32_t)(lastHeight - height), drawLength);
+            for (uint32_t k = 0; k < drawLength; ++k) {
+                row[written + k] = ckd_span_pixel(s.acc);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) s.acc[c] += s.step[c];
+            }
+            written += drawLength;
+            lastDrawn = height;
+        }
+        lastHeight = height;
+        lastColor = color;
+    }
+    const uint32_t sky = fog[0];
+    for (unsigned x = written; x < W; ++x) row[x] = sky;
+}
+
+ckd_status upload(const void *host, size_t bytes, void **dev)
+{
+    if (*dev) {
+        (void)hipFree(*dev);
+        *dev = nullptr;
+    }
+    if (hipMalloc(dev, bytes) != hipSuccess) {
+        *dev = nullptr;
+        return ckd::fail(CKD_ENOMEM, "map upload: hipMalloc(%zu) failed", bytes);
+    }
+    CKD_HIP(hipMemcpy(*dev, host, bytes, hipMemcpyHostToDevice));
+    return CKD_OK;
+}
+
+float ls_rayY(unsigned W, unsigned H)
+{
+#pragma clang fp contract(off)
+    const float kPI = 3.1415926535897932384626433832795f;
+    const float aspect = (float)H / (float)W;
+    const float viewLenScale = aspect * (kPI * 0.1f);
+    return 1024.f * viewLenScale;
+}
+
+int scale_rows(int v720, unsigned H) { return (int)(((long long)v720 * (long long)H) / 720); }
+
+} // namespace
+
+extern "C" {
+
+ckd_status ckd_landscape_create(ckd_ctx *ctx, const uint8_t *heightMap, const uint32_t *colorMap,
+                                const uint32_t *fog256)
+{
+    CKD_REQUIRE(heightMap && colorMap && fog256, "ckd_landscape_create: null map");
+    CKD_REQUIRE_CTX(ctx);
+    CKD_HIP(hipStreamSynchronize(ctx->stream));
+    ckd_status s = upload(heightMap, 1024 * 1024, (void **)&ctx->lsHeight);
+    if (s == CKD_OK) s = upload(colorMap, 1024 * 1024 * 4, (void **)&ctx->lsColor);
+    if (s == CKD_OK) s = upload(fog256, 256 * 4, (void **)&ctx->lsFog);
+    if (s == CKD_OK) ctx->lsFog0 = fog256[0];
+    return s;
+}
+
+ckd_status ckd_landscape_draw(ckd_ctx *ctx, uint32_t *dst, unsigned W, unsigned H,
+                              const ckd_landscape_cam *cams, unsigned nFrames)
+{
+    CKD_REQUIRE(dst && cams, "ckd_landscape_draw: null buffer");
+    CKD_REQUIRE(W >= 8 && H >= 8 && W <= 4096 && H <= 2160, "ckd_landscape_draw: bad size %ux%u",
+                W, H);
+    for (unsigned f = 0; f < nFrames; ++f)
+        CKD_REQUIRE(cams[f].mapTilt >= 0 && cams[f].mapTilt <= 180,
+                    "ckd_landscape_draw: frame %u: mapTilt %d outside 0..180", f, cams[f].mapTilt);
+    CKD_REQUIRE_CTX(ctx);
+    if (!ctx->lsHeight) return ckd::fail(CKD_ESTATE, "ckd_landscape_draw: no maps (ckd_landscape_create)");
+    LsLaunch P;
+    memset(&P, 0, sizeof(P));
+    P.rayY = ls_rayY(W, H);
+    P.colScale = 0.25f * (1280.f / (float)W);
+    P.halfW = (float)W * 0.5f;
+    P.mapScale = scale_rows(512, H);
+    for (unsigned f0 = 0; f0 < nFrames; f0 += kLsFramesPerLaunch) {
+        const unsigned n = nFrames - f0 < (unsigned)kLsFramesPerLaunch ? nFrames - f0
+                                                                       : (unsigned)kLsFramesPerLaunch;
+        for (unsigned k = 0; k < n; ++k) {
+            const ckd_landscape_cam &c = cams[f0 + k];
+            P.f[k] = LsFrame{c.X1, c.Y1, c.viewCos, c.viewSin, scale_rows(c.mapTilt, H)};
+        }
+        dim3 grid(ckd_div_up(W, 64), n);
+        k_landscape<<<grid, 64, 0, ctx->stream>>>(dst + (size_t)f0 * W * H, W, H, ctx->lsHeight,
+                                                  ctx->lsColor, ctx->lsFog, P);
+        ckd_status s = ckd::check_launch("landscape");
+        if (s != CKD_OK) return s;
+    }
+    return CKD_OK;
+}
+
+ckd_status ckd_tunnelscape_create(ckd_ctx *ctx, const uint8_t *heightMap, const uint32_t *colorMap,
+                                  const uint32_t *fog256)
+{
+    CKD_REQUIRE(heightMap && colorMap && fog256, "ckd_tunnelscape_create: null map");
+    for (size_t i = 0; i < 2048u * 2048u; ++i)
+        CKD_REQUIRE(heightMap[i] <= 159,
+                    "ckd_tunnelscape_create: height %u at %zu > 159 (negative span heights)",
+                    heightMap[i], i);
+    CKD_REQUIRE_CTX(ctx);
+    CKD_HIP(hipStreamSynchronize(ctx->stream));
+    ckd_status s = upload(heightMap, 2048 * 2048, (void **)&ctx->tsHeight);
+    if (s == CKD_OK) s = upload(colorMap, 2048 * 2048 * 4, (void **)&ctx->tsColor);
+    if (s == CKD_OK) s = upload(fog256, 256 * 4, (void **)&ctx->tsFog);
+    if (s == CKD_OK) ctx->tsFog0 = fog256[0];
+    return s;
+}
+
+ckd_status ckd_tunnelscape_rows(ckd_ctx *ctx, uint32_t *dst, unsigned W, unsigned H, float stepU,
+                                float stepV, float speed, float time)
+{
+    CKD_REQUIRE(dst, "ckd_tunnelscape_rows: null buffer");
+    CKD_REQUIRE(W >= 8 && H >= 8 && W <= 4096 && H <= 4096, "ckd_tunnelscape_rows: bad size %ux%u",
+                W, H);
+    CKD_REQUIRE_CTX(ctx);
+    if (!ctx->tsHeight)
+        return ckd::fail(CKD_ESTATE, "ckd_tunnelscape_rows: no maps (ckd_tunnelscape_create)");
+    // tscape (code/tunnelscape.cpp:103-118) frame constants, host FP32
+    float mapStepX, fromXAdd, kMapViewLenScale;
+    int dX, dY, fpFromY;
+    {
+#pragma clang fp contract(off)
+        const float kGoldenRatio = 1.61803398875f;
+        const float aspect = (float)H / (float)W;
+        const float kOneOverAspect = 1.f / aspect;
+        // kMapViewLenScale is the reference's 720p constant (code/tunnelscape.cpp:31)
+        kMapViewLenScale = (720.f / 1280.f) * 0.5f;
+        mapStepX = 2048.f / (float)(H - 1);
+        const float speedMul = sqrtf(stepU * stepU + stepV * stepV) * speed;
+        const float fromY = 1024.f + speedMul * time;
+        dX = (int)(stepV * 256.f);
+        dY = (int)((kOneOverAspect * stepU) * 256.f);
+        fpFromY = (int)(fromY * 256.f);
+        fromXAdd = stepU * time * kGoldenRatio;
+    }
+    k_tunnelscape<<<ckd_div_up(H, 64), 64, 0, ctx->stream>>>(dst, W, H, ctx->tsHeight, ctx->tsColor,
+                                                            ctx->tsFog, mapStepX, fromXAdd, fpFromY,
+                                                            dX, dY, kMapViewLenScale);
+    return ckd::check_launch("tunnelscape");
+}
+
+} // extern "C"

@@ -1,0 +1,183 @@
+// runtime.hip -- context, errors, device memory, events.
+#include "ckd_internal.h"
+
+#include <cstdio>
+#include <cstring>
+
+namespace {
+thread_local char g_lastError[512] = "";
+}
+
+namespace ckd {
+
+ckd_status fail(ckd_status s, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_lastError, sizeof(g_lastError), fmt, ap);
+    va_end(ap);
+    return s;
+}
+
+ckd_status check_launch(const char *what)
+{
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return fail(CKD_EDEVICE, "%s: launch failed: %s", what, hipGetErrorString(e));
+    return CKD_OK;
+}
+
+ckd_status stage(ckd_ctx *ctx, const void *host, size_t bytes, void **dev)
+{
+    if (bytes > ctx->stagingCap) {
+        // the previous staging buffer may still be read by queued work
+        if (ctx->staging) {
+            CKD_HIP(hipStreamSynchronize(ctx->stream));
+            CKD_HIP(hipFree(ctx->staging));
+            ctx->staging = nullptr;
+            ctx->stagingCap = 0;
+        }
+        size_t cap = 4096;
+        while (cap < bytes) cap *= 2;
+        if (hipMalloc(&ctx->staging, cap) != hipSuccess) {
+            ctx->staging = nullptr;
+            return fail(CKD_ENOMEM, "stage: hipMalloc(%zu) failed", cap);
+        }
+        ctx->stagingCap = cap;
+    }
+    // Reuse needs no wait: the copy is ordered on the context stream after
+    // every kernel that read the previous contents.
+    CKD_HIP(hipMemcpyAsync(ctx->staging, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+    *dev = ctx->staging;
+    return CKD_OK;
+}
+
+} // namespace ckd
+
+extern "C" {
+
+const char *ckd_last_error(void) { return g_lastError; }
+
+ckd_status ckd_create(int device, ckd_ctx **out)
+{
+    CKD_REQUIRE(out != nullptr, "ckd_create: null output pointer");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+        return ckd::fail(CKD_EDEVICE, "ckd_create: no HIP device available");
+    CKD_REQUIRE(device >= 0 && device < n, "ckd_create: device %d out of range (%d devices)",
+                device, n);
+    CKD_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    CKD_HIP(hipGetDeviceProperties(&prop, device));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return ckd::fail(CKD_EDEVICE, "ckd_create: device %d is %s, this library is built for gfx950",
+                         device, prop.gcnArchName);
+    ckd_ctx *ctx = new ckd_ctx();
+    ctx->device = device;
+    ctx->cuCount = prop.multiProcessorCount;
+    for (int i = 0; i < CKD_NUM_EVENTS; ++i) {
+        if (hipEventCreate(&ctx->events[i]) != hipSuccess) {
+            ckd_destroy(ctx);
+            return ckd::fail(CKD_EDEVICE, "ckd_create: hipEventCreate failed");
+        }
+    }
+    *out = ctx;
+    return CKD_OK;
+}
+
+void ckd_destroy(ckd_ctx *ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    for (int i = 0; i < CKD_NUM_EVENTS; ++i)
+        if (ctx->events[i]) (void)hipEventDestroy(ctx->events[i]);
+    void *bufs[] = {ctx->lsHeight, ctx->lsColor, ctx->lsFog, ctx->tsHeight, ctx->tsColor,
+                    ctx->tsFog, ctx->polarMap, ctx->polarInv, ctx->b26Scratch[0],
+                    ctx->b26Scratch[1], ctx->staging};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    delete ctx;
+}
+
+ckd_status ckd_set_stream(ckd_ctx *ctx, void *hipStream)
+{
+    CKD_REQUIRE_CTX(ctx);
+    ctx->stream = (hipStream_t)hipStream;
+    return CKD_OK;
+}
+
+ckd_status ckd_synchronize(ckd_ctx *ctx)
+{
+    CKD_REQUIRE_CTX(ctx);
+    CKD_HIP(hipStreamSynchronize(ctx->stream));
+    return CKD_OK;
+}
+
+ckd_status ckd_malloc(ckd_ctx *ctx, void **dptr, size_t bytes)
+{
+    CKD_REQUIRE(dptr != nullptr, "ckd_malloc: null output pointer");
+    CKD_REQUIRE_CTX(ctx);
+    *dptr = nullptr;
+    if (bytes == 0) return CKD_OK;
+    if (hipMalloc(dptr, bytes) != hipSuccess) {
+        *dptr = nullptr;
+        return ckd::fail(CKD_ENOMEM, "ckd_malloc: hipMalloc(%zu) failed", bytes);
+    }
+    return CKD_OK;
+}
+
+ckd_status ckd_free(ckd_ctx *ctx, void *dptr)
+{
+    CKD_REQUIRE_CTX(ctx);
+    if (dptr) CKD_HIP(hipFree(dptr));
+    return CKD_OK;
+}
+
+ckd_status ckd_memcpy_h2d(ckd_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    CKD_REQUIRE(bytes == 0 || (dst && src), "ckd_memcpy_h2d: null pointer");
+    CKD_REQUIRE_CTX(ctx);
+    if (bytes) CKD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return CKD_OK;
+}
+
+ckd_status ckd_memcpy_d2h(ckd_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    CKD_REQUIRE(bytes == 0 || (dst && src), "ckd_memcpy_d2h: null pointer");
+    CKD_REQUIRE_CTX(ctx);
+    if (bytes) {
+        CKD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        CKD_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    return CKD_OK;
+}
+
+ckd_status ckd_memcpy_d2d(ckd_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    CKD_REQUIRE(bytes == 0 || (dst && src), "ckd_memcpy_d2d: null pointer");
+    CKD_REQUIRE_CTX(ctx);
+    if (bytes) CKD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    return CKD_OK;
+}
+
+ckd_status ckd_event_record(ckd_ctx *ctx, int slot)
+{
+    CKD_REQUIRE(slot >= 0 && slot < CKD_NUM_EVENTS, "ckd_event_record: slot %d out of range", slot);
+    CKD_REQUIRE_CTX(ctx);
+    CKD_HIP(hipEventRecord(ctx->events[slot], ctx->stream));
+    return CKD_OK;
+}
+
+ckd_status ckd_event_elapsed_ms(ckd_ctx *ctx, int a, int b, float *ms)
+{
+    CKD_REQUIRE(a >= 0 && a < CKD_NUM_EVENTS && b >= 0 && b < CKD_NUM_EVENTS && ms,
+                "ckd_event_elapsed_ms: bad slot or null output");
+    CKD_REQUIRE_CTX(ctx);
+    CKD_HIP(hipEventSynchronize(ctx->events[b]));
+    CKD_HIP(hipEventElapsedTime(ms, ctx->events[a], ctx->events[b]));
+    return CKD_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,174 @@
+"""Thin Python wrapper over the C ABI: a context, device arrays and one
+method per entry point.  Used by the tests and bench.py; the C++ shim
+(cookiedough_amd/shim) is the drop-in for the reference's own C++ callers.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, ptr
+
+
+@dataclass
+class DevArray:
+    ctx: "Context"
+    ptr: int
+    shape: tuple
+    dtype: np.dtype
+
+    @property
+    def nbytes(self) -> int:
+        return int(np.prod(self.shape)) * np.dtype(self.dtype).itemsize
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        check(self.ctx.lib.ckd_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def view(self, offset_elems: int, shape) -> "DevArray":
+        it = np.dtype(self.dtype).itemsize
+        return DevArray(self.ctx, self.ptr + offset_elems * it, tuple(shape), self.dtype)
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.lib = _lib.load()
+        h = C.c_void_p()
+        check(self.lib.ckd_create(device, C.byref(h)))
+        self.h = h
+        self._owned: list[int] = []
+
+    # ---- memory ------------------------------------------------------------
+    def empty(self, shape, dtype=np.uint32) -> DevArray:
+        shape = tuple(shape) if isinstance(shape, (tuple, list)) else (shape,)
+        n = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        p = C.c_void_p()
+        check(self.lib.ckd_malloc(self.h, C.byref(p), max(n, 4)))
+        self._owned.append(p.value)
+        return DevArray(self, p.value, shape, np.dtype(dtype))
+
+    def upload(self, a: np.ndarray) -> DevArray:
+        a = np.ascontiguousarray(a)
+        d = self.empty(a.shape, a.dtype)
+        check(self.lib.ckd_memcpy_h2d(self.h, d.ptr, a.ctypes.data, a.nbytes))
+        self.sync()
+        return d
+
+    def write(self, d: DevArray, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=d.dtype)
+        assert a.nbytes == d.nbytes
+        check(self.lib.ckd_memcpy_h2d(self.h, d.ptr, a.ctypes.data, a.nbytes))
+        self.sync()
+
+    def free(self, d: DevArray) -> None:
+        if d.ptr in self._owned:
+            self._owned.remove(d.ptr)
+            check(self.lib.ckd_free(self.h, d.ptr))
+
+    def sync(self) -> None:
+        check(self.lib.ckd_synchronize(self.h))
+
+    def close(self) -> None:
+        if self.h:
+            for p in self._owned:
+                self.lib.ckd_free(self.h, p)
+            self._owned.clear()
+            self.lib.ckd_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- timing --------------------------------------------------------------
+    def record(self, slot: int) -> None:
+        check(self.lib.ckd_event_record(self.h, slot))
+
+    def elapsed_ms(self, a: int, b: int) -> float:
+        ms = C.c_float()
+        check(self.lib.ckd_event_elapsed_ms(self.h, a, b, C.byref(ms)))
+        return ms.value
+
+    # ---- entry points --------------------------------------------------------
+    def memset32(self, dst, value, n):
+        check(self.lib.ckd_memset32(self.h, ptr(dst), value, n))
+
+    def fx_blit_2x2(self, dst, fxmap, W, H):
+        check(self.lib.ckd_fx_blit_2x2(self.h, ptr(dst), ptr(fxmap), W, H))
+
+    def add32(self, dst, src, n):
+        check(self.lib.ckd_add32(self.h, ptr(dst), ptr(src), n))
+
+    def blit_add32(self, dst, src, destResX, srcResX, yRes):
+        check(self.lib.ckd_blit_add32(self.h, ptr(dst), ptr(src), destResX, srcResX, yRes))
+
+    def blit_add32a(self, dst, src, destResX, srcResX, yRes, alpha):
+        check(self.lib.ckd_blit_add32a(self.h, ptr(dst), ptr(src), destResX, srcResX, yRes, alpha))
+
+    def blur07_h(self, dst, src, W, H, strength):
+        check(self.lib.ckd_blur07_h(self.h, ptr(dst), ptr(src), W, H, strength))
+
+    def blur07_v(self, dst, src, W, H, strength):
+        check(self.lib.ckd_blur07_v(self.h, ptr(dst), ptr(src), W, H, strength))
+
+    def blur07(self, dst, src, W, H, strength):
+        check(self.lib.ckd_blur07(self.h, ptr(dst), ptr(src), W, H, strength))
+
+    def blur26_h(self, dst, src, W, H, strength, gain, passes):
+        check(self.lib.ckd_blur26_h(self.h, ptr(dst), ptr(src), W, H, strength, gain, passes))
+
+    def blur26_v(self, dst, src, W, H, strength, gain, passes):
+        check(self.lib.ckd_blur26_v(self.h, ptr(dst), ptr(src), W, H, strength, gain, passes))
+
+    def blur26(self, dst, src, W, H, strength, gain, passes):
+        check(self.lib.ckd_blur26(self.h, ptr(dst), ptr(src), W, H, strength, gain, passes))
+
+    def polar_create(self, W, H):
+        check(self.lib.ckd_polar_create(self.h, W, H))
+
+    def polar_blit(self, dst, src, W, H, inverse):
+        check(self.lib.ckd_polar_blit(self.h, ptr(dst), ptr(src), W, H, int(inverse)))
+
+    def grid_remap(self, dst, W, H, grid, tex, texLog2):
+        check(self.lib.ckd_grid_remap(self.h, ptr(dst), W, H, ptr(grid), ptr(tex), texLog2))
+
+    def landscape_create(self, hmap: np.ndarray, cmap: np.ndarray, fog: np.ndarray):
+        hmap = np.ascontiguousarray(hmap, np.uint8)
+        cmap = np.ascontiguousarray(cmap, np.uint32)
+        fog = np.ascontiguousarray(fog, np.uint32)
+        check(self.lib.ckd_landscape_create(self.h, hmap.ctypes.data, cmap.ctypes.data, fog.ctypes.data))
+
+    def landscape_draw(self, dst, W, H, cams):
+        arr = (_lib.LandscapeCam * len(cams))(*cams)
+        check(self.lib.ckd_landscape_draw(self.h, ptr(dst), W, H, arr, len(cams)))
+
+    def tunnelscape_create(self, hmap, cmap, fog):
+        hmap = np.ascontiguousarray(hmap, np.uint8)
+        cmap = np.ascontiguousarray(cmap, np.uint32)
+        fog = np.ascontiguousarray(fog, np.uint32)
+        check(self.lib.ckd_tunnelscape_create(self.h, hmap.ctypes.data, cmap.ctypes.data, fog.ctypes.data))
+
+    def tunnelscape_rows(self, dst, W, H, stepU, stepV, speed, time):
+        check(self.lib.ckd_tunnelscape_rows(self.h, ptr(dst), W, H, stepU, stepV, speed, time))
+
+    def sdf_render(self, dst, W, H, scene, cam12, shading, step_counter=None):
+        cam = (C.c_float * 12)(*[float(v) for v in cam12])
+        check(self.lib.ckd_sdf_render(self.h, ptr(dst), W, H, C.byref(scene), cam, shading,
+                                      ptr(step_counter)))
+
+    def tunnel_frame(self, dst, tmp0, tmp1, W, H, grid, tex, texLog2, radius):
+        check(self.lib.ckd_tunnel_frame(self.h, ptr(dst), ptr(tmp0), ptr(tmp1), W, H, ptr(grid),
+                                        ptr(tex), texLog2, radius))
+
+    def render_batch(self, requests, scene, shading, tex, texLog2, tmp, tmp_pixels,
+                     step_counter=None):
+        arr = (_lib.Request * len(requests))(*requests)
+        check(self.lib.ckd_render_batch(self.h, arr, len(requests),
+                                        C.byref(scene) if scene is not None else None, shading,
+                                        ptr(tex), texLog2, ptr(tmp), tmp_pixels, ptr(step_counter)))

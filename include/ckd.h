@@ -1,0 +1,231 @@
+/*
+ * ckd.h -- C ABI of the MI355X (gfx950) renderer for cookiedough's per-pixel
+ * effect loops.
+ *
+ * All entry points are extern "C", take plain pointers and sizes, and return
+ * a ckd_status; nothing throws across this boundary.  Pixel buffers are
+ * device pointers to uint32_t 0xAARRGGBB pixels (memory byte order B,G,R,A),
+ * rows contiguous.  Work is enqueued on the context's stream (hipStream_t
+ * passed as void*, default: the null stream) and is asynchronous unless the
+ * comment says otherwise.  A context is used by one host thread at a time.
+ *
+ * Each entry point names the reference interface it replaces as file:line
+ * of the reference repository (visualizersdotnl/cookiedough).
+ */
+#ifndef CKD_H
+#define CKD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CKD_API __attribute__((visibility("default")))
+
+typedef enum ckd_status {
+    CKD_OK = 0,
+    CKD_EINVAL = 1,  /* bad argument (size, pointer, parameter range)       */
+    CKD_ENOMEM = 2,  /* device allocation failed                            */
+    CKD_EDEVICE = 3, /* no usable device / HIP error                        */
+    CKD_ESTATE = 4   /* missing set-up (maps not created, ...)              */
+} ckd_status;
+
+typedef struct ckd_ctx ckd_ctx;
+
+/* ---- context ------------------------------------------------------------
+ * ckd_create replaces the set-up chain of code/main.cpp:263-279
+ * (CalculateCosLUT, Polar_Create, FxBlitter_Create, BoxBlur_Create) and
+ * ckd_destroy the teardown at code/main.cpp:372-374.  Errors are reported
+ * through ckd_last_error, the counterpart of SetLastError (code/main.h:49). */
+CKD_API ckd_status ckd_create(int device, ckd_ctx **out);
+CKD_API void ckd_destroy(ckd_ctx *ctx);
+/* message of the last failing call on this host thread ("" if none) */
+CKD_API const char *ckd_last_error(void);
+CKD_API ckd_status ckd_set_stream(ckd_ctx *ctx, void *hipStream);
+CKD_API ckd_status ckd_synchronize(ckd_ctx *ctx);
+
+/* ---- device memory and timing plumbing (no reference counterpart: the
+ * reference's buffers are host memory, code/main.cpp:307) ---------------- */
+CKD_API ckd_status ckd_malloc(ckd_ctx *ctx, void **dptr, size_t bytes);
+CKD_API ckd_status ckd_free(ckd_ctx *ctx, void *dptr);
+CKD_API ckd_status ckd_memcpy_h2d(ckd_ctx *ctx, void *dst, const void *src,
+                                  size_t bytes); /* stream-ordered        */
+CKD_API ckd_status ckd_memcpy_d2h(ckd_ctx *ctx, void *dst, const void *src,
+                                  size_t bytes); /* synchronous            */
+CKD_API ckd_status ckd_memcpy_d2d(ckd_ctx *ctx, void *dst, const void *src,
+                                  size_t bytes);
+/* HIP events on the context stream; slots 0..15 */
+CKD_API ckd_status ckd_event_record(ckd_ctx *ctx, int slot);
+CKD_API ckd_status ckd_event_elapsed_ms(ckd_ctx *ctx, int slotA, int slotB,
+                                        float *ms);
+
+/* ---- pixel primitives ---------------------------------------------------- */
+/* memset32, code/util.h:57-67 */
+CKD_API ckd_status ckd_memset32(ckd_ctx *ctx, uint32_t *dst, uint32_t value,
+                                size_t numInts);
+/* Fx_Blit_2x2, code/fx-blitter.cpp:27-75.  fxMap is (dstW/2+4) x (dstH/2+4)
+ * pixels (code/fx-blitter.h:16-17 at any even size); dstW/2 % 4 == 0. */
+CKD_API ckd_status ckd_fx_blit_2x2(ckd_ctx *ctx, uint32_t *dst,
+                                   const uint32_t *fxMap, unsigned dstW,
+                                   unsigned dstH);
+/* Add32, code/util.cpp:132-145 */
+CKD_API ckd_status ckd_add32(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src,
+                             unsigned numPixels);
+/* BlitAdd32, code/util.cpp:754-773 */
+CKD_API ckd_status ckd_blit_add32(ckd_ctx *ctx, uint32_t *dst,
+                                  const uint32_t *src, unsigned destResX,
+                                  unsigned srcResX, unsigned yRes);
+/* BlitAdd32A, code/util.cpp:775-796 */
+CKD_API ckd_status ckd_blit_add32a(ckd_ctx *ctx, uint32_t *dst,
+                                   const uint32_t *src, unsigned destResX,
+                                   unsigned srcResX, unsigned yRes, float alpha);
+
+/* ---- 2007 box blur, code/deprecated/boxblur.cpp -------------------------
+ * dst == src is allowed and reproduces the reference's in-place recurrence
+ * (the sliding-window subtract then reads already-blurred pixels).  The
+ * kernel span clampi(1,255,unsigned(strength*255)) must not exceed the pass
+ * length (xRes for H, yRes for V); strength must be finite and >= 0. */
+/* HorizontalBoxBlur32, code/deprecated/boxblur.cpp:44-126 */
+CKD_API ckd_status ckd_blur07_h(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src,
+                                unsigned xRes, unsigned yRes, float strength);
+/* VerticalBoxBlur32, code/deprecated/boxblur.cpp:128-220 */
+CKD_API ckd_status ckd_blur07_v(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src,
+                                unsigned xRes, unsigned yRes, float strength);
+/* BoxBlur32 (H, then V in place on dst), code/deprecated/boxblur.cpp:222-231 */
+CKD_API ckd_status ckd_blur07(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src,
+                              unsigned xRes, unsigned yRes, float strength);
+
+/* ---- 2026 box blur, code/boxblur.cpp -------------------------------------
+ * xRes, yRes <= 2048; strength in [0,100]; gain in [0,1]; passes >= 1;
+ * dst != src.  The two pixels the reference reads past the end of the last
+ * row of a pass (code/boxblur.cpp:185,195) are taken as zero. */
+/* BoxBlur_Horz32, code/boxblur.cpp:270-279 */
+CKD_API ckd_status ckd_blur26_h(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src,
+                                unsigned xRes, unsigned yRes, float strength,
+                                float gain, unsigned numPasses);
+/* BoxBlur_Vert32, code/boxblur.cpp:281-297 (xRes, yRes multiples of 4) */
+CKD_API ckd_status ckd_blur26_v(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src,
+                                unsigned xRes, unsigned yRes, float strength,
+                                float gain, unsigned numPasses);
+/* BoxBlur_32, code/boxblur.cpp:299-318 (xRes, yRes multiples of 4) */
+CKD_API ckd_status ckd_blur26(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src,
+                              unsigned xRes, unsigned yRes, float strength,
+                              float gain, unsigned numPasses);
+
+/* ---- polar UV-map remap, code/polar.cpp ----------------------------------
+ * ckd_polar_create computes the 24:8 maps of CalculateMaps
+ * (code/polar.cpp:18-59) on the host in FP32 for a W x H target and uploads
+ * them (Polar_Create, code/polar.cpp:61-72).  Polar_Blit
+ * (code/polar.cpp:135-154) then remaps src (W x H) into dst (W x H). */
+CKD_API ckd_status ckd_polar_create(ckd_ctx *ctx, unsigned W, unsigned H);
+CKD_API ckd_status ckd_polar_blit(ckd_ctx *ctx, uint32_t *dst,
+                                  const uint32_t *src, unsigned W, unsigned H,
+                                  int inverse);
+
+/* ---- per-8x8-block UV grid remap (no reference counterpart: the grid law
+ * of the tunnel benchmark; the per-pixel fetch is bsamp_prepUVs +
+ * bsamp32_32, code/bilinear.h:10-125).  grid holds (ceil(W/8)+1) x
+ * (ceil(H/8)+1) corner (U,V) pairs in 24:8; tex is 2^texLog2 square. */
+CKD_API ckd_status ckd_grid_remap(ckd_ctx *ctx, uint32_t *dst, unsigned W,
+                                  unsigned H, const int32_t *grid,
+                                  const uint32_t *tex, unsigned texLog2);
+
+/* ---- voxel landscape, code/landscape.cpp ---------------------------------
+ * Landscape_Create (code/landscape.cpp:196-222) with the maps passed in:
+ * 1024x1024 u8 heights, 1024x1024 ARGB colours and the 256-entry fog LUT,
+ * all host pointers, copied to device memory the context owns. */
+CKD_API ckd_status ckd_landscape_create(ckd_ctx *ctx, const uint8_t *heightMap,
+                                        const uint32_t *colorMap,
+                                        const uint32_t *fog256);
+typedef struct ckd_landscape_cam {
+    float X1, Y1;           /* origin in map texels (moveX+strafeX, ...)    */
+    float viewCos, viewSin; /* cosf/sinf of the view angle                  */
+    int mapTilt;            /* s_mapTilt at 720 rows (90 + int(tilt))       */
+} ckd_landscape_cam;
+/* Landscape_Draw without TapeWarp32 (code/landscape.cpp:228-243): fills dst
+ * (W x H, W <= 4096, H <= 2160) for each of nFrames cameras; frame f is
+ * written at dst + f*W*H.  The 720-row constants kMapScale and s_mapTilt
+ * are scaled by H/720 and the column spacing by 1280/W. */
+CKD_API ckd_status ckd_landscape_draw(ckd_ctx *ctx, uint32_t *dst, unsigned W,
+                                      unsigned H, const ckd_landscape_cam *cams,
+                                      unsigned nFrames);
+
+/* ---- tunnelscape rows, code/tunnelscape.cpp ------------------------------
+ * Tunnelscape_Create (code/tunnelscape.cpp:138-162) with 2048x2048 maps (all
+ * heights <= 159, see code/tunnelscape.cpp:75-84) and the fog LUT. */
+CKD_API ckd_status ckd_tunnelscape_create(ckd_ctx *ctx, const uint8_t *heightMap,
+                                          const uint32_t *colorMap,
+                                          const uint32_t *fog256);
+/* memset32 + tscape (code/tunnelscape.cpp:103-134, :170-171) into dst */
+CKD_API ckd_status ckd_tunnelscape_rows(ckd_ctx *ctx, uint32_t *dst, unsigned W,
+                                        unsigned H, float stepU, float stepV,
+                                        float speed, float time);
+
+/* ---- synthetic SDF scene (no reference counterpart: the sphere-tracing
+ * benchmark scene, structured like RenderSpikeyMap_2x2_Distant,
+ * code/shadertoy.cpp:525-598) -------------------------------------------- */
+typedef struct ckd_sdf_scene {
+    int nSpheres;           /* 1..32                                        */
+    float sphere[32][4];    /* cx, cy, cz, radius                           */
+    float albedo[33][3];    /* B,G,R of sphere i; entry nSpheres = ground   */
+    float groundY;
+    float light[3];         /* unit vector towards the light                */
+    float fogColor[3];      /* B,G,R                                        */
+    float fogDensity;       /* ExpFog scale: fog = 1-exp(-s*d^3)            */
+    float gamma;
+    float ambient;
+    float eps;              /* march stops when the distance <= eps         */
+    float tMax;             /* far clip                                     */
+    int maxSteps;           /* <= 1024                                      */
+    float camRadius, camHeight, camSpeed, focal;
+} ckd_sdf_scene;
+
+enum { CKD_SHADE_FP32 = 0, CKD_SHADE_FP16 = 1 };
+
+/* one frame per call ("plain"): dst W x H; cam is the float camera basis of
+ * the frame (origin, forward*focal, right, up: 12 floats, host pointer);
+ * stepCount (device, optional) accumulates the total number of march steps */
+CKD_API ckd_status ckd_sdf_render(ckd_ctx *ctx, uint32_t *dst, unsigned W,
+                                  unsigned H, const ckd_sdf_scene *scene,
+                                  const float *cam, int shading,
+                                  unsigned long long *stepCount);
+
+/* ---- c2 tunnel frame (grid remap, 2007 blur H then V out of place with
+ * span 2r+1, BlitAdd32 of the blur onto the remap); tmp0/tmp1 are W*H
+ * scratch images.  radius in 1..127 ---------------------------------------- */
+CKD_API ckd_status ckd_tunnel_frame(ckd_ctx *ctx, uint32_t *dst, uint32_t *tmp0,
+                                    uint32_t *tmp1, unsigned W, unsigned H,
+                                    const int32_t *grid, const uint32_t *tex,
+                                    unsigned texLog2, int radius);
+
+/* ---- batched renderer (no reference counterpart: many frames, effects and
+ * resolutions in one persistent tile-queue launch) ------------------------ */
+enum { CKD_FX_SDF = 0, CKD_FX_VOXEL = 1, CKD_FX_TUNNEL = 2 };
+
+typedef struct ckd_request {
+    int effect;             /* CKD_FX_*                                     */
+    unsigned W, H;
+    uint32_t *dst;          /* W*H output (device)                          */
+    float cam[12];          /* SDF: camera basis                            */
+    ckd_landscape_cam vcam; /* VOXEL: camera                                */
+    const int32_t *grid;    /* TUNNEL: corner grid (device)                 */
+    int radius;             /* TUNNEL: blur radius 1..127                   */
+} ckd_request;
+
+/* Renders requests[0..n) (host array).  SDF requests use `scene` and the
+ * given shading precision, VOXEL requests the landscape maps, TUNNEL
+ * requests `tex` (2^texLog2 square) and need scratch: tmp (device) of at
+ * least sum over TUNNEL requests of 2*W*H pixels.  stepCount as above. */
+CKD_API ckd_status ckd_render_batch(ckd_ctx *ctx, const ckd_request *requests,
+                                    unsigned n, const ckd_sdf_scene *scene,
+                                    int shading, const uint32_t *tex,
+                                    unsigned texLog2, uint32_t *tmp,
+                                    size_t tmpPixels,
+                                    unsigned long long *stepCount);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

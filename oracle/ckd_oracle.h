@@ -1,0 +1,171 @@
+/*
+ * ckd_oracle.h -- CPU restatement of cookiedough's per-pixel effect path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is linked or
+ * called only by tests/, by __graft_entry__.smoke() and by the cpu_baseline
+ * leg of bench.py.  The product (cookiedough_amd/libckd.so) never calls it.
+ *
+ * Every function restates one reference function and cites it as
+ * file:line relative to the reference repository root.  Conventions:
+ *   - pixels are uint32_t 0xAARRGGBB (memory byte order B,G,R,A);
+ *   - "24:8" is a signed int with 8 fractional bits;
+ *   - "u16 lanes" are the four channels widened to uint16_t, index 0 = B.
+ *
+ * Pinning status of each function (what checks it besides this restatement)
+ * is stated next to its declaration.
+ */
+#ifndef CKD_ORACLE_H
+#define CKD_ORACLE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- a1..a3 bilinear samplers (code/bilinear.h) ---------------------- */
+/* pinned: golden vectors from the compiled reference + exhaustive closed form */
+void orc_bsamp_prep(int U, int V, unsigned mapAnd, unsigned mapShift,
+                    unsigned *U0, unsigned *V0, unsigned *U1, unsigned *V1,
+                    unsigned *fracU, unsigned *fracV);
+unsigned orc_bsamp8(const uint8_t *tex, unsigned U0, unsigned V0, unsigned U1,
+                    unsigned V1, unsigned fracU, unsigned fracV);
+void orc_bsamp32_16(const uint32_t *tex, unsigned U0, unsigned V0, unsigned U1,
+                    unsigned V1, unsigned fracU, unsigned fracV, uint16_t out[4]);
+uint32_t orc_bsamp32_32_packed(const uint32_t *tex, unsigned U0, unsigned V0,
+                               unsigned U1, unsigned V1, unsigned fracU,
+                               unsigned fracV);
+
+/* ---- a4 colored span (code/cspan.h:47-78) ----------------------------- */
+/* pinned: golden vectors (lane quirks) from the compiled reference */
+void orc_cspan16(uint32_t *dst, ptrdiff_t destIncr, unsigned length,
+                 unsigned drawLength, const uint16_t A[4], const uint16_t B[4]);
+
+/* ---- a5..a7 voxel landscape (code/landscape.cpp) ---------------------- */
+typedef struct orc_landscape_cam {
+    float X1, Y1;          /* camera origin in map texels (float)           */
+    float viewCos, viewSin;/* cosf/sinf of the view angle (host libm)       */
+    int   mapTilt;         /* s_mapTilt at 720 rows (kMapTilt + int(tilt))  */
+} orc_landscape_cam;
+
+/* one column, reference vscape_ray (code/landscape.cpp:56-107) with the
+ * frame height, map scale and tilt as parameters (720 / 512 / s_mapTilt at
+ * the reference resolution).  pinned: golden frames from the compiled ref. */
+void orc_vscape_ray(uint32_t *pColumn, unsigned W, unsigned H,
+                    const uint8_t *heightMap, const uint32_t *colorMap,
+                    const uint32_t *fog256, int mapScale, int mapTilt,
+                    int curX, int curY, int dX, int dY, float fishMul);
+
+/* per-column FP32 ray set-up of vscape (code/landscape.cpp:164-190);
+ * outputs fpX1,fpY1 once and dX,dY,fishMul per column.
+ * pinned: straight restatement (the reference vscape() reads gamepad and
+ * Rocket state and cannot be called); invariants in tests. */
+void orc_vscape_setup(unsigned W, unsigned H, const orc_landscape_cam *cam,
+                      int *fpX1, int *fpY1, int *dX, int *dY, float *fishMul);
+
+/* whole frame: memset32(fog[0]) + vscape (code/landscape.cpp:228-243). */
+void orc_landscape_frame(uint32_t *dst, unsigned W, unsigned H,
+                         const uint8_t *heightMap, const uint32_t *colorMap,
+                         const uint32_t *fog256, const orc_landscape_cam *cam,
+                         int nthreads);
+
+/* ---- a8 tunnelscape rows (code/tunnelscape.cpp:44-134) ---------------- */
+void orc_tscape_ray(uint32_t *pRow, unsigned W, const uint8_t *heightMap,
+                    const uint32_t *colorMap, const uint32_t *fog256,
+                    int curX, int curY, int dX, int dY);
+void orc_tscape_frame(uint32_t *dst, unsigned W, unsigned H,
+                      const uint8_t *heightMap2048, const uint32_t *colorMap2048,
+                      const uint32_t *fog256, float stepU, float stepV,
+                      float speed, float time, int nthreads);
+
+/* ---- a9/a10 polar maps and blit (code/polar.cpp) ----------------------- */
+void orc_polar_maps(int *map, int *invMap, unsigned srcResX, unsigned srcResY,
+                    unsigned destResX, unsigned destResY);
+void orc_polar_blit(uint32_t *dst, const uint32_t *src, const int *map,
+                    unsigned W, unsigned H, unsigned srcStride, int nthreads);
+
+/* ---- a11 per-8x8-block UV grid remap (build-defined spec) -------------- */
+void orc_grid_remap(uint32_t *dst, unsigned W, unsigned H, const int *gridUV,
+                    const uint32_t *tex, unsigned texLog2, int nthreads);
+
+/* ---- a12 2007 box blur (code/deprecated/boxblur.cpp) ------------------- */
+/* dst may equal src (in-place recurrence, exactly as the reference). */
+int  orc_blur07_span(float strength);
+void orc_blur07_h(uint32_t *dst, const uint32_t *src, unsigned xRes,
+                  unsigned yRes, float strength, int nthreads);
+void orc_blur07_v(uint32_t *dst, const uint32_t *src, unsigned xRes,
+                  unsigned yRes, float strength, int nthreads);
+void orc_blur07(uint32_t *dst, const uint32_t *src, unsigned xRes,
+                unsigned yRes, float strength, int nthreads);
+
+/* ---- a13 2026 box blur (code/boxblur.cpp) ------------------------------ */
+/* scratch0/scratch1 each hold at least xRes*yRes pixels; src must be
+ * readable 2 pixels past the end of every row (the reference reads them). */
+void orc_blur26_h(uint32_t *dst, const uint32_t *src, unsigned xRes,
+                  unsigned yRes, float strength, float gain, unsigned passes,
+                  uint32_t *scratch0, uint32_t *scratch1);
+void orc_blur26_v(uint32_t *dst, const uint32_t *src, unsigned xRes,
+                  unsigned yRes, float strength, float gain, unsigned passes,
+                  uint32_t *scratch0, uint32_t *scratch1);
+void orc_blur26(uint32_t *dst, const uint32_t *src, unsigned xRes,
+                unsigned yRes, float strength, float gain, unsigned passes,
+                uint32_t *scratch0, uint32_t *scratch1);
+
+/* ---- a14..a16 blitters (code/fx-blitter.cpp, code/util.cpp, util.h) ---- */
+void orc_fx_blit_2x2(uint32_t *dst, const uint32_t *src, unsigned dstW,
+                     unsigned dstH);
+void orc_add32(uint32_t *dst, const uint32_t *src, unsigned n);
+void orc_blit_add32(uint32_t *dst, const uint32_t *src, unsigned destResX,
+                    unsigned srcResX, unsigned yRes);
+void orc_blit_add32a(uint32_t *dst, const uint32_t *src, unsigned destResX,
+                     unsigned srcResX, unsigned yRes, float alpha);
+void orc_memset32(uint32_t *dst, uint32_t value, size_t n);
+int  orc_ftofp24(float v);
+
+/* ---- c2 frame: grid remap -> blur07 H,V (out of place) -> BlitAdd32 ---- */
+void orc_tunnel_frame(uint32_t *dst, uint32_t *tmp0, uint32_t *tmp1,
+                      unsigned W, unsigned H, const int *gridUV,
+                      const uint32_t *tex, unsigned texLog2, int radius,
+                      int nthreads);
+
+/* ---- a20 synthetic SDF scene (build-defined spec) ----------------------- */
+typedef struct orc_sdf_scene {
+    int   nSpheres;
+    float sphere[32][4];     /* cx, cy, cz, radius                          */
+    float albedo[33][3];     /* B, G, R per sphere, last entry = ground     */
+    float groundY;
+    float light[3];          /* normalized direction towards the light      */
+    float fogColor[3];       /* B, G, R                                     */
+    float fogDensity;        /* ExpFog scale (cubic in distance)            */
+    float gamma;
+    float ambient;
+    float eps;               /* hit threshold                               */
+    float tMax;              /* far clip                                    */
+    int   maxSteps;
+    float camRadius, camHeight, camSpeed, focal;
+} orc_sdf_scene;
+
+/* render one frame in double (prec=64) or float (prec=32) arithmetic;
+ * steps (optional) receives the per-pixel march step count. */
+void orc_sdf_frame(uint32_t *dst, unsigned W, unsigned H,
+                   const orc_sdf_scene *scene, float t, int prec,
+                   uint16_t *steps, int nthreads);
+/* rows rowStart, rowStart+rowStep, ... only, compacted into dst */
+void orc_sdf_rows(uint32_t *dst, unsigned W, unsigned H,
+                  const orc_sdf_scene *scene, float t, int prec,
+                  unsigned rowStart, unsigned rowStep, uint16_t *steps,
+                  int nthreads);
+/* float camera basis of frame time t: origin[3], forward*focal[3],
+ * right[3], up[3] (computed in double, rounded to float) */
+void orc_sdf_camera(const orc_sdf_scene *scene, float t, float cam[12]);
+
+/* ---- a19 LUT trig (code/sincos-lut.h:13-33, code/sincos-lut.cpp:9-16) --- */
+void  orc_cos_lut(float lut[2049]);
+float orc_lutcosf(const float *lut, float angle);
+float orc_lutsinf(const float *lut, float angle);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

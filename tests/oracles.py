@@ -1,0 +1,140 @@
+"""ctypes bindings of the checkers: the oracle restatement
+(oracle/liboracle.so) and the reference harness (oracle/_ref/libckdref.so,
+built from the reference sources only where the reference checkout exists).
+
+Test infrastructure only.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from cookiedough_amd._lib import LandscapeCam, SdfScene
+
+ROOT = Path(__file__).resolve().parent.parent
+ORACLE_PATH = ROOT / "oracle" / "liboracle.so"
+REF_PATH = ROOT / "oracle" / "_ref" / "libckdref.so"
+
+_p, _u, _i, _f = C.c_void_p, C.c_uint, C.c_int, C.c_float
+
+
+class OrcCam(C.Structure):
+    _fields_ = LandscapeCam._fields_
+
+
+_ORC = {
+    "orc_bsamp_prep": (None, [_i, _i, _u, _u, _p, _p, _p, _p, _p, _p]),
+    "orc_bsamp8": (_u, [_p, _u, _u, _u, _u, _u, _u]),
+    "orc_bsamp32_16": (None, [_p, _u, _u, _u, _u, _u, _u, _p]),
+    "orc_bsamp32_32_packed": (C.c_uint32, [_p, _u, _u, _u, _u, _u, _u]),
+    "orc_cspan16": (None, [_p, C.c_ssize_t, _u, _u, _p, _p]),
+    "orc_vscape_ray": (None, [_p, _u, _u, _p, _p, _p, _i, _i, _i, _i, _i, _i, _f]),
+    "orc_vscape_setup": (None, [_u, _u, C.POINTER(OrcCam), _p, _p, _p, _p, _p]),
+    "orc_landscape_frame": (None, [_p, _u, _u, _p, _p, _p, C.POINTER(OrcCam), _i]),
+    "orc_tscape_ray": (None, [_p, _u, _p, _p, _p, _i, _i, _i, _i]),
+    "orc_tscape_frame": (None, [_p, _u, _u, _p, _p, _p, _f, _f, _f, _f, _i]),
+    "orc_polar_maps": (None, [_p, _p, _u, _u, _u, _u]),
+    "orc_polar_blit": (None, [_p, _p, _p, _u, _u, _u, _i]),
+    "orc_grid_remap": (None, [_p, _u, _u, _p, _p, _u, _i]),
+    "orc_blur07_span": (_i, [_f]),
+    "orc_blur07_h": (None, [_p, _p, _u, _u, _f, _i]),
+    "orc_blur07_v": (None, [_p, _p, _u, _u, _f, _i]),
+    "orc_blur07": (None, [_p, _p, _u, _u, _f, _i]),
+    "orc_blur26_h": (None, [_p, _p, _u, _u, _f, _f, _u, _p, _p]),
+    "orc_blur26_v": (None, [_p, _p, _u, _u, _f, _f, _u, _p, _p]),
+    "orc_blur26": (None, [_p, _p, _u, _u, _f, _f, _u, _p, _p]),
+    "orc_fx_blit_2x2": (None, [_p, _p, _u, _u]),
+    "orc_add32": (None, [_p, _p, _u]),
+    "orc_blit_add32": (None, [_p, _p, _u, _u, _u]),
+    "orc_blit_add32a": (None, [_p, _p, _u, _u, _u, _f]),
+    "orc_memset32": (None, [_p, C.c_uint32, C.c_size_t]),
+    "orc_ftofp24": (_i, [_f]),
+    "orc_tunnel_frame": (None, [_p, _p, _p, _u, _u, _p, _p, _u, _i, _i]),
+    "orc_sdf_frame": (None, [_p, _u, _u, C.POINTER(SdfScene), _f, _i, _p, _i]),
+    "orc_sdf_rows": (None, [_p, _u, _u, C.POINTER(SdfScene), _f, _i, _u, _u, _p, _i]),
+    "orc_sdf_camera": (None, [C.POINTER(SdfScene), _f, _p]),
+    "orc_cos_lut": (None, [_p]),
+    "orc_lutcosf": (_f, [_p, _f]),
+    "orc_lutsinf": (_f, [_p, _f]),
+}
+
+_REF = {
+    "ref_bsamp_prep": (None, [_i, _i, _u, _u, _p]),
+    "ref_bsamp8": (_u, [_p, _u, _u, _u, _u, _u, _u]),
+    "ref_bsamp32_16": (C.c_uint32, [_p, _u, _u, _u, _u, _u, _u]),
+    "ref_bsamp32_32": (C.c_uint32, [_p, _u, _u, _u, _u, _u, _u]),
+    "ref_cspan16": (None, [_p, _i, _u, _u, C.c_uint32, C.c_uint32]),
+    "ref_vscape_ray": (None, [_p, _p, _p, _p, _i, _i, _i, _i, _i, _f]),
+    "ref_memset32": (None, [_p, C.c_uint32, C.c_size_t]),
+    "ref_tscape_ray": (None, [_p, _p, _p, _p, _i, _i, _i, _i]),
+    "ref_polar_maps": (None, [_p, _p, _u, _u, _u, _u]),
+    "ref_polar_create": (_i, []),
+    "ref_polar_destroy": (None, []),
+    "ref_polar_blit": (None, [_p, _p, _i]),
+    "ref_blur07_h": (None, [_p, _p, _u, _u, _f]),
+    "ref_blur07_v": (None, [_p, _p, _u, _u, _f]),
+    "ref_blur07": (None, [_p, _p, _u, _u, _f]),
+    "ref_blur26_h": (None, [_p, _p, _u, _u, _f, _f, _u]),
+    "ref_blur26_v": (None, [_p, _p, _u, _u, _f, _f, _u]),
+    "ref_blur26": (None, [_p, _p, _u, _u, _f, _f, _u]),
+    "ref_fx_blit_2x2": (None, [_p, _p]),
+    "ref_add32": (None, [_p, _p, _u]),
+    "ref_blit_add32": (None, [_p, _p, _u, _u, _u]),
+    "ref_blit_add32a": (None, [_p, _p, _u, _u, _u, _f]),
+    "ref_cos_lut": (None, [_p]),
+    "ref_lutcosf": (_f, [_f]),
+    "ref_lutsinf": (_f, [_f]),
+    "ref_tape_warp32": (None, [_p, _p, _u, _u, _f, _f]),
+}
+
+
+def _bind(path: Path, table: dict) -> C.CDLL:
+    lib = C.CDLL(str(path))
+    for name, (res, args) in table.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_oracle = None
+_ref = None
+
+
+def oracle() -> C.CDLL:
+    global _oracle
+    if _oracle is None:
+        if not ORACLE_PATH.exists():
+            raise FileNotFoundError(f"{ORACLE_PATH} not built (make -C oracle)")
+        _oracle = _bind(ORACLE_PATH, _ORC)
+    return _oracle
+
+
+def ref_available() -> bool:
+    return REF_PATH.exists()
+
+
+def ref() -> C.CDLL:
+    global _ref
+    if _ref is None:
+        _ref = _bind(REF_PATH, _REF)
+    return _ref
+
+
+def P(a: np.ndarray) -> int:
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+def aligned_empty(n: int, dtype=np.uint32, align: int = 64, guard: int = 0) -> np.ndarray:
+    """zeroed 1-D array of n (+guard) elements whose data is `align`-aligned"""
+    it = np.dtype(dtype).itemsize
+    raw = np.zeros((n + guard) * it + align, np.uint8)
+    off = (-raw.ctypes.data) % align
+    return raw[off:off + (n + guard) * it].view(dtype)
+
+
+def orc_cam(cam: LandscapeCam) -> OrcCam:
+    return OrcCam(cam.X1, cam.Y1, cam.viewCos, cam.viewSin, cam.mapTilt)

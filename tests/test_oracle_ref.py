@@ -1,0 +1,262 @@
+"""Pins the oracle restatement against the reference itself: the reference
+harness (oracle/_ref/libckdref.so) is compiled from the unmodified reference
+sources, and every oracle function must match it bit for bit on brute-force
+and seeded inputs.  Skipped where the reference checkout is absent; the
+committed golden vectors (test_oracle_golden.py) pin the same functions
+there."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracles import P, aligned_empty, orc_cam
+from cookiedough_amd import workloads as wl
+
+pytestmark = pytest.mark.ref
+
+
+def test_bsamp_prep(orc, refl):
+    rng = np.random.default_rng(1)
+    vals = np.concatenate([rng.integers(-2**30, 2**30, 2000), [0, -1, -256, 255, 256, -257]])
+    for mask, shift in [(1023, 10), (2047, 11), (511, 9)]:
+        for U, V in zip(vals, vals[::-1]):
+            a = (C.c_uint * 6)()
+            refl.ref_bsamp_prep(int(U), int(V), mask, shift, a)
+            o = [C.c_uint() for _ in range(6)]
+            orc.orc_bsamp_prep(int(U), int(V), mask, shift, *[C.byref(x) for x in o])
+            assert list(a) == [x.value for x in o]
+
+
+def _tex_quads():
+    """2x2 textures with extreme channel pairs"""
+    rng = np.random.default_rng(2)
+    quads = [np.array([0, 0xffffffff, 0xffffffff, 0], np.uint32),
+             np.array([0xffffffff, 0, 0, 0xffffffff], np.uint32),
+             np.array([0x00ff00ff, 0xff00ff00, 0x80808080, 0x7f7f7f7f], np.uint32)]
+    quads += [rng.integers(0, 2**32, 4, dtype=np.uint64).astype(np.uint32) for _ in range(5)]
+    return quads
+
+
+def test_bilinear_exhaustive_fractions(orc, refl):
+    """all 256x256 (fu, fv) for a set of extreme texel quads"""
+    for q in _tex_quads():
+        tex = np.array([q[0], q[1], q[2], q[3]], np.uint32)  # 2x2, row stride 2
+        tex8 = (tex & 0xff).astype(np.uint8)
+        for fu in range(256):
+            for fv in range(0, 256, 5):
+                r32 = refl.ref_bsamp32_32(P(tex), 0, 0, 1, 2, fu * 0x01010101, fv * 0x01010101)
+                r16 = refl.ref_bsamp32_16(P(tex), 0, 0, 1, 2, fu * 0x01010101, fv * 0x01010101)
+                o32 = orc.orc_bsamp32_32_packed(P(tex), 0, 0, 1, 2, fu * 0x01010101, fv * 0x01010101)
+                assert r32 == o32 == r16
+                o16 = (C.c_uint16 * 4)()
+                orc.orc_bsamp32_16(P(tex), 0, 0, 1, 2, fu, fv, o16)
+                assert sum(int(o16[c]) << (8 * c) for c in range(4)) == r16
+                assert refl.ref_bsamp8(P(tex8), 0, 0, 1, 2, fu, fv) == \
+                    orc.orc_bsamp8(P(tex8), 0, 0, 1, 2, fu, fv)
+
+
+def _cspan_cases():
+    cols = [(0x00000000, 0xffffffff), (0xffffffff, 0x00000000), (0x12345678, 0x87654321),
+            (0x00ff00ff, 0xff00ff00), (0x80808080, 0x7f7f7f7f), (0xff010203, 0x00fffefd)]
+    for length in list(range(1, 41)) + [64, 100, 255, 256, 257, 511, 720, 1000]:
+        draws = range(1, length + 1) if length <= 40 else [1, 2, length // 2, length - 1, length]
+        for d in draws:
+            for a, b in cols:
+                yield length, d, a, b
+
+
+def test_cspan_lane_quirks(orc):
+    """cspanISSE16 vs its emulation over all lengths <= 40 and every visible
+    length, plus long spans (len 1 and 2 exercise the int16 wrap of 65536/len)"""
+    import oracles
+    refl = oracles.ref()
+    for length, d, a, b in _cspan_cases():
+        rdst = np.zeros(d, np.uint32)
+        odst = np.zeros(d, np.uint32)
+        refl.ref_cspan16(P(rdst), 1, length, d, a, b)
+        A = (C.c_uint16 * 4)(*[(a >> (8 * c)) & 0xff for c in range(4)])
+        B = (C.c_uint16 * 4)(*[(b >> (8 * c)) & 0xff for c in range(4)])
+        orc.orc_cspan16(P(odst), 1, length, d, A, B)
+        assert np.array_equal(rdst, odst), (length, d, hex(a), hex(b))
+
+
+@pytest.fixture(scope="module")
+def maps():
+    return wl.landscape_maps(0)
+
+
+def test_vscape_ray_columns(orc, refl, maps):
+    """reference vscape_ray vs the oracle on 64 columns of 3 cameras"""
+    H, W = 720, 1280
+    path = wl.landscape_path(0)
+    for t in [0.0, 0.7, 1.9]:
+        cam = wl.landscape_cam(path, t)
+        fpX1, fpY1 = C.c_int(), C.c_int()
+        dX = np.zeros(W, np.int32)
+        dY = np.zeros(W, np.int32)
+        fish = np.zeros(W, np.float32)
+        oc = orc_cam(cam)
+        orc.orc_vscape_setup(W, H, C.byref(oc), C.byref(fpX1), C.byref(fpY1), P(dX), P(dY), P(fish))
+        for col in list(range(0, W, 21)) + [W - 1]:
+            a = np.full(H * W, 7, np.uint32)
+            b = np.full(H * W, 7, np.uint32)
+            refl.ref_vscape_ray(P(a[col:]), P(maps.height), P(maps.color), P(maps.fog),
+                                cam.mapTilt, fpX1.value, fpY1.value, int(dX[col]), int(dY[col]),
+                                float(fish[col]))
+            orc.orc_vscape_ray(P(b[col:]), W, H, P(maps.height), P(maps.color), P(maps.fog),
+                               512, cam.mapTilt, fpX1.value, fpY1.value, int(dX[col]),
+                               int(dY[col]), float(fish[col]))
+            assert np.array_equal(a, b), (t, col)
+
+
+def test_landscape_frame_matches_reference_columns(orc, refl, maps):
+    """a whole 1280x720 oracle frame equals memset32 + 1280 reference rays"""
+    H, W = 720, 1280
+    cam = wl.landscape_cam(wl.landscape_path(0), 0.4)
+    oc = orc_cam(cam)
+    out = np.zeros(W * H, np.uint32)
+    orc.orc_landscape_frame(P(out), W, H, P(maps.height), P(maps.color), P(maps.fog), C.byref(oc), 8)
+    fpX1, fpY1 = C.c_int(), C.c_int()
+    dX = np.zeros(W, np.int32)
+    dY = np.zeros(W, np.int32)
+    fish = np.zeros(W, np.float32)
+    orc.orc_vscape_setup(W, H, C.byref(oc), C.byref(fpX1), C.byref(fpY1), P(dX), P(dY), P(fish))
+    ref = aligned_empty(W * H)
+    refl.ref_memset32(P(ref), int(maps.fog[0]), W * H)
+    for col in range(W):
+        refl.ref_vscape_ray(P(ref[col:]), P(maps.height), P(maps.color), P(maps.fog), cam.mapTilt,
+                            fpX1.value, fpY1.value, int(dX[col]), int(dY[col]), float(fish[col]))
+    assert np.array_equal(out, ref)
+
+
+def test_tscape_ray_rows(orc, refl):
+    m = wl.tunnelscape_maps(0)
+    W = 1280
+    rng = np.random.default_rng(3)
+    for _ in range(24):
+        cx, cy = int(rng.integers(0, 2048 * 256)), int(rng.integers(0, 2048 * 256))
+        dx, dy = int(rng.integers(-300, 300)), int(rng.integers(-300, 300))
+        a = np.full(W, 5, np.uint32)
+        b = np.full(W, 5, np.uint32)
+        refl.ref_tscape_ray(P(a), P(m.height), P(m.color), P(m.fog), cx, cy, dx, dy)
+        orc.orc_tscape_ray(P(b), W, P(m.height), P(m.color), P(m.fog), cx, cy, dx, dy)
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("W,H", [(1280, 720), (644, 364), (320, 180)])
+def test_polar_maps(orc, refl, W, H):
+    n = W * H * 2
+    a0, a1 = np.zeros(n, np.int32), np.zeros(n, np.int32)
+    b0, b1 = np.zeros(n, np.int32), np.zeros(n, np.int32)
+    refl.ref_polar_maps(P(a0), P(a1), W, H, W, H)
+    orc.orc_polar_maps(P(b0), P(b1), W, H, W, H)
+    assert np.array_equal(a0, b0) and np.array_equal(a1, b1)
+
+
+def test_polar_blit(orc, refl):
+    """Only the inverse blit is called: the forward Polar_Blit tiles 720 rows
+    with 32-row tiles (code/polar.cpp:137-142) and so reads its map and
+    writes its destination 16 rows past their ends.  The forward blit is the
+    same per-pixel fetch over the other map, which test_polar_maps pins."""
+    W, H = 1280, 720
+    assert refl.ref_polar_create() == 1
+    src = aligned_empty(W * H)
+    src[:] = np.random.default_rng(4).integers(0, 2**32, W * H, dtype=np.uint64).astype(np.uint32)
+    m0, m1 = np.zeros(W * H * 2, np.int32), np.zeros(W * H * 2, np.int32)
+    orc.orc_polar_maps(P(m0), P(m1), W, H, W, H)
+    for inv, m in [(1, m1)]:
+        r = aligned_empty(W * H)
+        o = np.zeros(W * H, np.uint32)
+        refl.ref_polar_blit(P(r), P(src), inv)
+        orc.orc_polar_blit(P(o), P(src), P(m), W, H, W, 4)
+        assert np.array_equal(r, o), inv
+    refl.ref_polar_destroy()
+
+
+BLUR07_SPANS = [1, 2, 3, 4, 5, 6, 7, 8, 9, 16, 17, 31, 32, 33, 64, 65, 100, 128, 129, 254, 255]
+
+
+@pytest.mark.parametrize("kind", ["h", "v", "2d"])
+@pytest.mark.parametrize("inplace", [False, True])
+def test_blur07(orc, refl, kind, inplace):
+    rng = np.random.default_rng(5)
+    W, H = 300, 260
+    for span in BLUR07_SPANS:
+        if span > min(W, H):
+            continue
+        strength = (span + 0.5) / 255.0
+        src = rng.integers(0, 2**32, W * H, dtype=np.uint64).astype(np.uint32)
+        if span % 3 == 0:
+            src[:] = 0xffffffff  # saturation corners
+        a = src.copy() if inplace else np.zeros(W * H, np.uint32)
+        b = src.copy() if inplace else np.zeros(W * H, np.uint32)
+        sa = a if inplace else src
+        sb = b if inplace else src
+        f_ref = {"h": refl.ref_blur07_h, "v": refl.ref_blur07_v, "2d": refl.ref_blur07}[kind]
+        f_orc = {"h": orc.orc_blur07_h, "v": orc.orc_blur07_v, "2d": orc.orc_blur07}[kind]
+        f_ref(P(a), P(sa), W, H, strength)
+        f_orc(P(b), P(sb), W, H, strength, 4)
+        assert np.array_equal(a, b), (span, kind, inplace)
+        assert orc.orc_blur07_span(strength) == span
+
+
+@pytest.mark.parametrize("kind", ["h", "v", "2d"])
+def test_blur26(orc, refl, kind):
+    rng = np.random.default_rng(6)
+    cases = [(64, 48, 10.0, 0.0, 1), (64, 48, 37.5, 0.25, 2), (128, 96, 100.0, 1.0, 3),
+             (200, 120, 3.3, 0.0, 3), (96, 200, 55.0, 0.5, 4), (64, 64, 0.0, 0.0, 1)]
+    for W, H, strength, gain, passes in cases:
+        src = aligned_empty(W * H, guard=4)
+        src[:W * H] = rng.integers(0, 2**32, W * H, dtype=np.uint64).astype(np.uint32)
+        a = aligned_empty(W * H, guard=4)
+        b = aligned_empty(W * H, guard=4)
+        s0 = aligned_empty(W * H, guard=4)
+        s1 = aligned_empty(W * H, guard=4)
+        f_ref = {"h": refl.ref_blur26_h, "v": refl.ref_blur26_v, "2d": refl.ref_blur26}[kind]
+        f_orc = {"h": orc.orc_blur26_h, "v": orc.orc_blur26_v, "2d": orc.orc_blur26}[kind]
+        f_ref(P(a), P(src), W, H, strength, gain, passes)
+        f_orc(P(b), P(src), W, H, strength, gain, passes, P(s0), P(s1))
+        assert np.array_equal(a[:W * H], b[:W * H]), (kind, W, H, strength, gain, passes)
+
+
+def test_fx_blit_2x2(orc, refl):
+    W, H = 1280, 720
+    mw, mh = W // 2 + 4, H // 2 + 4
+    src = aligned_empty(mw * mh)
+    src[:] = np.random.default_rng(7).integers(0, 2**32, mw * mh, dtype=np.uint64).astype(np.uint32)
+    a = aligned_empty(W * H)
+    b = np.zeros(W * H, np.uint32)
+    refl.ref_fx_blit_2x2(P(a), P(src))
+    orc.orc_fx_blit_2x2(P(b), P(src), W, H)
+    assert np.array_equal(a, b)
+
+
+def test_add_blits(orc, refl):
+    rng = np.random.default_rng(8)
+    d0 = rng.integers(0, 2**32, 97 * 31, dtype=np.uint64).astype(np.uint32)
+    s0 = rng.integers(0, 2**32, 97 * 31, dtype=np.uint64).astype(np.uint32)
+    a, b = d0.copy(), d0.copy()
+    refl.ref_add32(P(a), P(s0), a.size)
+    orc.orc_add32(P(b), P(s0), b.size)
+    assert np.array_equal(a, b)
+    for alpha in [None, 0.0, 0.3, 0.5, 1.0, 1.7]:
+        a, b = d0.copy(), d0.copy()
+        if alpha is None:
+            refl.ref_blit_add32(P(a), P(s0), 97, 60, 31)
+            orc.orc_blit_add32(P(b), P(s0), 97, 60, 31)
+        else:
+            refl.ref_blit_add32a(P(a), P(s0), 97, 60, 31, alpha)
+            orc.orc_blit_add32a(P(b), P(s0), 97, 60, 31, alpha)
+        assert np.array_equal(a, b), alpha
+
+
+def test_cos_lut(orc, refl):
+    a = np.zeros(2049, np.float32)
+    b = np.zeros(2049, np.float32)
+    refl.ref_cos_lut(P(a))
+    orc.orc_cos_lut(P(b))
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    xs = np.random.default_rng(9).uniform(-100, 100, 5000).astype(np.float32)
+    for x in xs:
+        assert np.float32(refl.ref_lutcosf(float(x))) == np.float32(orc.orc_lutcosf(P(b), float(x)))
+        assert np.float32(refl.ref_lutsinf(float(x))) == np.float32(orc.orc_lutsinf(P(b), float(x)))
