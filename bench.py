@@ -1,0 +1,288 @@
+#!/usr/bin/env python3
+"""Benchmark of the per-pixel effect renderer on one MI355X.
+
+Headline (one JSON line on stdout): Mpixel/s of the batched form -- 512
+frame requests mixing the SDF sphere tracer, the voxel landscape and the
+tunnel remap+blur+blit in a seeded 40/30/30 ratio at five resolutions,
+FP16 shading -- with inputs resident in HBM.  One step renders the whole
+batch.  Per-effect plain-FP32 (one frame per launch) vs batched-FP16
+figures, the FP16-vs-FP64 error of the 2560x1440 precision form, the
+roofline of the dominant kernel and a CPU baseline ride along in the
+same line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+from cookiedough_amd import workloads as wl  # noqa: E402
+from cookiedough_amd._lib import (CKD_FX_SDF, CKD_FX_TUNNEL, CKD_FX_VOXEL,  # noqa: E402
+                                  CKD_SHADE_FP16, CKD_SHADE_FP32)
+
+PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 vector (packed) peak
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def sdf_flops(total_steps: int, pixels: int, n_spheres: int) -> float:
+    """Algorithmic FLOPs of the sphere tracer: per march step F_sdf+7
+    (F_sdf = 11 per sphere + 2 for the plane), per pixel 6 SDF evaluations
+    for the normal and ~40 for shading (FMA = 2, sqrt/exp/log = 1)."""
+    f_sdf = 11 * n_spheres + 2
+    return total_steps * (f_sdf + 7) + pixels * (6 * f_sdf + 40)
+
+
+class Timer:
+    def __init__(self, ctx):
+        self.ctx = ctx
+
+    def run(self, fn, reps: int = 1) -> float:
+        """ms for `reps` calls of fn, HIP events on the context stream"""
+        self.ctx.sync()
+        self.ctx.record(0)
+        for _ in range(reps):
+            fn()
+        self.ctx.record(1)
+        self.ctx.sync()
+        return self.ctx.elapsed_ms(0, 1)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--requests", type=int, default=512)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(frames and requests are independent; see DESIGN.md for how the path shards)",
+              file=sys.stderr)
+        return 2
+
+    from cookiedough_amd.device import Context
+    ctx = Context(0)
+    T = Timer(ctx)
+
+    # ---------------------------------------------------------------- inputs
+    t_in = time.time()
+    maps = wl.landscape_maps(0)
+    ctx.landscape_create(maps.height, maps.color, maps.fog)
+    path = wl.landscape_path(0)
+    tex = wl.noise_texture(0, 9)
+    d_tex = ctx.upload(tex)
+    scene8 = wl.sdf_scene(0, 8, 64)
+    scene32 = wl.sdf_scene(0, 32, 128)
+    counter = ctx.upload(np.zeros(1, np.uint64))
+
+    items = wl.batch_requests(0, args.requests)
+    total_px = sum(i.W * i.H for i in items)
+    tmp_px = sum(2 * i.W * i.H for i in items if i.effect == CKD_FX_TUNNEL)
+    out = ctx.empty(total_px)
+    tmp = ctx.empty(max(1, tmp_px))
+    reqs, off, grids = [], 0, []
+    for it in items:
+        g = ctx.upload(wl.tunnel_grid(it.W, it.H, it.t)) if it.effect == CKD_FX_TUNNEL else None
+        grids.append(g)
+        reqs.append(wl.make_request(it, out.ptr + 4 * off, scene8, path, g.ptr if g else None))
+        off += it.W * it.H
+    log(f"inputs ready in {time.time() - t_in:.1f}s: {len(items)} requests, {total_px / 1e6:.1f} Mpx")
+
+    def batch_step():
+        ctx.render_batch(reqs, scene8, CKD_SHADE_FP16, d_tex, 9, tmp, tmp_px, counter)
+
+    # ------------------------------------------------------------- headline
+    for _ in range(args.warmup):
+        batch_step()
+    ctx.sync()
+    ctx.write(counter, np.zeros(1, np.uint64))
+    ms_total = T.run(batch_step, args.steps)
+    steps_total = int(counter.numpy()[0])
+    ms_step = ms_total / args.steps
+    value = total_px / (ms_step * 1e-3) / 1e6
+    log(f"batch: {ms_step:.2f} ms/step, {value:.0f} Mpx/s")
+
+    # ----------------------------------------------------- per effect (c0-c2)
+    W, H = 1280, 720
+    per = {}
+    frame = ctx.empty(W * H)
+
+    # SDF c0: 64 frames at t=k/60
+    cams = [wl.sdf_camera(scene8, k / 60) for k in range(64)]
+    fr_sdf = ctx.empty(64 * W * H)
+
+    def sdf_plain():
+        for k, c in enumerate(cams):
+            ctx.sdf_render(fr_sdf.view(k * W * H, (W * H,)), W, H, scene8, c, CKD_SHADE_FP32)
+
+    sdf_reqs = []
+    for k in range(64):
+        r = wl.make_request(wl.BatchItem(CKD_FX_SDF, W, H, k / 60), fr_sdf.ptr + 4 * k * W * H,
+                            scene8, path, None)
+        sdf_reqs.append(r)
+
+    def sdf_batched():
+        ctx.render_batch(sdf_reqs, scene8, CKD_SHADE_FP16, None, 9, None, 0)
+
+    # voxel c1: 120 frames on the Lissajous path
+    lcams = [wl.landscape_cam(path, k / 60) for k in range(120)]
+    fr_vox = ctx.empty(120 * W * H)
+
+    def vox_plain():
+        for k, c in enumerate(lcams):
+            ctx.landscape_draw(fr_vox.view(k * W * H, (W * H,)), W, H, [c])
+
+    def vox_batched():
+        ctx.landscape_draw(fr_vox, W, H, lcams)
+
+    # tunnel c2: 240 frames, radius U{1..32}
+    radii = wl.tunnel_radii(0, 240)
+    tgrids = [ctx.upload(wl.tunnel_grid(W, H, k / 60)) for k in range(240)]
+    fr_tun = ctx.empty(240 * W * H)
+    t0, t1 = ctx.empty(W * H), ctx.empty(W * H)
+    tun_tmp = ctx.empty(2 * 240 * W * H)
+
+    def tun_plain():
+        for k in range(240):
+            ctx.tunnel_frame(fr_tun.view(k * W * H, (W * H,)), t0, t1, W, H, tgrids[k], d_tex, 9,
+                             int(radii[k]))
+
+    tun_reqs = []
+    for k in range(240):
+        it = wl.BatchItem(CKD_FX_TUNNEL, W, H, k / 60, int(radii[k]))
+        tun_reqs.append(wl.make_request(it, fr_tun.ptr + 4 * k * W * H, scene8, path, tgrids[k].ptr))
+
+    def tun_batched():
+        ctx.render_batch(tun_reqs, scene8, CKD_SHADE_FP16, d_tex, 9, tun_tmp, 2 * 240 * W * H)
+
+    for name, frames, plain, batched in [("sdf_c0", 64, sdf_plain, sdf_batched),
+                                         ("voxel_c1", 120, vox_plain, vox_batched),
+                                         ("tunnel_c2", 240, tun_plain, tun_batched)]:
+        plain()
+        batched()
+        mp = T.run(plain)
+        mb = T.run(batched)
+        px = frames * W * H
+        per[name] = {"frames": frames, "plain_fp32_mpx_s": round(px / mp / 1e3, 1),
+                     "batched_fp16_mpx_s": round(px / mb / 1e3, 1),
+                     "speedup": round(mp / mb, 3)}
+        log(name, per[name])
+
+    # ------------------------------------------------- roofline: SDF kernel
+    # FP16-shaded tracer on the 2560x1440 precision form (32 spheres, 128
+    # steps), events around each launch
+    W3, H3 = 2560, 1440
+    f3 = ctx.empty(W3 * H3)
+    cams3 = [wl.sdf_camera(scene32, k / 60) for k in range(8)]
+    ctx.sdf_render(f3, W3, H3, scene32, cams3[0], CKD_SHADE_FP16)
+    ctx.write(counter, np.zeros(1, np.uint64))
+    ms3 = 0.0
+    for c in cams3:
+        ms3 += T.run(lambda: ctx.sdf_render(f3, W3, H3, scene32, c, CKD_SHADE_FP16, counter))
+    st3 = int(counter.numpy()[0])
+    flops3 = sdf_flops(st3, len(cams3) * W3 * H3, 32)
+    achieved = flops3 / (ms3 * 1e-3) / 1e12
+    roofline = {"bound": "valu", "kernel": "k_sdf<FP16> 2560x1440 32 spheres",
+                "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                "avg_launch_ms": round(ms3 / len(cams3), 3)}
+    log("roofline", roofline)
+
+    # ------------------------------------------------------------ CPU baseline
+    # The CPU leg is the only part of the bench that runs the oracle: the
+    # timed CPU baseline and the FP64 reference render the precision form's
+    # error is measured against.
+    cpu = None
+    es = None
+    if not args.no_cpu:
+        import oracles  # noqa: E402  (the checker)
+        from helpers import error_stats  # noqa: E402
+        orc = oracles.oracle()
+        nthr = min(16, os.cpu_count() or 1)
+        # precision form: FP16 shading vs the FP64 render on a row sample
+        errs = []
+        for k in [0, 63]:
+            cam = wl.sdf_camera(scene32, k / 60)
+            ctx.sdf_render(f3, W3, H3, scene32, cam, CKD_SHADE_FP16)
+            img = f3.numpy().reshape(H3, W3)
+            rows = np.arange(7, H3, 36)
+            ref = np.zeros(len(rows) * W3, np.uint32)
+            orc.orc_sdf_rows(oracles.P(ref), W3, H3, C.byref(scene32), k / 60, 64, 7, 36, None, nthr)
+            errs.append((img[rows].reshape(-1), ref))
+        es = error_stats(np.concatenate([e[0] for e in errs]), np.concatenate([e[1] for e in errs]))
+        log("precision", es)
+        # bounded sample of the batched workload: the first requests of each
+        # kind at their own resolutions, rendered by the oracle
+        sample = [it for it in items if it.effect == CKD_FX_SDF][:2] + \
+                 [it for it in items if it.effect == CKD_FX_VOXEL][:3] + \
+                 [it for it in items if it.effect == CKD_FX_TUNNEL][:3]
+        px = 0
+        t_c = time.perf_counter()
+        for it in sample:
+            n = it.W * it.H
+            buf = np.zeros(n, np.uint32)
+            if it.effect == CKD_FX_SDF:
+                orc.orc_sdf_frame(oracles.P(buf), it.W, it.H, C.byref(scene8), it.t, 32, None, nthr)
+            elif it.effect == CKD_FX_VOXEL:
+                oc = oracles.orc_cam(wl.landscape_cam(path, it.t))
+                orc.orc_landscape_frame(oracles.P(buf), it.W, it.H, oracles.P(maps.height),
+                                        oracles.P(maps.color), oracles.P(maps.fog), C.byref(oc), nthr)
+            else:
+                g = wl.tunnel_grid(it.W, it.H, it.t)
+                b0, b1 = np.zeros(n, np.uint32), np.zeros(n, np.uint32)
+                orc.orc_tunnel_frame(oracles.P(buf), oracles.P(b0), oracles.P(b1), it.W, it.H,
+                                     oracles.P(g), oracles.P(tex), 9, it.radius, nthr)
+            px += n
+        dt = time.perf_counter() - t_c
+        cpu = {"value": round(px / dt / 1e6, 2), "unit": "Mpixel/s", "cores": nthr, "kind": "port",
+               "sample": f"{len(sample)} requests of the batch (2 SDF FP32, 3 voxel, 3 tunnel), "
+                         f"{px / 1e6:.1f} Mpx, oracle C + OpenMP, {dt:.1f}s"}
+        log("cpu", cpu)
+
+    result = {
+        "metric": "Mpixel/s per effect at <=2/255 err: batched FP16 vs plain FP32, 1 MI355X",
+        "value": round(value, 1),
+        "unit": "Mpixel/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 march / fp16 shading; int (u8/u16/24:8) voxel, remap, blur",
+        "data": "synthetic (seeded)",
+        "config": {"workload": f"batched form: {len(items)} requests, 40/30/30 SDF/voxel/tunnel, "
+                               f"5 resolutions 320x180..2560x1440, t U[0,60], "
+                               f"{total_px / 1e6:.1f} Mpx per step",
+                   "march_steps_per_step": steps_total // max(1, args.steps)},
+        "per_effect": per,
+        "precision_c3": None if es is None else
+        {"shading": "fp16", "vs": "fp64 CPU render", "rows_sampled": "every 36th, frames 0 and 63", **es},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(result))
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.path.insert(0, str(ROOT / "tests"))
+    sys.exit(main())

@@ -1,0 +1,286 @@
+"""GPU parity: every entry point of libckd.so against the oracle on seeded
+inputs (bit-exact for the integer paths; the SDF float path within the
+stated tolerance of the FP64 render).  Calls go through the C ABI."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from cookiedough_amd import workloads as wl
+from cookiedough_amd._lib import CKD_SHADE_FP16, CKD_SHADE_FP32, CkdError
+from helpers import error_stats, rand_img
+from oracles import P, orc_cam
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+# ------------------------------------------------------------------ blits
+
+@pytest.mark.parametrize("W,H", [(1280, 720), (320, 180), (2560, 1440), (16, 2)])
+def test_fx_blit_2x2(ctx, orc, W, H):
+    _say("fx_blit", W, H)
+    mw, mh = W // 2 + 4, H // 2 + 4
+    src = rand_img(10 + W, mw * mh)
+    d_src = ctx.upload(src)
+    d_dst = ctx.empty(W * H)
+    ctx.fx_blit_2x2(d_dst, d_src, W, H)
+    ref = np.zeros(W * H, np.uint32)
+    orc.orc_fx_blit_2x2(P(ref), P(src), W, H)
+    assert np.array_equal(d_dst.numpy(), ref)
+
+
+def test_add_blits(ctx, orc):
+    _say("add blits")
+    d0, s0 = rand_img(20, 1280 * 720), rand_img(21, 1280 * 720)
+    dd, ds = ctx.upload(d0), ctx.upload(s0)
+    ctx.add32(dd, ds, d0.size)
+    r = d0.copy()
+    orc.orc_add32(P(r), P(s0), r.size)
+    assert np.array_equal(dd.numpy(), r)
+    for alpha in [None, 0.0, 0.37, 1.0, 1.6]:
+        ctx.write(dd, d0)
+        r = d0.copy()
+        if alpha is None:
+            ctx.blit_add32(dd, ds, 1280, 1000, 700)
+            orc.orc_blit_add32(P(r), P(s0), 1280, 1000, 700)
+        else:
+            ctx.blit_add32a(dd, ds, 1280, 1000, 700, alpha)
+            orc.orc_blit_add32a(P(r), P(s0), 1280, 1000, 700, alpha)
+        assert np.array_equal(dd.numpy(), r), alpha
+
+
+def test_memset32(ctx):
+    _say("memset32")
+    d = ctx.empty(1001)
+    ctx.write(d, np.zeros(1001, np.uint32))
+    ctx.memset32(d.view(3, (997,)), 0xdeadbeef, 997)
+    out = d.numpy()
+    assert (out[3:1000] == 0xdeadbeef).all() and (out[:3] == 0).all() and out[1000] == 0
+
+
+# ------------------------------------------------------------------- blur07
+
+SPANS = [1, 2, 3, 4, 5, 8, 9, 16, 33, 64, 65, 128, 129, 254, 255]
+
+
+@pytest.mark.parametrize("kind", ["h", "v", "2d"])
+@pytest.mark.parametrize("inplace", [False, True])
+def test_blur07(ctx, orc, kind, inplace):
+    W, H = 300, 260
+    src = rand_img(30, W * H)
+    d_src = ctx.upload(src)
+    d_dst = ctx.empty(W * H)
+    for span in SPANS:
+        _say("blur07", kind, "inplace" if inplace else "oop", span)
+        s = (span + 0.5) / 255.0
+        f_gpu = {"h": ctx.blur07_h, "v": ctx.blur07_v, "2d": ctx.blur07}[kind]
+        f_orc = {"h": orc.orc_blur07_h, "v": orc.orc_blur07_v, "2d": orc.orc_blur07}[kind]
+        ref = src.copy() if inplace else np.zeros(W * H, np.uint32)
+        f_orc(P(ref), P(ref if inplace else src), W, H, s, 8)
+        if inplace:
+            ctx.write(d_dst, src)
+            f_gpu(d_dst, d_dst, W, H, s)
+        else:
+            f_gpu(d_dst, d_src, W, H, s)
+        assert np.array_equal(d_dst.numpy(), ref), (kind, inplace, span)
+
+
+def test_blur07_full_frame(ctx, orc):
+    W, H = 1280, 720
+    src = rand_img(31, W * H)
+    d_src, d_dst = ctx.upload(src), ctx.empty(W * H)
+    for r in [1, 7, 32]:
+        _say("blur07 frame", r)
+        s = (2 * r + 1.5) / 255
+        ctx.blur07(d_dst, d_src, W, H, s)
+        ref = np.zeros(W * H, np.uint32)
+        orc.orc_blur07(P(ref), P(src), W, H, s, 8)
+        assert np.array_equal(d_dst.numpy(), ref), r
+
+
+def test_blur07_rejects_bad_args(ctx):
+    d = ctx.empty(64 * 8)
+    with pytest.raises(CkdError):
+        ctx.blur07_v(d, d, 64, 8, 10 / 255)   # span 10 > 8 rows
+    with pytest.raises(CkdError):
+        ctx.blur07_h(d, d, 64, 8, -1.0)
+
+
+# ------------------------------------------------------------------- blur26
+
+@pytest.mark.parametrize("kind", ["h", "v", "2d"])
+def test_blur26(ctx, orc, kind):
+    cases = [(64, 48, 10.0, 0.0, 1), (64, 48, 37.5, 0.25, 2), (128, 96, 100.0, 1.0, 3),
+             (200, 120, 3.3, 0.0, 3), (96, 200, 55.0, 0.5, 4), (1280, 720, 12.0, 0.0, 3),
+             (2048, 64, 100.0, 0.0, 1)]
+    for W, H, strength, gain, passes in cases:
+        _say("blur26", kind, W, H, strength, gain, passes)
+        src = np.zeros(W * H + 4, np.uint32)
+        src[:W * H] = rand_img(40 + W, W * H)
+        ref = np.zeros(W * H + 4, np.uint32)
+        s0 = np.zeros(W * H + 4, np.uint32)
+        s1 = np.zeros(W * H + 4, np.uint32)
+        f_orc = {"h": orc.orc_blur26_h, "v": orc.orc_blur26_v, "2d": orc.orc_blur26}[kind]
+        f_orc(P(ref), P(src), W, H, strength, gain, passes, P(s0), P(s1))
+        d_src, d_dst = ctx.upload(src[:W * H]), ctx.empty(W * H)
+        f_gpu = {"h": ctx.blur26_h, "v": ctx.blur26_v, "2d": ctx.blur26}[kind]
+        f_gpu(d_dst, d_src, W, H, strength, gain, passes)
+        assert np.array_equal(d_dst.numpy(), ref[:W * H]), (kind, W, H, strength, gain, passes)
+        ctx.free(d_src)
+        ctx.free(d_dst)
+
+
+# -------------------------------------------------------------------- remap
+
+@pytest.mark.parametrize("W,H", [(1280, 720), (640, 360)])
+def test_polar(ctx, orc, W, H):
+    _say("polar", W, H)
+    ctx.polar_create(W, H)
+    src = rand_img(50, W * H)
+    m0, m1 = np.zeros(W * H * 2, np.int32), np.zeros(W * H * 2, np.int32)
+    orc.orc_polar_maps(P(m0), P(m1), W, H, W, H)
+    d_src, d_dst = ctx.upload(src), ctx.empty(W * H)
+    for inv, m in [(0, m0), (1, m1)]:
+        ctx.polar_blit(d_dst, d_src, W, H, inv)
+        ref = np.zeros(W * H, np.uint32)
+        orc.orc_polar_blit(P(ref), P(src), P(m), W, H, W, 8)
+        assert np.array_equal(d_dst.numpy(), ref), inv
+
+
+@pytest.mark.parametrize("W,H", [(1280, 720), (320, 180), (1920, 1080)])
+def test_grid_remap(ctx, orc, W, H):
+    _say("grid remap", W, H)
+    tex = wl.noise_texture(0, 9)
+    grid = wl.tunnel_grid(W, H, 3.7)
+    d_tex, d_grid, d_dst = ctx.upload(tex), ctx.upload(grid), ctx.empty(W * H)
+    ctx.grid_remap(d_dst, W, H, d_grid, d_tex, 9)
+    ref = np.zeros(W * H, np.uint32)
+    orc.orc_grid_remap(P(ref), W, H, P(grid), P(tex), 9, 8)
+    assert np.array_equal(d_dst.numpy(), ref)
+
+
+@pytest.mark.parametrize("W,H,r", [(1280, 720, 1), (1280, 720, 17), (1280, 720, 32), (320, 180, 32)])
+def test_tunnel_frame(ctx, orc, W, H, r):
+    _say("tunnel frame", W, H, r)
+    tex = wl.noise_texture(0, 9)
+    grid = wl.tunnel_grid(W, H, 11.0 + r)
+    d_tex, d_grid = ctx.upload(tex), ctx.upload(grid)
+    d_dst, t0, t1 = ctx.empty(W * H), ctx.empty(W * H), ctx.empty(W * H)
+    ctx.tunnel_frame(d_dst, t0, t1, W, H, d_grid, d_tex, 9, r)
+    ref, o0, o1 = (np.zeros(W * H, np.uint32) for _ in range(3))
+    orc.orc_tunnel_frame(P(ref), P(o0), P(o1), W, H, P(grid), P(tex), 9, r, 8)
+    assert np.array_equal(d_dst.numpy(), ref)
+
+
+# -------------------------------------------------------------------- voxel
+
+@pytest.fixture(scope="module")
+def lmaps(ctx):
+    m = wl.landscape_maps(0)
+    ctx.landscape_create(m.height, m.color, m.fog)
+    return m
+
+
+@pytest.mark.parametrize("W,H", [(1280, 720), (320, 180), (1920, 1080), (2560, 1440)])
+def test_landscape(ctx, orc, lmaps, W, H):
+    path = wl.landscape_path(0)
+    times = [0.0, 0.5, 1.25, 7.0]
+    cams = [wl.landscape_cam(path, t) for t in times]
+    _say("landscape", W, H)
+    d = ctx.empty(len(cams) * W * H)
+    ctx.landscape_draw(d, W, H, cams)
+    out = d.numpy().reshape(len(cams), W * H)
+    for k, cam in enumerate(cams):
+        ref = np.zeros(W * H, np.uint32)
+        oc = orc_cam(cam)
+        orc.orc_landscape_frame(P(ref), W, H, P(lmaps.height), P(lmaps.color), P(lmaps.fog),
+                                C.byref(oc), 8)
+        assert np.array_equal(out[k], ref), (W, H, times[k])
+
+
+def test_tunnelscape(ctx, orc):
+    _say("tunnelscape")
+    m = wl.tunnelscape_maps(0)
+    ctx.tunnelscape_create(m.height, m.color, m.fog)
+    W, H = 1280, 720
+    d = ctx.empty(W * H)
+    for (su, sv, sp, t) in [(0.3, 1.2, 2.0, 0.0), (-0.7, 0.9, 1.0, 3.5)]:
+        ctx.tunnelscape_rows(d, W, H, su, sv, sp, t)
+        ref = np.zeros(W * H, np.uint32)
+        orc.orc_tscape_frame(P(ref), W, H, P(m.height), P(m.color), P(m.fog), su, sv, sp, t, 8)
+        assert np.array_equal(d.numpy(), ref)
+
+
+# ---------------------------------------------------------------------- SDF
+
+@pytest.mark.parametrize("nsph,steps,W,H", [(8, 64, 320, 180), (8, 64, 640, 360), (32, 128, 320, 180)])
+@pytest.mark.parametrize("shading", [CKD_SHADE_FP32, CKD_SHADE_FP16])
+def test_sdf_vs_fp64(ctx, orc, nsph, steps, W, H, shading):
+    sc = wl.sdf_scene(0, nsph, steps)
+    d = ctx.empty(W * H)
+    cnt = ctx.upload(np.zeros(1, np.uint64))
+    for t in [0.0, 0.55, 31.0]:
+        cam = wl.sdf_camera(sc, t)
+        ctx.write(cnt, np.zeros(1, np.uint64))
+        ctx.sdf_render(d, W, H, sc, cam, shading, cnt)
+        out = d.numpy()
+        ref = np.zeros(W * H, np.uint32)
+        st = np.zeros(W * H, np.uint16)
+        orc.orc_sdf_frame(P(ref), W, H, C.byref(sc), t, 64, P(st), 8)
+        es = error_stats(out, ref)
+        gpu_steps = int(cnt.numpy()[0])
+        _say("sdf", nsph, W, H, "fp16" if shading else "fp32", t, es, "steps", gpu_steps,
+             int(st.sum()))
+        assert es["frac_over_tol"] <= 0.005, es
+        assert es["mean"] <= 0.25, es
+        assert abs(gpu_steps - int(st.sum())) <= 0.01 * st.sum()
+
+
+def test_sdf_camera_matches_cpu_renderer(orc):
+    sc = wl.sdf_scene(0, 8, 64)
+    for t in [0.0, 1.0 / 60, 0.55, 59.9]:
+        a = np.zeros(12, np.float32)
+        orc.orc_sdf_camera(C.byref(sc), t, P(a))
+        assert np.array_equal(a, np.array(wl.sdf_camera(sc, t), np.float32))
+
+
+# -------------------------------------------------------------------- batch
+
+def test_batch_matches_single_calls(ctx, lmaps):
+    _say("batch")
+    from cookiedough_amd._lib import CKD_FX_TUNNEL
+    items = wl.batch_requests(3, 24)
+    sc = wl.sdf_scene(0, 8, 64)
+    path = wl.landscape_path(0)
+    tex = wl.noise_texture(0, 9)
+    d_tex = ctx.upload(tex)
+    total = sum(i.W * i.H for i in items)
+    tmp_px = sum(2 * i.W * i.H for i in items if i.effect == CKD_FX_TUNNEL)
+    out = ctx.empty(total)
+    tmp = ctx.empty(max(tmp_px, 1))
+    reqs, off, grids = [], 0, []
+    for it in items:
+        g = ctx.upload(wl.tunnel_grid(it.W, it.H, it.t)) if it.effect == CKD_FX_TUNNEL else None
+        grids.append(g)
+        reqs.append(wl.make_request(it, out.ptr + 4 * off, sc, path, g.ptr if g else None))
+        off += it.W * it.H
+    ctx.render_batch(reqs, sc, CKD_SHADE_FP16, d_tex, 9, tmp, tmp_px)
+    got = out.numpy()
+    # the same requests one by one through the single-frame entry points
+    off = 0
+    single = ctx.empty(max(i.W * i.H for i in items))
+    t0, t1 = ctx.empty(single.shape), ctx.empty(single.shape)
+    for it, g in zip(items, grids):
+        n = it.W * it.H
+        if it.effect == 0:
+            ctx.sdf_render(single, it.W, it.H, sc, wl.sdf_camera(sc, it.t), CKD_SHADE_FP16)
+        elif it.effect == 1:
+            ctx.landscape_draw(single, it.W, it.H, [wl.landscape_cam(path, it.t)])
+        else:
+            ctx.tunnel_frame(single, t0, t1, it.W, it.H, g, d_tex, 9, it.radius)
+        assert np.array_equal(single.numpy()[:n], got[off:off + n]), (it,)
+        off += n
