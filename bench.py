@@ -93,7 +93,7 @@ def main() -> int:
 
     items = wl.batch_requests(0, args.requests)
     total_px = sum(i.W * i.H for i in items)
-    tmp_px = sum(2 * i.W * i.H for i in items if i.effect == CKD_FX_TUNNEL)
+    tmp_px = sum(i.W * i.H for i in items if i.effect == CKD_FX_TUNNEL)
     out = ctx.empty(total_px)
     tmp = ctx.empty(max(1, tmp_px))
     reqs, off, grids = [], 0, []
@@ -155,12 +155,12 @@ def main() -> int:
     radii = wl.tunnel_radii(0, 240)
     tgrids = [ctx.upload(wl.tunnel_grid(W, H, k / 60)) for k in range(240)]
     fr_tun = ctx.empty(240 * W * H)
-    t0, t1 = ctx.empty(W * H), ctx.empty(W * H)
-    tun_tmp = ctx.empty(2 * 240 * W * H)
+    t0 = ctx.empty(W * H)
+    tun_tmp = ctx.empty(240 * W * H)
 
     def tun_plain():
         for k in range(240):
-            ctx.tunnel_frame(fr_tun.view(k * W * H, (W * H,)), t0, t1, W, H, tgrids[k], d_tex, 9,
+            ctx.tunnel_frame(fr_tun.view(k * W * H, (W * H,)), t0, W, H, tgrids[k], d_tex, 9,
                              int(radii[k]))
 
     tun_reqs = []
@@ -169,7 +169,7 @@ def main() -> int:
         tun_reqs.append(wl.make_request(it, fr_tun.ptr + 4 * k * W * H, scene8, path, tgrids[k].ptr))
 
     def tun_batched():
-        ctx.render_batch(tun_reqs, scene8, CKD_SHADE_FP16, d_tex, 9, tun_tmp, 2 * 240 * W * H)
+        ctx.render_batch(tun_reqs, scene8, CKD_SHADE_FP16, d_tex, 9, tun_tmp, 240 * W * H)
 
     for name, frames, plain, batched in [("sdf_c0", 64, sdf_plain, sdf_batched),
                                          ("voxel_c1", 120, vox_plain, vox_batched),

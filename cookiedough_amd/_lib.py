@@ -83,7 +83,7 @@ SIGNATURES = {
     "ckd_tunnelscape_create": [_p, _p, _p, _p],
     "ckd_tunnelscape_rows": [_p, _p, _u, _u, _f, _f, _f, _f],
     "ckd_sdf_render": [_p, _p, _u, _u, C.POINTER(SdfScene), C.POINTER(_f), _i, _p],
-    "ckd_tunnel_frame": [_p, _p, _p, _p, _u, _u, _p, _p, _u, _i],
+    "ckd_tunnel_frame": [_p, _p, _p, _u, _u, _p, _p, _u, _i],
     "ckd_render_batch": [_p, C.POINTER(Request), _u, C.POINTER(SdfScene), _i, _p, _u, _p,
                          C.c_size_t, _p],
 }

@@ -19,7 +19,7 @@ CSRC = PKG / "csrc"
 OBJDIR = PKG / "_build"
 LIB = PKG / "libckd.so"
 
-SOURCES = ["runtime.hip", "k_blit.hip", "k_blur07.hip", "k_blur26.hip", "k_remap.hip",
+SOURCES = ["runtime.hip", "k_blit.hip", "k_blur07.hip", "k_blur26.hip", "k_remap.hip", "k_tunnel.hip",
            "k_voxel.hip", "k_sdf.hip", "k_batch.hip"]
 
 HIPCC_FLAGS = [
@@ -69,6 +69,14 @@ def build(verbose: bool = True) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # guard over the generated assembly (build container only)
+    guard = PKG / "isa_check.py"
+    if guard.exists():
+        import importlib.util
+        spec = importlib.util.spec_from_file_location("ckd_isa_check", guard)
+        mod = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(mod)
+        mod.check(hipcc(), HIPCC_FLAGS, CSRC, SOURCES)
     if verbose:
         print(f"built {LIB}")
     return LIB

@@ -118,3 +118,34 @@ CKD_DEV uint32_t ckd_span_pixel(const uint32_t acc[4])
     }
     return px;
 }
+
+// ---- 2007 box blur helpers --------------------------------------------
+// Four 8-bit channels spread into the 16-bit fields of a u64: window sums
+// of up to 255 pixels stay below 2^16 per field, so sums and differences of
+// spread values (prefix sums included) are exact field by field.
+CKD_DEV uint64_t ckd_spread(uint32_t px)
+{
+    return (uint64_t)(px & 0xffu) | ((uint64_t)(px & 0xff00u) << 8) |
+           ((uint64_t)(px & 0xff0000u) << 16) | ((uint64_t)(px & 0xff000000u) << 24);
+}
+
+// Div (code/deprecated/boxblur.cpp:39-42): mulhi_epu16(acc, div) per field,
+// then packus_epi16
+CKD_DEV uint32_t ckd_b07_div_pack(uint64_t acc, uint32_t div)
+{
+    uint32_t px = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t a = (uint32_t)(acc >> (16 * k)) & 0xffffu;
+        const uint32_t q = (a * div) >> 16;
+        px |= (q > 32767u ? 0u : (q > 255u ? 255u : q)) << (8 * k);
+    }
+    return px;
+}
+
+// WeightToDiv(16*count) (code/deprecated/boxblur.cpp:11-14) stored through
+// _mm_set1_epi16: the divisor of an odd-span window of `count` pixels
+CKD_DEV uint32_t ckd_b07_count_div(uint32_t count)
+{
+    return count ? ((((1u << 24) / (16u * count)) >> 4) & 0xffffu) : 0u;
+}

@@ -52,7 +52,8 @@ CKD_DEV int32_t lerp_term(int32_t A, int32_t B, int32_t alpha)
     return A + ((int32_t)((uint32_t)(B - A) * (uint32_t)alpha) >> 16);
 }
 
-// two channels (c, c+2 of byte order B,G,R,A -> fields lo/hi) per u64
+// two channels (c, c+2 of byte order B,G,R,A -> fields lo/hi) per u64; the
+// prefix sums of T stay below 2^32 per field, so fields never carry
 CKD_DEV uint64_t pack2(int32_t lo, int32_t hi) { return (uint64_t)(uint32_t)lo | ((uint64_t)(uint32_t)hi << 32); }
 
 CKD_DEV uint32_t div_lane(int32_t sum, uint32_t k)
@@ -166,11 +167,14 @@ __global__ __launch_bounds__(kBlock) void k_b26_pass(uint32_t *__restrict__ out,
     for (int o = threadIdx.x; o < L; o += kBlock) {
         const int hEnd = min(iSpan + o, L) + 1;
         const int tEnd = max(o - iSpan, 0);
-        const uint64_t d0 = PT0[hEnd] - Hbase0 - PT0[tEnd];
-        const uint64_t d1 = PT1[hEnd] - Hbase1 - PT1[tEnd];
-        const int32_t sum[4] = {S0[0] + (int32_t)(uint32_t)d0, S0[1] + (int32_t)(uint32_t)d1,
-                                S0[2] + (int32_t)(uint32_t)(d0 >> 32),
-                                S0[3] + (int32_t)(uint32_t)(d1 >> 32)};
+        // heads minus tails can be negative, so subtract field by field (a
+        // u64 difference would borrow across the 32-bit fields)
+        const uint64_t a0 = PT0[hEnd], a1 = PT1[hEnd], t0 = PT0[tEnd], t1 = PT1[tEnd];
+        const int32_t sum[4] = {
+            S0[0] + (int32_t)((uint32_t)a0 - (uint32_t)Hbase0 - (uint32_t)t0),
+            S0[1] + (int32_t)((uint32_t)a1 - (uint32_t)Hbase1 - (uint32_t)t1),
+            S0[2] + (int32_t)((uint32_t)(a0 >> 32) - (uint32_t)(Hbase0 >> 32) - (uint32_t)(t0 >> 32)),
+            S0[3] + (int32_t)((uint32_t)(a1 >> 32) - (uint32_t)(Hbase1 >> 32) - (uint32_t)(t1 >> 32))};
         const uint32_t k = o < iSpan ? P.spanScales[o]
                                      : (o >= L - iSpan ? P.spanScales[L - 1 - o] : P.iScale);
         uint32_t pix = 0;

@@ -5,6 +5,8 @@
 // texels, blend them with ckd_bilerp32 (exactly bsamp32_32) and store.
 #include "ckd_internal.h"
 #include "ckd_pixel.h"
+#include "ckd_grid.h"
+#include "ckd_jobs.h"
 
 #include <cfloat>
 #include <cmath>
@@ -66,34 +68,6 @@ __global__ __launch_bounds__(256) void k_polar_blit(uint32_t *__restrict__ dst,
 
 } // namespace
 
-namespace ckd {
-
-// Grid remap of one pixel (build-defined law):
-//   corners C00,C10,C01,C11 of block (x>>3, y>>3), local (i,j) = (x&7, y&7)
-//   L = C00*(8-j) + C01*j, R = C10*(8-j) + C11*j, UV = (L*(8-i) + R*i) >> 6
-// then bsamp_prepUVs (code/bilinear.h:10-31) with wrap mask 2^texLog2-1 and
-// bsamp32_32 (code/bilinear.h:86-125).
-CKD_DEV uint32_t grid_pixel(const int2 *__restrict__ grid, unsigned GW,
-                            const uint32_t *__restrict__ tex, unsigned texLog2, unsigned x,
-                            unsigned y)
-{
-    const unsigned bx = x >> 3, by = y >> 3;
-    const int i = (int)(x & 7), j = (int)(y & 7);
-    const int2 *g = grid + (size_t)by * GW + bx;
-    const int2 c00 = g[0], c10 = g[1], c01 = g[GW], c11 = g[GW + 1];
-    const int Lu = c00.x * (8 - j) + c01.x * j, Ru = c10.x * (8 - j) + c11.x * j;
-    const int Lv = c00.y * (8 - j) + c01.y * j, Rv = c10.y * (8 - j) + c11.y * j;
-    const int U = (Lu * (8 - i) + Ru * i) >> 6;
-    const int V = (Lv * (8 - i) + Rv * i) >> 6;
-    const unsigned mask = (1u << texLog2) - 1;
-    const unsigned u0 = ((unsigned)(U >> 8)) & mask, u1 = ((unsigned)(U >> 8) + 1) & mask;
-    const unsigned v0 = (((unsigned)(V >> 8)) & mask) << texLog2;
-    const unsigned v1 = (((unsigned)(V >> 8) + 1) & mask) << texLog2;
-    return ckd_bilerp32(tex[u0 + v0], tex[u1 + v0], tex[u0 + v1], tex[u1 + v1],
-                        (uint32_t)U & 0xff, (uint32_t)V & 0xff);
-}
-
-} // namespace ckd
 
 namespace {
 
@@ -112,6 +86,12 @@ __global__ __launch_bounds__(256) void k_grid_remap(uint32_t *__restrict__ dst, 
 } // namespace
 
 namespace ckd {
+void tunnel_job_geometry(TunJob &J, uint32_t *dst, uint32_t *tmp, const int32_t *grid, unsigned W,
+                         unsigned H, int radius);
+uint32_t tunnel_vblocks(unsigned W, unsigned H);
+ckd_status launch_tunnel_jobs(ckd_ctx *ctx, const TunJob *jobs, int nJobs, const TunJob &single,
+                              uint32_t totalRows, uint32_t totalVBlocks, unsigned maxW,
+                              const uint32_t *tex, unsigned texLog2);
 ckd_status blur07_hv_oop(ckd_ctx *ctx, uint32_t *tmp, uint32_t *dst, const uint32_t *src,
                          unsigned W, unsigned H, float strength);
 ckd_status launch_blit_add(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, unsigned destStride,
@@ -181,22 +161,20 @@ ckd_status ckd_grid_remap(ckd_ctx *ctx, uint32_t *dst, unsigned W, unsigned H,
     return ckd::launch_grid_remap(ctx, dst, W, H, grid, tex, texLog2);
 }
 
-ckd_status ckd_tunnel_frame(ckd_ctx *ctx, uint32_t *dst, uint32_t *tmp0, uint32_t *tmp1,
-                            unsigned W, unsigned H, const int32_t *grid, const uint32_t *tex,
-                            unsigned texLog2, int radius)
+ckd_status ckd_tunnel_frame(ckd_ctx *ctx, uint32_t *dst, uint32_t *tmp, unsigned W, unsigned H,
+                            const int32_t *grid, const uint32_t *tex, unsigned texLog2, int radius)
 {
-    CKD_REQUIRE(dst && tmp0 && tmp1 && grid && tex, "ckd_tunnel_frame: null buffer");
+    CKD_REQUIRE(dst && tmp && grid && tex, "ckd_tunnel_frame: null buffer");
     CKD_REQUIRE(W > 0 && H > 0 && W <= 4096 && H <= 4096, "ckd_tunnel_frame: bad size %ux%u", W, H);
     CKD_REQUIRE(texLog2 >= 1 && texLog2 <= 14, "ckd_tunnel_frame: texLog2 %u out of 1..14", texLog2);
     CKD_REQUIRE(radius >= 1 && radius <= 127 && (unsigned)(2 * radius + 1) <= (W < H ? W : H),
                 "ckd_tunnel_frame: radius %d out of range", radius);
     CKD_REQUIRE_CTX(ctx);
-    ckd_status s = ckd::launch_grid_remap(ctx, dst, W, H, grid, tex, texLog2);
-    if (s != CKD_OK) return s;
-    const float strength = (2.f * (float)radius + 1.5f) / 255.f;
-    s = ckd::blur07_hv_oop(ctx, tmp0, tmp1, dst, W, H, strength);
-    if (s != CKD_OK) return s;
-    return ckd::launch_blit_add(ctx, dst, tmp1, W, W, W, H);
+    TunJob J;
+    ckd::tunnel_job_geometry(J, dst, tmp, grid, W, H, radius);
+    J.rowBase = 0;
+    J.vBase = 0;
+    return ckd::launch_tunnel_jobs(ctx, nullptr, 1, J, H, ckd::tunnel_vblocks(W, H), W, tex, texLog2);
 }
 
 } // extern "C"

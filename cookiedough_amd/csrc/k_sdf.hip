@@ -16,11 +16,16 @@
 // lanes of a wave march neighbouring rays.
 #include "ckd_internal.h"
 #include "ckd_pixel.h"
+#include "ckd_jobs.h"
 
 
 namespace {
 
 typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+
+// v_sqrt_f32 (1 ulp) for the distance field; the default sqrtf is the
+// correctly rounded multi-instruction sequence
+CKD_DEV float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 struct SdfCam {
     float o[3], f[3], r[3], u[3];
@@ -33,7 +38,7 @@ CKD_DEV float scene_dist(const ckd_sdf_scene &sc, float px, float py, float pz)
         const float dx = px - sc.sphere[i][0];
         const float dy = py - sc.sphere[i][1];
         const float dz = pz - sc.sphere[i][2];
-        const float l = sqrtf(dx * dx + dy * dy + dz * dz) - sc.sphere[i][3];
+        const float l = fsqrt(dx * dx + dy * dy + dz * dz) - sc.sphere[i][3];
         d = fminf(d, l);
     }
     return d;
@@ -47,7 +52,7 @@ CKD_DEV int scene_id(const ckd_sdf_scene &sc, float px, float py, float pz)
         const float dx = px - sc.sphere[i][0];
         const float dy = py - sc.sphere[i][1];
         const float dz = pz - sc.sphere[i][2];
-        const float l = sqrtf(dx * dx + dy * dy + dz * dz) - sc.sphere[i][3];
+        const float l = fsqrt(dx * dx + dy * dy + dz * dz) - sc.sphere[i][3];
         if (l < d) {
             d = l;
             best = i;
@@ -77,7 +82,7 @@ CKD_DEV MarchResult march(const ckd_sdf_scene &sc, const SdfCam &cam, unsigned W
     dx = cam.f[0] + u * cam.r[0] + v * cam.u[0];
     dy = cam.f[1] + u * cam.r[1] + v * cam.u[1];
     dz = cam.f[2] + u * cam.r[2] + v * cam.u[2];
-    const float il = 1.f / sqrtf(dx * dx + dy * dy + dz * dz);
+    const float il = __builtin_amdgcn_rsqf(dx * dx + dy * dy + dz * dz);
     dx *= il;
     dy *= il;
     dz *= il;
@@ -100,16 +105,12 @@ CKD_DEV MarchResult march(const ckd_sdf_scene &sc, const SdfCam &cam, unsigned W
 }
 
 template <bool kHalf>
-CKD_DEV uint32_t shade(const ckd_sdf_scene &sc, const MarchResult &m)
+CKD_DEV uint32_t shade_lit(const ckd_sdf_scene &sc, const MarchResult &m, float nx, float ny,
+                           float nz, int id)
 {
-    const float h = 1e-3f;
-    float nx = scene_dist(sc, m.hx + h, m.hy, m.hz) - scene_dist(sc, m.hx - h, m.hy, m.hz);
-    float ny = scene_dist(sc, m.hx, m.hy + h, m.hz) - scene_dist(sc, m.hx, m.hy - h, m.hz);
-    float nz = scene_dist(sc, m.hx, m.hy, m.hz + h) - scene_dist(sc, m.hx, m.hy, m.hz - h);
-    const int id = scene_id(sc, m.hx, m.hy, m.hz);
     const float t3 = m.total * m.total * m.total;
     if (!kHalf) {
-        const float nl = sqrtf(nx * nx + ny * ny + nz * nz);
+        const float nl = fsqrt(nx * nx + ny * ny + nz * nz);
         const float inl = nl > 0.f ? 1.f / nl : 0.f;
         float diff = (nx * sc.light[0] + ny * sc.light[1] + nz * sc.light[2]) * inl;
         diff = diff > 0.f ? diff : 0.f;
@@ -157,6 +158,17 @@ CKD_DEV uint32_t shade(const ckd_sdf_scene &sc, const MarchResult &m)
     }
 }
 
+template <bool kHalf>
+CKD_DEV uint32_t shade(const ckd_sdf_scene &sc, const MarchResult &m)
+{
+    const float h = 1e-3f;
+    float nx = scene_dist(sc, m.hx + h, m.hy, m.hz) - scene_dist(sc, m.hx - h, m.hy, m.hz);
+    float ny = scene_dist(sc, m.hx, m.hy + h, m.hz) - scene_dist(sc, m.hx, m.hy - h, m.hz);
+    float nz = scene_dist(sc, m.hx, m.hy, m.hz + h) - scene_dist(sc, m.hx, m.hy, m.hz - h);
+    const int id = scene_id(sc, m.hx, m.hy, m.hz);
+    return shade_lit<kHalf>(sc, m, nx, ny, nz, id);
+}
+
 CKD_DEV void count_steps(unsigned long long *stepCount, int steps)
 {
     if (!stepCount) return;
@@ -183,6 +195,233 @@ __global__ __launch_bounds__(256) void k_sdf(uint32_t *__restrict__ dst, unsigne
         dst[(size_t)y * W + x] = shade<kHalf>(sc, m);
     }
     count_steps(stepCount, steps);
+}
+
+// ---- persistent batched tracer ------------------------------------------
+//
+// One launch renders every SDF request of a batch.  Each wave pulls 32x32
+// pixel chunks from a global queue (one atomic per chunk).  Inside a chunk
+// the wave keeps its 64 lanes busy: lanes whose ray has finished are
+// refilled with the chunk's next pixels (ballot + mbcnt ranks), so a
+// long-marching ray no longer holds 63 idle lanes.  Finished rays are
+// pushed to a per-wave LDS queue and shaded 64 at a time, so shading also
+// runs with full waves.  The march evaluates |o + t*d - c|^2 as
+// t^2 + t*B + C with B = 2 d.(o-c) per ray and C = |o-c|^2 per chunk:
+// two VALU ops, a square root, a subtract and a min per sphere and step.
+
+struct ShadeEntry {
+    uint32_t job, pix;
+    float tHit, total;
+};
+
+constexpr int kChunkSide = 32;
+constexpr int kStepsPerRefill = 4;
+constexpr int kQueueCap = 128;
+constexpr int kWavesPerBlock = 4;
+
+CKD_DEV uint32_t lane_id() { return __lane_id(); }
+
+// direction of pixel (x, y) through the camera basis
+CKD_DEV void ray_dir(const float *cam, uint32_t W, uint32_t H, uint32_t x, uint32_t y, float &dx,
+                     float &dy, float &dz)
+{
+    const float fH = (float)H;
+    const float u = (2.f * (float)x - (float)W) / fH;
+    const float v = (fH - 2.f * (float)y) / fH;
+    dx = cam[3] + u * cam[6] + v * cam[9];
+    dy = cam[4] + u * cam[7] + v * cam[10];
+    dz = cam[5] + u * cam[8] + v * cam[11];
+    const float il = __builtin_amdgcn_rsqf(dx * dx + dy * dy + dz * dz);
+    dx *= il;
+    dy *= il;
+    dz *= il;
+}
+
+// shading of one finished ray: hit point, central-difference normal with
+// the six offsets taken from one |p-c|^2 per sphere
+// (|p +- h e_k - c|^2 = q + h^2 +- 2 h (p-c)_k), argmin id, Lambert, fog,
+// gamma, pack
+template <bool kHalf, int kMaxSph>
+CKD_DEV uint32_t shade_hit(const ckd_sdf_scene &sc, const float *cam, float dx, float dy, float dz,
+                           float tHit, float total)
+{
+    const float hx = cam[0] + dx * tHit, hy = cam[1] + dy * tHit, hz = cam[2] + dz * tHit;
+    const float h = 1e-3f, hh = h * h, h2 = 2.f * h;
+    const float pl = hy - sc.groundY;
+    float mxp = pl, mxm = pl, myp = pl + h, mym = pl - h, mzp = pl, mzm = pl;
+    float best = pl;
+    int id = sc.nSpheres;
+#pragma unroll
+    for (int i = 0; i < kMaxSph; ++i) {
+        if (i < sc.nSpheres) {
+            const float ex = hx - sc.sphere[i][0], ey = hy - sc.sphere[i][1], ez = hz - sc.sphere[i][2];
+            const float r = sc.sphere[i][3];
+            const float q = ex * ex + ey * ey + ez * ez;
+            const float qh = q + hh;
+            const float c = fsqrt(q) - r;
+            if (c < best) {
+                best = c;
+                id = i;
+            }
+            mxp = fminf(mxp, fsqrt(qh + h2 * ex) - r);
+            mxm = fminf(mxm, fsqrt(qh - h2 * ex) - r);
+            myp = fminf(myp, fsqrt(qh + h2 * ey) - r);
+            mym = fminf(mym, fsqrt(qh - h2 * ey) - r);
+            mzp = fminf(mzp, fsqrt(qh + h2 * ez) - r);
+            mzm = fminf(mzm, fsqrt(qh - h2 * ez) - r);
+        }
+    }
+    const float nx = mxp - mxm, ny = myp - mym, nz = mzp - mzm;
+    MarchResult m;
+    m.hx = hx;
+    m.hy = hy;
+    m.hz = hz;
+    m.total = total;
+    m.steps = 0;
+    return shade_lit<kHalf>(sc, m, nx, ny, nz, id);
+}
+
+template <bool kHalf, int kMaxSph>
+__global__ __launch_bounds__(256) void k_sdf_persistent(uint32_t *__restrict__ queueCounter,
+                                                        uint32_t totalChunks,
+                                                        const SdfJob *__restrict__ jobs, int nJobs,
+                                                        const ckd_sdf_scene sc,
+                                                        unsigned long long *__restrict__ stepCount)
+{
+    __shared__ ShadeEntry queue[kWavesPerBlock][kQueueCap];
+    const uint32_t lane = lane_id();
+    const uint32_t wave = threadIdx.x >> 6;
+    ShadeEntry *q = queue[wave];
+    int qn = 0; // entries waiting in this wave's queue (uniform)
+
+    // chunk state (uniform)
+    int job = -1;
+    uint32_t next = 0, end = 0, cx0 = 0, cy0 = 0, cw = 1;
+    float C[kMaxSph];
+    float cam[12];
+    uint32_t W = 0, H = 0;
+
+    // ray state (per lane)
+    bool active = false;
+    uint32_t pix = 0;
+    float t = 0.f, d = 1.f, tHit = 0.f, dx = 0.f, dy = 0.f, dz = 0.f;
+    int steps = 0;
+    float B[kMaxSph];
+    unsigned long long mySteps = 0;
+
+    while (true) {
+        // refill finished lanes from the current chunk
+        const uint64_t idle = __ballot(!active);
+        if (idle && next < end) {
+            const uint32_t rank = __popcll(idle & ((1ull << lane) - 1));
+            if (!active) {
+                const uint32_t k = next + rank;
+                if (k < end) {
+                    const uint32_t x = cx0 + k % cw, y = cy0 + k / cw;
+                    pix = y * W + x;
+                    ray_dir(cam, W, H, x, y, dx, dy, dz);
+#pragma unroll
+                    for (int i = 0; i < kMaxSph; ++i) {
+                        const float ox = cam[0] - sc.sphere[i][0], oy = cam[1] - sc.sphere[i][1],
+                                    oz = cam[2] - sc.sphere[i][2];
+                        B[i] = 2.f * (dx * ox + dy * oy + dz * oz);
+                    }
+                    t = 0.f;
+                    d = 1.f;
+                    tHit = 0.f;
+                    steps = 0;
+                    active = true;
+                }
+            }
+            next = min(end, next + (uint32_t)__popcll(idle));
+        }
+        if (!__any(active)) {
+            // chunk drained: take the next one
+            uint32_t c = 0;
+            if (lane == 0) c = atomicAdd(queueCounter, 1u);
+            c = __shfl(c, 0, 64);
+            if (c >= totalChunks) break;
+            job = ckd_find_job(jobs, nJobs, c, &SdfJob::chunkBase);
+            const SdfJob &J = jobs[job];
+            W = J.W;
+            H = J.H;
+            for (int k = 0; k < 12; ++k) cam[k] = J.cam[k];
+            const uint32_t local = c - J.chunkBase;
+            cx0 = (local % J.chunksX) * kChunkSide;
+            cy0 = (local / J.chunksX) * kChunkSide;
+            cw = min((uint32_t)kChunkSide, W - cx0);
+            const uint32_t chh = min((uint32_t)kChunkSide, H - cy0);
+            next = 0;
+            end = cw * chh;
+#pragma unroll
+            for (int i = 0; i < kMaxSph; ++i) {
+                const float ox = cam[0] - sc.sphere[i][0], oy = cam[1] - sc.sphere[i][1],
+                            oz = cam[2] - sc.sphere[i][2];
+                C[i] = ox * ox + oy * oy + oz * oz;
+            }
+            continue;
+        }
+        // march a few steps
+        const float oyg = cam[1] - sc.groundY;
+#pragma unroll
+        for (int k = 0; k < kStepsPerRefill; ++k) {
+            if (active && d > sc.eps && steps < sc.maxSteps && t < sc.tMax) {
+                const float tt = t * t;
+                float dist = fmaf(t, dy, oyg);
+#pragma unroll
+                for (int i = 0; i < kMaxSph; ++i) {
+                    if (i < sc.nSpheres) {
+                        const float qq = fmaf(t, B[i], C[i] + tt);
+                        dist = fminf(dist, fsqrt(qq) - sc.sphere[i][3]);
+                    }
+                }
+                tHit = t;
+                d = dist;
+                t += dist;
+                ++steps;
+            }
+        }
+        // retire finished rays into the shade queue
+        const bool fin = active && !(d > sc.eps && steps < sc.maxSteps && t < sc.tMax);
+        const uint64_t finMask = __ballot(fin);
+        if (finMask) {
+            if (fin) {
+                const uint32_t slot = qn + __popcll(finMask & ((1ull << lane) - 1));
+                q[slot] = ShadeEntry{(uint32_t)job, pix, tHit, t};
+                mySteps += (unsigned long long)steps;
+                active = false;
+            }
+            qn += __popcll(finMask);
+            __builtin_amdgcn_wave_barrier();
+            if (qn >= 64) {
+                // shade 64 queued hits with the whole wave
+                const ShadeEntry e = q[qn - 64 + lane];
+                const SdfJob &J = jobs[e.job];
+                const uint32_t x = e.pix % J.W, y = e.pix / J.W;
+                float ex, ey, ez;
+                ray_dir(J.cam, J.W, J.H, x, y, ex, ey, ez);
+                J.dst[e.pix] = shade_hit<kHalf, kMaxSph>(sc, J.cam, ex, ey, ez, e.tHit, e.total);
+                qn -= 64;
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+    }
+    // leftover hits
+    __builtin_amdgcn_wave_barrier();
+    if ((int)lane < qn) {
+        const ShadeEntry e = q[lane];
+        const SdfJob &J = jobs[e.job];
+        const uint32_t x = e.pix % J.W, y = e.pix / J.W;
+        float ex, ey, ez;
+        ray_dir(J.cam, J.W, J.H, x, y, ex, ey, ez);
+        J.dst[e.pix] = shade_hit<kHalf, kMaxSph>(sc, J.cam, ex, ey, ez, e.tHit, e.total);
+    }
+    if (stepCount) {
+        unsigned long long s = mySteps;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        if (lane == 0) atomicAdd(stepCount, s);
+    }
 }
 
 ckd_status validate_scene(const ckd_sdf_scene *sc)
@@ -216,6 +455,31 @@ ckd_status launch_sdf(ckd_ctx *ctx, uint32_t *dst, unsigned W, unsigned H,
     else
         k_sdf<false><<<grid, 256, 0, ctx->stream>>>(dst, W, H, sc, cam, stepCount);
     return check_launch("sdf");
+}
+
+ckd_status launch_sdf_persistent(ckd_ctx *ctx, uint32_t *queueCounter, uint32_t totalChunks,
+                                 const SdfJob *jobs, int nJobs, const ckd_sdf_scene &sc,
+                                 int shading, unsigned long long *stepCount)
+{
+    if (totalChunks == 0) return CKD_OK;
+    // persistent: enough 4-wave blocks to fill every CU several times over;
+    // surplus blocks find the queue empty and exit
+    unsigned blocks = (unsigned)ctx->cuCount * 8;
+    const unsigned useful = (totalChunks + kWavesPerBlock - 1) / kWavesPerBlock;
+    if (blocks > useful) blocks = useful;
+    const bool big = sc.nSpheres > 8;
+    if (shading == CKD_SHADE_FP16) {
+        if (big)
+            k_sdf_persistent<true, 32><<<blocks, 256, 0, ctx->stream>>>(queueCounter, totalChunks, jobs, nJobs, sc, stepCount);
+        else
+            k_sdf_persistent<true, 8><<<blocks, 256, 0, ctx->stream>>>(queueCounter, totalChunks, jobs, nJobs, sc, stepCount);
+    } else {
+        if (big)
+            k_sdf_persistent<false, 32><<<blocks, 256, 0, ctx->stream>>>(queueCounter, totalChunks, jobs, nJobs, sc, stepCount);
+        else
+            k_sdf_persistent<false, 8><<<blocks, 256, 0, ctx->stream>>>(queueCounter, totalChunks, jobs, nJobs, sc, stepCount);
+    }
+    return check_launch("sdf persistent");
 }
 
 } // namespace ckd

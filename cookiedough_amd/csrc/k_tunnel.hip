@@ -1,0 +1,143 @@
+// k_tunnel.hip -- the tunnel frame pipeline: per-8x8-block grid remap, the
+// 2007 box blur with odd span 2e+1 (horizontal then vertical, out of place)
+// and BlitAdd32 of the blur onto the remap (code/deprecated/boxblur.cpp,
+// code/util.cpp:754-773), in two fused kernels:
+//
+//   T1  one block per row: remap the row into LDS and into dst, prefix-sum
+//       it, write the horizontal blur to tmp
+//   T2  one wave per 64 columns x 32 rows: vertical blur of tmp and the
+//       saturating add into dst
+//
+// For an odd span the window of x is [x-e, x+e] clipped to the line and the
+// reference's divisor is WeightToDiv(16*count) of that window's pixel count
+// (edge ramps and full window alike), so out = div(sum, count).
+#include "ckd_internal.h"
+#include "ckd_pixel.h"
+#include "ckd_grid.h"
+#include "ckd_jobs.h"
+
+namespace {
+
+constexpr int kT1Block = 256;
+constexpr int kT2Seg = 32;
+
+CKD_DEV const TunJob &pick(const TunJob *jobs, int nJobs, const TunJob &single, uint32_t unit,
+                           uint32_t TunJob::*base)
+{
+    if (!jobs) return single;
+    return jobs[ckd_find_job(jobs, nJobs, unit, base)];
+}
+
+__global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict__ jobs, int nJobs,
+                                                       const TunJob single,
+                                                       const uint32_t *__restrict__ tex,
+                                                       unsigned texLog2)
+{
+    extern __shared__ uint64_t lds[];
+    __shared__ uint32_t D[256];
+    __shared__ uint64_t waveTot[kT1Block / 64];
+    const TunJob &J = pick(jobs, nJobs, single, blockIdx.x, &TunJob::rowBase);
+    const int W = (int)J.W;
+    const unsigned y = blockIdx.x - J.rowBase;
+    uint64_t *P = lds;
+    uint32_t *px = reinterpret_cast<uint32_t *>(lds + W + 1);
+
+    D[threadIdx.x] = ckd_b07_count_div(threadIdx.x);
+    const unsigned GW = (J.W + 7) / 8 + 1;
+    const int2 *grid = reinterpret_cast<const int2 *>(J.grid);
+    uint32_t *drow = J.dst + (size_t)y * W;
+    for (int x = threadIdx.x; x < W; x += kT1Block) {
+        const uint32_t v = ckd::grid_pixel(grid, GW, tex, texLog2, (unsigned)x, y);
+        px[x] = v;
+        drow[x] = v;
+    }
+    __syncthreads();
+
+    const int chunk = (W + kT1Block - 1) / kT1Block;
+    const int c0 = threadIdx.x * chunk, c1 = min(c0 + chunk, W);
+    uint64_t local = 0;
+    for (int i = c0; i < c1; ++i) local += ckd_spread(px[i]);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint64_t incl = local;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint64_t v = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += v;
+    }
+    if (lane == 63) waveTot[wave] = incl;
+    __syncthreads();
+    uint64_t run = incl - local;
+    for (int w = 0; w < wave; ++w) run += waveTot[w];
+    if (threadIdx.x == 0) P[0] = 0;
+    for (int i = c0; i < c1; ++i) {
+        run += ckd_spread(px[i]);
+        P[i + 1] = run;
+    }
+    __syncthreads();
+
+    const int e = (int)J.edge;
+    uint32_t *trow = J.tmp + (size_t)y * W;
+    for (int x = threadIdx.x; x < W; x += kT1Block) {
+        const int lo = max(x - e, 0), hi = min(x + e, W - 1);
+        trow[x] = ckd_b07_div_pack(P[hi + 1] - P[lo], D[hi - lo + 1]);
+    }
+}
+
+__global__ __launch_bounds__(64) void k_tunnel_v(const TunJob *__restrict__ jobs, int nJobs,
+                                                 const TunJob single)
+{
+    __shared__ uint32_t D[256];
+    const TunJob &J = pick(jobs, nJobs, single, blockIdx.x, &TunJob::vBase);
+    for (int i = threadIdx.x; i < 256; i += 64) D[i] = ckd_b07_count_div((uint32_t)i);
+    __syncthreads();
+    const uint32_t local = blockIdx.x - J.vBase;
+    const int x = (int)((local % J.vBlocksX) * 64 + threadIdx.x);
+    const int y0 = (int)((local / J.vBlocksX) * kT2Seg);
+    const int W = (int)J.W, H = (int)J.H, e = (int)J.edge;
+    if (x >= W) return;
+    const uint32_t *col = J.tmp + x;
+    const int y1 = min(y0 + kT2Seg, H);
+    uint64_t F = 0;
+    for (int r = max(0, y0 - e); r <= min(H - 1, y0 + e); ++r) F += ckd_spread(col[(size_t)r * W]);
+    for (int y = y0; y < y1; ++y) {
+        const int lo = max(y - e, 0), hi = min(y + e, H - 1);
+        const uint32_t b = ckd_b07_div_pack(F, D[hi - lo + 1]);
+        uint32_t *d = J.dst + (size_t)y * W + x;
+        *d = ckd_adds_u8x4(*d, b);
+        if (y + e + 1 <= H - 1) F += ckd_spread(col[(size_t)(y + e + 1) * W]);
+        if (y - e >= 0) F -= ckd_spread(col[(size_t)(y - e) * W]);
+    }
+}
+
+} // namespace
+
+namespace ckd {
+
+void tunnel_job_geometry(TunJob &J, uint32_t *dst, uint32_t *tmp, const int32_t *grid, unsigned W,
+                         unsigned H, int radius)
+{
+    J.dst = dst;
+    J.tmp = tmp;
+    J.grid = grid;
+    J.W = W;
+    J.H = H;
+    J.edge = (uint32_t)radius;
+    J.vBlocksX = (W + 63) / 64;
+}
+
+uint32_t tunnel_vblocks(unsigned W, unsigned H) { return ((W + 63) / 64) * ((H + kT2Seg - 1) / kT2Seg); }
+
+ckd_status launch_tunnel_jobs(ckd_ctx *ctx, const TunJob *jobs, int nJobs, const TunJob &single,
+                              uint32_t totalRows, uint32_t totalVBlocks, unsigned maxW,
+                              const uint32_t *tex, unsigned texLog2)
+{
+    if (totalRows == 0) return CKD_OK;
+    const size_t lds = (size_t)(maxW + 1) * sizeof(uint64_t) + (size_t)maxW * sizeof(uint32_t);
+    k_tunnel_h<<<totalRows, kT1Block, lds, ctx->stream>>>(jobs, nJobs, single, tex, texLog2);
+    ckd_status s = check_launch("tunnel remap+H");
+    if (s != CKD_OK) return s;
+    k_tunnel_v<<<totalVBlocks, 64, 0, ctx->stream>>>(jobs, nJobs, single);
+    return check_launch("tunnel V+add");
+}
+
+} // namespace ckd

@@ -8,6 +8,7 @@
 // each pixel is written exactly once.
 #include "ckd_internal.h"
 #include "ckd_pixel.h"
+#include "ckd_jobs.h"
 
 #include <cstring>
 #include <vector>
@@ -48,17 +49,16 @@ CKD_DEV uint32_t udiv24(uint32_t num, uint32_t den)
 // contraction and correctly rounded division and square root, so the
 // integer ray parameters equal the host's bit for bit.
 #pragma clang fp contract(off)
-CKD_DEV void ls_setup(const LsLaunch &P, const LsFrame &F, unsigned iRay, int &dX, int &dY,
-                      int &fpFishMul)
+CKD_DEV void ls_setup(float rayY, float colScale, float halfW, float X1, float Y1, float viewCos,
+                      float viewSin, unsigned iRay, int &dX, int &dY, int &fpFishMul)
 {
-    const float rayX = P.colScale * ((float)iRay - P.halfW);
-    const float rayY = P.rayY;
-    const float rotRayX = F.viewCos * rayX - F.viewSin * rayY;
-    const float rotRayY = F.viewSin * rayX + F.viewCos * rayY;
-    const float X2 = F.X1 + rotRayX;
-    const float Y2 = F.Y1 + rotRayY;
-    float fdX = X2 - F.X1;
-    float fdY = Y2 - F.Y1;
+    const float rayX = colScale * ((float)iRay - halfW);
+    const float rotRayX = viewCos * rayX - viewSin * rayY;
+    const float rotRayY = viewSin * rayX + viewCos * rayY;
+    const float X2 = X1 + rotRayX;
+    const float Y2 = Y1 + rotRayY;
+    float fdX = X2 - X1;
+    float fdY = Y2 - Y1;
     if (fabsf(fdX + fdY) > 1.1920928955078125e-07f) {
         const float length = 1.f / sqrtf(fdX * fdX + fdY * fdY);
         fdX *= length;
@@ -71,20 +71,17 @@ CKD_DEV void ls_setup(const LsLaunch &P, const LsFrame &F, unsigned iRay, int &d
 }
 #pragma clang fp contract(on)
 
-// vscape_ray (code/landscape.cpp:56-107) for column iRay of frame blockIdx.y
-__global__ __launch_bounds__(64) void k_landscape(uint32_t *__restrict__ dst, unsigned W,
-                                                  unsigned H, const uint8_t *__restrict__ hmap,
-                                                  const uint32_t *__restrict__ cmap,
-                                                  const uint32_t *__restrict__ fog, LsLaunch P)
+// vscape_ray (code/landscape.cpp:56-107) for column iRay of one frame, then
+// the fused memset32(fog[0]) for the rows no span reached
+CKD_DEV void ls_column(uint32_t *__restrict__ col, unsigned W, unsigned H, unsigned iRay,
+                       float rayY, float colScale, float halfW, int mapScale, float X1, float Y1,
+                       float viewCos, float viewSin, int mapTilt,
+                       const uint8_t *__restrict__ hmap, const uint32_t *__restrict__ cmap,
+                       const uint32_t *__restrict__ fog)
 {
-    const unsigned iRay = blockIdx.x * 64 + threadIdx.x;
-    if (iRay >= W) return;
-    const LsFrame F = P.f[blockIdx.y];
-    uint32_t *col = dst + (size_t)blockIdx.y * W * H + iRay;
-
     int dX, dY, fpFishMul;
-    ls_setup(P, F, iRay, dX, dY, fpFishMul);
-    int curX = (int)(F.X1 * 256.f), curY = (int)(F.Y1 * 256.f);
+    ls_setup(rayY, colScale, halfW, X1, Y1, viewCos, viewSin, iRay, dX, dY, fpFishMul);
+    int curX = (int)(X1 * 256.f), curY = (int)(Y1 * 256.f);
 
     int lastHeight = (int)H, lastDrawn = (int)H;
     uint32_t lastColor =
@@ -106,27 +103,52 @@ __global__ __launch_bounds__(64) void k_landscape(uint32_t *__restrict__ dst, un
 
         // ((255-h)<<16) / (fpFishMul*(iStep+1)) is an unsigned division
         const uint32_t q = udiv24((255u - h) << 16, (uint32_t)fpFishMul * (iStep + 1));
-        const int hgt = (((int)q * P.mapScale) >> 8) + F.mapTilt;
+        const int hgt = (((int)q * mapScale) >> 8) + mapTilt;
 
         if (hgt < lastDrawn) {
             const uint32_t drawLength = (uint32_t)(lastDrawn - hgt);
-            CkdSpan s;
-            ckd_span_setup(s, color, lastColor, (uint32_t)(lastHeight - hgt), drawLength);
+            CkdSpan sp;
+            ckd_span_setup(sp, color, lastColor, (uint32_t)(lastHeight - hgt), drawLength);
             uint32_t *p = col + (size_t)hgt * W;
             for (uint32_t k = 0; k < drawLength; ++k) {
-                *p = ckd_span_pixel(s.acc);
+                *p = ckd_span_pixel(sp.acc);
                 p += W;
 #pragma unroll
-                for (int c = 0; c < 4; ++c) s.acc[c] += s.step[c];
+                for (int c = 0; c < 4; ++c) sp.acc[c] += sp.step[c];
             }
             lastDrawn = hgt;
         }
         lastHeight = hgt;
         lastColor = color;
     }
-    // the fused memset32(fog[0]) of Landscape_Draw for the rows never drawn
     const uint32_t sky = fog[0];
     for (int y = 0; y < lastDrawn; ++y) col[(size_t)y * W] = sky;
+}
+
+__global__ __launch_bounds__(64) void k_landscape(uint32_t *__restrict__ dst, unsigned W,
+                                                  unsigned H, const uint8_t *__restrict__ hmap,
+                                                  const uint32_t *__restrict__ cmap,
+                                                  const uint32_t *__restrict__ fog, LsLaunch P)
+{
+    const unsigned iRay = blockIdx.x * 64 + threadIdx.x;
+    if (iRay >= W) return;
+    const LsFrame &F = P.f[blockIdx.y];
+    ls_column(dst + (size_t)blockIdx.y * W * H + iRay, W, H, iRay, P.rayY, P.colScale, P.halfW,
+              P.mapScale, F.X1, F.Y1, F.viewCos, F.viewSin, F.mapTilt, hmap, cmap, fog);
+}
+
+// every voxel request of a batch in one launch: block -> job by binary search
+__global__ __launch_bounds__(64) void k_landscape_jobs(const VoxJob *__restrict__ jobs, int nJobs,
+                                                       const uint8_t *__restrict__ hmap,
+                                                       const uint32_t *__restrict__ cmap,
+                                                       const uint32_t *__restrict__ fog)
+{
+    const int j = ckd_find_job(jobs, nJobs, blockIdx.x, &VoxJob::blockBase);
+    const VoxJob &J = jobs[j];
+    const unsigned iRay = (blockIdx.x - J.blockBase) * 64 + threadIdx.x;
+    if (iRay >= J.W) return;
+    ls_column(J.dst + iRay, J.W, J.H, iRay, J.rayY, J.colScale, J.halfW, J.mapScale, J.X1, J.Y1,
+              J.viewCos, J.viewSin, J.mapTilt, hmap, cmap, fog);
 }
 
 // ---- tunnelscape ---------------------------------------------------------
@@ -223,6 +245,35 @@ float ls_rayY(unsigned W, unsigned H)
 int scale_rows(int v720, unsigned H) { return (int)(((long long)v720 * (long long)H) / 720); }
 
 } // namespace
+
+namespace ckd {
+
+void voxel_job_geometry(VoxJob &J, unsigned W, unsigned H, const ckd_landscape_cam &c)
+{
+    J.W = W;
+    J.H = H;
+    J.rayY = ls_rayY(W, H);
+    J.colScale = 0.25f * (1280.f / (float)W);
+    J.halfW = (float)W * 0.5f;
+    J.mapScale = scale_rows(512, H);
+    J.X1 = c.X1;
+    J.Y1 = c.Y1;
+    J.viewCos = c.viewCos;
+    J.viewSin = c.viewSin;
+    J.mapTilt = scale_rows(c.mapTilt, H);
+}
+
+ckd_status launch_landscape_jobs(ckd_ctx *ctx, const VoxJob *jobs, int nJobs, uint32_t totalBlocks)
+{
+    if (totalBlocks == 0) return CKD_OK;
+    if (!ctx->lsHeight)
+        return fail(CKD_ESTATE, "voxel requests need the landscape maps (ckd_landscape_create)");
+    k_landscape_jobs<<<totalBlocks, 64, 0, ctx->stream>>>(jobs, nJobs, ctx->lsHeight, ctx->lsColor,
+                                                          ctx->lsFog);
+    return check_launch("landscape jobs");
+}
+
+} // namespace ckd
 
 extern "C" {
 

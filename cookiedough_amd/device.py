@@ -162,9 +162,9 @@ class Context:
         check(self.lib.ckd_sdf_render(self.h, ptr(dst), W, H, C.byref(scene), cam, shading,
                                       ptr(step_counter)))
 
-    def tunnel_frame(self, dst, tmp0, tmp1, W, H, grid, tex, texLog2, radius):
-        check(self.lib.ckd_tunnel_frame(self.h, ptr(dst), ptr(tmp0), ptr(tmp1), W, H, ptr(grid),
-                                        ptr(tex), texLog2, radius))
+    def tunnel_frame(self, dst, tmp, W, H, grid, tex, texLog2, radius):
+        check(self.lib.ckd_tunnel_frame(self.h, ptr(dst), ptr(tmp), W, H, ptr(grid), ptr(tex),
+                                        texLog2, radius))
 
     def render_batch(self, requests, scene, shading, tex, texLog2, tmp, tmp_pixels,
                      step_counter=None):

@@ -192,13 +192,16 @@ CKD_API ckd_status ckd_sdf_render(ckd_ctx *ctx, uint32_t *dst, unsigned W,
                                   const float *cam, int shading,
                                   unsigned long long *stepCount);
 
-/* ---- c2 tunnel frame (grid remap, 2007 blur H then V out of place with
- * span 2r+1, BlitAdd32 of the blur onto the remap); tmp0/tmp1 are W*H
- * scratch images.  radius in 1..127 ---------------------------------------- */
-CKD_API ckd_status ckd_tunnel_frame(ckd_ctx *ctx, uint32_t *dst, uint32_t *tmp0,
-                                    uint32_t *tmp1, unsigned W, unsigned H,
-                                    const int32_t *grid, const uint32_t *tex,
-                                    unsigned texLog2, int radius);
+/* ---- tunnel frame (no reference counterpart as a whole: the tunnel
+ * benchmark's pipeline of grid remap, 2007 blur H then V out of place with
+ * span 2r+1 -- HorizontalBoxBlur32 / VerticalBoxBlur32,
+ * code/deprecated/boxblur.cpp:44-220 -- and BlitAdd32 of the blur onto the
+ * remap, code/util.cpp:754-773).  tmp is a W*H scratch image; radius in
+ * 1..127 with 2r+1 <= min(W, H). */
+CKD_API ckd_status ckd_tunnel_frame(ckd_ctx *ctx, uint32_t *dst, uint32_t *tmp,
+                                    unsigned W, unsigned H, const int32_t *grid,
+                                    const uint32_t *tex, unsigned texLog2,
+                                    int radius);
 
 /* ---- batched renderer (no reference counterpart: many frames, effects and
  * resolutions in one persistent tile-queue launch) ------------------------ */
@@ -217,7 +220,9 @@ typedef struct ckd_request {
 /* Renders requests[0..n) (host array).  SDF requests use `scene` and the
  * given shading precision, VOXEL requests the landscape maps, TUNNEL
  * requests `tex` (2^texLog2 square) and need scratch: tmp (device) of at
- * least sum over TUNNEL requests of 2*W*H pixels.  stepCount as above. */
+ * least sum over TUNNEL requests of W*H pixels.  stepCount as above.
+ * SDF requests run in one persistent launch over a queue of 32x32-pixel
+ * chunks, VOXEL requests in one launch, TUNNEL requests in two. */
 CKD_API ckd_status ckd_render_batch(ckd_ctx *ctx, const ckd_request *requests,
                                     unsigned n, const ckd_sdf_scene *scene,
                                     int shading, const uint32_t *tex,

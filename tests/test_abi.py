@@ -52,7 +52,7 @@ def test_create_without_device_fails_cleanly_or_succeeds_on_gpu():
     (lambda l: l.ckd_blit_add32(None, 1, 1, 10, 20, 4), "srcResX"),
     (lambda l: l.ckd_blit_add32a(None, 1, 1, 20, 10, 4, -1.0), "alpha"),
     (lambda l: l.ckd_grid_remap(None, 1, 64, 64, 1, 1, 20), "texLog2"),
-    (lambda l: l.ckd_tunnel_frame(None, 1, 1, 1, 320, 180, 1, 1, 9, 100), "radius"),
+    (lambda l: l.ckd_tunnel_frame(None, 1, 1, 320, 180, 1, 1, 9, 100), "radius"),
     (lambda l: l.ckd_polar_create(None, 3, 3), "size"),
     (lambda l: l.ckd_event_record(None, 99), "slot"),
 ])

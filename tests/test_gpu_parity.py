@@ -129,7 +129,12 @@ def test_blur26(ctx, orc, kind):
         d_src, d_dst = ctx.upload(src[:W * H]), ctx.empty(W * H)
         f_gpu = {"h": ctx.blur26_h, "v": ctx.blur26_v, "2d": ctx.blur26}[kind]
         f_gpu(d_dst, d_src, W, H, strength, gain, passes)
-        assert np.array_equal(d_dst.numpy(), ref[:W * H]), (kind, W, H, strength, gain, passes)
+        got = d_dst.numpy()
+        bad = np.nonzero(got != ref[:W * H])[0]
+        if len(bad):
+            _say("blur26 mismatch", len(bad), "first", [(int(b) % W, int(b) // W, hex(int(got[b])),
+                                                          hex(int(ref[b]))) for b in bad[:6]])
+        assert len(bad) == 0, (kind, W, H, strength, gain, passes)
         ctx.free(d_src)
         ctx.free(d_dst)
 
@@ -169,8 +174,8 @@ def test_tunnel_frame(ctx, orc, W, H, r):
     tex = wl.noise_texture(0, 9)
     grid = wl.tunnel_grid(W, H, 11.0 + r)
     d_tex, d_grid = ctx.upload(tex), ctx.upload(grid)
-    d_dst, t0, t1 = ctx.empty(W * H), ctx.empty(W * H), ctx.empty(W * H)
-    ctx.tunnel_frame(d_dst, t0, t1, W, H, d_grid, d_tex, 9, r)
+    d_dst, t0 = ctx.empty(W * H), ctx.empty(W * H)
+    ctx.tunnel_frame(d_dst, t0, W, H, d_grid, d_tex, 9, r)
     ref, o0, o1 = (np.zeros(W * H, np.uint32) for _ in range(3))
     orc.orc_tunnel_frame(P(ref), P(o0), P(o1), W, H, P(grid), P(tex), 9, r, 8)
     assert np.array_equal(d_dst.numpy(), ref)
@@ -259,7 +264,7 @@ def test_batch_matches_single_calls(ctx, lmaps):
     tex = wl.noise_texture(0, 9)
     d_tex = ctx.upload(tex)
     total = sum(i.W * i.H for i in items)
-    tmp_px = sum(2 * i.W * i.H for i in items if i.effect == CKD_FX_TUNNEL)
+    tmp_px = sum(i.W * i.H for i in items if i.effect == CKD_FX_TUNNEL)
     out = ctx.empty(total)
     tmp = ctx.empty(max(tmp_px, 1))
     reqs, off, grids = [], 0, []
@@ -273,7 +278,7 @@ def test_batch_matches_single_calls(ctx, lmaps):
     # the same requests one by one through the single-frame entry points
     off = 0
     single = ctx.empty(max(i.W * i.H for i in items))
-    t0, t1 = ctx.empty(single.shape), ctx.empty(single.shape)
+    t0 = ctx.empty(single.shape)
     for it, g in zip(items, grids):
         n = it.W * it.H
         if it.effect == 0:
@@ -281,6 +286,41 @@ def test_batch_matches_single_calls(ctx, lmaps):
         elif it.effect == 1:
             ctx.landscape_draw(single, it.W, it.H, [wl.landscape_cam(path, it.t)])
         else:
-            ctx.tunnel_frame(single, t0, t1, it.W, it.H, g, d_tex, 9, it.radius)
-        assert np.array_equal(single.numpy()[:n], got[off:off + n]), (it,)
+            ctx.tunnel_frame(single, t0, it.W, it.H, g, d_tex, 9, it.radius)
+        if it.effect == 0:
+            es = error_stats(got[off:off + n], single.numpy()[:n])
+            assert es["frac_over_tol"] <= 0.005 and es["mean"] <= 0.25, (it, es)
+        else:
+            assert np.array_equal(single.numpy()[:n], got[off:off + n]), (it,)
         off += n
+
+
+@pytest.mark.parametrize("nsph,steps", [(8, 64), (32, 128)])
+@pytest.mark.parametrize("shading", [CKD_SHADE_FP32, CKD_SHADE_FP16])
+def test_batch_sdf_vs_fp64(ctx, orc, nsph, steps, shading):
+    """the persistent tracer (chunk queue, lane refill, deferred shading)
+    against the FP64 render, over requests of several sizes in one launch"""
+    sc = wl.sdf_scene(0, nsph, steps)
+    sizes = [(320, 180, 0.0), (200, 120, 3.3), (64, 40, 12.0), (333, 77, 45.5)]
+    total = sum(w * h for w, h, _ in sizes)
+    out = ctx.empty(total)
+    cnt = ctx.upload(np.zeros(1, np.uint64))
+    reqs, off = [], 0
+    for w, h, t in sizes:
+        reqs.append(wl.make_request(wl.BatchItem(0, w, h, t), out.ptr + 4 * off, sc, None, None))
+        off += w * h
+    ctx.render_batch(reqs, sc, shading, None, 9, None, 0, cnt)
+    got = out.numpy()
+    off, ref_steps = 0, 0
+    for w, h, t in sizes:
+        ref = np.zeros(w * h, np.uint32)
+        st = np.zeros(w * h, np.uint16)
+        orc.orc_sdf_frame(P(ref), w, h, C.byref(sc), t, 64, P(st), 8)
+        es = error_stats(got[off:off + w * h], ref)
+        _say("batch sdf", nsph, shading, (w, h, t), es)
+        assert es["frac_over_tol"] <= 0.005 and es["mean"] <= 0.25, es
+        ref_steps += int(st.sum())
+        off += w * h
+    gpu_steps = int(cnt.numpy()[0])
+    _say("steps", gpu_steps, ref_steps)
+    assert abs(gpu_steps - ref_steps) <= 0.01 * ref_steps
