@@ -89,7 +89,6 @@ def main() -> int:
     d_tex = ctx.upload(tex)
     scene8 = wl.sdf_scene(0, 8, 64)
     scene32 = wl.sdf_scene(0, 32, 128)
-    counter = ctx.upload(np.zeros(1, np.uint64))
 
     items = wl.batch_requests(0, args.requests)
     total_px = sum(i.W * i.H for i in items)
@@ -102,18 +101,34 @@ def main() -> int:
         grids.append(g)
         reqs.append(wl.make_request(it, out.ptr + 4 * off, scene8, path, g.ptr if g else None))
         off += it.W * it.H
+    reqs = ctx.request_array(reqs)
     log(f"inputs ready in {time.time() - t_in:.1f}s: {len(items)} requests, {total_px / 1e6:.1f} Mpx")
 
+    # every launch of the persistent SDF kernel in this run is timed with HIP
+    # events (the roofline's kernel); pcount accumulates its march steps and
+    # psdf_px the pixels it shades
+    pcount = ctx.upload(np.zeros(1, np.uint64))
+    psdf = {"px": 0}
+    sdf_px_batch = sum(i.W * i.H for i in items if i.effect == CKD_FX_SDF)
+    # launch shapes tuned on this batch (untimed), then the batch captured
+    # once as a hipGraph; a step is one replay
+    t_tune = time.time()
+    tuning = ctx.autotune_batch(reqs, scene8, CKD_SHADE_FP16, d_tex, 9, tmp, tmp_px).as_dict()
+    log(f"autotuned in {time.time() - t_tune:.1f}s: {tuning}")
+    graph = ctx.batch_graph(reqs, scene8, CKD_SHADE_FP16, d_tex, 9, tmp, tmp_px, pcount)
+    ctx.kernel_timing(True)
+
     def batch_step():
-        ctx.render_batch(reqs, scene8, CKD_SHADE_FP16, d_tex, 9, tmp, tmp_px, counter)
+        graph.launch()
+        psdf["px"] += sdf_px_batch
 
     # ------------------------------------------------------------- headline
     for _ in range(args.warmup):
         batch_step()
     ctx.sync()
-    ctx.write(counter, np.zeros(1, np.uint64))
+    st_before = int(pcount.numpy()[0])
     ms_total = T.run(batch_step, args.steps)
-    steps_total = int(counter.numpy()[0])
+    steps_total = int(pcount.numpy()[0]) - st_before
     ms_step = ms_total / args.steps
     value = total_px / (ms_step * 1e-3) / 1e6
     log(f"batch: {ms_step:.2f} ms/step, {value:.0f} Mpx/s")
@@ -121,7 +136,6 @@ def main() -> int:
     # ----------------------------------------------------- per effect (c0-c2)
     W, H = 1280, 720
     per = {}
-    frame = ctx.empty(W * H)
 
     # SDF c0: 64 frames at t=k/60
     cams = [wl.sdf_camera(scene8, k / 60) for k in range(64)]
@@ -136,9 +150,11 @@ def main() -> int:
         r = wl.make_request(wl.BatchItem(CKD_FX_SDF, W, H, k / 60), fr_sdf.ptr + 4 * k * W * H,
                             scene8, path, None)
         sdf_reqs.append(r)
+    sdf_reqs = ctx.request_array(sdf_reqs)
 
     def sdf_batched():
-        ctx.render_batch(sdf_reqs, scene8, CKD_SHADE_FP16, None, 9, None, 0)
+        ctx.render_batch(sdf_reqs, scene8, CKD_SHADE_FP16, None, 9, None, 0, pcount)
+        psdf["px"] += 64 * W * H
 
     # voxel c1: 120 frames on the Lissajous path
     lcams = [wl.landscape_cam(path, k / 60) for k in range(120)]
@@ -167,6 +183,7 @@ def main() -> int:
     for k in range(240):
         it = wl.BatchItem(CKD_FX_TUNNEL, W, H, k / 60, int(radii[k]))
         tun_reqs.append(wl.make_request(it, fr_tun.ptr + 4 * k * W * H, scene8, path, tgrids[k].ptr))
+    tun_reqs = ctx.request_array(tun_reqs)
 
     def tun_batched():
         ctx.render_batch(tun_reqs, scene8, CKD_SHADE_FP16, d_tex, 9, tun_tmp, 240 * W * H)
@@ -184,27 +201,48 @@ def main() -> int:
                      "speedup": round(mp / mb, 3)}
         log(name, per[name])
 
-    # ------------------------------------------------- roofline: SDF kernel
-    # FP16-shaded tracer on the 2560x1440 precision form (32 spheres, 128
-    # steps), events around each launch
-    W3, H3 = 2560, 1440
-    f3 = ctx.empty(W3 * H3)
-    cams3 = [wl.sdf_camera(scene32, k / 60) for k in range(8)]
-    ctx.sdf_render(f3, W3, H3, scene32, cams3[0], CKD_SHADE_FP16)
-    ctx.write(counter, np.zeros(1, np.uint64))
-    ms3 = 0.0
-    for c in cams3:
-        ms3 += T.run(lambda: ctx.sdf_render(f3, W3, H3, scene32, c, CKD_SHADE_FP16, counter))
-    st3 = int(counter.numpy()[0])
-    flops3 = sdf_flops(st3, len(cams3) * W3 * H3, 32)
-    achieved = flops3 / (ms3 * 1e-3) / 1e12
-    roofline = {"bound": "valu", "kernel": "k_sdf<FP16> 2560x1440 32 spheres",
+    # ------------------------------------------- roofline: persistent tracer
+    ms_k, launches = ctx.kernel_timing_read(1)  # CKD_KT_SDF_PERSISTENT
+    ctx.kernel_timing(False)
+    steps_k = int(pcount.numpy()[0])
+    flops_k = sdf_flops(steps_k, psdf["px"], 8)
+    achieved = flops_k / (ms_k * 1e-3) / 1e12
+    roofline = {"bound": "valu", "kernel": "k_sdf_persistent<fp16 shading, 8 spheres>",
                 "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
-                "avg_launch_ms": round(ms3 / len(cams3), 3)}
+                "launches": launches, "avg_launch_ms": round(ms_k / max(1, launches), 4),
+                "algorithmic": "per launch: sum(steps)*(F_sdf+7) + px*(6*F_sdf+40), "
+                               "F_sdf = 11*8+2 = 90 FLOP"}
     log("roofline", roofline)
 
-    # ------------------------------------------------------------ CPU baseline
+    # --------------------------------------- c3 precision form (2560x1440)
+    W3, H3 = 2560, 1440
+    n3 = 64
+    fr3 = ctx.empty(n3 * W3 * H3)
+    cams3 = [wl.sdf_camera(scene32, k / 60) for k in range(n3)]
+
+    def c3_plain():
+        for k, c in enumerate(cams3):
+            ctx.sdf_render(fr3.view(k * W3 * H3, (W3 * H3,)), W3, H3, scene32, c, CKD_SHADE_FP32)
+
+    reqs3 = [wl.make_request(wl.BatchItem(CKD_FX_SDF, W3, H3, k / 60), fr3.ptr + 4 * k * W3 * H3,
+                             scene32, path, None) for k in range(n3)]
+    reqs3 = ctx.request_array(reqs3)
+
+    def c3_batched():
+        ctx.render_batch(reqs3, scene32, CKD_SHADE_FP16, None, 9, None, 0)
+
+    c3_plain()
+    c3_batched()
+    mp, mb = T.run(c3_plain), T.run(c3_batched)
+    px3 = n3 * W3 * H3
+    per["sdf_c3_precision"] = {"frames": n3, "plain_fp32_mpx_s": round(px3 / mp / 1e3, 1),
+                               "batched_fp16_mpx_s": round(px3 / mb / 1e3, 1),
+                               "speedup": round(mp / mb, 3)}
+    log("sdf_c3", per["sdf_c3_precision"])
+
+    f3 = fr3
+
     # The CPU leg is the only part of the bench that runs the oracle: the
     # timed CPU baseline and the FP64 reference render the precision form's
     # error is measured against.
@@ -217,42 +255,48 @@ def main() -> int:
         nthr = min(16, os.cpu_count() or 1)
         # precision form: FP16 shading vs the FP64 render on a row sample
         errs = []
+        c3_batched()
         for k in [0, 63]:
-            cam = wl.sdf_camera(scene32, k / 60)
-            ctx.sdf_render(f3, W3, H3, scene32, cam, CKD_SHADE_FP16)
-            img = f3.numpy().reshape(H3, W3)
+            img = fr3.view(k * W3 * H3, (W3 * H3,)).numpy().reshape(H3, W3)
             rows = np.arange(7, H3, 36)
             ref = np.zeros(len(rows) * W3, np.uint32)
             orc.orc_sdf_rows(oracles.P(ref), W3, H3, C.byref(scene32), k / 60, 64, 7, 36, None, nthr)
             errs.append((img[rows].reshape(-1), ref))
         es = error_stats(np.concatenate([e[0] for e in errs]), np.concatenate([e[1] for e in errs]))
         log("precision", es)
-        # bounded sample of the batched workload: the first requests of each
-        # kind at their own resolutions, rendered by the oracle
-        sample = [it for it in items if it.effect == CKD_FX_SDF][:2] + \
-                 [it for it in items if it.effect == CKD_FX_VOXEL][:3] + \
-                 [it for it in items if it.effect == CKD_FX_TUNNEL][:3]
-        px = 0
-        t_c = time.perf_counter()
-        for it in sample:
+        # bounded sample of the batched workload: its requests in order, as
+        # many as fit in about 10 s of CPU time, rendered by the oracle
+        def cpu_render(it, nthreads):
             n = it.W * it.H
             buf = np.zeros(n, np.uint32)
             if it.effect == CKD_FX_SDF:
-                orc.orc_sdf_frame(oracles.P(buf), it.W, it.H, C.byref(scene8), it.t, 32, None, nthr)
+                orc.orc_sdf_frame(oracles.P(buf), it.W, it.H, C.byref(scene8), it.t, 32, None, nthreads)
             elif it.effect == CKD_FX_VOXEL:
                 oc = oracles.orc_cam(wl.landscape_cam(path, it.t))
                 orc.orc_landscape_frame(oracles.P(buf), it.W, it.H, oracles.P(maps.height),
-                                        oracles.P(maps.color), oracles.P(maps.fog), C.byref(oc), nthr)
+                                        oracles.P(maps.color), oracles.P(maps.fog), C.byref(oc),
+                                        nthreads)
             else:
                 g = wl.tunnel_grid(it.W, it.H, it.t)
                 b0, b1 = np.zeros(n, np.uint32), np.zeros(n, np.uint32)
                 orc.orc_tunnel_frame(oracles.P(buf), oracles.P(b0), oracles.P(b1), it.W, it.H,
-                                     oracles.P(g), oracles.P(tex), 9, it.radius, nthr)
-            px += n
-        dt = time.perf_counter() - t_c
+                                     oracles.P(g), oracles.P(tex), 9, it.radius, nthreads)
+            return n
+
+        def cpu_rate(nthreads, budget_s):
+            px, k, t_c = 0, 0, time.perf_counter()
+            while k < len(items) and time.perf_counter() - t_c < budget_s:
+                px += cpu_render(items[k], nthreads)
+                k += 1
+            return px, k, time.perf_counter() - t_c
+
+        px, k, dt = cpu_rate(nthr, 10.0)
+        px1, k1, dt1 = cpu_rate(1, 5.0)
         cpu = {"value": round(px / dt / 1e6, 2), "unit": "Mpixel/s", "cores": nthr, "kind": "port",
-               "sample": f"{len(sample)} requests of the batch (2 SDF FP32, 3 voxel, 3 tunnel), "
-                         f"{px / 1e6:.1f} Mpx, oracle C + OpenMP, {dt:.1f}s"}
+               "sample": f"first {k} of the {len(items)} batch requests ({px / 1e6:.1f} Mpx) "
+                         f"rendered by the oracle (C, OpenMP) in {dt:.1f}s",
+               "value_1_thread": round(px1 / dt1 / 1e6, 2),
+               "sample_1_thread": f"first {k1} requests in {dt1:.1f}s"}
         log("cpu", cpu)
 
     result = {
@@ -270,8 +314,9 @@ def main() -> int:
         "data": "synthetic (seeded)",
         "config": {"workload": f"batched form: {len(items)} requests, 40/30/30 SDF/voxel/tunnel, "
                                f"5 resolutions 320x180..2560x1440, t U[0,60], "
-                               f"{total_px / 1e6:.1f} Mpx per step",
-                   "march_steps_per_step": steps_total // max(1, args.steps)},
+                               f"{total_px / 1e6:.1f} Mpx per step, one hipGraph replay per step",
+                   "march_steps_per_step": steps_total // max(1, args.steps),
+                   "tuning": tuning},
         "per_effect": per,
         "precision_c3": None if es is None else
         {"shading": "fp16", "vs": "fp64 CPU render", "rows_sampled": "every 36th, frames 0 and 63", **es},
@@ -279,6 +324,7 @@ def main() -> int:
         "cpu_baseline": cpu,
     }
     print(json.dumps(result))
+    graph.close()
     ctx.close()
     return 0
 

@@ -9,9 +9,9 @@ exports the reference's own entry points.
 """
 from ._lib import (CKD_EDEVICE, CKD_EINVAL, CKD_ENOMEM, CKD_ESTATE, CKD_FX_SDF, CKD_FX_TUNNEL,
                    CKD_FX_VOXEL, CKD_OK, CKD_SHADE_FP16, CKD_SHADE_FP32, CkdError, LandscapeCam,
-                   Request, SdfScene, load)
+                   Request, SdfScene, Tuning, load)
 
-__all__ = ["Context", "CkdError", "LandscapeCam", "Request", "SdfScene", "load",
+__all__ = ["Context", "CkdError", "LandscapeCam", "Request", "SdfScene", "Tuning", "load",
            "CKD_OK", "CKD_EINVAL", "CKD_ENOMEM", "CKD_EDEVICE", "CKD_ESTATE",
            "CKD_SHADE_FP32", "CKD_SHADE_FP16", "CKD_FX_SDF", "CKD_FX_VOXEL", "CKD_FX_TUNNEL"]
 

@@ -15,6 +15,8 @@ LIB_PATH = Path(__file__).resolve().parent / "libckd.so"
 CKD_OK, CKD_EINVAL, CKD_ENOMEM, CKD_EDEVICE, CKD_ESTATE = range(5)
 CKD_SHADE_FP32, CKD_SHADE_FP16 = 0, 1
 CKD_FX_SDF, CKD_FX_VOXEL, CKD_FX_TUNNEL = 0, 1, 2
+(CKD_KT_SDF_PLAIN, CKD_KT_SDF_PERSISTENT, CKD_KT_LANDSCAPE, CKD_KT_TUNNEL_H,
+ CKD_KT_TUNNEL_V) = range(5)
 
 
 class CkdError(RuntimeError):
@@ -37,6 +39,14 @@ class SdfScene(C.Structure):
                 ("eps", C.c_float), ("tMax", C.c_float), ("maxSteps", C.c_int),
                 ("camRadius", C.c_float), ("camHeight", C.c_float), ("camSpeed", C.c_float),
                 ("focal", C.c_float)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("sdfStepsPerRefill", C.c_int), ("sdfChunk", C.c_int),
+                ("sdfBlocksPerCU", C.c_int), ("tunnelVSeg", C.c_int)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class Request(C.Structure):
@@ -64,6 +74,8 @@ SIGNATURES = {
     "ckd_memcpy_d2d": [_p, _p, _p, C.c_size_t],
     "ckd_event_record": [_p, _i],
     "ckd_event_elapsed_ms": [_p, _i, _i, C.POINTER(_f)],
+    "ckd_kernel_timing": [_p, _i],
+    "ckd_kernel_timing_read": [_p, _i, C.POINTER(C.c_double), C.POINTER(C.c_ulonglong)],
     "ckd_memset32": [_p, _p, C.c_uint32, C.c_size_t],
     "ckd_fx_blit_2x2": [_p, _p, _p, _u, _u],
     "ckd_add32": [_p, _p, _p, _u],
@@ -86,8 +98,16 @@ SIGNATURES = {
     "ckd_tunnel_frame": [_p, _p, _p, _u, _u, _p, _p, _u, _i],
     "ckd_render_batch": [_p, C.POINTER(Request), _u, C.POINTER(SdfScene), _i, _p, _u, _p,
                          C.c_size_t, _p],
+    "ckd_get_tuning": [_p, C.POINTER(Tuning)],
+    "ckd_set_tuning": [_p, C.POINTER(Tuning)],
+    "ckd_autotune_batch": [_p, C.POINTER(Request), _u, C.POINTER(SdfScene), _i, _p, _u, _p,
+                           C.c_size_t, C.POINTER(Tuning)],
+    "ckd_batch_graph_create": [_p, C.POINTER(Request), _u, C.POINTER(SdfScene), _i, _p, _u, _p,
+                               C.c_size_t, _p, C.POINTER(_p)],
+    "ckd_graph_launch": [_p],
+    "ckd_graph_destroy": [_p],
 }
-_RESTYPES = {"ckd_destroy": None, "ckd_last_error": C.c_char_p}
+_RESTYPES = {"ckd_destroy": None, "ckd_last_error": C.c_char_p, "ckd_graph_destroy": None}
 
 _lib = None
 

@@ -7,6 +7,8 @@
 #include <cstdarg>
 #include <cstddef>
 #include <cstdint>
+#include <utility>
+#include <vector>
 
 #include "../../include/ckd.h"
 
@@ -17,6 +19,10 @@ struct ckd_ctx {
     int cuCount = 0;
     hipStream_t stream = nullptr;
     hipEvent_t events[CKD_NUM_EVENTS] = {};
+    // second stream of the batched renderer (voxel and tunnel work runs
+    // beside the persistent SDF kernel), joined through two events
+    hipStream_t aux = nullptr;
+    hipEvent_t forkEv = nullptr, joinEv = nullptr;
 
     // landscape maps (Landscape_Create)
     uint8_t *lsHeight = nullptr;
@@ -42,6 +48,20 @@ struct ckd_ctx {
     // per-call parameter staging (device), grown on demand
     void *staging = nullptr;
     size_t stagingCap = 0;
+
+    // launch-shape tuning (ckd_autotune_batch / ckd_set_tuning)
+    ckd_tuning tuning = {2, 32, 8, 32};
+
+    // per-kernel HIP-event timing (ckd_kernel_timing)
+    bool timing = false;
+    struct KtPending {
+        hipEvent_t begin, end;
+        bool pooled; // false: the events belong to a graph
+    };
+    std::vector<KtPending> ktPending[CKD_KT_COUNT];
+    double ktMs[CKD_KT_COUNT] = {};
+    unsigned long long ktLaunches[CKD_KT_COUNT] = {};
+    std::vector<hipEvent_t> eventPool;
 };
 
 namespace ckd {
@@ -56,6 +76,36 @@ ckd_status check_launch(const char *what);
 // Device buffer of at least `bytes` owned by the context for per-call
 // parameters; enqueues the host->device copy on the context stream.
 ckd_status stage(ckd_ctx *ctx, const void *host, size_t bytes, void **dev);
+
+// Waits for every pending kernel-timing event pair and adds it up.
+ckd_status drain_timing(ckd_ctx *ctx);
+
+// Redirects ctx->stream for the lifetime of the guard.
+class StreamSwap {
+public:
+    StreamSwap(ckd_ctx *ctx, hipStream_t s) : ctx_(ctx), saved_(ctx->stream) { ctx->stream = s; }
+    ~StreamSwap() { ctx_->stream = saved_; }
+    StreamSwap(const StreamSwap &) = delete;
+    StreamSwap &operator=(const StreamSwap &) = delete;
+
+private:
+    ckd_ctx *ctx_;
+    hipStream_t saved_;
+};
+
+// Brackets one kernel launch with events when kernel timing is on.
+class KernelTimer {
+public:
+    KernelTimer(ckd_ctx *ctx, int kernel);
+    ~KernelTimer();
+    KernelTimer(const KernelTimer &) = delete;
+    KernelTimer &operator=(const KernelTimer &) = delete;
+
+private:
+    ckd_ctx *ctx_;
+    int kernel_;
+    hipEvent_t begin_ = nullptr;
+};
 
 } // namespace ckd
 

@@ -1,8 +1,14 @@
-// k_batch.hip -- ckd_render_batch: many frame requests of mixed effects and
-// resolutions.  The host sorts the requests into three job tables (SDF,
-// voxel, tunnel), stages them in one device buffer together with the SDF
-// chunk-queue counter, and launches one persistent SDF kernel, one voxel
-// kernel over every voxel column and the two fused tunnel kernels.
+// k_batch.hip -- the batched renderer: many frame requests of mixed effects
+// and resolutions.
+//
+// The host sorts the requests into three job tables (SDF, voxel, tunnel)
+// and stages them in one device buffer together with the SDF chunk-queue
+// counter.  One persistent SDF kernel runs on the context stream; the voxel
+// kernel (every voxel column of every request) and the two fused tunnel
+// kernels run beside it on the auxiliary stream.  The same launch sequence
+// is replayed eagerly (ckd_render_batch), timed over candidate launch
+// shapes (ckd_autotune_batch) or captured once as a hipGraph
+// (ckd_batch_graph_create / ckd_graph_launch).
 #include "ckd_internal.h"
 #include "ckd_jobs.h"
 
@@ -17,73 +23,86 @@ void voxel_job_geometry(VoxJob &J, unsigned W, unsigned H, const ckd_landscape_c
 ckd_status launch_landscape_jobs(ckd_ctx *ctx, const VoxJob *jobs, int nJobs, uint32_t totalBlocks);
 void tunnel_job_geometry(TunJob &J, uint32_t *dst, uint32_t *tmp, const int32_t *grid, unsigned W,
                          unsigned H, int radius);
-uint32_t tunnel_vblocks(unsigned W, unsigned H);
+uint32_t tunnel_vblocks(unsigned W, unsigned H, int seg);
 ckd_status launch_tunnel_jobs(ckd_ctx *ctx, const TunJob *jobs, int nJobs, const TunJob &single,
                               uint32_t totalRows, uint32_t totalVBlocks, unsigned maxW,
                               const uint32_t *tex, unsigned texLog2);
 } // namespace ckd
 
-namespace {
+struct ckd_graph {
+    ckd_ctx *ctx = nullptr;
+    void *tables = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    // event-record nodes around the persistent SDF kernel, read by the
+    // context's kernel timing on every replay
+    hipEvent_t sdfBegin = nullptr, sdfEnd = nullptr;
+    bool timedBefore = false;
+};
 
-constexpr int kSdfChunk = 32;
+namespace {
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-} // namespace
+// host image of the staged tables: [queue counter][SDF][voxel][tunnel]
+struct Plan {
+    std::vector<unsigned char> host;
+    size_t oSdf = 256, oVox = 0, oTun = 0;
+    int nSdf = 0, nVox = 0, nTun = 0;
+    uint32_t chunks = 0, voxBlocks = 0, tunRows = 0, tunV = 0;
+    unsigned tunMaxW = 0;
+};
 
-extern "C" {
+__global__ void k_reset_counter(uint32_t *c) { *c = 0u; }
 
-ckd_status ckd_render_batch(ckd_ctx *ctx, const ckd_request *requests, unsigned n,
-                            const ckd_sdf_scene *scene, int shading, const uint32_t *tex,
-                            unsigned texLog2, uint32_t *tmp, size_t tmpPixels,
-                            unsigned long long *stepCount)
+ckd_status validate(const char *fn, const ckd_request *requests, unsigned n,
+                    const ckd_sdf_scene *scene, int shading, const uint32_t *tex, unsigned texLog2,
+                    const uint32_t *tmp, size_t tmpPixels)
 {
-    CKD_REQUIRE(n == 0 || requests, "ckd_render_batch: null request array");
+    CKD_REQUIRE(n == 0 || requests, "%s: null request array", fn);
     size_t need = 0;
     for (unsigned i = 0; i < n; ++i) {
         const ckd_request &r = requests[i];
-        CKD_REQUIRE(r.dst != nullptr, "ckd_render_batch: request %u: null dst", i);
+        CKD_REQUIRE(r.dst != nullptr, "%s: request %u: null dst", fn, i);
         CKD_REQUIRE(r.W >= 8 && r.H >= 8 && r.W <= 4096 && r.H <= 2160,
-                    "ckd_render_batch: request %u: bad size %ux%u", i, r.W, r.H);
+                    "%s: request %u: bad size %ux%u", fn, i, r.W, r.H);
         switch (r.effect) {
         case CKD_FX_SDF:
-            CKD_REQUIRE(scene != nullptr, "ckd_render_batch: request %u: SDF needs a scene", i);
+            CKD_REQUIRE(scene != nullptr, "%s: request %u: SDF needs a scene", fn, i);
             break;
         case CKD_FX_VOXEL:
             CKD_REQUIRE(r.vcam.mapTilt >= 0 && r.vcam.mapTilt <= 180,
-                        "ckd_render_batch: request %u: mapTilt %d outside 0..180", i, r.vcam.mapTilt);
+                        "%s: request %u: mapTilt %d outside 0..180", fn, i, r.vcam.mapTilt);
             break;
         case CKD_FX_TUNNEL:
-            CKD_REQUIRE(r.grid && tex, "ckd_render_batch: request %u: TUNNEL needs grid and texture", i);
-            CKD_REQUIRE(texLog2 >= 1 && texLog2 <= 14, "ckd_render_batch: texLog2 %u out of 1..14",
-                        texLog2);
+            CKD_REQUIRE(r.grid && tex, "%s: request %u: TUNNEL needs grid and texture", fn, i);
+            CKD_REQUIRE(texLog2 >= 1 && texLog2 <= 14, "%s: texLog2 %u out of 1..14", fn, texLog2);
             CKD_REQUIRE(r.radius >= 1 && r.radius <= 127 &&
                             (unsigned)(2 * r.radius + 1) <= (r.W < r.H ? r.W : r.H),
-                        "ckd_render_batch: request %u: radius %d out of range", i, r.radius);
+                        "%s: request %u: radius %d out of range", fn, i, r.radius);
             need += (size_t)r.W * r.H;
             break;
         default:
-            return ckd::fail(CKD_EINVAL, "ckd_render_batch: request %u: unknown effect %d", i, r.effect);
+            return ckd::fail(CKD_EINVAL, "%s: request %u: unknown effect %d", fn, i, r.effect);
         }
     }
     CKD_REQUIRE(need <= tmpPixels && (need == 0 || tmp),
-                "ckd_render_batch: TUNNEL requests need %zu scratch pixels, got %zu", need, tmpPixels);
+                "%s: TUNNEL requests need %zu scratch pixels, got %zu", fn, need, tmpPixels);
     CKD_REQUIRE(shading == CKD_SHADE_FP32 || shading == CKD_SHADE_FP16,
-                "ckd_render_batch: unknown shading mode %d", shading);
-    if (scene) {
+                "%s: unknown shading mode %d", fn, shading);
+    if (scene)
         CKD_REQUIRE(scene->nSpheres >= 0 && scene->nSpheres <= 32 && scene->maxSteps >= 1 &&
                         scene->maxSteps <= 1024,
-                    "ckd_render_batch: bad scene (nSpheres %d, maxSteps %d)", scene->nSpheres,
-                    scene->maxSteps);
-    }
-    CKD_REQUIRE_CTX(ctx);
-    if (n == 0) return CKD_OK;
+                    "%s: bad scene (nSpheres %d, maxSteps %d)", fn, scene->nSpheres, scene->maxSteps);
+    return CKD_OK;
+}
 
+void make_plan(const ckd_ctx *ctx, const ckd_request *requests, unsigned n, uint32_t *tmp, Plan &P)
+{
     std::vector<SdfJob> sdf;
     std::vector<VoxJob> vox;
     std::vector<TunJob> tun;
-    uint32_t chunks = 0, voxBlocks = 0, tunRows = 0, tunV = 0;
-    unsigned tunMaxW = 0;
+    const uint32_t chunk = (uint32_t)ctx->tuning.sdfChunk;
     size_t off = 0;
     for (unsigned i = 0; i < n; ++i) {
         const ckd_request &r = requests[i];
@@ -92,63 +111,291 @@ ckd_status ckd_render_batch(ckd_ctx *ctx, const ckd_request *requests, unsigned 
             J.dst = r.dst;
             J.W = r.W;
             J.H = r.H;
-            J.chunksX = (r.W + kSdfChunk - 1) / kSdfChunk;
-            J.chunkBase = chunks;
+            J.chunksX = (r.W + chunk - 1) / chunk;
+            J.chunkBase = P.chunks;
             memcpy(J.cam, r.cam, sizeof(J.cam));
-            chunks += J.chunksX * ((r.H + kSdfChunk - 1) / kSdfChunk);
+            P.chunks += J.chunksX * ((r.H + chunk - 1) / chunk);
             sdf.push_back(J);
         } else if (r.effect == CKD_FX_VOXEL) {
             VoxJob J;
             J.dst = r.dst;
             ckd::voxel_job_geometry(J, r.W, r.H, r.vcam);
-            J.blockBase = voxBlocks;
-            voxBlocks += (r.W + 63) / 64;
+            J.blockBase = P.voxBlocks;
+            P.voxBlocks += (r.W + 63) / 64;
             vox.push_back(J);
         } else {
             TunJob J;
             ckd::tunnel_job_geometry(J, r.dst, tmp + off, r.grid, r.W, r.H, r.radius);
             off += (size_t)r.W * r.H;
-            J.rowBase = tunRows;
-            J.vBase = tunV;
-            tunRows += r.H;
-            tunV += ckd::tunnel_vblocks(r.W, r.H);
-            if (r.W > tunMaxW) tunMaxW = r.W;
+            J.rowBase = P.tunRows;
+            J.vBase = P.tunV;
+            P.tunRows += r.H;
+            P.tunV += ckd::tunnel_vblocks(r.W, r.H, ctx->tuning.tunnelVSeg);
+            if (r.W > P.tunMaxW) P.tunMaxW = r.W;
             tun.push_back(J);
         }
     }
-    // one staging buffer: [queue counter][SDF jobs][voxel jobs][tunnel jobs]
-    const size_t oSdf = 256;
-    const size_t oVox = oSdf + align256(sdf.size() * sizeof(SdfJob));
-    const size_t oTun = oVox + align256(vox.size() * sizeof(VoxJob));
-    const size_t bytes = oTun + align256(tun.size() * sizeof(TunJob));
-    std::vector<unsigned char> host(bytes, 0);
-    if (!sdf.empty()) memcpy(host.data() + oSdf, sdf.data(), sdf.size() * sizeof(SdfJob));
-    if (!vox.empty()) memcpy(host.data() + oVox, vox.data(), vox.size() * sizeof(VoxJob));
-    if (!tun.empty()) memcpy(host.data() + oTun, tun.data(), tun.size() * sizeof(TunJob));
-    void *dev = nullptr;
-    ckd_status s = ckd::stage(ctx, host.data(), bytes, &dev);
-    if (s != CKD_OK) return s;
-    unsigned char *d = static_cast<unsigned char *>(dev);
+    P.nSdf = (int)sdf.size();
+    P.nVox = (int)vox.size();
+    P.nTun = (int)tun.size();
+    P.oVox = P.oSdf + align256(sdf.size() * sizeof(SdfJob));
+    P.oTun = P.oVox + align256(vox.size() * sizeof(VoxJob));
+    P.host.assign(P.oTun + align256(tun.size() * sizeof(TunJob)), 0);
+    if (!sdf.empty()) memcpy(P.host.data() + P.oSdf, sdf.data(), sdf.size() * sizeof(SdfJob));
+    if (!vox.empty()) memcpy(P.host.data() + P.oVox, vox.data(), vox.size() * sizeof(VoxJob));
+    if (!tun.empty()) memcpy(P.host.data() + P.oTun, tun.data(), tun.size() * sizeof(TunJob));
+}
 
-    if (!vox.empty()) {
-        s = ckd::launch_landscape_jobs(ctx, reinterpret_cast<const VoxJob *>(d + oVox), (int)vox.size(),
-                                       voxBlocks);
-        if (s != CKD_OK) return s;
+// the launch sequence over tables staged at `dev`
+ckd_status launch_plan(ckd_ctx *ctx, const Plan &P, unsigned char *dev, const ckd_sdf_scene *scene,
+                       int shading, const uint32_t *tex, unsigned texLog2,
+                       unsigned long long *stepCount, hipEvent_t sdfBegin = nullptr,
+                       hipEvent_t sdfEnd = nullptr)
+{
+    ckd_status s = CKD_OK;
+    const bool fork = P.nSdf > 0 && (P.nVox > 0 || P.nTun > 0);
+    if (fork) {
+        CKD_HIP(hipEventRecord(ctx->forkEv, ctx->stream));
+        CKD_HIP(hipStreamWaitEvent(ctx->aux, ctx->forkEv, 0));
     }
-    if (!tun.empty()) {
-        TunJob none;
-        memset(&none, 0, sizeof(none));
-        s = ckd::launch_tunnel_jobs(ctx, reinterpret_cast<const TunJob *>(d + oTun), (int)tun.size(),
-                                    none, tunRows, tunV, tunMaxW, tex, texLog2);
-        if (s != CKD_OK) return s;
-    }
-    if (!sdf.empty()) {
-        s = ckd::launch_sdf_persistent(ctx, reinterpret_cast<uint32_t *>(d), chunks,
-                                       reinterpret_cast<const SdfJob *>(d + oSdf), (int)sdf.size(),
+    if (P.nSdf > 0) {
+        if (sdfBegin) CKD_HIP(hipEventRecord(sdfBegin, ctx->stream));
+        s = ckd::launch_sdf_persistent(ctx, reinterpret_cast<uint32_t *>(dev), P.chunks,
+                                       reinterpret_cast<const SdfJob *>(dev + P.oSdf), P.nSdf,
                                        *scene, shading, stepCount);
         if (s != CKD_OK) return s;
+        if (sdfEnd) CKD_HIP(hipEventRecord(sdfEnd, ctx->stream));
+    }
+    {
+        ckd::StreamSwap swap(ctx, fork ? ctx->aux : ctx->stream);
+        if (P.nVox > 0) {
+            s = ckd::launch_landscape_jobs(ctx, reinterpret_cast<const VoxJob *>(dev + P.oVox),
+                                           P.nVox, P.voxBlocks);
+            if (s != CKD_OK) return s;
+        }
+        if (P.nTun > 0) {
+            TunJob none;
+            memset(&none, 0, sizeof(none));
+            s = ckd::launch_tunnel_jobs(ctx, reinterpret_cast<const TunJob *>(dev + P.oTun), P.nTun,
+                                        none, P.tunRows, P.tunV, P.tunMaxW, tex, texLog2);
+            if (s != CKD_OK) return s;
+        }
+    }
+    if (fork) {
+        CKD_HIP(hipEventRecord(ctx->joinEv, ctx->aux));
+        CKD_HIP(hipStreamWaitEvent(ctx->stream, ctx->joinEv, 0));
     }
     return CKD_OK;
+}
+
+ckd_status run_eager(ckd_ctx *ctx, const ckd_request *requests, unsigned n,
+                     const ckd_sdf_scene *scene, int shading, const uint32_t *tex,
+                     unsigned texLog2, uint32_t *tmp, unsigned long long *stepCount)
+{
+    Plan P;
+    make_plan(ctx, requests, n, tmp, P);
+    void *dev = nullptr;
+    ckd_status s = ckd::stage(ctx, P.host.data(), P.host.size(), &dev);
+    if (s != CKD_OK) return s;
+    return launch_plan(ctx, P, static_cast<unsigned char *>(dev), scene, shading, tex, texLog2,
+                       stepCount);
+}
+
+ckd_status valid_tuning(const ckd_tuning *t)
+{
+    CKD_REQUIRE(t != nullptr, "tuning: null pointer");
+    CKD_REQUIRE(t->sdfStepsPerRefill == 1 || t->sdfStepsPerRefill == 2 || t->sdfStepsPerRefill == 4,
+                "tuning: sdfStepsPerRefill %d not in {1,2,4}", t->sdfStepsPerRefill);
+    CKD_REQUIRE(t->sdfChunk == 16 || t->sdfChunk == 32, "tuning: sdfChunk %d not in {16,32}",
+                t->sdfChunk);
+    CKD_REQUIRE(t->sdfBlocksPerCU >= 1 && t->sdfBlocksPerCU <= 32,
+                "tuning: sdfBlocksPerCU %d outside 1..32", t->sdfBlocksPerCU);
+    CKD_REQUIRE(t->tunnelVSeg >= 8 && t->tunnelVSeg <= 128, "tuning: tunnelVSeg %d outside 8..128",
+                t->tunnelVSeg);
+    return CKD_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+ckd_status ckd_get_tuning(ckd_ctx *ctx, ckd_tuning *out)
+{
+    CKD_REQUIRE(out != nullptr, "ckd_get_tuning: null output");
+    CKD_REQUIRE_CTX(ctx);
+    *out = ctx->tuning;
+    return CKD_OK;
+}
+
+ckd_status ckd_set_tuning(ckd_ctx *ctx, const ckd_tuning *t)
+{
+    ckd_status s = valid_tuning(t);
+    if (s != CKD_OK) return s;
+    CKD_REQUIRE_CTX(ctx);
+    ctx->tuning = *t;
+    return CKD_OK;
+}
+
+ckd_status ckd_render_batch(ckd_ctx *ctx, const ckd_request *requests, unsigned n,
+                            const ckd_sdf_scene *scene, int shading, const uint32_t *tex,
+                            unsigned texLog2, uint32_t *tmp, size_t tmpPixels,
+                            unsigned long long *stepCount)
+{
+    ckd_status s = validate("ckd_render_batch", requests, n, scene, shading, tex, texLog2, tmp,
+                            tmpPixels);
+    if (s != CKD_OK) return s;
+    CKD_REQUIRE_CTX(ctx);
+    if (n == 0) return CKD_OK;
+    return run_eager(ctx, requests, n, scene, shading, tex, texLog2, tmp, stepCount);
+}
+
+ckd_status ckd_autotune_batch(ckd_ctx *ctx, const ckd_request *requests, unsigned n,
+                              const ckd_sdf_scene *scene, int shading, const uint32_t *tex,
+                              unsigned texLog2, uint32_t *tmp, size_t tmpPixels, ckd_tuning *best)
+{
+    ckd_status s = validate("ckd_autotune_batch", requests, n, scene, shading, tex, texLog2, tmp,
+                            tmpPixels);
+    if (s != CKD_OK) return s;
+    CKD_REQUIRE_CTX(ctx);
+    if (n == 0) return CKD_OK;
+    const int steps[] = {1, 2, 4}, chunks[] = {16, 32}, bpc[] = {4, 8, 16}, segs[] = {16, 32, 64};
+    const ckd_tuning saved = ctx->tuning;
+    const bool timing = ctx->timing;
+    ctx->timing = false;
+    ckd_tuning bestT = saved;
+    float bestMs = 3.4e38f;
+    hipEvent_t a = ctx->events[CKD_NUM_EVENTS - 2], b = ctx->events[CKD_NUM_EVENTS - 1];
+    for (int st : steps)
+        for (int ch : chunks)
+            for (int bp : bpc)
+                for (int sg : segs) {
+                    ctx->tuning = ckd_tuning{st, ch, bp, sg};
+                    // warm once, then time the second run
+                    s = run_eager(ctx, requests, n, scene, shading, tex, texLog2, tmp, nullptr);
+                    if (s == CKD_OK) {
+                        if (hipEventRecord(a, ctx->stream) != hipSuccess) s = CKD_EDEVICE;
+                    }
+                    if (s == CKD_OK)
+                        s = run_eager(ctx, requests, n, scene, shading, tex, texLog2, tmp, nullptr);
+                    float ms = 0.f;
+                    if (s == CKD_OK && (hipEventRecord(b, ctx->stream) != hipSuccess ||
+                                        hipEventSynchronize(b) != hipSuccess ||
+                                        hipEventElapsedTime(&ms, a, b) != hipSuccess))
+                        s = ckd::fail(CKD_EDEVICE, "ckd_autotune_batch: event timing failed");
+                    if (s != CKD_OK) {
+                        ctx->tuning = saved;
+                        ctx->timing = timing;
+                        return s;
+                    }
+                    if (ms < bestMs) {
+                        bestMs = ms;
+                        bestT = ctx->tuning;
+                    }
+                }
+    ctx->tuning = bestT;
+    ctx->timing = timing;
+    if (best) *best = bestT;
+    return CKD_OK;
+}
+
+ckd_status ckd_batch_graph_create(ckd_ctx *ctx, const ckd_request *requests, unsigned n,
+                                  const ckd_sdf_scene *scene, int shading, const uint32_t *tex,
+                                  unsigned texLog2, uint32_t *tmp, size_t tmpPixels,
+                                  unsigned long long *stepCount, ckd_graph **out)
+{
+    CKD_REQUIRE(out != nullptr, "ckd_batch_graph_create: null output");
+    *out = nullptr;
+    ckd_status s = validate("ckd_batch_graph_create", requests, n, scene, shading, tex, texLog2,
+                            tmp, tmpPixels);
+    if (s != CKD_OK) return s;
+    CKD_REQUIRE(n > 0, "ckd_batch_graph_create: empty batch");
+    CKD_REQUIRE_CTX(ctx);
+    Plan P;
+    make_plan(ctx, requests, n, tmp, P);
+    ckd_graph *g = new ckd_graph();
+    g->ctx = ctx;
+    if (hipMalloc(&g->tables, P.host.size()) != hipSuccess) {
+        delete g;
+        return ckd::fail(CKD_ENOMEM, "ckd_batch_graph_create: table allocation failed");
+    }
+    if (hipMemcpy(g->tables, P.host.data(), P.host.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        ckd_graph_destroy(g);
+        return ckd::fail(CKD_EDEVICE, "ckd_batch_graph_create: table upload failed");
+    }
+    // capture on a private stream so nothing else on the context stream is
+    // recorded; the kernel-timing events are not part of the graph
+    hipStream_t cap = nullptr;
+    if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess) {
+        ckd_graph_destroy(g);
+        return ckd::fail(CKD_EDEVICE, "ckd_batch_graph_create: stream creation failed");
+    }
+    if (P.nSdf > 0 && (hipEventCreate(&g->sdfBegin) != hipSuccess ||
+                       hipEventCreate(&g->sdfEnd) != hipSuccess)) {
+        (void)hipStreamDestroy(cap);
+        ckd_graph_destroy(g);
+        return ckd::fail(CKD_EDEVICE, "ckd_batch_graph_create: event creation failed");
+    }
+    const bool timing = ctx->timing;
+    ctx->timing = false;
+    hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        ckd::StreamSwap swap(ctx, cap);
+        unsigned char *dev = static_cast<unsigned char *>(g->tables);
+        // the queue counter is reset by a kernel node on every replay
+        k_reset_counter<<<1, 1, 0, cap>>>(reinterpret_cast<uint32_t *>(dev));
+        s = launch_plan(ctx, P, dev, scene, shading, tex, texLog2, stepCount, g->sdfBegin,
+                        g->sdfEnd);
+        const hipError_t e2 = hipStreamEndCapture(cap, &g->graph);
+        if (s == CKD_OK && e2 != hipSuccess)
+            s = ckd::fail(CKD_EDEVICE, "ckd_batch_graph_create: capture failed: %s",
+                          hipGetErrorString(e2));
+    } else {
+        s = ckd::fail(CKD_EDEVICE, "ckd_batch_graph_create: hipStreamBeginCapture: %s",
+                      hipGetErrorString(e));
+    }
+    ctx->timing = timing;
+    (void)hipStreamDestroy(cap);
+    if (s == CKD_OK && hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
+        s = ckd::fail(CKD_EDEVICE, "ckd_batch_graph_create: hipGraphInstantiate failed");
+    if (s != CKD_OK) {
+        ckd_graph_destroy(g);
+        return s;
+    }
+    *out = g;
+    return CKD_OK;
+}
+
+ckd_status ckd_graph_launch(ckd_graph *g)
+{
+    CKD_REQUIRE(g && g->exec, "ckd_graph_launch: null graph");
+    ckd_ctx *ctx = g->ctx;
+    const bool timed = ctx->timing && g->sdfBegin;
+    // the graph's timing events are re-recorded by every replay: the
+    // previous replay's pair is read before they are reused
+    if (g->timedBefore) {
+        ckd_status s = ckd::drain_timing(ctx);
+        if (s != CKD_OK) return s;
+        g->timedBefore = false;
+    }
+    CKD_HIP(hipGraphLaunch(g->exec, ctx->stream));
+    if (timed) {
+        ctx->ktPending[CKD_KT_SDF_PERSISTENT].push_back({g->sdfBegin, g->sdfEnd, false});
+        g->timedBefore = true;
+    }
+    return CKD_OK;
+}
+
+void ckd_graph_destroy(ckd_graph *g)
+{
+    if (!g) return;
+    if (g->timedBefore) (void)ckd::drain_timing(g->ctx);
+    if (g->ctx) (void)hipStreamSynchronize(g->ctx->stream);
+    if (g->sdfBegin) (void)hipEventDestroy(g->sdfBegin);
+    if (g->sdfEnd) (void)hipEventDestroy(g->sdfEnd);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->tables) (void)hipFree(g->tables);
+    delete g;
 }
 
 } // extern "C"

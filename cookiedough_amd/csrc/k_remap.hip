@@ -88,7 +88,7 @@ __global__ __launch_bounds__(256) void k_grid_remap(uint32_t *__restrict__ dst, 
 namespace ckd {
 void tunnel_job_geometry(TunJob &J, uint32_t *dst, uint32_t *tmp, const int32_t *grid, unsigned W,
                          unsigned H, int radius);
-uint32_t tunnel_vblocks(unsigned W, unsigned H);
+uint32_t tunnel_vblocks(unsigned W, unsigned H, int seg);
 ckd_status launch_tunnel_jobs(ckd_ctx *ctx, const TunJob *jobs, int nJobs, const TunJob &single,
                               uint32_t totalRows, uint32_t totalVBlocks, unsigned maxW,
                               const uint32_t *tex, unsigned texLog2);
@@ -174,7 +174,7 @@ ckd_status ckd_tunnel_frame(ckd_ctx *ctx, uint32_t *dst, uint32_t *tmp, unsigned
     ckd::tunnel_job_geometry(J, dst, tmp, grid, W, H, radius);
     J.rowBase = 0;
     J.vBase = 0;
-    return ckd::launch_tunnel_jobs(ctx, nullptr, 1, J, H, ckd::tunnel_vblocks(W, H), W, tex, texLog2);
+    return ckd::launch_tunnel_jobs(ctx, nullptr, 1, J, H, ckd::tunnel_vblocks(W, H, ctx->tuning.tunnelVSeg), W, tex, texLog2);
 }
 
 } // extern "C"

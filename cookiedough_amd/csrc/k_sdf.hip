@@ -22,6 +22,7 @@
 namespace {
 
 typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+typedef float f4 __attribute__((ext_vector_type(4)));
 
 // v_sqrt_f32 (1 ulp) for the distance field; the default sqrtf is the
 // correctly rounded multi-instruction sequence
@@ -214,12 +215,26 @@ struct ShadeEntry {
     float tHit, total;
 };
 
-constexpr int kChunkSide = 32;
-constexpr int kStepsPerRefill = 4;
 constexpr int kQueueCap = 128;
 constexpr int kWavesPerBlock = 4;
 
 CKD_DEV uint32_t lane_id() { return __lane_id(); }
+
+// a zero the compiler cannot see through: indexing LDS tables with it keeps
+// their loads inside the loop instead of hoisting every table entry into
+// registers for the whole kernel
+CKD_DEV uint32_t opaque_zero()
+{
+    uint32_t z;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(z));
+    return z;
+}
+
+// a wave-uniform float kept in a scalar register
+CKD_DEV float uniform_f(float v)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
 
 // direction of pixel (x, y) through the camera basis
 CKD_DEV void ray_dir(const float *cam, uint32_t W, uint32_t H, uint32_t x, uint32_t y, float &dx,
@@ -251,7 +266,7 @@ CKD_DEV uint32_t shade_hit(const ckd_sdf_scene &sc, const float *cam, float dx, 
     float mxp = pl, mxm = pl, myp = pl + h, mym = pl - h, mzp = pl, mzm = pl;
     float best = pl;
     int id = sc.nSpheres;
-#pragma unroll
+#pragma unroll 4
     for (int i = 0; i < kMaxSph; ++i) {
         if (i < sc.nSpheres) {
             const float ex = hx - sc.sphere[i][0], ey = hy - sc.sphere[i][1], ez = hz - sc.sphere[i][2];
@@ -281,14 +296,20 @@ CKD_DEV uint32_t shade_hit(const ckd_sdf_scene &sc, const float *cam, float dx, 
     return shade_lit<kHalf>(sc, m, nx, ny, nz, id);
 }
 
-template <bool kHalf, int kMaxSph>
+template <bool kHalf, int kMaxSph, int kStepsPerRefill>
 __global__ __launch_bounds__(256) void k_sdf_persistent(uint32_t *__restrict__ queueCounter,
                                                         uint32_t totalChunks,
                                                         const SdfJob *__restrict__ jobs, int nJobs,
                                                         const ckd_sdf_scene sc,
-                                                        unsigned long long *__restrict__ stepCount)
+                                                        unsigned long long *__restrict__ stepCount,
+                                                        uint32_t kChunkSide)
 {
     __shared__ ShadeEntry queue[kWavesPerBlock][kQueueCap];
+    // per-chunk march constants (C_i, r_i) of each wave: read every step with
+    // broadcast LDS loads instead of occupying 2*kMaxSph scalar registers
+    __shared__ f4 sphCR[kWavesPerBlock][kMaxSph / 2]; // {C_2k, r_2k, C_2k+1, r_2k+1}
+    // o - c_i of the chunk's camera, read when a lane starts a ray
+    __shared__ f4 sphOC[kWavesPerBlock][kMaxSph];
     const uint32_t lane = lane_id();
     const uint32_t wave = threadIdx.x >> 6;
     ShadeEntry *q = queue[wave];
@@ -297,7 +318,6 @@ __global__ __launch_bounds__(256) void k_sdf_persistent(uint32_t *__restrict__ q
     // chunk state (uniform)
     int job = -1;
     uint32_t next = 0, end = 0, cx0 = 0, cy0 = 0, cw = 1;
-    float C[kMaxSph];
     float cam[12];
     uint32_t W = 0, H = 0;
 
@@ -320,11 +340,11 @@ __global__ __launch_bounds__(256) void k_sdf_persistent(uint32_t *__restrict__ q
                     const uint32_t x = cx0 + k % cw, y = cy0 + k / cw;
                     pix = y * W + x;
                     ray_dir(cam, W, H, x, y, dx, dy, dz);
+                    const uint32_t z = opaque_zero();
 #pragma unroll
                     for (int i = 0; i < kMaxSph; ++i) {
-                        const float ox = cam[0] - sc.sphere[i][0], oy = cam[1] - sc.sphere[i][1],
-                                    oz = cam[2] - sc.sphere[i][2];
-                        B[i] = 2.f * (dx * ox + dy * oy + dz * oz);
+                        const f4 oc = sphOC[wave][i + z];
+                        B[i] = 2.f * (dx * oc.x + dy * oc.y + dz * oc.z);
                     }
                     t = 0.f;
                     d = 1.f;
@@ -345,7 +365,8 @@ __global__ __launch_bounds__(256) void k_sdf_persistent(uint32_t *__restrict__ q
             const SdfJob &J = jobs[job];
             W = J.W;
             H = J.H;
-            for (int k = 0; k < 12; ++k) cam[k] = J.cam[k];
+#pragma unroll
+            for (int k = 0; k < 12; ++k) cam[k] = uniform_f(J.cam[k]);
             const uint32_t local = c - J.chunkBase;
             cx0 = (local % J.chunksX) * kChunkSide;
             cy0 = (local / J.chunksX) * kChunkSide;
@@ -353,12 +374,23 @@ __global__ __launch_bounds__(256) void k_sdf_persistent(uint32_t *__restrict__ q
             const uint32_t chh = min((uint32_t)kChunkSide, H - cy0);
             next = 0;
             end = cw * chh;
-#pragma unroll
-            for (int i = 0; i < kMaxSph; ++i) {
-                const float ox = cam[0] - sc.sphere[i][0], oy = cam[1] - sc.sphere[i][1],
-                            oz = cam[2] - sc.sphere[i][2];
-                C[i] = ox * ox + oy * oy + oz * oz;
+            // lane i stages sphere i; spheres past nSpheres become a neutral
+            // sphere whose distance is never the minimum
+            if (lane < (uint32_t)kMaxSph) {
+                float c = 1e30f, r = 0.f, ox = 0.f, oy = 0.f, oz = 0.f;
+                if ((int)lane < sc.nSpheres) {
+                    ox = cam[0] - sc.sphere[lane][0];
+                    oy = cam[1] - sc.sphere[lane][1];
+                    oz = cam[2] - sc.sphere[lane][2];
+                    c = ox * ox + oy * oy + oz * oz;
+                    r = sc.sphere[lane][3];
+                }
+                float *cr = reinterpret_cast<float *>(&sphCR[wave][lane >> 1]) + 2 * (lane & 1);
+                cr[0] = c;
+                cr[1] = r;
+                sphOC[wave][lane] = f4{ox, oy, oz, 0.f};
             }
+            __builtin_amdgcn_wave_barrier();
             continue;
         }
         // march a few steps
@@ -368,12 +400,14 @@ __global__ __launch_bounds__(256) void k_sdf_persistent(uint32_t *__restrict__ q
             if (active && d > sc.eps && steps < sc.maxSteps && t < sc.tMax) {
                 const float tt = t * t;
                 float dist = fmaf(t, dy, oyg);
+                const uint32_t z = opaque_zero();
 #pragma unroll
-                for (int i = 0; i < kMaxSph; ++i) {
-                    if (i < sc.nSpheres) {
-                        const float qq = fmaf(t, B[i], C[i] + tt);
-                        dist = fminf(dist, fsqrt(qq) - sc.sphere[i][3]);
-                    }
+                for (int i = 0; i < kMaxSph / 2; ++i) {
+                    const f4 cr = sphCR[wave][i + z];
+                    const float q0 = fmaf(t, B[2 * i], cr.x + tt);
+                    const float q1 = fmaf(t, B[2 * i + 1], cr.z + tt);
+                    dist = fminf(dist, fsqrt(q0) - cr.y);
+                    dist = fminf(dist, fsqrt(q1) - cr.w);
                 }
                 tHit = t;
                 d = dist;
@@ -450,11 +484,25 @@ ckd_status launch_sdf(ckd_ctx *ctx, uint32_t *dst, unsigned W, unsigned H,
         cam.u[i] = cam12[9 + i];
     }
     dim3 grid(ckd_div_up(W, 16), ckd_div_up(H, 16));
+    KernelTimer kt(ctx, CKD_KT_SDF_PLAIN);
     if (shading == CKD_SHADE_FP16)
         k_sdf<true><<<grid, 256, 0, ctx->stream>>>(dst, W, H, sc, cam, stepCount);
     else
         k_sdf<false><<<grid, 256, 0, ctx->stream>>>(dst, W, H, sc, cam, stepCount);
     return check_launch("sdf");
+}
+
+template <bool kHalf, int kMaxSph>
+void launch_persistent_steps(int steps, unsigned blocks, hipStream_t st, uint32_t *queueCounter,
+                             uint32_t totalChunks, const SdfJob *jobs, int nJobs,
+                             const ckd_sdf_scene &sc, unsigned long long *stepCount, uint32_t chunk)
+{
+    if (steps <= 1)
+        k_sdf_persistent<kHalf, kMaxSph, 1><<<blocks, 256, 0, st>>>(queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk);
+    else if (steps == 2)
+        k_sdf_persistent<kHalf, kMaxSph, 2><<<blocks, 256, 0, st>>>(queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk);
+    else
+        k_sdf_persistent<kHalf, kMaxSph, 4><<<blocks, 256, 0, st>>>(queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk);
 }
 
 ckd_status launch_sdf_persistent(ckd_ctx *ctx, uint32_t *queueCounter, uint32_t totalChunks,
@@ -464,20 +512,20 @@ ckd_status launch_sdf_persistent(ckd_ctx *ctx, uint32_t *queueCounter, uint32_t 
     if (totalChunks == 0) return CKD_OK;
     // persistent: enough 4-wave blocks to fill every CU several times over;
     // surplus blocks find the queue empty and exit
-    unsigned blocks = (unsigned)ctx->cuCount * 8;
+    unsigned blocks = (unsigned)ctx->cuCount * (unsigned)ctx->tuning.sdfBlocksPerCU;
     const unsigned useful = (totalChunks + kWavesPerBlock - 1) / kWavesPerBlock;
     if (blocks > useful) blocks = useful;
+    if (blocks == 0) blocks = 1;
     const bool big = sc.nSpheres > 8;
+    const int steps = ctx->tuning.sdfStepsPerRefill;
+    const uint32_t chunk = (uint32_t)ctx->tuning.sdfChunk;
+    KernelTimer kt(ctx, CKD_KT_SDF_PERSISTENT);
     if (shading == CKD_SHADE_FP16) {
-        if (big)
-            k_sdf_persistent<true, 32><<<blocks, 256, 0, ctx->stream>>>(queueCounter, totalChunks, jobs, nJobs, sc, stepCount);
-        else
-            k_sdf_persistent<true, 8><<<blocks, 256, 0, ctx->stream>>>(queueCounter, totalChunks, jobs, nJobs, sc, stepCount);
+        if (big) launch_persistent_steps<true, 32>(steps, blocks, ctx->stream, queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk);
+        else launch_persistent_steps<true, 8>(steps, blocks, ctx->stream, queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk);
     } else {
-        if (big)
-            k_sdf_persistent<false, 32><<<blocks, 256, 0, ctx->stream>>>(queueCounter, totalChunks, jobs, nJobs, sc, stepCount);
-        else
-            k_sdf_persistent<false, 8><<<blocks, 256, 0, ctx->stream>>>(queueCounter, totalChunks, jobs, nJobs, sc, stepCount);
+        if (big) launch_persistent_steps<false, 32>(steps, blocks, ctx->stream, queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk);
+        else launch_persistent_steps<false, 8>(steps, blocks, ctx->stream, queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk);
     }
     return check_launch("sdf persistent");
 }

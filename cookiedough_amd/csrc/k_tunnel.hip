@@ -19,7 +19,6 @@
 namespace {
 
 constexpr int kT1Block = 256;
-constexpr int kT2Seg = 32;
 
 CKD_DEV const TunJob &pick(const TunJob *jobs, int nJobs, const TunJob &single, uint32_t unit,
                            uint32_t TunJob::*base)
@@ -84,7 +83,7 @@ __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict_
 }
 
 __global__ __launch_bounds__(64) void k_tunnel_v(const TunJob *__restrict__ jobs, int nJobs,
-                                                 const TunJob single)
+                                                 const TunJob single, int kT2Seg)
 {
     __shared__ uint32_t D[256];
     const TunJob &J = pick(jobs, nJobs, single, blockIdx.x, &TunJob::vBase);
@@ -125,7 +124,7 @@ void tunnel_job_geometry(TunJob &J, uint32_t *dst, uint32_t *tmp, const int32_t 
     J.vBlocksX = (W + 63) / 64;
 }
 
-uint32_t tunnel_vblocks(unsigned W, unsigned H) { return ((W + 63) / 64) * ((H + kT2Seg - 1) / kT2Seg); }
+uint32_t tunnel_vblocks(unsigned W, unsigned H, int seg) { return ((W + 63) / 64) * ((H + seg - 1) / seg); }
 
 ckd_status launch_tunnel_jobs(ckd_ctx *ctx, const TunJob *jobs, int nJobs, const TunJob &single,
                               uint32_t totalRows, uint32_t totalVBlocks, unsigned maxW,
@@ -133,10 +132,14 @@ ckd_status launch_tunnel_jobs(ckd_ctx *ctx, const TunJob *jobs, int nJobs, const
 {
     if (totalRows == 0) return CKD_OK;
     const size_t lds = (size_t)(maxW + 1) * sizeof(uint64_t) + (size_t)maxW * sizeof(uint32_t);
-    k_tunnel_h<<<totalRows, kT1Block, lds, ctx->stream>>>(jobs, nJobs, single, tex, texLog2);
+    {
+        KernelTimer kt(ctx, CKD_KT_TUNNEL_H);
+        k_tunnel_h<<<totalRows, kT1Block, lds, ctx->stream>>>(jobs, nJobs, single, tex, texLog2);
+    }
     ckd_status s = check_launch("tunnel remap+H");
     if (s != CKD_OK) return s;
-    k_tunnel_v<<<totalVBlocks, 64, 0, ctx->stream>>>(jobs, nJobs, single);
+    KernelTimer kt(ctx, CKD_KT_TUNNEL_V);
+    k_tunnel_v<<<totalVBlocks, 64, 0, ctx->stream>>>(jobs, nJobs, single, ctx->tuning.tunnelVSeg);
     return check_launch("tunnel V+add");
 }
 

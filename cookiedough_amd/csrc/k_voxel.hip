@@ -268,6 +268,7 @@ ckd_status launch_landscape_jobs(ckd_ctx *ctx, const VoxJob *jobs, int nJobs, ui
     if (totalBlocks == 0) return CKD_OK;
     if (!ctx->lsHeight)
         return fail(CKD_ESTATE, "voxel requests need the landscape maps (ckd_landscape_create)");
+    KernelTimer kt(ctx, CKD_KT_LANDSCAPE);
     k_landscape_jobs<<<totalBlocks, 64, 0, ctx->stream>>>(jobs, nJobs, ctx->lsHeight, ctx->lsColor,
                                                           ctx->lsFog);
     return check_launch("landscape jobs");
@@ -315,6 +316,7 @@ ckd_status ckd_landscape_draw(ckd_ctx *ctx, uint32_t *dst, unsigned W, unsigned 
             P.f[k] = LsFrame{c.X1, c.Y1, c.viewCos, c.viewSin, scale_rows(c.mapTilt, H)};
         }
         dim3 grid(ckd_div_up(W, 64), n);
+        ckd::KernelTimer kt(ctx, CKD_KT_LANDSCAPE);
         k_landscape<<<grid, 64, 0, ctx->stream>>>(dst + (size_t)f0 * W * H, W, H, ctx->lsHeight,
                                                   ctx->lsColor, ctx->lsFog, P);
         ckd_status s = ckd::check_launch("landscape");

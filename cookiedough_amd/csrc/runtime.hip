@@ -52,9 +52,93 @@ ckd_status stage(ckd_ctx *ctx, const void *host, size_t bytes, void **dev)
     return CKD_OK;
 }
 
+KernelTimer::KernelTimer(ckd_ctx *ctx, int kernel) : ctx_(ctx), kernel_(kernel)
+{
+    if (!ctx->timing) return;
+    hipEvent_t e = nullptr;
+    if (!ctx->eventPool.empty()) {
+        e = ctx->eventPool.back();
+        ctx->eventPool.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+        return;
+    }
+    if (hipEventRecord(e, ctx->stream) != hipSuccess) {
+        ctx->eventPool.push_back(e);
+        return;
+    }
+    begin_ = e;
+}
+
+KernelTimer::~KernelTimer()
+{
+    if (!begin_) return;
+    hipEvent_t e = nullptr;
+    if (!ctx_->eventPool.empty()) {
+        e = ctx_->eventPool.back();
+        ctx_->eventPool.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+        ctx_->eventPool.push_back(begin_);
+        return;
+    }
+    (void)hipEventRecord(e, ctx_->stream);
+    ctx_->ktPending[kernel_].push_back({begin_, e, true});
+}
+
+ckd_status drain_timing(ckd_ctx *ctx)
+{
+    for (int k = 0; k < CKD_KT_COUNT; ++k) {
+        for (auto &pr : ctx->ktPending[k]) {
+            CKD_HIP(hipEventSynchronize(pr.end));
+            float ms = 0.f;
+            CKD_HIP(hipEventElapsedTime(&ms, pr.begin, pr.end));
+            ctx->ktMs[k] += ms;
+            ctx->ktLaunches[k] += 1;
+            if (pr.pooled) {
+                ctx->eventPool.push_back(pr.begin);
+                ctx->eventPool.push_back(pr.end);
+            }
+        }
+        ctx->ktPending[k].clear();
+    }
+    return CKD_OK;
+}
+
 } // namespace ckd
 
+namespace {
+using ckd::drain_timing;
+} // namespace
+
 extern "C" {
+
+ckd_status ckd_kernel_timing(ckd_ctx *ctx, int enable)
+{
+    CKD_REQUIRE_CTX(ctx);
+    ckd_status s = drain_timing(ctx);
+    if (s != CKD_OK) return s;
+    if (enable) {
+        for (int k = 0; k < CKD_KT_COUNT; ++k) {
+            ctx->ktMs[k] = 0.0;
+            ctx->ktLaunches[k] = 0;
+        }
+    }
+    ctx->timing = enable != 0;
+    return CKD_OK;
+}
+
+ckd_status ckd_kernel_timing_read(ckd_ctx *ctx, int kernel, double *totalMs,
+                                  unsigned long long *launches)
+{
+    CKD_REQUIRE(kernel >= 0 && kernel < CKD_KT_COUNT && totalMs && launches,
+                "ckd_kernel_timing_read: bad kernel id %d or null output", kernel);
+    CKD_REQUIRE_CTX(ctx);
+    ckd_status s = drain_timing(ctx);
+    if (s != CKD_OK) return s;
+    *totalMs = ctx->ktMs[kernel];
+    *launches = ctx->ktLaunches[kernel];
+    return CKD_OK;
+}
+
 
 const char *ckd_last_error(void) { return g_lastError; }
 
@@ -82,6 +166,12 @@ ckd_status ckd_create(int device, ckd_ctx **out)
             return ckd::fail(CKD_EDEVICE, "ckd_create: hipEventCreate failed");
         }
     }
+    if (hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->forkEv, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->joinEv, hipEventDisableTiming) != hipSuccess) {
+        ckd_destroy(ctx);
+        return ckd::fail(CKD_EDEVICE, "ckd_create: stream/event creation failed");
+    }
     *out = ctx;
     return CKD_OK;
 }
@@ -93,6 +183,16 @@ void ckd_destroy(ckd_ctx *ctx)
     (void)hipDeviceSynchronize();
     for (int i = 0; i < CKD_NUM_EVENTS; ++i)
         if (ctx->events[i]) (void)hipEventDestroy(ctx->events[i]);
+    for (int k = 0; k < CKD_KT_COUNT; ++k)
+        for (auto &pr : ctx->ktPending[k])
+            if (pr.pooled) {
+                (void)hipEventDestroy(pr.begin);
+                (void)hipEventDestroy(pr.end);
+            }
+    for (hipEvent_t e : ctx->eventPool) (void)hipEventDestroy(e);
+    if (ctx->forkEv) (void)hipEventDestroy(ctx->forkEv);
+    if (ctx->joinEv) (void)hipEventDestroy(ctx->joinEv);
+    if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
     void *bufs[] = {ctx->lsHeight, ctx->lsColor, ctx->lsFog, ctx->tsHeight, ctx->tsColor,
                     ctx->tsFog, ctx->polarMap, ctx->polarInv, ctx->b26Scratch[0],
                     ctx->b26Scratch[1], ctx->staging};

@@ -95,6 +95,14 @@ class Context:
         check(self.lib.ckd_event_elapsed_ms(self.h, a, b, C.byref(ms)))
         return ms.value
 
+    def kernel_timing(self, enable: bool) -> None:
+        check(self.lib.ckd_kernel_timing(self.h, int(enable)))
+
+    def kernel_timing_read(self, kernel: int) -> tuple[float, int]:
+        ms, n = C.c_double(), C.c_ulonglong()
+        check(self.lib.ckd_kernel_timing_read(self.h, kernel, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
     # ---- entry points --------------------------------------------------------
     def memset32(self, dst, value, n):
         check(self.lib.ckd_memset32(self.h, ptr(dst), value, n))
@@ -166,9 +174,54 @@ class Context:
         check(self.lib.ckd_tunnel_frame(self.h, ptr(dst), ptr(tmp), W, H, ptr(grid), ptr(tex),
                                         texLog2, radius))
 
+    def request_array(self, requests):
+        """a ctypes array of requests, built once and reusable across calls"""
+        return (_lib.Request * len(requests))(*requests)
+
     def render_batch(self, requests, scene, shading, tex, texLog2, tmp, tmp_pixels,
                      step_counter=None):
-        arr = (_lib.Request * len(requests))(*requests)
+        arr = requests if isinstance(requests, C.Array) else self.request_array(requests)
         check(self.lib.ckd_render_batch(self.h, arr, len(requests),
                                         C.byref(scene) if scene is not None else None, shading,
                                         ptr(tex), texLog2, ptr(tmp), tmp_pixels, ptr(step_counter)))
+
+    # ---- launch shapes and graphs --------------------------------------------
+    def get_tuning(self) -> "_lib.Tuning":
+        t = _lib.Tuning()
+        check(self.lib.ckd_get_tuning(self.h, C.byref(t)))
+        return t
+
+    def set_tuning(self, t: "_lib.Tuning") -> None:
+        check(self.lib.ckd_set_tuning(self.h, C.byref(t)))
+
+    def autotune_batch(self, requests, scene, shading, tex, texLog2, tmp, tmp_pixels):
+        arr = requests if isinstance(requests, C.Array) else self.request_array(requests)
+        t = _lib.Tuning()
+        check(self.lib.ckd_autotune_batch(self.h, arr, len(arr),
+                                          C.byref(scene) if scene is not None else None, shading,
+                                          ptr(tex), texLog2, ptr(tmp), tmp_pixels, C.byref(t)))
+        return t
+
+    def batch_graph(self, requests, scene, shading, tex, texLog2, tmp, tmp_pixels,
+                    step_counter=None) -> "BatchGraph":
+        arr = requests if isinstance(requests, C.Array) else self.request_array(requests)
+        g = C.c_void_p()
+        check(self.lib.ckd_batch_graph_create(self.h, arr, len(arr),
+                                              C.byref(scene) if scene is not None else None,
+                                              shading, ptr(tex), texLog2, ptr(tmp), tmp_pixels,
+                                              ptr(step_counter), C.byref(g)))
+        # the request array must outlive the graph
+        return BatchGraph(self, g, arr, scene)
+
+
+class BatchGraph:
+    def __init__(self, ctx: Context, handle, keep_alive, scene):
+        self.ctx, self.h, self._keep = ctx, handle, (keep_alive, scene)
+
+    def launch(self) -> None:
+        check(self.ctx.lib.ckd_graph_launch(self.h))
+
+    def close(self) -> None:
+        if self.h:
+            self.ctx.lib.ckd_graph_destroy(self.h)
+            self.h = None

@@ -61,6 +61,23 @@ CKD_API ckd_status ckd_event_record(ckd_ctx *ctx, int slot);
 CKD_API ckd_status ckd_event_elapsed_ms(ckd_ctx *ctx, int slotA, int slotB,
                                         float *ms);
 
+/* Per-kernel timing: with timing on, every launch of the kernels below is
+ * bracketed by HIP events on the context stream; ckd_kernel_timing_read
+ * waits for them and returns the total milliseconds and launch count since
+ * timing was switched on (enable = 1 resets the totals, 0 stops). */
+enum {
+    CKD_KT_SDF_PLAIN = 0,      /* ckd_sdf_render                           */
+    CKD_KT_SDF_PERSISTENT = 1, /* SDF requests of ckd_render_batch         */
+    CKD_KT_LANDSCAPE = 2,      /* ckd_landscape_draw and voxel requests    */
+    CKD_KT_TUNNEL_H = 3,       /* tunnel remap + horizontal blur           */
+    CKD_KT_TUNNEL_V = 4,       /* tunnel vertical blur + additive blit     */
+    CKD_KT_COUNT = 5
+};
+CKD_API ckd_status ckd_kernel_timing(ckd_ctx *ctx, int enable);
+CKD_API ckd_status ckd_kernel_timing_read(ckd_ctx *ctx, int kernel,
+                                          double *totalMs,
+                                          unsigned long long *launches);
+
 /* ---- pixel primitives ---------------------------------------------------- */
 /* memset32, code/util.h:57-67 */
 CKD_API ckd_status ckd_memset32(ckd_ctx *ctx, uint32_t *dst, uint32_t value,
@@ -217,6 +234,17 @@ typedef struct ckd_request {
     int radius;             /* TUNNEL: blur radius 1..127                   */
 } ckd_request;
 
+/* Launch shapes of the batched renderer, chosen by ckd_autotune_batch (or
+ * set directly).  Defaults: {2, 32, 8, 32}. */
+typedef struct ckd_tuning {
+    int sdfStepsPerRefill; /* march steps between lane refills: 1, 2 or 4   */
+    int sdfChunk;          /* side of the square pixel chunks: 16 or 32     */
+    int sdfBlocksPerCU;    /* persistent 4-wave blocks per CU: 1..32         */
+    int tunnelVSeg;        /* rows per lane in the vertical blur: 8..128     */
+} ckd_tuning;
+CKD_API ckd_status ckd_get_tuning(ckd_ctx *ctx, ckd_tuning *out);
+CKD_API ckd_status ckd_set_tuning(ckd_ctx *ctx, const ckd_tuning *t);
+
 /* Renders requests[0..n) (host array).  SDF requests use `scene` and the
  * given shading precision, VOXEL requests the landscape maps, TUNNEL
  * requests `tex` (2^texLog2 square) and need scratch: tmp (device) of at
@@ -229,6 +257,32 @@ CKD_API ckd_status ckd_render_batch(ckd_ctx *ctx, const ckd_request *requests,
                                     unsigned texLog2, uint32_t *tmp,
                                     size_t tmpPixels,
                                     unsigned long long *stepCount);
+
+/* Times every candidate launch shape on this batch (each run twice, HIP
+ * events), keeps the fastest in the context and returns it in *best (may
+ * be NULL).  The outputs of the batch are overwritten. */
+CKD_API ckd_status ckd_autotune_batch(ckd_ctx *ctx, const ckd_request *requests,
+                                      unsigned n, const ckd_sdf_scene *scene,
+                                      int shading, const uint32_t *tex,
+                                      unsigned texLog2, uint32_t *tmp,
+                                      size_t tmpPixels, ckd_tuning *best);
+
+/* A batch captured once as a hipGraph (job tables staged in device memory
+ * the graph owns, the chunk-queue reset as a kernel node, the voxel and
+ * tunnel branch on the auxiliary stream) and replayed with ckd_graph_launch
+ * on the context stream.  The request array, scene, texture and scratch
+ * must stay valid and unchanged while the graph lives; the launch shapes
+ * are those of the context at capture time. */
+typedef struct ckd_graph ckd_graph;
+CKD_API ckd_status ckd_batch_graph_create(ckd_ctx *ctx, const ckd_request *requests,
+                                          unsigned n, const ckd_sdf_scene *scene,
+                                          int shading, const uint32_t *tex,
+                                          unsigned texLog2, uint32_t *tmp,
+                                          size_t tmpPixels,
+                                          unsigned long long *stepCount,
+                                          ckd_graph **out);
+CKD_API ckd_status ckd_graph_launch(ckd_graph *graph);
+CKD_API void ckd_graph_destroy(ckd_graph *graph);
 
 #ifdef __cplusplus
 }

@@ -93,3 +93,14 @@ def test_batch_validation():
     arr = (_lib.Request * 1)(r)
     assert lib.ckd_render_batch(None, arr, 1, None, 0, 8, 9, None, 0, None) == CKD_EINVAL
     assert "scratch" in _lib.last_error()
+
+
+def test_tuning_validation_without_gpu():
+    lib = _lib.load()
+    for bad in [(3, 32, 8, 32), (2, 24, 8, 32), (2, 32, 0, 32), (2, 32, 8, 4)]:
+        t = _lib.Tuning(*bad)
+        assert lib.ckd_set_tuning(None, C.byref(t)) == CKD_EINVAL
+        assert "tuning" in _lib.last_error()
+    t = _lib.Tuning(2, 32, 8, 32)
+    assert lib.ckd_set_tuning(None, C.byref(t)) == CKD_EINVAL
+    assert "null context" in _lib.last_error()

@@ -324,3 +324,65 @@ def test_batch_sdf_vs_fp64(ctx, orc, nsph, steps, shading):
     gpu_steps = int(cnt.numpy()[0])
     _say("steps", gpu_steps, ref_steps)
     assert abs(gpu_steps - ref_steps) <= 0.01 * ref_steps
+
+
+def _mixed_batch(ctx, seed=5, n=20):
+    from cookiedough_amd._lib import CKD_FX_TUNNEL
+    items = wl.batch_requests(seed, n)
+    items = [wl.BatchItem(i.effect, i.W // 2 if i.W > 1280 else i.W,
+                          i.H // 2 if i.H > 720 else i.H, i.t, i.radius) for i in items]
+    sc = wl.sdf_scene(0, 8, 64)
+    path = wl.landscape_path(0)
+    tex = ctx.upload(wl.noise_texture(0, 9))
+    total = sum(i.W * i.H for i in items)
+    tmp_px = sum(i.W * i.H for i in items if i.effect == CKD_FX_TUNNEL)
+    out = ctx.empty(total)
+    tmp = ctx.empty(max(tmp_px, 1))
+    reqs, off, keep = [], 0, []
+    for it in items:
+        g = ctx.upload(wl.tunnel_grid(it.W, it.H, it.t)) if it.effect == CKD_FX_TUNNEL else None
+        keep.append(g)
+        reqs.append(wl.make_request(it, out.ptr + 4 * off, sc, path, g.ptr if g else None))
+        off += it.W * it.H
+    return ctx.request_array(reqs), sc, tex, out, tmp, tmp_px, keep
+
+
+def test_batch_graph_replays_match_eager(ctx, lmaps):
+    """the captured batch replayed several times, with eager work and
+    synchronisations in between; every replay equals the eager render"""
+    _say("graph")
+    reqs, sc, tex, out, tmp, tmp_px, keep = _mixed_batch(ctx)
+    ctx.render_batch(reqs, sc, CKD_SHADE_FP16, tex, 9, tmp, tmp_px)
+    eager = out.numpy()
+    graph = ctx.batch_graph(reqs, sc, CKD_SHADE_FP16, tex, 9, tmp, tmp_px)
+    try:
+        for rep in range(4):
+            ctx.memset32(out, 0x12345678, out.shape[0])
+            ctx.sync()
+            graph.launch()
+            ctx.sync()
+            assert np.array_equal(out.numpy(), eager), rep
+            ctx.render_batch(reqs, sc, CKD_SHADE_FP16, tex, 9, tmp, tmp_px)
+            ctx.sync()
+    finally:
+        graph.close()
+
+
+def test_autotune_keeps_results_identical(ctx, lmaps):
+    _say("autotune")
+    reqs, sc, tex, out, tmp, tmp_px, keep = _mixed_batch(ctx, seed=7, n=12)
+    ctx.set_tuning(wl_tuning_default())
+    ctx.render_batch(reqs, sc, CKD_SHADE_FP32, tex, 9, tmp, tmp_px)
+    before = out.numpy()
+    best = ctx.autotune_batch(reqs, sc, CKD_SHADE_FP32, tex, 9, tmp, tmp_px)
+    _say("best tuning", best.as_dict())
+    assert best.as_dict() == ctx.get_tuning().as_dict()
+    ctx.render_batch(reqs, sc, CKD_SHADE_FP32, tex, 9, tmp, tmp_px)
+    # launch shapes change the order of work, not its arithmetic
+    assert np.array_equal(out.numpy(), before)
+    ctx.set_tuning(wl_tuning_default())
+
+
+def wl_tuning_default():
+    from cookiedough_amd._lib import Tuning
+    return Tuning(2, 32, 8, 32)
