@@ -81,6 +81,7 @@ SIGNATURES = {
     "ckd_add32": [_p, _p, _p, _u],
     "ckd_blit_add32": [_p, _p, _p, _u, _u, _u],
     "ckd_blit_add32a": [_p, _p, _p, _u, _u, _u, _f],
+    "ckd_tape_warp32": [_p, _p, _p, _u, _u, _f, _f],
     "ckd_blur07_h": [_p, _p, _p, _u, _u, _f],
     "ckd_blur07_v": [_p, _p, _p, _u, _u, _f],
     "ckd_blur07": [_p, _p, _p, _u, _u, _f],

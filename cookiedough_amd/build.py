@@ -82,6 +82,20 @@ def build(verbose: bool = True) -> Path:
     return LIB
 
 
+def build_shim_test(verbose: bool = True) -> Path:
+    """The C++ drop-in shim linked into its test driver (tests/shim/shim_test)."""
+    exe = ROOT / "tests" / "shim" / "shim_test"
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", f"-I{ROOT / 'include'}",
+           str(PKG / "shim" / "ckd_shim.cpp"), str(ROOT / "tests" / "shim" / "shim_test.cpp"),
+           str(LIB), f"-Wl,-rpath,{PKG}", "-o", str(exe)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"shim build failed:\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {exe}")
+    return exe
+
+
 def build_oracle(verbose: bool = True) -> None:
     """Compile the oracle (test infrastructure) and, where the reference
     checkout exists, the reference harness oracle/_ref/libckdref.so."""
@@ -99,4 +113,5 @@ def build_oracle(verbose: bool = True) -> None:
 
 if __name__ == "__main__":
     build()
+    build_shim_test()
     build_oracle()

@@ -24,6 +24,9 @@ struct ckd_ctx {
     hipStream_t aux = nullptr;
     hipEvent_t forkEv = nullptr, joinEv = nullptr;
 
+    // interpolated cosine LUT, 2049 floats (CalculateCosLUT)
+    float *cosLut = nullptr;
+
     // landscape maps (Landscape_Create)
     uint8_t *lsHeight = nullptr;
     uint32_t *lsColor = nullptr;

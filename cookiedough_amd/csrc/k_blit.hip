@@ -83,6 +83,46 @@ __global__ __launch_bounds__(256) void k_add32_flat(uint4 *__restrict__ dst,
     }
 }
 
+// TapeWarp32 (code/util.cpp:552-603) with lutsinf/lutcosf
+// (code/sincos-lut.h:13-26): FP32 without contraction, so the 24:8 sample
+// positions equal the host's bit for bit.
+#pragma clang fp contract(off)
+CKD_DEV float lutcos(const float *__restrict__ lut, float angle, float scale)
+{
+    angle = fabsf(angle);
+    angle *= scale;
+    const int index = (int)angle & 2047;
+    const float frac = angle - truncf(angle);
+    return lut[index] + (lut[index + 1] - lut[index]) * frac;
+}
+
+__global__ __launch_bounds__(256) void k_tape_warp(uint32_t *__restrict__ dst,
+                                                   const uint32_t *__restrict__ src,
+                                                   const float *__restrict__ lut, unsigned xRes,
+                                                   unsigned yRes, float strength, float speed,
+                                                   float scale, float halfPi)
+{
+    const unsigned iX = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned iY = blockIdx.y;
+    if (iX >= xRes) return;
+    // ARRESTED_DEV_LEGACY lutsinf(a) = lutcosf(a + pi/2)
+    const float dX = lutcos(lut, (float)(int)iY * speed + halfPi, scale) * strength * 1.f;
+    const float dY = lutcos(lut, (float)iX * speed, scale) * strength * 1.f;
+    float tX = (float)iX + dX;
+    float tY = (float)iY + dY;
+    if (tX < 0.f) tX = 0.f;
+    else if (tX >= (float)xRes - 1.f) tX = (float)xRes - 2.f;
+    if (tY < 0.f) tY = 0.f;
+    else if (tY >= (float)yRes - 1.f) tY = (float)yRes - 2.f;
+    const int U = (int)(tX * 256.f), V = (int)(tY * 256.f);
+    const unsigned U0 = (unsigned)(U >> 8);
+    const unsigned V0 = (unsigned)(V >> 8) * xRes;
+    const uint32_t fu = (uint32_t)U & 0xff, fv = (uint32_t)V & 0xff;
+    dst[(size_t)iY * xRes + iX] = ckd_bilerp32(src[U0 + V0], src[U0 + 1 + V0], src[U0 + V0 + xRes],
+                                               src[U0 + 1 + V0 + xRes], fu, fv);
+}
+#pragma clang fp contract(on)
+
 unsigned grid_for(size_t items, unsigned block, unsigned cap)
 {
     size_t g = (items + block - 1) / block;
@@ -134,6 +174,30 @@ ckd_status launch_blit_add(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, uns
 } // namespace ckd
 
 extern "C" {
+
+ckd_status ckd_tape_warp32(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, unsigned xRes,
+                           unsigned yRes, float strength, float speed)
+{
+    CKD_REQUIRE(dst && src, "ckd_tape_warp32: null buffer");
+    CKD_REQUIRE(dst != src, "ckd_tape_warp32: dst must differ from src");
+    CKD_REQUIRE(xRes >= 2 && yRes >= 2 && xRes <= 16384 && yRes <= 16384,
+                "ckd_tape_warp32: bad size %ux%u", xRes, yRes);
+    CKD_REQUIRE(strength == strength && speed == speed && strength < 1e30f && strength > -1e30f &&
+                    speed < 1e30f && speed > -1e30f,
+                "ckd_tape_warp32: strength and speed must be finite");
+    CKD_REQUIRE_CTX(ctx);
+    float scale, halfPi;
+    {
+#pragma clang fp contract(off)
+        const float kPI = 3.1415926535897932384626433832795f;
+        scale = (1.f / (2.f * kPI)) * 2048.f;
+        halfPi = kPI * 0.5f;
+    }
+    dim3 grid(ckd_div_up(xRes, 256), yRes);
+    k_tape_warp<<<grid, 256, 0, ctx->stream>>>(dst, src, ctx->cosLut, xRes, yRes, strength, speed,
+                                                scale, halfPi);
+    return ckd::check_launch("tape_warp32");
+}
 
 ckd_status ckd_memset32(ckd_ctx *ctx, uint32_t *dst, uint32_t value, size_t numInts)
 {

@@ -1,6 +1,7 @@
 // runtime.hip -- context, errors, device memory, events.
 #include "ckd_internal.h"
 
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 
@@ -166,6 +167,19 @@ ckd_status ckd_create(int device, ckd_ctx **out)
             return ckd::fail(CKD_EDEVICE, "ckd_create: hipEventCreate failed");
         }
     }
+    {
+        // CalculateCosLUT (code/sincos-lut.cpp:9-16) with the host libm
+        const float kPI = 3.1415926535897932384626433832795f;
+        const float k2PI = 2.f * kPI;
+        float lut[2049];
+        for (unsigned i = 0; i < 2048; ++i) lut[i] = cosf((float)i * (k2PI / 2048.f));
+        lut[2048] = lut[0];
+        if (hipMalloc(&ctx->cosLut, sizeof(lut)) != hipSuccess ||
+            hipMemcpy(ctx->cosLut, lut, sizeof(lut), hipMemcpyHostToDevice) != hipSuccess) {
+            ckd_destroy(ctx);
+            return ckd::fail(CKD_ENOMEM, "ckd_create: cosine LUT upload failed");
+        }
+    }
     if (hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->forkEv, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->joinEv, hipEventDisableTiming) != hipSuccess) {
@@ -193,7 +207,7 @@ void ckd_destroy(ckd_ctx *ctx)
     if (ctx->forkEv) (void)hipEventDestroy(ctx->forkEv);
     if (ctx->joinEv) (void)hipEventDestroy(ctx->joinEv);
     if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
-    void *bufs[] = {ctx->lsHeight, ctx->lsColor, ctx->lsFog, ctx->tsHeight, ctx->tsColor,
+    void *bufs[] = {ctx->cosLut, ctx->lsHeight, ctx->lsColor, ctx->lsFog, ctx->tsHeight, ctx->tsColor,
                     ctx->tsFog, ctx->polarMap, ctx->polarInv, ctx->b26Scratch[0],
                     ctx->b26Scratch[1], ctx->staging};
     for (void *b : bufs)

@@ -119,6 +119,9 @@ class Context:
     def blit_add32a(self, dst, src, destResX, srcResX, yRes, alpha):
         check(self.lib.ckd_blit_add32a(self.h, ptr(dst), ptr(src), destResX, srcResX, yRes, alpha))
 
+    def tape_warp32(self, dst, src, W, H, strength, speed):
+        check(self.lib.ckd_tape_warp32(self.h, ptr(dst), ptr(src), W, H, strength, speed))
+
     def blur07_h(self, dst, src, W, H, strength):
         check(self.lib.ckd_blur07_h(self.h, ptr(dst), ptr(src), W, H, strength))
 

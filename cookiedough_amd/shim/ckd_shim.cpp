@@ -1,0 +1,384 @@
+// ckd_shim.cpp -- the reference's effect / filter / blit entry points over
+// the libckd C ABI.  See ckd_shim.h.
+//
+// One process-wide context is created by the first *_Create of any part and
+// destroyed by the last *_Destroy.  Host buffers are uploaded into device
+// staging images the shim owns, processed, and downloaded before the call
+// returns; errors go to SetLastError, as the reference's Create functions
+// report theirs (code/main.h:49).
+#include "ckd_shim.h"
+
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "../../include/ckd.h"
+#include "ckd_host_api.h"
+
+namespace {
+
+constexpr unsigned kResX = 1280, kResY = 720;                  // code/main.h:37-38
+constexpr unsigned kFxMapResX = kResX / 2 + 4, kFxMapResY = kResY / 2 + 4; // code/fx-blitter.h:16-17
+constexpr float kPI = 3.1415926535897932384626433832795f;
+
+ckd_ctx *g_ctx = nullptr;
+int g_users = 0;
+
+void report(const char *what)
+{
+    SetLastError(std::string(what) + ": " + ckd_last_error());
+}
+
+bool acquire()
+{
+    if (!g_ctx && ckd_create(0, &g_ctx) != CKD_OK) {
+        report("libckd context");
+        return false;
+    }
+    ++g_users;
+    return true;
+}
+
+void release()
+{
+    if (g_users > 0 && --g_users == 0) {
+        ckd_destroy(g_ctx);
+        g_ctx = nullptr;
+    }
+}
+
+// device image that grows on demand
+struct DevImage {
+    uint32_t *p = nullptr;
+    size_t cap = 0;
+    uint32_t *get(size_t pixels)
+    {
+        if (pixels <= cap) return p;
+        if (p) ckd_free(g_ctx, p);
+        p = nullptr;
+        cap = 0;
+        void *q = nullptr;
+        if (ckd_malloc(g_ctx, &q, pixels * 4) != CKD_OK) return nullptr;
+        p = static_cast<uint32_t *>(q);
+        cap = pixels;
+        return p;
+    }
+};
+
+DevImage g_a, g_b;
+
+bool up(uint32_t *dev, const uint32_t *host, size_t pixels)
+{
+    return ckd_memcpy_h2d(g_ctx, dev, host, pixels * 4) == CKD_OK;
+}
+
+bool down(uint32_t *host, const uint32_t *dev, size_t pixels)
+{
+    return ckd_memcpy_d2h(g_ctx, host, dev, pixels * 4) == CKD_OK;
+}
+
+// unary filter: upload src, f(devDst, devSrc), download; dst == src keeps
+// the reference's in-place semantics where f supports them
+template <typename F>
+void filter(const char *name, uint32_t *pDest, const uint32_t *pSrc, size_t pixels, bool inPlace,
+            F f)
+{
+    if (!g_ctx && !acquire()) return;
+    uint32_t *a = g_a.get(pixels);
+    uint32_t *b = inPlace ? a : g_b.get(pixels);
+    if (!a || !b || !up(a, pSrc, pixels) || f(b, a) != CKD_OK || !down(pDest, b, pixels))
+        report(name);
+}
+
+// binary blit: dst = f(dst, src)
+template <typename F>
+void blit2(const char *name, uint32_t *pDest, size_t dstPixels, const uint32_t *pSrc,
+           size_t srcPixels, F f)
+{
+    if (!g_ctx && !acquire()) return;
+    uint32_t *a = g_a.get(dstPixels), *b = g_b.get(srcPixels);
+    if (!a || !b || !up(a, pDest, dstPixels) || !up(b, pSrc, srcPixels) || f(a, b) != CKD_OK ||
+        !down(pDest, a, dstPixels))
+        report(name);
+}
+
+// ---- landscape state (code/landscape.cpp:15-47, 109-168) -----------------
+bool s_lsReady = false;
+SyncTrack trackVoxelScapeForward, trackVoxelScapeTilt, trackWarpSpeed, trackWarpStrength;
+float s_padTilt = 0.f, s_viewAngle = 0.f, s_strafeX = 0.f, s_strafeY = 0.f;
+float s_padMoveX = 0.f, s_padMoveY = 0.f;
+
+float getf(SyncTrack t) { return (float)Rocket::get(t); }
+
+float clampf(float lo, float hi, float v) { return std::fmax(lo, std::fmin(hi, v)); }
+
+// ---- tunnelscape state (code/tunnelscape.cpp:19-20) ----------------------
+bool s_tsReady = false;
+SyncTrack trackStarsStepU, trackStarsStepV, trackStarsSpeed, trackStarsBlur;
+
+} // namespace
+
+// ===================================================================== parts
+
+uint32_t *g_pFxMap[kShimNumFxMaps] = {nullptr};
+
+bool Landscape_Create()
+{
+    if (!acquire()) return false;
+    const uint8_t *hm = Image_Load8("assets/scape/D17.png");
+    const uint32_t *cm = Image_Load32("assets/scape/C17W-edit.png");
+    if (!hm || !cm) return false;
+    const uint32_t *fog = Image_Load32("assets/scape/foggradient.jpg");
+    if (!fog) return false;
+    if (ckd_landscape_create(g_ctx, hm, cm, fog) != CKD_OK) {
+        report("Landscape_Create");
+        return false;
+    }
+    trackVoxelScapeForward = Rocket::AddTrack("voxelScape:Forward");
+    trackVoxelScapeTilt = Rocket::AddTrack("voxelScape:Tilt");
+    trackWarpSpeed = Rocket::AddTrack("voxelScape:WarpSpeed");
+    trackWarpStrength = Rocket::AddTrack("voxelScape:WarpStrength");
+    s_lsReady = true;
+    return true;
+}
+
+void Landscape_Destroy()
+{
+    if (s_lsReady) release();
+    s_lsReady = false;
+}
+
+void Landscape_Draw(uint32_t *pDest, float time, float delta)
+{
+    (void)time;
+    if (!s_lsReady) return;
+    // the camera of vscape (code/landscape.cpp:111-168)
+    PadState pad;
+    Gamepad_Update(pad);
+    const float kMaxTiltDiff = 90.f;
+    const float tiltStep = std::fmin(delta, 1.f);
+    if (s_padTilt < kMaxTiltDiff && pad.rightY > 0.f) s_padTilt += tiltStep;
+    if (s_padTilt > -kMaxTiltDiff && pad.rightY < 0.f) s_padTilt -= tiltStep;
+    const float tilt = clampf(-kMaxTiltDiff, kMaxTiltDiff, getf(trackVoxelScapeTilt) + s_padTilt);
+    const int mapTilt = 90 + int(tilt);
+    const float kAspect = (float)kResY / kResX;
+    float viewMul = 1.f / kAspect;
+    viewMul *= delta * 0.01f;
+    s_viewAngle += viewMul * pad.lShoulder;
+    s_viewAngle -= viewMul * pad.rShoulder;
+    const float viewCos = cosf(s_viewAngle);
+    const float viewSin = sinf(s_viewAngle);
+    if (pad.rightX != 0.f) {
+        const float strafe = pad.rightX * delta;
+        s_strafeX += viewCos * strafe;
+        s_strafeY += viewSin * strafe;
+    }
+    const float padMove = (pad.leftY != 0.f) ? -pad.leftY * delta : 0.f;
+    s_padMoveX += -viewSin * padMove;
+    s_padMoveY += viewCos * padMove;
+    const float fwd = getf(trackVoxelScapeForward);
+    float moveX = -viewSin * fwd;
+    float moveY = viewCos * fwd;
+    moveX += s_padMoveX;
+    moveY += s_padMoveY;
+    ckd_landscape_cam cam{moveX + s_strafeX, moveY + s_strafeY, viewCos, viewSin, mapTilt};
+
+    // Landscape_Draw (code/landscape.cpp:228-243)
+    const size_t n = (size_t)kResX * kResY;
+    const float warpStrength = getf(trackWarpStrength);
+    uint32_t *a = g_a.get(n), *b = g_b.get(n);
+    if (!a || !b || ckd_landscape_draw(g_ctx, a, kResX, kResY, &cam, 1) != CKD_OK) {
+        report("Landscape_Draw");
+        return;
+    }
+    uint32_t *out = a;
+    if (warpStrength != 0.f) {
+        if (ckd_tape_warp32(g_ctx, b, a, kResX, kResY, getf(trackWarpSpeed), warpStrength) != CKD_OK) {
+            report("Landscape_Draw (TapeWarp32)");
+            return;
+        }
+        out = b;
+    }
+    if (!down(pDest, out, n)) report("Landscape_Draw");
+}
+
+bool Tunnelscape_Create()
+{
+    if (!acquire()) return false;
+    const uint8_t *hm = Image_Load8("assets/scape/tscape-D7-edit.png");
+    const uint32_t *cm = Image_Load32("assets/scape/tscape-C7W-edit.png");
+    if (!hm || !cm) return false;
+    const uint32_t *fog = Image_Load32("assets/scape/foggradient.jpg");
+    if (!fog) return false;
+    if (ckd_tunnelscape_create(g_ctx, hm, cm, fog) != CKD_OK ||
+        ckd_polar_create(g_ctx, kResX, kResY) != CKD_OK) {
+        report("Tunnelscape_Create");
+        return false;
+    }
+    trackStarsStepU = Rocket::AddTrack("starsTunnel:stepU");
+    trackStarsStepV = Rocket::AddTrack("starsTunnel:stepV");
+    trackStarsSpeed = Rocket::AddTrack("starsTunnel:Speed");
+    trackStarsBlur = Rocket::AddTrack("starsTunnel:Blur");
+    s_tsReady = true;
+    return true;
+}
+
+void Tunnelscape_Destroy()
+{
+    if (s_tsReady) release();
+    s_tsReady = false;
+}
+
+void Tunnelscape_Draw(uint32_t *pDest, float time, float delta)
+{
+    (void)delta;
+    if (!s_tsReady) return;
+    const size_t n = (size_t)kResX * kResY;
+    uint32_t *a = g_a.get(n), *b = g_b.get(n);
+    if (!a || !b ||
+        ckd_tunnelscape_rows(g_ctx, a, kResX, kResY, getf(trackStarsStepU), getf(trackStarsStepV),
+                             getf(trackStarsSpeed), time) != CKD_OK ||
+        ckd_polar_blit(g_ctx, b, a, kResX, kResY, 1) != CKD_OK) {
+        report("Tunnelscape_Draw");
+        return;
+    }
+    float blur = getf(trackStarsBlur);
+    if (blur != 0.f) {
+        // BoxBlurScale (code/deprecated/boxblur.h:13-19), then BoxBlur32 in
+        // place twice (code/tunnelscape.cpp:176-185)
+        blur = clampf(1.f, 100.f, blur) * 0.01f;
+        if (ckd_blur07(g_ctx, b, b, kResX, kResY, blur) != CKD_OK ||
+            ckd_blur07(g_ctx, b, b, kResX, kResY, blur) != CKD_OK) {
+            report("Tunnelscape_Draw (blur)");
+            return;
+        }
+    }
+    if (!down(pDest, b, n)) report("Tunnelscape_Draw");
+}
+
+bool Polar_Create()
+{
+    if (!acquire()) return false;
+    if (ckd_polar_create(g_ctx, kResX, kResY) != CKD_OK) {
+        report("Polar_Create");
+        return false;
+    }
+    return true;
+}
+
+void Polar_Destroy() { release(); }
+
+void Polar_Blit(uint32_t *pDest, const uint32_t *pSrc, bool inverse)
+{
+    filter("Polar_Blit", pDest, pSrc, (size_t)kResX * kResY, false, [&](uint32_t *d, uint32_t *s) {
+        return ckd_polar_blit(g_ctx, d, s, kResX, kResY, inverse ? 1 : 0);
+    });
+}
+
+bool FxBlitter_Create()
+{
+    if (!acquire()) return false;
+    // one allocation for the four maps, 16-byte aligned (code/fx-blitter.cpp:10-20)
+    const size_t mapPixels = (size_t)kFxMapResX * kFxMapResY;
+    void *base = nullptr;
+    if (posix_memalign(&base, 16, mapPixels * 4 * kShimNumFxMaps) != 0) return false;
+    memset(base, 0, mapPixels * 4 * kShimNumFxMaps);
+    for (size_t i = 0; i < kShimNumFxMaps; ++i)
+        g_pFxMap[i] = static_cast<uint32_t *>(base) + i * mapPixels;
+    return true;
+}
+
+void FxBlitter_Destroy()
+{
+    free(g_pFxMap[0]);
+    for (auto &m : g_pFxMap) m = nullptr;
+    release();
+}
+
+void Fx_Blit_2x2(uint32_t *pDest, const uint32_t *pSrc)
+{
+    if (!g_ctx && !acquire()) return;
+    const size_t mapPixels = (size_t)kFxMapResX * kFxMapResY, n = (size_t)kResX * kResY;
+    uint32_t *a = g_a.get(mapPixels), *b = g_b.get(n);
+    if (!a || !b || !up(a, pSrc, mapPixels) || ckd_fx_blit_2x2(g_ctx, b, a, kResX, kResY) != CKD_OK ||
+        !down(pDest, b, n))
+        report("Fx_Blit_2x2");
+}
+
+bool BoxBlur_Create() { return acquire(); }
+void BoxBlur_Destroy() { release(); }
+
+void BoxBlur_Horz32(uint32_t *pDest, const uint32_t *pSrc, unsigned xRes, unsigned yRes,
+                    float strength, float gain, unsigned numPasses)
+{
+    filter("BoxBlur_Horz32", pDest, pSrc, (size_t)xRes * yRes, false, [&](uint32_t *d, uint32_t *s) {
+        return ckd_blur26_h(g_ctx, d, s, xRes, yRes, strength, gain, numPasses);
+    });
+}
+
+void BoxBlur_Vert32(uint32_t *pDest, const uint32_t *pSrc, unsigned xRes, unsigned yRes,
+                    float strength, float gain, unsigned numPasses)
+{
+    filter("BoxBlur_Vert32", pDest, pSrc, (size_t)xRes * yRes, false, [&](uint32_t *d, uint32_t *s) {
+        return ckd_blur26_v(g_ctx, d, s, xRes, yRes, strength, gain, numPasses);
+    });
+}
+
+void BoxBlur_32(uint32_t *pDest, const uint32_t *pSrc, unsigned xRes, unsigned yRes, float strength,
+                float gain, unsigned numPasses)
+{
+    filter("BoxBlur_32", pDest, pSrc, (size_t)xRes * yRes, false, [&](uint32_t *d, uint32_t *s) {
+        return ckd_blur26(g_ctx, d, s, xRes, yRes, strength, gain, numPasses);
+    });
+}
+
+void HorizontalBoxBlur32(uint32_t *pDest, const uint32_t *pSrc, unsigned int xRes,
+                         unsigned int yRes, float strength)
+{
+    filter("HorizontalBoxBlur32", pDest, pSrc, (size_t)xRes * yRes, pDest == pSrc,
+           [&](uint32_t *d, uint32_t *s) { return ckd_blur07_h(g_ctx, d, s, xRes, yRes, strength); });
+}
+
+void VerticalBoxBlur32(uint32_t *pDest, const uint32_t *pSrc, unsigned int xRes, unsigned int yRes,
+                       float strength)
+{
+    filter("VerticalBoxBlur32", pDest, pSrc, (size_t)xRes * yRes, pDest == pSrc,
+           [&](uint32_t *d, uint32_t *s) { return ckd_blur07_v(g_ctx, d, s, xRes, yRes, strength); });
+}
+
+void BoxBlur32(uint32_t *pDest, const uint32_t *pSrc, unsigned int xRes, unsigned int yRes,
+               float strength)
+{
+    filter("BoxBlur32", pDest, pSrc, (size_t)xRes * yRes, pDest == pSrc,
+           [&](uint32_t *d, uint32_t *s) { return ckd_blur07(g_ctx, d, s, xRes, yRes, strength); });
+}
+
+void Add32(uint32_t *pDest, const uint32_t *pSrc, unsigned numPixels)
+{
+    blit2("Add32", pDest, numPixels, pSrc, numPixels,
+          [&](uint32_t *d, uint32_t *s) { return ckd_add32(g_ctx, d, s, numPixels); });
+}
+
+void BlitAdd32(uint32_t *pDest, const uint32_t *pSrc, unsigned destResX, unsigned srcResX,
+               unsigned yRes)
+{
+    blit2("BlitAdd32", pDest, (size_t)destResX * yRes, pSrc, (size_t)srcResX * yRes,
+          [&](uint32_t *d, uint32_t *s) { return ckd_blit_add32(g_ctx, d, s, destResX, srcResX, yRes); });
+}
+
+void BlitAdd32A(uint32_t *pDest, const uint32_t *pSrc, unsigned destResX, unsigned srcResX,
+                unsigned yRes, float alpha)
+{
+    blit2("BlitAdd32A", pDest, (size_t)destResX * yRes, pSrc, (size_t)srcResX * yRes,
+          [&](uint32_t *d, uint32_t *s) {
+              return ckd_blit_add32a(g_ctx, d, s, destResX, srcResX, yRes, alpha);
+          });
+}
+
+void TapeWarp32(uint32_t *pDest, const uint32_t *pSrc, unsigned xRes, unsigned yRes,
+                float strength, float speed)
+{
+    filter("TapeWarp32", pDest, pSrc, (size_t)xRes * yRes, false, [&](uint32_t *d, uint32_t *s) {
+        return ckd_tape_warp32(g_ctx, d, s, xRes, yRes, strength, speed);
+    });
+}

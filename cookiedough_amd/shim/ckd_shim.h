@@ -1,0 +1,61 @@
+// ckd_shim.h -- the reference's effect, filter and blit entry points,
+// implemented over the libckd C ABI (include/ckd.h).  Same names and
+// signatures as the reference headers, so code/demo.cpp links against this
+// shim in place of landscape.cpp, tunnelscape.cpp, polar.cpp,
+// fx-blitter.cpp, boxblur.cpp, deprecated/boxblur.cpp and the blits of
+// util.cpp.  Buffers are host memory, as in the reference (1280x720 frames,
+// 644x364 FX maps); every call uploads its inputs, renders on the GPU and
+// downloads the result before returning.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+// code/landscape.h:7-9
+bool Landscape_Create();
+void Landscape_Destroy();
+void Landscape_Draw(uint32_t *pDest, float time, float delta);
+
+// code/tunnelscape.h:7-9
+bool Tunnelscape_Create();
+void Tunnelscape_Destroy();
+void Tunnelscape_Draw(uint32_t *pDest, float time, float delta);
+
+// code/polar.h:7-10
+bool Polar_Create();
+void Polar_Destroy();
+void Polar_Blit(uint32_t *pDest, const uint32_t *pSrc, bool inverse = false);
+
+// code/fx-blitter.h:26-33
+constexpr size_t kShimNumFxMaps = 4;
+extern uint32_t *g_pFxMap[kShimNumFxMaps];
+bool FxBlitter_Create();
+void FxBlitter_Destroy();
+void Fx_Blit_2x2(uint32_t *pDest, const uint32_t *pSrc);
+
+// code/boxblur.h:7-20
+bool BoxBlur_Create();
+void BoxBlur_Destroy();
+void BoxBlur_Horz32(uint32_t *pDest, const uint32_t *pSrc, unsigned xRes, unsigned yRes,
+                    float strength, float gain, unsigned numPasses);
+void BoxBlur_Vert32(uint32_t *pDest, const uint32_t *pSrc, unsigned xRes, unsigned yRes,
+                    float strength, float gain, unsigned numPasses);
+void BoxBlur_32(uint32_t *pDest, const uint32_t *pSrc, unsigned xRes, unsigned yRes,
+                float strength, float gain, unsigned numPasses);
+
+// code/deprecated/boxblur.h:21-41
+void HorizontalBoxBlur32(uint32_t *pDest, const uint32_t *pSrc, unsigned int xRes,
+                         unsigned int yRes, float strength);
+void VerticalBoxBlur32(uint32_t *pDest, const uint32_t *pSrc, unsigned int xRes,
+                       unsigned int yRes, float strength);
+void BoxBlur32(uint32_t *pDest, const uint32_t *pSrc, unsigned int xRes, unsigned int yRes,
+               float strength);
+
+// code/util.h:79,118-119,93
+void Add32(uint32_t *pDest, const uint32_t *pSrc, unsigned numPixels);
+void BlitAdd32(uint32_t *pDest, const uint32_t *pSrc, unsigned destResX, unsigned srcResX,
+               unsigned yRes);
+void BlitAdd32A(uint32_t *pDest, const uint32_t *pSrc, unsigned destResX, unsigned srcResX,
+                unsigned yRes, float alpha);
+void TapeWarp32(uint32_t *pDest, const uint32_t *pSrc, unsigned xRes, unsigned yRes,
+                float strength, float speed);

@@ -99,6 +99,15 @@ CKD_API ckd_status ckd_blit_add32a(ckd_ctx *ctx, uint32_t *dst,
                                    const uint32_t *src, unsigned destResX,
                                    unsigned srcResX, unsigned yRes, float alpha);
 
+/* TapeWarp32, code/util.cpp:552-603: per-pixel FP32 offsets from the
+ * interpolated cosine LUT (code/sincos-lut.h:13-26, built by ckd_create as
+ * CalculateCosLUT does, code/sincos-lut.cpp:9-16), clamped, converted to
+ * 24:8 and sampled bilinearly.  The row stride of the fetch is xRes (the
+ * reference uses kResX = 1280, the only width it calls this with). */
+CKD_API ckd_status ckd_tape_warp32(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src,
+                                   unsigned xRes, unsigned yRes, float strength,
+                                   float speed);
+
 /* ---- 2007 box blur, code/deprecated/boxblur.cpp -------------------------
  * dst == src is allowed and reproduces the reference's in-place recurrence
  * (the sliding-window subtract then reads already-blurred pixels).  The

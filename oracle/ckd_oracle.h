@@ -164,6 +164,9 @@ void orc_sdf_camera(const orc_sdf_scene *scene, float t, float cam[12]);
 void  orc_cos_lut(float lut[2049]);
 float orc_lutcosf(const float *lut, float angle);
 float orc_lutsinf(const float *lut, float angle);
+/* code/util.cpp:552-603; pinned: reference harness (TapeWarp32 itself) */
+void  orc_tape_warp32(uint32_t *dst, const uint32_t *src, unsigned xRes, unsigned yRes,
+                      float strength, float speed, const float *lut);
 
 #ifdef __cplusplus
 }

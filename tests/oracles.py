@@ -58,6 +58,7 @@ _ORC = {
     "orc_cos_lut": (None, [_p]),
     "orc_lutcosf": (_f, [_p, _f]),
     "orc_lutsinf": (_f, [_p, _f]),
+    "orc_tape_warp32": (None, [_p, _p, _u, _u, _f, _f, _p]),
 }
 
 _REF = {

@@ -1,0 +1,151 @@
+// shim_test.cpp -- drives the C++ drop-in shim (cookiedough_amd/shim) the
+// way code/demo.cpp does, with test doubles for the host application's
+// Rocket, image loader, gamepad and error reporting:
+//   Rocket tracks   values from CKD_SHIM_TRACKS="name=value;name=value"
+//   Image_Load8/32  raw files $CKD_SHIM_ASSETS/<file name>.bin
+//   Gamepad         always idle
+// Usage:
+//   shim_test landscape <time> <delta> <out>        (1280x720 frame)
+//   shim_test polar <in> <inverse 0|1> <out>
+//   shim_test fxblit <in 644x364> <out>
+//   shim_test blur07 <in> <W> <H> <strength> <h|v|2d> <inplace 0|1> <out>
+//   shim_test tapewarp <in> <W> <H> <strength> <speed> <out>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../cookiedough_amd/shim/ckd_host_api.h"
+#include "../../cookiedough_amd/shim/ckd_shim.h"
+
+struct sync_track {
+    std::string name;
+    double value;
+};
+
+static std::map<std::string, sync_track> g_tracks;
+static bool g_failed = false;
+
+namespace Rocket {
+const sync_track *AddTrack(const char *name)
+{
+    sync_track &t = g_tracks[name];
+    t.name = name;
+    t.value = 0.0;
+    if (const char *env = getenv("CKD_SHIM_TRACKS")) {
+        std::string s(env), key = std::string(name) + "=";
+        size_t p = s.find(key);
+        if (p != std::string::npos) t.value = atof(s.c_str() + p + key.size());
+    }
+    return &t;
+}
+double get(const sync_track *track) { return track ? track->value : 0.0; }
+} // namespace Rocket
+
+static std::vector<unsigned char> read_file(const std::string &path)
+{
+    std::vector<unsigned char> out;
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f) return out;
+    fseek(f, 0, SEEK_END);
+    out.resize((size_t)ftell(f));
+    fseek(f, 0, SEEK_SET);
+    if (fread(out.data(), 1, out.size(), f) != out.size()) out.clear();
+    fclose(f);
+    return out;
+}
+
+static void *load_asset(const std::string &path)
+{
+    const char *dir = getenv("CKD_SHIM_ASSETS");
+    const std::string base = path.substr(path.find_last_of('/') + 1);
+    std::vector<unsigned char> d = read_file(std::string(dir ? dir : ".") + "/" + base + ".bin");
+    if (d.empty()) return nullptr;
+    void *p = malloc(d.size());
+    memcpy(p, d.data(), d.size());
+    return p;
+}
+
+uint32_t *Image_Load32(const std::string &path) { return static_cast<uint32_t *>(load_asset(path)); }
+uint8_t *Image_Load8(const std::string &path) { return static_cast<uint8_t *>(load_asset(path)); }
+
+bool Gamepad_Update(PadState &state)
+{
+    memset(&state, 0, sizeof(state));
+    return false;
+}
+
+void SetLastError(const std::string &description)
+{
+    fprintf(stderr, "SetLastError: %s\n", description.c_str());
+    g_failed = true;
+}
+
+static std::vector<uint32_t> read_pixels(const char *path)
+{
+    std::vector<unsigned char> d = read_file(path);
+    std::vector<uint32_t> px(d.size() / 4);
+    memcpy(px.data(), d.data(), px.size() * 4);
+    return px;
+}
+
+static void write_pixels(const char *path, const uint32_t *px, size_t n)
+{
+    FILE *f = fopen(path, "wb");
+    fwrite(px, 4, n, f);
+    fclose(f);
+}
+
+int main(int argc, char **argv)
+{
+    if (argc < 2) return 2;
+    const std::string cmd = argv[1];
+    const size_t n = 1280 * 720;
+    std::vector<uint32_t> out(n, 0);
+    if (cmd == "landscape" && argc == 5) {
+        if (!Landscape_Create()) return 1;
+        Landscape_Draw(out.data(), (float)atof(argv[2]), (float)atof(argv[3]));
+        Landscape_Destroy();
+        write_pixels(argv[4], out.data(), n);
+    } else if (cmd == "polar" && argc == 5) {
+        std::vector<uint32_t> in = read_pixels(argv[2]);
+        if (in.size() != n || !Polar_Create()) return 1;
+        Polar_Blit(out.data(), in.data(), atoi(argv[3]) != 0);
+        Polar_Destroy();
+        write_pixels(argv[4], out.data(), n);
+    } else if (cmd == "fxblit" && argc == 4) {
+        std::vector<uint32_t> in = read_pixels(argv[2]);
+        if (in.size() != 644 * 364 || !FxBlitter_Create()) return 1;
+        memcpy(g_pFxMap[0], in.data(), in.size() * 4);
+        Fx_Blit_2x2(out.data(), g_pFxMap[0]);
+        FxBlitter_Destroy();
+        write_pixels(argv[3], out.data(), n);
+    } else if (cmd == "blur07" && argc == 9) {
+        std::vector<uint32_t> in = read_pixels(argv[2]);
+        const unsigned W = (unsigned)atoi(argv[3]), H = (unsigned)atoi(argv[4]);
+        const float s = (float)atof(argv[5]);
+        const std::string mode = argv[6];
+        const bool inplace = atoi(argv[7]) != 0;
+        if (in.size() != (size_t)W * H || !BoxBlur_Create()) return 1;
+        std::vector<uint32_t> res(in.size());
+        uint32_t *dst = inplace ? in.data() : res.data();
+        if (mode == "h") HorizontalBoxBlur32(dst, in.data(), W, H, s);
+        else if (mode == "v") VerticalBoxBlur32(dst, in.data(), W, H, s);
+        else BoxBlur32(dst, in.data(), W, H, s);
+        BoxBlur_Destroy();
+        write_pixels(argv[8], dst, in.size());
+    } else if (cmd == "tapewarp" && argc == 8) {
+        std::vector<uint32_t> in = read_pixels(argv[2]);
+        const unsigned W = (unsigned)atoi(argv[3]), H = (unsigned)atoi(argv[4]);
+        if (in.size() != (size_t)W * H || !BoxBlur_Create()) return 1;
+        std::vector<uint32_t> res(in.size());
+        TapeWarp32(res.data(), in.data(), W, H, (float)atof(argv[5]), (float)atof(argv[6]));
+        BoxBlur_Destroy();
+        write_pixels(argv[7], res.data(), res.size());
+    } else {
+        return 2;
+    }
+    return g_failed ? 1 : 0;
+}

@@ -386,3 +386,17 @@ def test_autotune_keeps_results_identical(ctx, lmaps):
 def wl_tuning_default():
     from cookiedough_amd._lib import Tuning
     return Tuning(2, 32, 8, 32)
+
+
+def test_tape_warp32(ctx, orc):
+    _say("tape warp")
+    W, H = 1280, 720
+    lut = np.zeros(2049, np.float32)
+    orc.orc_cos_lut(P(lut))
+    src = rand_img(60, W * H)
+    d_src, d_dst = ctx.upload(src), ctx.empty(W * H)
+    for strength, speed in [(3.0, 0.05), (12.5, 0.013), (0.0, 1.0), (40.0, -0.2)]:
+        ctx.tape_warp32(d_dst, d_src, W, H, strength, speed)
+        ref = np.zeros(W * H, np.uint32)
+        orc.orc_tape_warp32(P(ref), P(src), W, H, strength, speed, P(lut))
+        assert np.array_equal(d_dst.numpy(), ref), (strength, speed)

@@ -1,0 +1,122 @@
+"""The C++ drop-in shim (the reference's own entry points over the C ABI),
+driven like code/demo.cpp drives the effects, against the oracle."""
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from cookiedough_amd import workloads as wl
+from cookiedough_amd._lib import LandscapeCam
+from helpers import rand_img
+from oracles import P, orc_cam
+
+pytestmark = pytest.mark.gpu
+
+EXE = Path(__file__).resolve().parent / "shim" / "shim_test"
+
+
+def run(args, env=None, tmp=None):
+    e = dict(os.environ)
+    e.update(env or {})
+    r = subprocess.run([str(EXE), *map(str, args)], capture_output=True, text=True, env=e,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    return r
+
+
+@pytest.fixture(scope="module")
+def assets(tmp_path_factory):
+    d = tmp_path_factory.mktemp("assets")
+    m = wl.landscape_maps(0)
+    (d / "D17.png.bin").write_bytes(m.height.tobytes())
+    (d / "C17W-edit.png.bin").write_bytes(m.color.tobytes())
+    (d / "foggradient.jpg.bin").write_bytes(m.fog.tobytes())
+    return d, m
+
+
+@pytest.mark.parametrize("fwd,tilt", [(12.5, 0.0), (300.0, -25.0), (-77.0, 60.0)])
+def test_landscape_draw(orc, assets, tmp_path, fwd, tilt):
+    d, m = assets
+    out = tmp_path / "out.bin"
+    run(["landscape", 1.0, 1.0, out],
+        env={"CKD_SHIM_ASSETS": str(d),
+             "CKD_SHIM_TRACKS": f"voxelScape:Forward={fwd};voxelScape:Tilt={tilt}"})
+    got = np.fromfile(out, np.uint32)
+    # vscape with an idle gamepad: view angle 0, origin (-0*fwd, fwd)
+    cam = LandscapeCam(np.float32(-0.0 * fwd), np.float32(fwd), 1.0, 0.0, 90 + int(tilt))
+    ref = np.zeros(1280 * 720, np.uint32)
+    oc = orc_cam(cam)
+    orc.orc_landscape_frame(P(ref), 1280, 720, P(m.height), P(m.color), P(m.fog), C.byref(oc), 8)
+    assert np.array_equal(got, ref)
+
+
+def test_landscape_draw_with_tape_warp(orc, assets, tmp_path):
+    d, m = assets
+    out = tmp_path / "out.bin"
+    run(["landscape", 1.0, 1.0, out],
+        env={"CKD_SHIM_ASSETS": str(d),
+             "CKD_SHIM_TRACKS": "voxelScape:Forward=40;voxelScape:Tilt=5;"
+                                "voxelScape:WarpSpeed=6.5;voxelScape:WarpStrength=0.02"})
+    got = np.fromfile(out, np.uint32)
+    cam = LandscapeCam(np.float32(-0.0), 40.0, 1.0, 0.0, 95)
+    frame = np.zeros(1280 * 720, np.uint32)
+    oc = orc_cam(cam)
+    orc.orc_landscape_frame(P(frame), 1280, 720, P(m.height), P(m.color), P(m.fog), C.byref(oc), 8)
+    lut = np.zeros(2049, np.float32)
+    orc.orc_cos_lut(P(lut))
+    ref = np.zeros_like(frame)
+    # Landscape_Draw passes (WarpSpeed, WarpStrength) as (strength, speed)
+    orc.orc_tape_warp32(P(ref), P(frame), 1280, 720, 6.5, 0.02, P(lut))
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("inverse", [0, 1])
+def test_polar_blit(orc, tmp_path, inverse):
+    src = rand_img(70, 1280 * 720)
+    inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
+    src.tofile(inp)
+    run(["polar", inp, inverse, out])
+    m0, m1 = np.zeros(1280 * 720 * 2, np.int32), np.zeros(1280 * 720 * 2, np.int32)
+    orc.orc_polar_maps(P(m0), P(m1), 1280, 720, 1280, 720)
+    ref = np.zeros(1280 * 720, np.uint32)
+    orc.orc_polar_blit(P(ref), P(src), P(m1 if inverse else m0), 1280, 720, 1280, 8)
+    assert np.array_equal(np.fromfile(out, np.uint32), ref)
+
+
+def test_fx_blit(orc, tmp_path):
+    src = rand_img(71, 644 * 364)
+    inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
+    src.tofile(inp)
+    run(["fxblit", inp, out])
+    ref = np.zeros(1280 * 720, np.uint32)
+    orc.orc_fx_blit_2x2(P(ref), P(src), 1280, 720)
+    assert np.array_equal(np.fromfile(out, np.uint32), ref)
+
+
+@pytest.mark.parametrize("mode,inplace", [("h", 0), ("v", 1), ("2d", 0), ("2d", 1)])
+def test_blur07(orc, tmp_path, mode, inplace):
+    W, H, s = 640, 360, 23.5 / 255
+    src = rand_img(72, W * H)
+    inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
+    src.tofile(inp)
+    run(["blur07", inp, W, H, s, mode, inplace, out])
+    f = {"h": orc.orc_blur07_h, "v": orc.orc_blur07_v, "2d": orc.orc_blur07}[mode]
+    ref = src.copy() if inplace else np.zeros_like(src)
+    f(P(ref), P(ref if inplace else src), W, H, s, 8)
+    assert np.array_equal(np.fromfile(out, np.uint32), ref)
+
+
+def test_tape_warp(orc, tmp_path):
+    W, H = 1280, 720
+    src = rand_img(73, W * H)
+    inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
+    src.tofile(inp)
+    run(["tapewarp", inp, W, H, 9.0, 0.031, out])
+    lut = np.zeros(2049, np.float32)
+    orc.orc_cos_lut(P(lut))
+    ref = np.zeros_like(src)
+    orc.orc_tape_warp32(P(ref), P(src), W, H, 9.0, 0.031, P(lut))
+    assert np.array_equal(np.fromfile(out, np.uint32), ref)

@@ -260,3 +260,17 @@ def test_cos_lut(orc, refl):
     for x in xs:
         assert np.float32(refl.ref_lutcosf(float(x))) == np.float32(orc.orc_lutcosf(P(b), float(x)))
         assert np.float32(refl.ref_lutsinf(float(x))) == np.float32(orc.orc_lutsinf(P(b), float(x)))
+
+
+def test_tape_warp32(orc, refl):
+    W, H = 1280, 720
+    lut = np.zeros(2049, np.float32)
+    refl.ref_cos_lut(P(lut))
+    src = aligned_empty(W * H)
+    src[:] = np.random.default_rng(11).integers(0, 2**32, W * H, dtype=np.uint64).astype(np.uint32)
+    for strength, speed in [(3.0, 0.05), (12.5, 0.013), (0.0, 1.0), (40.0, -0.2)]:
+        a = aligned_empty(W * H)
+        b = np.zeros(W * H, np.uint32)
+        refl.ref_tape_warp32(P(a), P(src), W, H, strength, speed)
+        orc.orc_tape_warp32(P(b), P(src), W, H, strength, speed, P(lut))
+        assert np.array_equal(a, b), (strength, speed)
