@@ -34,10 +34,7 @@ struct ckd_graph {
     void *tables = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    // event-record nodes around the persistent SDF kernel, read by the
-    // context's kernel timing on every replay
-    hipEvent_t sdfBegin = nullptr, sdfEnd = nullptr;
-    bool timedBefore = false;
+    bool hasSdf = false;
 };
 
 namespace {
@@ -149,8 +146,7 @@ void make_plan(const ckd_ctx *ctx, const ckd_request *requests, unsigned n, uint
 // the launch sequence over tables staged at `dev`
 ckd_status launch_plan(ckd_ctx *ctx, const Plan &P, unsigned char *dev, const ckd_sdf_scene *scene,
                        int shading, const uint32_t *tex, unsigned texLog2,
-                       unsigned long long *stepCount, hipEvent_t sdfBegin = nullptr,
-                       hipEvent_t sdfEnd = nullptr)
+                       unsigned long long *stepCount)
 {
     ckd_status s = CKD_OK;
     const bool fork = P.nSdf > 0 && (P.nVox > 0 || P.nTun > 0);
@@ -159,12 +155,10 @@ ckd_status launch_plan(ckd_ctx *ctx, const Plan &P, unsigned char *dev, const ck
         CKD_HIP(hipStreamWaitEvent(ctx->aux, ctx->forkEv, 0));
     }
     if (P.nSdf > 0) {
-        if (sdfBegin) CKD_HIP(hipEventRecord(sdfBegin, ctx->stream));
         s = ckd::launch_sdf_persistent(ctx, reinterpret_cast<uint32_t *>(dev), P.chunks,
                                        reinterpret_cast<const SdfJob *>(dev + P.oSdf), P.nSdf,
                                        *scene, shading, stepCount);
         if (s != CKD_OK) return s;
-        if (sdfEnd) CKD_HIP(hipEventRecord(sdfEnd, ctx->stream));
     }
     {
         ckd::StreamSwap swap(ctx, fork ? ctx->aux : ctx->stream);
@@ -329,12 +323,7 @@ ckd_status ckd_batch_graph_create(ckd_ctx *ctx, const ckd_request *requests, uns
         ckd_graph_destroy(g);
         return ckd::fail(CKD_EDEVICE, "ckd_batch_graph_create: stream creation failed");
     }
-    if (P.nSdf > 0 && (hipEventCreate(&g->sdfBegin) != hipSuccess ||
-                       hipEventCreate(&g->sdfEnd) != hipSuccess)) {
-        (void)hipStreamDestroy(cap);
-        ckd_graph_destroy(g);
-        return ckd::fail(CKD_EDEVICE, "ckd_batch_graph_create: event creation failed");
-    }
+    g->hasSdf = P.nSdf > 0;
     const bool timing = ctx->timing;
     ctx->timing = false;
     hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
@@ -343,8 +332,7 @@ ckd_status ckd_batch_graph_create(ckd_ctx *ctx, const ckd_request *requests, uns
         unsigned char *dev = static_cast<unsigned char *>(g->tables);
         // the queue counter is reset by a kernel node on every replay
         k_reset_counter<<<1, 1, 0, cap>>>(reinterpret_cast<uint32_t *>(dev));
-        s = launch_plan(ctx, P, dev, scene, shading, tex, texLog2, stepCount, g->sdfBegin,
-                        g->sdfEnd);
+        s = launch_plan(ctx, P, dev, scene, shading, tex, texLog2, stepCount);
         const hipError_t e2 = hipStreamEndCapture(cap, &g->graph);
         if (s == CKD_OK && e2 != hipSuccess)
             s = ckd::fail(CKD_EDEVICE, "ckd_batch_graph_create: capture failed: %s",
@@ -369,29 +357,19 @@ ckd_status ckd_graph_launch(ckd_graph *g)
 {
     CKD_REQUIRE(g && g->exec, "ckd_graph_launch: null graph");
     ckd_ctx *ctx = g->ctx;
-    const bool timed = ctx->timing && g->sdfBegin;
-    // the graph's timing events are re-recorded by every replay: the
-    // previous replay's pair is read before they are reused
-    if (g->timedBefore) {
-        ckd_status s = ckd::drain_timing(ctx);
-        if (s != CKD_OK) return s;
-        g->timedBefore = false;
-    }
+    // with kernel timing on, a replay is timed as one persistent-SDF launch:
+    // events on the context stream around the whole graph, whose critical
+    // path is the SDF kernel (the counter reset and the fork/join add
+    // microseconds; the voxel and tunnel branch runs beside it)
+    ckd::KernelTimer kt(ctx, g->hasSdf ? CKD_KT_SDF_PERSISTENT : -1);
     CKD_HIP(hipGraphLaunch(g->exec, ctx->stream));
-    if (timed) {
-        ctx->ktPending[CKD_KT_SDF_PERSISTENT].push_back({g->sdfBegin, g->sdfEnd, false});
-        g->timedBefore = true;
-    }
     return CKD_OK;
 }
 
 void ckd_graph_destroy(ckd_graph *g)
 {
     if (!g) return;
-    if (g->timedBefore) (void)ckd::drain_timing(g->ctx);
     if (g->ctx) (void)hipStreamSynchronize(g->ctx->stream);
-    if (g->sdfBegin) (void)hipEventDestroy(g->sdfBegin);
-    if (g->sdfEnd) (void)hipEventDestroy(g->sdfEnd);
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
     if (g->tables) (void)hipFree(g->tables);

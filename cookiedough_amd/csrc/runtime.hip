@@ -55,7 +55,7 @@ ckd_status stage(ckd_ctx *ctx, const void *host, size_t bytes, void **dev)
 
 KernelTimer::KernelTimer(ckd_ctx *ctx, int kernel) : ctx_(ctx), kernel_(kernel)
 {
-    if (!ctx->timing) return;
+    if (!ctx->timing || kernel < 0 || kernel >= CKD_KT_COUNT) return;
     hipEvent_t e = nullptr;
     if (!ctx->eventPool.empty()) {
         e = ctx->eventPool.back();

@@ -279,7 +279,8 @@ CKD_API ckd_status ckd_autotune_batch(ckd_ctx *ctx, const ckd_request *requests,
 /* A batch captured once as a hipGraph (job tables staged in device memory
  * the graph owns, the chunk-queue reset as a kernel node, the voxel and
  * tunnel branch on the auxiliary stream) and replayed with ckd_graph_launch
- * on the context stream.  The request array, scene, texture and scratch
+ * on the context stream.  With kernel timing on, each replay counts as one
+ * CKD_KT_SDF_PERSISTENT launch timed around the whole graph.  The request array, scene, texture and scratch
  * must stay valid and unchanged while the graph lives; the launch shapes
  * are those of the context at capture time. */
 typedef struct ckd_graph ckd_graph;
