@@ -168,6 +168,29 @@ float orc_lutsinf(const float *lut, float angle);
 void  orc_tape_warp32(uint32_t *dst, const uint32_t *src, unsigned xRes, unsigned yRes,
                       float strength, float speed, const float *lut);
 
+/* ---- a17-a19 the reference's raymarched effects (code/shadertoy.cpp) ---- */
+enum { ORC_ST_PLASMA = 0, ORC_ST_NAUTILUS = 1, ORC_ST_SPIKEY_DISTANT = 2,
+       ORC_ST_SPIKEY_CLOSE = 3, ORC_ST_SINUSES = 4, ORC_ST_LAURA = 5,
+       ORC_ST_FDTUNNEL = 6 };
+/* One FX map ((resX/2+4) x (resY/2+4)) of an effect.  params: the Rocket
+ * track values in the order of the C ABI's parameter structs (time first);
+ * glow, tex and texHi are used by ORC_ST_FDTUNNEL only (1024x1024 ARGB).
+ * rsqrtMode 0 = 1/sqrtf, 1 = x86 rsqrtss (as the reference on this CPU).
+ * pinned: the compiled reference driven by its own Rocket player. */
+void orc_st_map(int effect, uint32_t *dst, uint32_t *glow, unsigned resX, unsigned resY,
+                const float *params, const float *lut, const uint32_t *tex,
+                const uint32_t *texHi, int rsqrtMode, int nthreads);
+/* the effect's Draw function into a resX x resY frame (0, or -1 when the
+ * close-up's mix-blur layer is asked for) */
+int orc_st_draw(int effect, uint32_t *dst, unsigned resX, unsigned resY, const float *params,
+                const float *lut, const uint32_t *tex, const uint32_t *texHi, int rsqrtMode,
+                int nthreads);
+/* one lane of exp_ps / log_ps (3rdparty/sse_mathfun.h) and ToPixel4 of one
+ * pixel (code/shadertoy-util.h:136-146) */
+float orc_log_ps(float x);
+float orc_exp_ps(float x);
+uint32_t orc_topixel(const float c[4]);
+
 #ifdef __cplusplus
 }
 #endif
