@@ -29,7 +29,7 @@ sys.path.insert(0, str(ROOT))
 
 from cookiedough_amd import workloads as wl  # noqa: E402
 from cookiedough_amd._lib import (CKD_FX_SDF, CKD_FX_TUNNEL, CKD_FX_VOXEL,  # noqa: E402
-                                  CKD_SHADE_FP16, CKD_SHADE_FP32)
+                                  CKD_SHADE_FP16, CKD_SHADE_FP32, Tuning)
 
 PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 vector (packed) peak
 PEAK_HBM_GBS = 8000.0
@@ -69,10 +69,15 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--requests", type=int, default=512)
+    ap.add_argument("--tuning", default=None,
+                    help="steps,chunk,blocksPerCU,vSeg: fixed launch shapes instead of autotuning")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the batched headline (for kernel profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(frames and requests are independent; see DESIGN.md for how the path shards)",
+              "(frames and requests are independent, so several GPUs would each take a share "
+              "of the request queue)",
               file=sys.stderr)
         return 2
 
@@ -113,8 +118,14 @@ def main() -> int:
     # launch shapes tuned on this batch (untimed), then the batch captured
     # once as a hipGraph; a step is one replay
     t_tune = time.time()
-    tuning = ctx.autotune_batch(reqs, scene8, CKD_SHADE_FP16, d_tex, 9, tmp, tmp_px).as_dict()
-    log(f"autotuned in {time.time() - t_tune:.1f}s: {tuning}")
+    if args.tuning:
+        s_, c_, b_, v_ = (int(x) for x in args.tuning.split(","))
+        ctx.set_tuning(Tuning(s_, c_, b_, v_))
+        tuning = ctx.get_tuning().as_dict()
+        log(f"fixed tuning: {tuning}")
+    else:
+        tuning = ctx.autotune_batch(reqs, scene8, CKD_SHADE_FP16, d_tex, 9, tmp, tmp_px).as_dict()
+        log(f"autotuned in {time.time() - t_tune:.1f}s: {tuning}")
     graph = ctx.batch_graph(reqs, scene8, CKD_SHADE_FP16, d_tex, 9, tmp, tmp_px, pcount)
     ctx.kernel_timing(True)
 
@@ -123,6 +134,7 @@ def main() -> int:
         psdf["px"] += sdf_px_batch
 
     # ------------------------------------------------------------- headline
+    st_zero = int(pcount.numpy()[0])
     for _ in range(args.warmup):
         batch_step()
     ctx.sync()
@@ -133,78 +145,11 @@ def main() -> int:
     value = total_px / (ms_step * 1e-3) / 1e6
     log(f"batch: {ms_step:.2f} ms/step, {value:.0f} Mpx/s")
 
-    # ----------------------------------------------------- per effect (c0-c2)
-    W, H = 1280, 720
-    per = {}
-
-    # SDF c0: 64 frames at t=k/60
-    cams = [wl.sdf_camera(scene8, k / 60) for k in range(64)]
-    fr_sdf = ctx.empty(64 * W * H)
-
-    def sdf_plain():
-        for k, c in enumerate(cams):
-            ctx.sdf_render(fr_sdf.view(k * W * H, (W * H,)), W, H, scene8, c, CKD_SHADE_FP32)
-
-    sdf_reqs = []
-    for k in range(64):
-        r = wl.make_request(wl.BatchItem(CKD_FX_SDF, W, H, k / 60), fr_sdf.ptr + 4 * k * W * H,
-                            scene8, path, None)
-        sdf_reqs.append(r)
-    sdf_reqs = ctx.request_array(sdf_reqs)
-
-    def sdf_batched():
-        ctx.render_batch(sdf_reqs, scene8, CKD_SHADE_FP16, None, 9, None, 0, pcount)
-        psdf["px"] += 64 * W * H
-
-    # voxel c1: 120 frames on the Lissajous path
-    lcams = [wl.landscape_cam(path, k / 60) for k in range(120)]
-    fr_vox = ctx.empty(120 * W * H)
-
-    def vox_plain():
-        for k, c in enumerate(lcams):
-            ctx.landscape_draw(fr_vox.view(k * W * H, (W * H,)), W, H, [c])
-
-    def vox_batched():
-        ctx.landscape_draw(fr_vox, W, H, lcams)
-
-    # tunnel c2: 240 frames, radius U{1..32}
-    radii = wl.tunnel_radii(0, 240)
-    tgrids = [ctx.upload(wl.tunnel_grid(W, H, k / 60)) for k in range(240)]
-    fr_tun = ctx.empty(240 * W * H)
-    t0 = ctx.empty(W * H)
-    tun_tmp = ctx.empty(240 * W * H)
-
-    def tun_plain():
-        for k in range(240):
-            ctx.tunnel_frame(fr_tun.view(k * W * H, (W * H,)), t0, W, H, tgrids[k], d_tex, 9,
-                             int(radii[k]))
-
-    tun_reqs = []
-    for k in range(240):
-        it = wl.BatchItem(CKD_FX_TUNNEL, W, H, k / 60, int(radii[k]))
-        tun_reqs.append(wl.make_request(it, fr_tun.ptr + 4 * k * W * H, scene8, path, tgrids[k].ptr))
-    tun_reqs = ctx.request_array(tun_reqs)
-
-    def tun_batched():
-        ctx.render_batch(tun_reqs, scene8, CKD_SHADE_FP16, d_tex, 9, tun_tmp, 240 * W * H)
-
-    for name, frames, plain, batched in [("sdf_c0", 64, sdf_plain, sdf_batched),
-                                         ("voxel_c1", 120, vox_plain, vox_batched),
-                                         ("tunnel_c2", 240, tun_plain, tun_batched)]:
-        plain()
-        batched()
-        mp = T.run(plain)
-        mb = T.run(batched)
-        px = frames * W * H
-        per[name] = {"frames": frames, "plain_fp32_mpx_s": round(px / mp / 1e3, 1),
-                     "batched_fp16_mpx_s": round(px / mb / 1e3, 1),
-                     "speedup": round(mp / mb, 3)}
-        log(name, per[name])
-
     # ------------------------------------------- roofline: persistent tracer
+    # every warmup and timed replay of the headline graph, and nothing else
     ms_k, launches = ctx.kernel_timing_read(1)  # CKD_KT_SDF_PERSISTENT
     ctx.kernel_timing(False)
-    steps_k = int(pcount.numpy()[0])
+    steps_k = int(pcount.numpy()[0]) - st_zero
     flops_k = sdf_flops(steps_k, psdf["px"], 8)
     achieved = flops_k / (ms_k * 1e-3) / 1e12
     roofline = {"bound": "valu", "kernel": "k_sdf_persistent<fp16 shading, 8 spheres>",
@@ -215,89 +160,156 @@ def main() -> int:
                                "F_sdf = 11*8+2 = 90 FLOP"}
     log("roofline", roofline)
 
-    # --------------------------------------- c3 precision form (2560x1440)
-    W3, H3 = 2560, 1440
-    n3 = 64
-    fr3 = ctx.empty(n3 * W3 * H3)
-    cams3 = [wl.sdf_camera(scene32, k / 60) for k in range(n3)]
+    per, es, cpu = {}, None, None
+    if not args.headline_only:
+        # ----------------------------------------------------- per effect (c0-c2)
+        W, H = 1280, 720
 
-    def c3_plain():
-        for k, c in enumerate(cams3):
-            ctx.sdf_render(fr3.view(k * W3 * H3, (W3 * H3,)), W3, H3, scene32, c, CKD_SHADE_FP32)
+        # SDF c0: 64 frames at t=k/60
+        cams = [wl.sdf_camera(scene8, k / 60) for k in range(64)]
+        fr_sdf = ctx.empty(64 * W * H)
 
-    reqs3 = [wl.make_request(wl.BatchItem(CKD_FX_SDF, W3, H3, k / 60), fr3.ptr + 4 * k * W3 * H3,
-                             scene32, path, None) for k in range(n3)]
-    reqs3 = ctx.request_array(reqs3)
+        def sdf_plain():
+            for k, c in enumerate(cams):
+                ctx.sdf_render(fr_sdf.view(k * W * H, (W * H,)), W, H, scene8, c, CKD_SHADE_FP32)
 
-    def c3_batched():
-        ctx.render_batch(reqs3, scene32, CKD_SHADE_FP16, None, 9, None, 0)
+        sdf_reqs = []
+        for k in range(64):
+            r = wl.make_request(wl.BatchItem(CKD_FX_SDF, W, H, k / 60), fr_sdf.ptr + 4 * k * W * H,
+                                scene8, path, None)
+            sdf_reqs.append(r)
+        sdf_reqs = ctx.request_array(sdf_reqs)
 
-    c3_plain()
-    c3_batched()
-    mp, mb = T.run(c3_plain), T.run(c3_batched)
-    px3 = n3 * W3 * H3
-    per["sdf_c3_precision"] = {"frames": n3, "plain_fp32_mpx_s": round(px3 / mp / 1e3, 1),
-                               "batched_fp16_mpx_s": round(px3 / mb / 1e3, 1),
-                               "speedup": round(mp / mb, 3)}
-    log("sdf_c3", per["sdf_c3_precision"])
+        def sdf_batched():
+            ctx.render_batch(sdf_reqs, scene8, CKD_SHADE_FP16, None, 9, None, 0, pcount)
+            psdf["px"] += 64 * W * H
 
-    f3 = fr3
+        # voxel c1: 120 frames on the Lissajous path
+        lcams = [wl.landscape_cam(path, k / 60) for k in range(120)]
+        fr_vox = ctx.empty(120 * W * H)
 
-    # The CPU leg is the only part of the bench that runs the oracle: the
-    # timed CPU baseline and the FP64 reference render the precision form's
-    # error is measured against.
-    cpu = None
-    es = None
-    if not args.no_cpu:
-        import oracles  # noqa: E402  (the checker)
-        from helpers import error_stats  # noqa: E402
-        orc = oracles.oracle()
-        nthr = min(16, os.cpu_count() or 1)
-        # precision form: FP16 shading vs the FP64 render on a row sample
-        errs = []
+        def vox_plain():
+            for k, c in enumerate(lcams):
+                ctx.landscape_draw(fr_vox.view(k * W * H, (W * H,)), W, H, [c])
+
+        def vox_batched():
+            ctx.landscape_draw(fr_vox, W, H, lcams)
+
+        # tunnel c2: 240 frames, radius U{1..32}
+        radii = wl.tunnel_radii(0, 240)
+        tgrids = [ctx.upload(wl.tunnel_grid(W, H, k / 60)) for k in range(240)]
+        fr_tun = ctx.empty(240 * W * H)
+        t0 = ctx.empty(W * H)
+        tun_tmp = ctx.empty(240 * W * H)
+
+        def tun_plain():
+            for k in range(240):
+                ctx.tunnel_frame(fr_tun.view(k * W * H, (W * H,)), t0, W, H, tgrids[k], d_tex, 9,
+                                 int(radii[k]))
+
+        tun_reqs = []
+        for k in range(240):
+            it = wl.BatchItem(CKD_FX_TUNNEL, W, H, k / 60, int(radii[k]))
+            tun_reqs.append(wl.make_request(it, fr_tun.ptr + 4 * k * W * H, scene8, path, tgrids[k].ptr))
+        tun_reqs = ctx.request_array(tun_reqs)
+
+        def tun_batched():
+            ctx.render_batch(tun_reqs, scene8, CKD_SHADE_FP16, d_tex, 9, tun_tmp, 240 * W * H)
+
+        for name, frames, plain, batched in [("sdf_c0", 64, sdf_plain, sdf_batched),
+                                             ("voxel_c1", 120, vox_plain, vox_batched),
+                                             ("tunnel_c2", 240, tun_plain, tun_batched)]:
+            plain()
+            batched()
+            mp = T.run(plain)
+            mb = T.run(batched)
+            px = frames * W * H
+            per[name] = {"frames": frames, "plain_fp32_mpx_s": round(px / mp / 1e3, 1),
+                         "batched_fp16_mpx_s": round(px / mb / 1e3, 1),
+                         "speedup": round(mp / mb, 3)}
+            log(name, per[name])
+
+        # --------------------------------------- c3 precision form (2560x1440)
+        W3, H3 = 2560, 1440
+        n3 = 64
+        fr3 = ctx.empty(n3 * W3 * H3)
+        cams3 = [wl.sdf_camera(scene32, k / 60) for k in range(n3)]
+
+        def c3_plain():
+            for k, c in enumerate(cams3):
+                ctx.sdf_render(fr3.view(k * W3 * H3, (W3 * H3,)), W3, H3, scene32, c, CKD_SHADE_FP32)
+
+        reqs3 = [wl.make_request(wl.BatchItem(CKD_FX_SDF, W3, H3, k / 60), fr3.ptr + 4 * k * W3 * H3,
+                                 scene32, path, None) for k in range(n3)]
+        reqs3 = ctx.request_array(reqs3)
+
+        def c3_batched():
+            ctx.render_batch(reqs3, scene32, CKD_SHADE_FP16, None, 9, None, 0)
+
+        c3_plain()
         c3_batched()
-        for k in [0, 63]:
-            img = fr3.view(k * W3 * H3, (W3 * H3,)).numpy().reshape(H3, W3)
-            rows = np.arange(7, H3, 36)
-            ref = np.zeros(len(rows) * W3, np.uint32)
-            orc.orc_sdf_rows(oracles.P(ref), W3, H3, C.byref(scene32), k / 60, 64, 7, 36, None, nthr)
-            errs.append((img[rows].reshape(-1), ref))
-        es = error_stats(np.concatenate([e[0] for e in errs]), np.concatenate([e[1] for e in errs]))
-        log("precision", es)
-        # bounded sample of the batched workload: its requests in order, as
-        # many as fit in about 10 s of CPU time, rendered by the oracle
-        def cpu_render(it, nthreads):
-            n = it.W * it.H
-            buf = np.zeros(n, np.uint32)
-            if it.effect == CKD_FX_SDF:
-                orc.orc_sdf_frame(oracles.P(buf), it.W, it.H, C.byref(scene8), it.t, 32, None, nthreads)
-            elif it.effect == CKD_FX_VOXEL:
-                oc = oracles.orc_cam(wl.landscape_cam(path, it.t))
-                orc.orc_landscape_frame(oracles.P(buf), it.W, it.H, oracles.P(maps.height),
-                                        oracles.P(maps.color), oracles.P(maps.fog), C.byref(oc),
-                                        nthreads)
-            else:
-                g = wl.tunnel_grid(it.W, it.H, it.t)
-                b0, b1 = np.zeros(n, np.uint32), np.zeros(n, np.uint32)
-                orc.orc_tunnel_frame(oracles.P(buf), oracles.P(b0), oracles.P(b1), it.W, it.H,
-                                     oracles.P(g), oracles.P(tex), 9, it.radius, nthreads)
-            return n
+        mp, mb = T.run(c3_plain), T.run(c3_batched)
+        px3 = n3 * W3 * H3
+        per["sdf_c3_precision"] = {"frames": n3, "plain_fp32_mpx_s": round(px3 / mp / 1e3, 1),
+                                   "batched_fp16_mpx_s": round(px3 / mb / 1e3, 1),
+                                   "speedup": round(mp / mb, 3)}
+        log("sdf_c3", per["sdf_c3_precision"])
 
-        def cpu_rate(nthreads, budget_s):
-            px, k, t_c = 0, 0, time.perf_counter()
-            while k < len(items) and time.perf_counter() - t_c < budget_s:
-                px += cpu_render(items[k], nthreads)
-                k += 1
-            return px, k, time.perf_counter() - t_c
+        f3 = fr3
 
-        px, k, dt = cpu_rate(nthr, 10.0)
-        px1, k1, dt1 = cpu_rate(1, 5.0)
-        cpu = {"value": round(px / dt / 1e6, 2), "unit": "Mpixel/s", "cores": nthr, "kind": "port",
-               "sample": f"first {k} of the {len(items)} batch requests ({px / 1e6:.1f} Mpx) "
-                         f"rendered by the oracle (C, OpenMP) in {dt:.1f}s",
-               "value_1_thread": round(px1 / dt1 / 1e6, 2),
-               "sample_1_thread": f"first {k1} requests in {dt1:.1f}s"}
-        log("cpu", cpu)
+        # The CPU leg is the only part of the bench that runs the oracle: the
+        # timed CPU baseline and the FP64 reference render the precision form's
+        # error is measured against.
+        if not args.no_cpu:
+            import oracles  # noqa: E402  (the checker)
+            from helpers import error_stats  # noqa: E402
+            orc = oracles.oracle()
+            nthr = min(16, os.cpu_count() or 1)
+            # precision form: FP16 shading vs the FP64 render on a row sample
+            errs = []
+            c3_batched()
+            for k in [0, 63]:
+                img = fr3.view(k * W3 * H3, (W3 * H3,)).numpy().reshape(H3, W3)
+                rows = np.arange(7, H3, 36)
+                ref = np.zeros(len(rows) * W3, np.uint32)
+                orc.orc_sdf_rows(oracles.P(ref), W3, H3, C.byref(scene32), k / 60, 64, 7, 36, None, nthr)
+                errs.append((img[rows].reshape(-1), ref))
+            es = error_stats(np.concatenate([e[0] for e in errs]), np.concatenate([e[1] for e in errs]))
+            log("precision", es)
+            # bounded sample of the batched workload: its requests in order, as
+            # many as fit in about 10 s of CPU time, rendered by the oracle
+            def cpu_render(it, nthreads):
+                n = it.W * it.H
+                buf = np.zeros(n, np.uint32)
+                if it.effect == CKD_FX_SDF:
+                    orc.orc_sdf_frame(oracles.P(buf), it.W, it.H, C.byref(scene8), it.t, 32, None, nthreads)
+                elif it.effect == CKD_FX_VOXEL:
+                    oc = oracles.orc_cam(wl.landscape_cam(path, it.t))
+                    orc.orc_landscape_frame(oracles.P(buf), it.W, it.H, oracles.P(maps.height),
+                                            oracles.P(maps.color), oracles.P(maps.fog), C.byref(oc),
+                                            nthreads)
+                else:
+                    g = wl.tunnel_grid(it.W, it.H, it.t)
+                    b0, b1 = np.zeros(n, np.uint32), np.zeros(n, np.uint32)
+                    orc.orc_tunnel_frame(oracles.P(buf), oracles.P(b0), oracles.P(b1), it.W, it.H,
+                                         oracles.P(g), oracles.P(tex), 9, it.radius, nthreads)
+                return n
+
+            def cpu_rate(nthreads, budget_s):
+                px, k, t_c = 0, 0, time.perf_counter()
+                while k < len(items) and time.perf_counter() - t_c < budget_s:
+                    px += cpu_render(items[k], nthreads)
+                    k += 1
+                return px, k, time.perf_counter() - t_c
+
+            px, k, dt = cpu_rate(nthr, 10.0)
+            px1, k1, dt1 = cpu_rate(1, 5.0)
+            cpu = {"value": round(px / dt / 1e6, 2), "unit": "Mpixel/s", "cores": nthr, "kind": "port",
+                   "sample": f"first {k} of the {len(items)} batch requests ({px / 1e6:.1f} Mpx) "
+                             f"rendered by the oracle (C, OpenMP) in {dt:.1f}s",
+                   "value_1_thread": round(px1 / dt1 / 1e6, 2),
+                   "sample_1_thread": f"first {k1} requests in {dt1:.1f}s"}
+            log("cpu", cpu)
 
     result = {
         "metric": "Mpixel/s per effect at <=2/255 err: batched FP16 vs plain FP32, 1 MI355X",

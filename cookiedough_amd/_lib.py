@@ -16,7 +16,7 @@ CKD_OK, CKD_EINVAL, CKD_ENOMEM, CKD_EDEVICE, CKD_ESTATE = range(5)
 CKD_SHADE_FP32, CKD_SHADE_FP16 = 0, 1
 CKD_FX_SDF, CKD_FX_VOXEL, CKD_FX_TUNNEL = 0, 1, 2
 (CKD_KT_SDF_PLAIN, CKD_KT_SDF_PERSISTENT, CKD_KT_LANDSCAPE, CKD_KT_TUNNEL_H,
- CKD_KT_TUNNEL_V) = range(5)
+ CKD_KT_TUNNEL_V, CKD_KT_SHADERTOY) = range(6)
 
 
 class CkdError(RuntimeError):
@@ -48,6 +48,56 @@ class Tuning(C.Structure):
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
 
+
+
+def _params(name: str, tracks: list) -> type:
+    """ctypes struct of one frame's parameters: `time`, then one float per
+    Rocket track (field name, track name)."""
+    fields = [("time", C.c_float)] + [(f, C.c_float) for f, _ in tracks]
+    cls = type(name, (C.Structure,), {"_fields_": fields, "TRACKS": dict(tracks)})
+    cls.as_array = lambda self: np.frombuffer(bytes(self), np.float32).copy()
+    return cls
+
+
+# the parameter structs of the raymarched effects (include/ckd.h), with the
+# track each field is read from in code/shadertoy.cpp
+PlasmaParams = _params("PlasmaParams", [
+    ("speed", "plasma:Speed"), ("hue", "plasma:Hue"), ("gamma", "plasma:Gamma"),
+    ("desaturation", "plasma:Desaturation")])
+NautilusParams = _params("NautilusParams", [
+    ("roll", "nautilus:Roll"), ("hue", "nautilus:Hue"), ("speed", "nautilus:Speed"),
+    ("desaturation", "nautilus:Desat"), ("blur", "nautilus:Blur")])
+SpikeyParams = _params("SpikeyParams", [
+    ("speed", "spike:Speed"), ("roll", "spike:Roll"), ("specular", "spike:Specular"),
+    ("desaturation", "spike:Desaturation"), ("hue", "spike:Hue"), ("gamma", "spike:Gamma"),
+    ("distXOffs", "distSpike:xOffs"), ("distYOffs", "distSpike:yOffs"),
+    ("distZOffs", "distSpike:zOffs"), ("distWarmup", "distSpike:Warmup"),
+    ("closeXOffs", "closeSpike:xOffs"), ("closeYOffs", "closeSpike:yOffs"),
+    ("closeZOffs", "closeSpike:zOffs"), ("closeZOffsScale", "closeSpike:zOffsScale"),
+    ("closeNormalGrain", "closeSpike:NormalGrain"), ("closeScale", "closeSpike:Scale"),
+    ("closeRim", "closeSpike:Rim"), ("closeAspectMul", "closeSpike:AspectMul"),
+    ("closeMixBlurMap", "closeSpike:MixBlurMap"), ("closeMixBlur", "closeSpike:MixBlur"),
+    ("closeMixMapBlur", "closeSpike:MixMapBlur"),
+    ("closeMixBlurOpacity", "closeSpike:MixBlurOpacity")])
+SinusesParams = _params("SinusesParams", [
+    ("specular", "sinusesTunnel:Specular"), ("roll", "sinusesTunnel:Roll"),
+    ("speed", "sinusesTunnel:Speed"), ("offsX", "sinusesTunnel:OffsX"),
+    ("gamma", "sinusesTunnel:Gamma"), ("hue", "sinusesTunnel:Hue"),
+    ("desaturation", "sinusesTunnel:Desaturation")])
+LauraParams = _params("LauraParams", [
+    ("speed", "laura:Speed"), ("yaw", "laura:Yaw"), ("pitch", "laura:Pitch"),
+    ("roll", "laura:Roll"), ("hue", "laura:Hue"), ("saturate", "laura:Saturate")])
+FdTunnelParams = _params("FdTunnelParams", [
+    ("boxy", "tunnel:Boxy"), ("flowerScale", "tunnel:FlowerScale"),
+    ("flowerFreq", "tunnel:FlowerFreq"), ("flowerPhase", "tunnel:FlowerPhase"),
+    ("speed", "tunnel:Speed"), ("roll", "tunnel:Roll"), ("pitch", "tunnel:Pitch"),
+    ("radius", "tunnel:Radius"), ("mulU", "tunnel:MulU"), ("mulV", "tunnel:MulV"),
+    ("litTiles", "tunnel:LitTiles"), ("litBlur", "tunnel:LitBlur"), ("fog1", "tunnel:Fog1"),
+    ("fog2", "tunnel:Fog2")])
+
+# effect name -> parameter struct; "spikey" takes a close flag
+ST_PARAMS = {"plasma": PlasmaParams, "nautilus": NautilusParams, "spikey": SpikeyParams,
+             "sinuses": SinusesParams, "laura": LauraParams, "fdtunnel": FdTunnelParams}
 
 class Request(C.Structure):
     _fields_ = [("effect", C.c_int), ("W", C.c_uint), ("H", C.c_uint),
@@ -107,6 +157,19 @@ SIGNATURES = {
                                C.c_size_t, _p, C.POINTER(_p)],
     "ckd_graph_launch": [_p],
     "ckd_graph_destroy": [_p],
+    "ckd_fdtunnel_create": [_p, _p, _p],
+    "ckd_fdtunnel_map": [_p, _p, _p, _u, _u, C.POINTER(FdTunnelParams), _u],
+    "ckd_fdtunnel_draw": [_p, _p, _u, _u, C.POINTER(FdTunnelParams), _u],
+    "ckd_spikey_map": [_p, _p, _u, _u, C.POINTER(SpikeyParams), _u, _i],
+    "ckd_spikey_draw": [_p, _p, _u, _u, C.POINTER(SpikeyParams), _u, _i],
+    "ckd_plasma_map": [_p, _p, _u, _u, C.POINTER(PlasmaParams), _u],
+    "ckd_plasma_draw": [_p, _p, _u, _u, C.POINTER(PlasmaParams), _u],
+    "ckd_nautilus_map": [_p, _p, _u, _u, C.POINTER(NautilusParams), _u],
+    "ckd_nautilus_draw": [_p, _p, _u, _u, C.POINTER(NautilusParams), _u],
+    "ckd_sinuses_map": [_p, _p, _u, _u, C.POINTER(SinusesParams), _u],
+    "ckd_sinuses_draw": [_p, _p, _u, _u, C.POINTER(SinusesParams), _u],
+    "ckd_laura_map": [_p, _p, _u, _u, C.POINTER(LauraParams), _u],
+    "ckd_laura_draw": [_p, _p, _u, _u, C.POINTER(LauraParams), _u],
 }
 _RESTYPES = {"ckd_destroy": None, "ckd_last_error": C.c_char_p, "ckd_graph_destroy": None}
 

@@ -24,8 +24,16 @@ struct ckd_ctx {
     hipStream_t aux = nullptr;
     hipEvent_t forkEv = nullptr, joinEv = nullptr;
 
-    // interpolated cosine LUT, 2049 floats (CalculateCosLUT)
+    // interpolated cosine LUT, 2049 floats (CalculateCosLUT), on the
+    // device and the host copy the per-frame set-up reads
     float *cosLut = nullptr;
+    float hostLut[2049] = {};
+
+    // the free-directional tunnel's two 1024x1024 textures
+    // (Shadertoy_Create) and the FX-map scratch of the effects' Draw calls
+    uint32_t *fdTex = nullptr, *fdTexHi = nullptr;
+    uint32_t *stScratch = nullptr;
+    size_t stScratchCap = 0;
 
     // landscape maps (Landscape_Create)
     uint8_t *lsHeight = nullptr;

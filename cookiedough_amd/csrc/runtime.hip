@@ -174,6 +174,7 @@ ckd_status ckd_create(int device, ckd_ctx **out)
         float lut[2049];
         for (unsigned i = 0; i < 2048; ++i) lut[i] = cosf((float)i * (k2PI / 2048.f));
         lut[2048] = lut[0];
+        for (unsigned i = 0; i < 2049; ++i) ctx->hostLut[i] = lut[i];
         if (hipMalloc(&ctx->cosLut, sizeof(lut)) != hipSuccess ||
             hipMemcpy(ctx->cosLut, lut, sizeof(lut), hipMemcpyHostToDevice) != hipSuccess) {
             ckd_destroy(ctx);
@@ -209,7 +210,8 @@ void ckd_destroy(ckd_ctx *ctx)
     if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
     void *bufs[] = {ctx->cosLut, ctx->lsHeight, ctx->lsColor, ctx->lsFog, ctx->tsHeight, ctx->tsColor,
                     ctx->tsFog, ctx->polarMap, ctx->polarInv, ctx->b26Scratch[0],
-                    ctx->b26Scratch[1], ctx->staging};
+                    ctx->b26Scratch[1], ctx->staging, ctx->fdTex, ctx->fdTexHi,
+                    ctx->stScratch};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete ctx;

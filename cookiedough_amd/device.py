@@ -155,6 +155,43 @@ class Context:
         fog = np.ascontiguousarray(fog, np.uint32)
         check(self.lib.ckd_landscape_create(self.h, hmap.ctypes.data, cmap.ctypes.data, fog.ctypes.data))
 
+    # ---- the reference's raymarched effects (code/shadertoy.cpp)
+    def fdtunnel_create(self, tex, tex_highlights):
+        tex = np.ascontiguousarray(tex, np.uint32)
+        hi = np.ascontiguousarray(tex_highlights, np.uint32)
+        if tex.size != 1024 * 1024 or hi.size != 1024 * 1024:
+            raise ValueError("fdtunnel textures are 1024x1024")
+        check(self.lib.ckd_fdtunnel_create(self.h, tex.ctypes.data, hi.ctypes.data))
+
+    @staticmethod
+    def _frames(effect, frames):
+        cls = _lib.ST_PARAMS[effect]
+        if isinstance(frames, C.Array):
+            return frames, len(frames)
+        if isinstance(frames, cls):
+            frames = [frames]
+        return (cls * len(frames))(*frames), len(frames)
+
+    def st_map(self, effect, maps, resX, resY, frames, glow=None, close=False):
+        """FX maps of nFrames frames ((resX/2+4) x (resY/2+4) each)"""
+        arr, n = self._frames(effect, frames)
+        fn = getattr(self.lib, f"ckd_{effect}_map")
+        if effect == "spikey":
+            check(fn(self.h, ptr(maps), resX, resY, arr, n, int(bool(close))))
+        elif effect == "fdtunnel":
+            check(fn(self.h, ptr(maps), ptr(glow), resX, resY, arr, n))
+        else:
+            check(fn(self.h, ptr(maps), resX, resY, arr, n))
+
+    def st_draw(self, effect, dst, resX, resY, frames, close=False):
+        """the effect's Draw function into nFrames resX x resY frames"""
+        arr, n = self._frames(effect, frames)
+        fn = getattr(self.lib, f"ckd_{effect}_draw")
+        if effect == "spikey":
+            check(fn(self.h, ptr(dst), resX, resY, arr, n, int(bool(close))))
+        else:
+            check(fn(self.h, ptr(dst), resX, resY, arr, n))
+
     def landscape_draw(self, dst, W, H, cams):
         arr = (_lib.LandscapeCam * len(cams))(*cams)
         check(self.lib.ckd_landscape_draw(self.h, ptr(dst), W, H, arr, len(cams)))

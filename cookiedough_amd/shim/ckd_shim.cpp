@@ -116,6 +116,68 @@ float clampf(float lo, float hi, float v) { return std::fmax(lo, std::fmin(hi, v
 bool s_tsReady = false;
 SyncTrack trackStarsStepU, trackStarsStepV, trackStarsSpeed, trackStarsBlur;
 
+// ---- raymarched effects (code/shadertoy.cpp:25-172) ----------------------
+// Each effect's tracks in the order of its ckd_*_params fields after
+// `time`; a Draw call reads them into the struct.
+const char *const kPlasmaTracks[] = {"plasma:Speed", "plasma:Hue", "plasma:Gamma",
+                                     "plasma:Desaturation"};
+const char *const kNautilusTracks[] = {"nautilus:Roll", "nautilus:Hue", "nautilus:Speed",
+                                       "nautilus:Desat", "nautilus:Blur"};
+const char *const kSpikeyTracks[] = {
+    "spike:Speed", "spike:Roll", "spike:Specular", "spike:Desaturation", "spike:Hue",
+    "spike:Gamma", "distSpike:xOffs", "distSpike:yOffs", "distSpike:zOffs", "distSpike:Warmup",
+    "closeSpike:xOffs", "closeSpike:yOffs", "closeSpike:zOffs", "closeSpike:zOffsScale",
+    "closeSpike:NormalGrain", "closeSpike:Scale", "closeSpike:Rim", "closeSpike:AspectMul",
+    "closeSpike:MixBlurMap", "closeSpike:MixBlur", "closeSpike:MixMapBlur",
+    "closeSpike:MixBlurOpacity"};
+const char *const kSinusesTracks[] = {"sinusesTunnel:Specular", "sinusesTunnel:Roll",
+                                      "sinusesTunnel:Speed", "sinusesTunnel:OffsX",
+                                      "sinusesTunnel:Gamma", "sinusesTunnel:Hue",
+                                      "sinusesTunnel:Desaturation"};
+const char *const kLauraTracks[] = {"laura:Speed", "laura:Yaw", "laura:Pitch",
+                                    "laura:Roll", "laura:Hue", "laura:Saturate"};
+const char *const kTunnelTracks[] = {
+    "tunnel:Boxy", "tunnel:FlowerScale", "tunnel:FlowerFreq", "tunnel:FlowerPhase",
+    "tunnel:Speed", "tunnel:Roll", "tunnel:Pitch", "tunnel:Radius", "tunnel:MulU",
+    "tunnel:MulV", "tunnel:LitTiles", "tunnel:LitBlur", "tunnel:Fog1", "tunnel:Fog2"};
+
+template <size_t N> struct StTracks {
+    SyncTrack t[N] = {};
+    void add(const char *const (&names)[N])
+    {
+        for (size_t i = 0; i < N; ++i) t[i] = Rocket::AddTrack(names[i]);
+    }
+    // time, then the track values, into a ckd_*_params struct of floats
+    template <typename P> P read(float time) const
+    {
+        static_assert(sizeof(P) == (N + 1) * sizeof(float), "parameter struct layout");
+        float v[N + 1];
+        v[0] = time;
+        for (size_t i = 0; i < N; ++i) v[i + 1] = (float)Rocket::get(t[i]);
+        P p;
+        memcpy(&p, v, sizeof(p));
+        return p;
+    }
+};
+
+bool s_stReady = false;
+StTracks<4> s_plasmaTracks;
+StTracks<5> s_nautilusTracks;
+StTracks<22> s_spikeyTracks;
+StTracks<7> s_sinusesTracks;
+StTracks<6> s_lauraTracks;
+StTracks<14> s_tunnelTracks;
+uint32_t *s_spikeBlurMaps[2] = {nullptr, nullptr}; // loaded as the reference does
+
+// renders one frame on the GPU into the staging image and downloads it
+template <typename F> void st_draw(const char *name, uint32_t *pDest, F render)
+{
+    if (!s_stReady) return;
+    const size_t n = (size_t)kResX * kResY;
+    uint32_t *a = g_a.get(n);
+    if (!a || render(a) != CKD_OK || !down(pDest, a, n)) report(name);
+}
+
 } // namespace
 
 // ===================================================================== parts
@@ -254,6 +316,89 @@ void Tunnelscape_Draw(uint32_t *pDest, float time, float delta)
         }
     }
     if (!down(pDest, b, n)) report("Tunnelscape_Draw");
+}
+
+bool Shadertoy_Create()
+{
+    if (!acquire()) return false;
+    s_plasmaTracks.add(kPlasmaTracks);
+    s_nautilusTracks.add(kNautilusTracks);
+    s_spikeyTracks.add(kSpikeyTracks);
+    s_sinusesTracks.add(kSinusesTracks);
+    s_lauraTracks.add(kLauraTracks);
+    s_tunnelTracks.add(kTunnelTracks);
+    // the free-directional tunnel's 1024x1024 textures and the close-up's
+    // blur maps (code/shadertoy.cpp:174-185)
+    const uint32_t *tex = Image_Load32("assets/shadertoy/nytrik-hextexture.png");
+    const uint32_t *texHi = Image_Load32("assets/shadertoy/nytrik-hextexture-fx.png");
+    if (!tex || !texHi) return false;
+    s_spikeBlurMaps[0] = Image_Load32("assets/shadertoy/close-up-blur-map-1.png");
+    s_spikeBlurMaps[1] = Image_Load32("assets/shadertoy/close-up-blur-map-2.png");
+    if (!s_spikeBlurMaps[0] || !s_spikeBlurMaps[1]) return false;
+    if (ckd_fdtunnel_create(g_ctx, tex, texHi) != CKD_OK) {
+        report("Shadertoy_Create");
+        return false;
+    }
+    s_stReady = true;
+    return true;
+}
+
+void Shadertoy_Destroy()
+{
+    if (s_stReady) release();
+    s_stReady = false;
+}
+
+void Plasma_Draw(uint32_t *pDest, float time, float delta)
+{
+    (void)delta;
+    const ckd_plasma_params p = s_plasmaTracks.read<ckd_plasma_params>(time);
+    st_draw("Plasma_Draw", pDest,
+            [&](uint32_t *d) { return ckd_plasma_draw(g_ctx, d, kResX, kResY, &p, 1); });
+}
+
+void Nautilus_Draw(uint32_t *pDest, float time, float delta)
+{
+    (void)delta;
+    const ckd_nautilus_params p = s_nautilusTracks.read<ckd_nautilus_params>(time);
+    st_draw("Nautilus_Draw", pDest,
+            [&](uint32_t *d) { return ckd_nautilus_draw(g_ctx, d, kResX, kResY, &p, 1); });
+}
+
+// The close-up's mix-blur layer (closeSpike:MixBlurOpacity > 0,
+// code/shadertoy.cpp:668-701) is not provided: such a call reports an
+// error through SetLastError and leaves pDest as it was.
+void Spikey_Draw(uint32_t *pDest, float time, float delta, bool close)
+{
+    (void)delta;
+    const ckd_spikey_params p = s_spikeyTracks.read<ckd_spikey_params>(time);
+    st_draw("Spikey_Draw", pDest, [&](uint32_t *d) {
+        return ckd_spikey_draw(g_ctx, d, kResX, kResY, &p, 1, close ? 1 : 0);
+    });
+}
+
+void Sinuses_Draw(uint32_t *pDest, float time, float delta)
+{
+    (void)delta;
+    const ckd_sinuses_params p = s_sinusesTracks.read<ckd_sinuses_params>(time);
+    st_draw("Sinuses_Draw", pDest,
+            [&](uint32_t *d) { return ckd_sinuses_draw(g_ctx, d, kResX, kResY, &p, 1); });
+}
+
+void Laura_Draw(uint32_t *pDest, float time, float delta)
+{
+    (void)delta;
+    const ckd_laura_params p = s_lauraTracks.read<ckd_laura_params>(time);
+    st_draw("Laura_Draw", pDest,
+            [&](uint32_t *d) { return ckd_laura_draw(g_ctx, d, kResX, kResY, &p, 1); });
+}
+
+void Tunnel_Draw(uint32_t *pDest, float time, float delta)
+{
+    (void)delta;
+    const ckd_fdtunnel_params p = s_tunnelTracks.read<ckd_fdtunnel_params>(time);
+    st_draw("Tunnel_Draw", pDest,
+            [&](uint32_t *d) { return ckd_fdtunnel_draw(g_ctx, d, kResX, kResY, &p, 1); });
 }
 
 bool Polar_Create()

@@ -2,8 +2,8 @@
 // implemented over the libckd C ABI (include/ckd.h).  Same names and
 // signatures as the reference headers, so code/demo.cpp links against this
 // shim in place of landscape.cpp, tunnelscape.cpp, polar.cpp,
-// fx-blitter.cpp, boxblur.cpp, deprecated/boxblur.cpp and the blits of
-// util.cpp.  Buffers are host memory, as in the reference (1280x720 frames,
+// fx-blitter.cpp, boxblur.cpp, deprecated/boxblur.cpp, shadertoy.cpp and
+// the blits of util.cpp.  Buffers are host memory, as in the reference (1280x720 frames,
 // 644x364 FX maps); every call uploads its inputs, renders on the GPU and
 // downloads the result before returning.
 #pragma once
@@ -50,6 +50,16 @@ void VerticalBoxBlur32(uint32_t *pDest, const uint32_t *pSrc, unsigned int xRes,
                        unsigned int yRes, float strength);
 void BoxBlur32(uint32_t *pDest, const uint32_t *pSrc, unsigned int xRes, unsigned int yRes,
                float strength);
+
+// code/shadertoy.h:5-15
+bool Shadertoy_Create();
+void Shadertoy_Destroy();
+void Nautilus_Draw(uint32_t *pDest, float time, float delta);
+void Spikey_Draw(uint32_t *pDest, float time, float delta, bool close = true);
+void Sinuses_Draw(uint32_t *pDest, float time, float delta);
+void Laura_Draw(uint32_t *pDest, float time, float delta);
+void Plasma_Draw(uint32_t *pDest, float time, float delta);
+void Tunnel_Draw(uint32_t *pDest, float time, float delta);
 
 // code/util.h:79,118-119,93
 void Add32(uint32_t *pDest, const uint32_t *pSrc, unsigned numPixels);

@@ -71,7 +71,8 @@ enum {
     CKD_KT_LANDSCAPE = 2,      /* ckd_landscape_draw and voxel requests    */
     CKD_KT_TUNNEL_H = 3,       /* tunnel remap + horizontal blur           */
     CKD_KT_TUNNEL_V = 4,       /* tunnel vertical blur + additive blit     */
-    CKD_KT_COUNT = 5
+    CKD_KT_SHADERTOY = 5,      /* FX maps of the raymarched effects        */
+    CKD_KT_COUNT = 6
 };
 CKD_API ckd_status ckd_kernel_timing(ckd_ctx *ctx, int enable);
 CKD_API ckd_status ckd_kernel_timing_read(ckd_ctx *ctx, int kernel,
@@ -293,6 +294,151 @@ CKD_API ckd_status ckd_batch_graph_create(ckd_ctx *ctx, const ckd_request *reque
                                           ckd_graph **out);
 CKD_API ckd_status ckd_graph_launch(ckd_graph *graph);
 CKD_API void ckd_graph_destroy(ckd_graph *graph);
+
+/* ---- the reference's raymarched effects (code/shadertoy.cpp) ------------
+ *
+ * Each effect renders an FX map of (resX/2+4) x (resY/2+4) pixels for an
+ * output of resX x resY (code/fx-blitter.h:13-17; aspect resY/resX as
+ * kAspect, code/main.h:43-44), then the Draw form post-processes it and
+ * blits it 2x2 into the output as the reference's *_Draw does.  resX is a
+ * multiple of 8 and resY even, both 64..8192; 1280x720 gives the
+ * reference's 644x364 map.
+ *
+ * Parameters: one struct per frame (host memory, copied per call), holding
+ * the frame's `time` (the Draw functions' time argument) and the Rocket
+ * track values the reference reads, under the track's name.  All must be
+ * finite.  _map writes nFrames consecutive maps, _draw nFrames consecutive
+ * frames; both run asynchronously on the context stream.
+ *
+ * Results equal the reference's with two exceptions: vNorm4's
+ * _mm_rsqrt_ps estimate is a correctly rounded 1/sqrtf, and powf, expf
+ * and atan2f are the device library's (within 2 ulp of the host libm). */
+
+/* RenderPlasmaMap / Plasma_Draw (code/shadertoy.cpp:200-280) */
+typedef struct ckd_plasma_params {
+    float time;
+    float speed;        /* plasma:Speed        */
+    float hue;          /* plasma:Hue          */
+    float gamma;        /* plasma:Gamma        */
+    float desaturation; /* plasma:Desaturation */
+} ckd_plasma_params;
+CKD_API ckd_status ckd_plasma_map(ckd_ctx *ctx, uint32_t *maps, unsigned resX, unsigned resY,
+                                  const ckd_plasma_params *frames, unsigned nFrames);
+CKD_API ckd_status ckd_plasma_draw(ckd_ctx *ctx, uint32_t *dst, unsigned resX, unsigned resY,
+                                   const ckd_plasma_params *frames, unsigned nFrames);
+
+/* RenderNautilusMap_2x2 / Nautilus_Draw (code/shadertoy.cpp:288-396); the
+ * Draw form box-blurs the output in place when BoxBlurScale(blur) != 0 */
+typedef struct ckd_nautilus_params {
+    float time;
+    float roll;         /* nautilus:Roll  */
+    float hue;          /* nautilus:Hue   */
+    float speed;        /* nautilus:Speed */
+    float desaturation; /* nautilus:Desat */
+    float blur;         /* nautilus:Blur  */
+} ckd_nautilus_params;
+CKD_API ckd_status ckd_nautilus_map(ckd_ctx *ctx, uint32_t *maps, unsigned resX, unsigned resY,
+                                    const ckd_nautilus_params *frames, unsigned nFrames);
+CKD_API ckd_status ckd_nautilus_draw(ckd_ctx *ctx, uint32_t *dst, unsigned resX, unsigned resY,
+                                     const ckd_nautilus_params *frames, unsigned nFrames);
+
+/* Spikey_Draw (code/shadertoy.cpp:661-727) and its renderers:
+ * close = 0: RenderSpikeyMap_2x2_Distant (:525-598), or when distWarmup is
+ *   non-zero _Distant_SpecularOnly (:600-659) followed by the in-place
+ *   horizontal box blur of the map;
+ * close = 1: RenderSpikeyMap_2x2_Close (:432-523).  Its mix-blur layer
+ *   (closeMixBlurOpacity > 0) blends the reference's blur-map images and is
+ *   not provided: ckd_spikey_draw returns CKD_EINVAL for it. */
+typedef struct ckd_spikey_params {
+    float time;
+    float speed;               /* spike:Speed               */
+    float roll;                /* spike:Roll                */
+    float specular;            /* spike:Specular            */
+    float desaturation;        /* spike:Desaturation        */
+    float hue;                 /* spike:Hue                 */
+    float gamma;               /* spike:Gamma               */
+    float distXOffs;           /* distSpike:xOffs           */
+    float distYOffs;           /* distSpike:yOffs           */
+    float distZOffs;           /* distSpike:zOffs           */
+    float distWarmup;          /* distSpike:Warmup          */
+    float closeXOffs;          /* closeSpike:xOffs          */
+    float closeYOffs;          /* closeSpike:yOffs          */
+    float closeZOffs;          /* closeSpike:zOffs          */
+    float closeZOffsScale;     /* closeSpike:zOffsScale     */
+    float closeNormalGrain;    /* closeSpike:NormalGrain    */
+    float closeScale;          /* closeSpike:Scale          */
+    float closeRim;            /* closeSpike:Rim            */
+    float closeAspectMul;      /* closeSpike:AspectMul      */
+    float closeMixBlurMap;     /* closeSpike:MixBlurMap     */
+    float closeMixBlur;        /* closeSpike:MixBlur        */
+    float closeMixMapBlur;     /* closeSpike:MixMapBlur     */
+    float closeMixBlurOpacity; /* closeSpike:MixBlurOpacity */
+} ckd_spikey_params;
+CKD_API ckd_status ckd_spikey_map(ckd_ctx *ctx, uint32_t *maps, unsigned resX, unsigned resY,
+                                  const ckd_spikey_params *frames, unsigned nFrames, int close);
+CKD_API ckd_status ckd_spikey_draw(ckd_ctx *ctx, uint32_t *dst, unsigned resX, unsigned resY,
+                                   const ckd_spikey_params *frames, unsigned nFrames, int close);
+
+/* RenderSinMap_2x2 / Sinuses_Draw (code/shadertoy.cpp:862-988) */
+typedef struct ckd_sinuses_params {
+    float time;
+    float specular;     /* sinusesTunnel:Specular     */
+    float roll;         /* sinusesTunnel:Roll         */
+    float speed;        /* sinusesTunnel:Speed        */
+    float offsX;        /* sinusesTunnel:OffsX        */
+    float gamma;        /* sinusesTunnel:Gamma        */
+    float hue;          /* sinusesTunnel:Hue          */
+    float desaturation; /* sinusesTunnel:Desaturation */
+} ckd_sinuses_params;
+CKD_API ckd_status ckd_sinuses_map(ckd_ctx *ctx, uint32_t *maps, unsigned resX, unsigned resY,
+                                   const ckd_sinuses_params *frames, unsigned nFrames);
+CKD_API ckd_status ckd_sinuses_draw(ckd_ctx *ctx, uint32_t *dst, unsigned resX, unsigned resY,
+                                    const ckd_sinuses_params *frames, unsigned nFrames);
+
+/* RenderLaura_2x2 / Laura_Draw (code/shadertoy.cpp:990-1109) */
+typedef struct ckd_laura_params {
+    float time;
+    float speed;    /* laura:Speed    */
+    float yaw;      /* laura:Yaw      */
+    float pitch;    /* laura:Pitch    */
+    float roll;     /* laura:Roll     */
+    float hue;      /* laura:Hue      */
+    float saturate; /* laura:Saturate */
+} ckd_laura_params;
+CKD_API ckd_status ckd_laura_map(ckd_ctx *ctx, uint32_t *maps, unsigned resX, unsigned resY,
+                                 const ckd_laura_params *frames, unsigned nFrames);
+CKD_API ckd_status ckd_laura_draw(ckd_ctx *ctx, uint32_t *dst, unsigned resX, unsigned resY,
+                                  const ckd_laura_params *frames, unsigned nFrames);
+
+/* RenderTunnelMap_2x2 / Tunnel_Draw (code/shadertoy.cpp:735-861), the
+ * free-directional tunnel.  Its two 1024x1024 ARGB textures (host memory,
+ * copied) are those Shadertoy_Create loads (code/shadertoy.cpp:174-178).
+ * _map writes the colour maps and the glow maps; the Draw form box-blurs
+ * the glow in place when litBlur >= 1 and adds it when litTiles != 0. */
+typedef struct ckd_fdtunnel_params {
+    float time;
+    float boxy;        /* tunnel:Boxy        */
+    float flowerScale; /* tunnel:FlowerScale */
+    float flowerFreq;  /* tunnel:FlowerFreq  */
+    float flowerPhase; /* tunnel:FlowerPhase */
+    float speed;       /* tunnel:Speed       */
+    float roll;        /* tunnel:Roll        */
+    float pitch;       /* tunnel:Pitch       */
+    float radius;      /* tunnel:Radius      */
+    float mulU;        /* tunnel:MulU        */
+    float mulV;        /* tunnel:MulV        */
+    float litTiles;    /* tunnel:LitTiles    */
+    float litBlur;     /* tunnel:LitBlur     */
+    float fog1;        /* tunnel:Fog1        */
+    float fog2;        /* tunnel:Fog2        */
+} ckd_fdtunnel_params;
+CKD_API ckd_status ckd_fdtunnel_create(ckd_ctx *ctx, const uint32_t *tex1024,
+                                       const uint32_t *texHighlights1024);
+CKD_API ckd_status ckd_fdtunnel_map(ckd_ctx *ctx, uint32_t *maps, uint32_t *glowMaps,
+                                    unsigned resX, unsigned resY,
+                                    const ckd_fdtunnel_params *frames, unsigned nFrames);
+CKD_API ckd_status ckd_fdtunnel_draw(ckd_ctx *ctx, uint32_t *dst, unsigned resX, unsigned resY,
+                                     const ckd_fdtunnel_params *frames, unsigned nFrames);
 
 #ifdef __cplusplus
 }

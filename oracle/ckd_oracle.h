@@ -190,6 +190,8 @@ int orc_st_draw(int effect, uint32_t *dst, unsigned resX, unsigned resY, const f
 float orc_log_ps(float x);
 float orc_exp_ps(float x);
 uint32_t orc_topixel(const float c[4]);
+/* x86 rsqrtss of this CPU */
+float orc_rsqrtss(float x);
 
 #ifdef __cplusplus
 }

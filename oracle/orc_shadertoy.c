@@ -63,6 +63,9 @@ static float dot3(const float a[4], const float b[4])
     return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
 
+/* the x86 rsqrtss estimate of this CPU (probe for fixtures written with it) */
+float orc_rsqrtss(float x) { return _mm_cvtss_f32(_mm_rsqrt_ss(_mm_set_ss(x))); }
+
 static float rsqrt_of(const st_ctx *c, float x)
 {
     if (c->rsq) return _mm_cvtss_f32(_mm_rsqrt_ss(_mm_set_ss(x)));

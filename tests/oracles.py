@@ -59,6 +59,12 @@ _ORC = {
     "orc_lutcosf": (_f, [_p, _f]),
     "orc_lutsinf": (_f, [_p, _f]),
     "orc_tape_warp32": (None, [_p, _p, _u, _u, _f, _f, _p]),
+    "orc_st_map": (None, [_i, _p, _p, _u, _u, _p, _p, _p, _p, _i, _i]),
+    "orc_st_draw": (_i, [_i, _p, _u, _u, _p, _p, _p, _p, _i, _i]),
+    "orc_log_ps": (_f, [_f]),
+    "orc_exp_ps": (_f, [_f]),
+    "orc_topixel": (C.c_uint32, [_p]),
+    "orc_rsqrtss": (_f, [_f]),
 }
 
 _REF = {
@@ -88,7 +94,36 @@ _REF = {
     "ref_lutcosf": (_f, [_f]),
     "ref_lutsinf": (_f, [_f]),
     "ref_tape_warp32": (None, [_p, _p, _u, _u, _f, _f]),
+    "ref_st_init": (_i, []),
+    "ref_st_rocket": (_i, [C.c_char_p]),
+    "ref_st_row": (None, [C.c_double]),
+    "ref_st_get": (C.c_double, [C.c_char_p]),
+    "ref_st_textures": (None, [_p, _p]),
+    "ref_st_map": (None, [_i, _p, _p, _f]),
+    "ref_st_draw": (None, [_i, _p, _f, _f]),
+    "ref_st_gamma": (None, [_p, _f, _p]),
+    "ref_st_topixel4": (None, [_p, _p]),
 }
+
+# oracle / harness effect ids (oracle/ckd_oracle.h) of the effects' map
+# renderers; the spike has two (distant, close-up)
+ST_IDS = {"plasma": 0, "nautilus": 1, "spikey": 2, "spikey_close": 3, "sinuses": 4,
+          "laura": 5, "fdtunnel": 6}
+ST_STRUCT = {"spikey_close": "spikey"}  # id name -> parameter struct name
+
+
+def write_rocket_tracks(root: Path, values: dict) -> None:
+    """Writes GNU Rocket player track files (root/sync/_<name>.track, the
+    layout 3rdparty/rocket-stripped/lib/device.c reads): for each track
+    name, one step key per row 0..K-1 holding values[name][row]."""
+    import struct
+    d = Path(root) / "sync"
+    d.mkdir(parents=True, exist_ok=True)
+    for name, vals in values.items():
+        enc = "".join(ch if (ch.isalnum() or ch in "._/") else f"-{ord(ch):02X}" for ch in name)
+        blob = struct.pack("<i", len(vals)) + b"".join(
+            struct.pack("<ifb", row, float(v), 0) for row, v in enumerate(vals))
+        (d / f"_{enc}.track").write_bytes(blob)
 
 
 def _bind(path: Path, table: dict) -> C.CDLL:

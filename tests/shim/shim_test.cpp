@@ -10,6 +10,8 @@
 //   shim_test fxblit <in 644x364> <out>
 //   shim_test blur07 <in> <W> <H> <strength> <h|v|2d> <inplace 0|1> <out>
 //   shim_test tapewarp <in> <W> <H> <strength> <speed> <out>
+//   shim_test shadertoy <plasma|nautilus|spikey|spikey_close|sinuses|laura|tunnel>
+//             <time> <out>                            (1280x720 frame)
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -35,7 +37,7 @@ const sync_track *AddTrack(const char *name)
     t.name = name;
     t.value = 0.0;
     if (const char *env = getenv("CKD_SHIM_TRACKS")) {
-        std::string s(env), key = std::string(name) + "=";
+        std::string s = std::string(";") + env, key = std::string(";") + name + "=";
         size_t p = s.find(key);
         if (p != std::string::npos) t.value = atof(s.c_str() + p + key.size());
     }
@@ -144,6 +146,20 @@ int main(int argc, char **argv)
         TapeWarp32(res.data(), in.data(), W, H, (float)atof(argv[5]), (float)atof(argv[6]));
         BoxBlur_Destroy();
         write_pixels(argv[7], res.data(), res.size());
+    } else if (cmd == "shadertoy" && argc == 5) {
+        const std::string fx = argv[2];
+        const float t = (float)atof(argv[3]);
+        if (!Shadertoy_Create()) return 1;
+        if (fx == "plasma") Plasma_Draw(out.data(), t, 1.f);
+        else if (fx == "nautilus") Nautilus_Draw(out.data(), t, 1.f);
+        else if (fx == "spikey") Spikey_Draw(out.data(), t, 1.f, false);
+        else if (fx == "spikey_close") Spikey_Draw(out.data(), t, 1.f, true);
+        else if (fx == "sinuses") Sinuses_Draw(out.data(), t, 1.f);
+        else if (fx == "laura") Laura_Draw(out.data(), t, 1.f);
+        else if (fx == "tunnel") Tunnel_Draw(out.data(), t, 1.f);
+        else return 2;
+        Shadertoy_Destroy();
+        write_pixels(argv[4], out.data(), n);
     } else {
         return 2;
     }

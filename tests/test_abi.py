@@ -104,3 +104,33 @@ def test_tuning_validation_without_gpu():
     t = _lib.Tuning(2, 32, 8, 32)
     assert lib.ckd_set_tuning(None, C.byref(t)) == CKD_EINVAL
     assert "null context" in _lib.last_error()
+
+
+def test_shadertoy_argument_validation_without_gpu():
+    lib = _lib.load()
+    p = _lib.PlasmaParams(time=1.0, speed=1.0, hue=24.0, gamma=2.0)
+    assert lib.ckd_plasma_map(None, 8, 1282, 720, C.byref(p), 1) == CKD_EINVAL
+    assert "multiple of 8" in _lib.last_error()
+    assert lib.ckd_plasma_draw(None, 8, 1280, 720, C.byref(p), 1) == CKD_EINVAL
+    assert "null context" in _lib.last_error()
+    p.gamma = float("nan")
+    assert lib.ckd_plasma_map(None, 8, 1280, 720, C.byref(p), 1) == CKD_EINVAL
+    assert "finite" in _lib.last_error()
+    s = _lib.SpikeyParams(time=1.0, closeMixBlurOpacity=0.25)
+    assert lib.ckd_spikey_draw(None, 8, 1280, 720, C.byref(s), 1, 1) == CKD_EINVAL
+    assert "mix-blur" in _lib.last_error()
+    t = _lib.FdTunnelParams(time=1.0)
+    assert lib.ckd_fdtunnel_map(None, 8, None, 1280, 720, C.byref(t), 1) == CKD_EINVAL
+    assert "glow" in _lib.last_error()
+
+
+def test_shadertoy_param_structs_match_the_header():
+    """field order of each ctypes struct == the C struct in include/ckd.h"""
+    text = (ROOT / "include" / "ckd.h").read_text()
+    for name, cls in _lib.ST_PARAMS.items():
+        body = re.search(r"typedef struct ckd_%s_params \{(.*?)\} ckd_%s_params;" % (name, name),
+                         text, re.S).group(1)
+        fields = re.findall(r"float (\w+);", body)
+        assert fields == [f for f, _ in cls._fields_], name
+        tracks = re.findall(r"/\* ([\w:]+)\s*\*/", body)
+        assert tracks == list(cls.TRACKS.values()), name

@@ -1,0 +1,260 @@
+"""The oracle of the reference's raymarched effects (code/shadertoy.cpp):
+pinned to the compiled reference (golden hashes and crops written from it
+with the demo's own Rocket tracks, and, where the reference checkout is
+present, live runs driven through the reference's Rocket player), and to
+what the mathematics fixes for GammaAdj's Cephes exp/log and ToPixel4's
+lane packing."""
+from __future__ import annotations
+
+import ctypes as C
+import hashlib
+import json
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracles
+from oracles import P, aligned_empty
+from helpers import error_stats
+from cookiedough_amd import _lib
+from cookiedough_amd import workloads as wl
+
+GOLD = Path(__file__).resolve().parent / "golden"
+W, H = 1280, 720
+MW, MH = W // 2 + 4, H // 2 + 4
+
+
+def sha(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.fixture(scope="module")
+def gold():
+    return (json.loads((GOLD / "shadertoy.json").read_text()),
+            np.load(GOLD / "shadertoy_arrays.npz"))
+
+
+@pytest.fixture(scope="module")
+def st_inputs(orc):
+    lut = np.zeros(2049, np.float32)
+    orc.orc_cos_lut(P(lut))
+    tex = np.ascontiguousarray(wl.noise_texture(11, 10).reshape(-1))
+    tex_hi = np.ascontiguousarray(wl.noise_texture(12, 10).reshape(-1))
+    return lut, tex, tex_hi
+
+
+def orc_map(orc, st_inputs, effect, params, mode, W=W, H=H):
+    lut, tex, tex_hi = st_inputs
+    mw, mh = W // 2 + 4, H // 2 + 4
+    m, g = aligned_empty(mw * mh), aligned_empty(mw * mh)
+    p = np.ascontiguousarray(params, np.float32)
+    orc.orc_st_map(oracles.ST_IDS[effect], P(m), P(g), W, H, P(p), P(lut), P(tex), P(tex_hi), mode, 8)
+    return m, g
+
+
+def probe_matches(orc, gold_json) -> bool:
+    pr = gold_json["rsqrtss_probe"]
+    return all(np.float32(orc.orc_rsqrtss(x)) == np.float32(y) for x, y in zip(pr["x"], pr["y"]))
+
+
+def test_fixture_covers_every_effect_and_feature(gold):
+    g, _ = gold
+    effects = {c["effect"] for c in g["cases"]}
+    assert effects == set(oracles.ST_IDS)
+    by = lambda e, f: [dict(zip(c["fields"], c["params"]))[f] for c in g["cases"] if c["effect"] == e]
+    assert any(v != 0 for v in by("spikey", "distWarmup"))      # specular-only renderer
+    assert any(v == 0 for v in by("spikey", "distWarmup"))      # distant renderer
+    assert {0.0, 1.0} <= set(by("spikey_close", "closeAspectMul"))
+    assert any(v >= 1 for v in by("fdtunnel", "litBlur"))       # glow blur + add
+    assert any(v != 0 for v in by("nautilus", "blur"))          # frame blur
+
+
+def test_oracle_matches_reference_hashes(orc, gold, st_inputs):
+    """oracle with this CPU's rsqrtss == the compiled reference, bit for bit,
+    maps and Draw frames (only where the rsqrtss table matches the one the
+    fixture was written with)"""
+    g, _ = gold
+    if not probe_matches(orc, g):
+        pytest.skip("this CPU's rsqrtss table differs from the fixture's")
+    lut, tex, tex_hi = st_inputs
+    for c in g["cases"]:
+        m, gl = orc_map(orc, st_inputs, c["effect"], c["params"], 1)
+        assert sha(m) == c["map_sha256"], (c["effect"], c["row"])
+        if "glow_sha256" in c:
+            assert sha(gl) == c["glow_sha256"], (c["effect"], c["row"])
+        if "draw_sha256" in c:
+            d = aligned_empty(W * H)
+            p = np.ascontiguousarray(c["params"], np.float32)
+            assert orc.orc_st_draw(oracles.ST_IDS[c["effect"]], P(d), W, H, P(p), P(lut), P(tex),
+                                   P(tex_hi), 1, 8) == 0
+            assert sha(d) == c["draw_sha256"], (c["effect"], c["row"], "draw")
+
+
+# vNorm4's rsqrtps estimate vs a correctly rounded 1/sqrt: rays differ in the
+# last bits, which shows as 1/255 steps and, where a ray grazes a surface,
+# rare large differences
+PORTABLE_MEAN, PORTABLE_FRAC_OVER_2 = 0.5, 0.01
+
+
+def test_portable_oracle_near_reference(orc, gold, st_inputs):
+    g, arrays = gold
+    (r0, r1), (c0, c1) = g["crop"]
+    for k, c in enumerate(g["cases"]):
+        m, gl = orc_map(orc, st_inputs, c["effect"], c["params"], 0)
+        es = error_stats(m.reshape(MH, MW)[r0:r1, c0:c1].reshape(-1),
+                         arrays[f"map_crop_{k}"].reshape(-1))
+        assert es["mean"] <= PORTABLE_MEAN and es["frac_over_tol"] <= PORTABLE_FRAC_OVER_2, (
+            c["effect"], c["row"], es)
+        if c["effect"] == "plasma":   # no reciprocal square root on its path
+            assert es["max"] == 0
+
+
+def f32(x):
+    return float(np.float32(x))
+
+
+def test_cephes_log_exp_accuracy(orc):
+    """exp_ps/log_ps are Cephes' single-precision polynomials: within a few
+    ulp of the true functions over their range (a dropped or transposed
+    coefficient breaks this)"""
+    xs = np.geomspace(1e-37, 3e38, 4001).astype(np.float32)
+    for x in xs:
+        got = orc.orc_log_ps(float(x))
+        want = math.log(float(x))
+        assert abs(got - want) <= 3e-7 * max(1.0, abs(want)), (x, got, want)
+    for x in np.linspace(-87.0, 88.0, 4001).astype(np.float32):
+        got = orc.orc_exp_ps(float(x))
+        want = math.exp(float(x))
+        assert abs(got - want) <= 4e-7 * want, (x, got, want)
+
+
+def test_cephes_special_lanes(orc):
+    flt_min = float(np.float32(1.17549435e-38))
+    assert math.isnan(orc.orc_log_ps(0.0)) and math.isnan(orc.orc_log_ps(-1.0))
+    # NaN passes maxps as FLT_MIN: log(NaN) = log(FLT_MIN), about -87.34
+    assert orc.orc_log_ps(float("nan")) == orc.orc_log_ps(flt_min)
+    assert abs(orc.orc_log_ps(flt_min) - math.log(flt_min)) < 1e-4
+    # exp clamps to +-88.3762626647949; NaN takes the upper bound
+    assert orc.orc_exp_ps(float("nan")) == orc.orc_exp_ps(100.0) == orc.orc_exp_ps(88.3762626647949)
+    assert orc.orc_exp_ps(-100.0) == 0.0
+    assert orc.orc_exp_ps(0.0) == 1.0 and orc.orc_log_ps(1.0) == 0.0
+
+
+def pack(orc, c):
+    arr = (C.c_float * 4)(*c)
+    return orc.orc_topixel(C.addressof(arr))
+
+
+def test_topixel_lane_packing(orc):
+    """ToPixel4: 255*c rounded to nearest even, negatives and NaN to 0,
+    packus_epi32 to [0, 65535], then packus_epi16 reads that as int16"""
+    cases = [(1.0, 255), (0.0, 0), (-0.5, 0), (float("nan"), 0), (float("inf"), 0),
+             (128.5 / 255, 128), (0.5 / 255, 0), (1.5 / 255, 2), (2.0, 255),
+             (32767.0 / 255, 255), (32768.0 / 255, 0), (65535.0 / 255, 0), (70000.0 / 255, 0),
+             (1e30, 0)]
+    for v, want in cases:
+        px = pack(orc, [v, 0.0, 1.0, 0.0])
+        assert px & 0xff == want, (v, px)
+        assert (px >> 16) & 0xff == 255
+    assert pack(orc, [0.0, 0.0, 0.0, 1.0]) == 0xff000000
+
+
+# ---- live runs of the compiled reference (only with the checkout) -------
+
+REF_TARGET = Path("/root/reference/target")
+
+
+@pytest.mark.ref
+def test_gamma_and_pack_lanes_vs_reference(orc, refl):
+    vals = [0.0, -0.0, -1.0, 1e-45, 1e-38, 1.17549435e-38, 1e-20, 0.001, 0.5, 0.7071067, 1.0,
+            1.5, 2.0, 100.0, 3e38, float("inf"), float("nan")]
+    for gamma in [0.5, 1.0, 1.44, 2.2, -1.0, 60.0]:
+        for i in range(0, len(vals), 4):
+            quad = (vals[i:i + 4] + [1.0] * 4)[:4]
+            a, out = (C.c_float * 4)(*quad), (C.c_float * 4)()
+            refl.ref_st_gamma(C.addressof(a), gamma, C.addressof(out))
+            for x, r in zip(quad, out):
+                o = orc.orc_exp_ps(float(np.float32(gamma) * np.float32(orc.orc_log_ps(x))))
+                assert np.float32(o).tobytes() == np.float32(r).tobytes(), (x, gamma, o, r)
+    rng = np.random.default_rng(5)
+    lanes = np.concatenate([rng.uniform(-2, 2, 4000), rng.uniform(100, 300, 400),
+                            [float("nan"), float("inf"), -float("inf"), 128.5 / 255, 0.5 / 255]])
+    lanes = np.resize(lanes.astype(np.float32), (len(lanes) + 15) // 16 * 16)
+    for i in range(0, len(lanes), 16):
+        q, out = (C.c_float * 16)(*lanes[i:i + 16]), (C.c_uint32 * 4)()
+        refl.ref_st_topixel4(C.addressof(q), C.addressof(out))
+        for k in range(4):
+            assert pack(orc, lanes[i + 4 * k:i + 4 * k + 4]) == out[k]
+
+
+def random_tracks(seed: int, rows: int) -> dict:
+    """plausible values of every effect track for `rows` rows"""
+    r = np.random.default_rng(seed)
+    u = lambda a, b: list(r.uniform(a, b, rows))
+    pick = lambda *v: list(r.choice(v, rows))
+    return {
+        "plasma:Speed": u(0.1, 2), "plasma:Hue": u(0, 80), "plasma:Gamma": u(0.5, 3),
+        "plasma:Desaturation": u(-1, 1),
+        "nautilus:Roll": u(-0.1, 0.1), "nautilus:Hue": u(0, 80), "nautilus:Speed": u(1, 4),
+        "nautilus:Desat": u(0, 1), "nautilus:Blur": pick(0.0, 5.0, 20.0),
+        "spike:Speed": u(0.5, 14), "spike:Roll": u(-3, 3), "spike:Specular": pick(3.14, 4.0, 22.0),
+        "spike:Desaturation": u(-1, 4), "spike:Hue": u(0, 70), "spike:Gamma": u(0.5, 2),
+        "distSpike:xOffs": u(-0.3, 0.3), "distSpike:yOffs": u(-0.3, 0.3),
+        "distSpike:zOffs": u(0, 1), "distSpike:Warmup": pick(0.0, 0.0, 1.5, 3.0),
+        "closeSpike:xOffs": u(-0.3, 0.3), "closeSpike:yOffs": u(-0.3, 0.3),
+        "closeSpike:zOffs": u(-1, 1), "closeSpike:zOffsScale": u(0, 1),
+        "closeSpike:NormalGrain": u(0.1, 1), "closeSpike:Scale": u(0.3, 1.3),
+        "closeSpike:Rim": pick(0.0, 1.0), "closeSpike:AspectMul": pick(0.0, 1.0),
+        "sinusesTunnel:Specular": u(1, 30), "sinusesTunnel:Roll": u(-3, 0),
+        "sinusesTunnel:Speed": u(-5, 3), "sinusesTunnel:OffsX": u(0, 0.4),
+        "sinusesTunnel:Gamma": u(2, 8), "sinusesTunnel:Hue": u(40, 55),
+        "sinusesTunnel:Desaturation": u(-8, 3),
+        "laura:Speed": u(-5, 5), "laura:Yaw": u(-1, 1), "laura:Pitch": u(-1, 1),
+        "laura:Roll": u(-0.02, 0.02), "laura:Hue": u(10, 30), "laura:Saturate": u(0.5, 4),
+        "tunnel:Boxy": u(0, 1), "tunnel:FlowerScale": u(0, 0.3), "tunnel:FlowerFreq": pick(3.0, 4.0, 5.0),
+        "tunnel:FlowerPhase": u(-1, 1), "tunnel:Speed": u(0.5, 2), "tunnel:Roll": u(-0.2, 0.2),
+        "tunnel:Pitch": u(-0.2, 0.2), "tunnel:Radius": u(0.5, 2), "tunnel:MulU": pick(1.0, 2.0),
+        "tunnel:MulV": u(0.5, 2), "tunnel:LitTiles": pick(0.0, 1.0), "tunnel:LitBlur": u(0, 10),
+        "tunnel:Fog1": u(0, 255), "tunnel:Fog2": u(0, 255),
+    }
+
+
+@pytest.mark.ref
+def test_oracle_vs_reference_through_rocket(orc, refl, st_inputs, tmp_path):
+    """random track values, written as Rocket track files and read by the
+    reference's own player: oracle (this CPU's rsqrtss) == reference, maps
+    and Draw frames of every effect"""
+    lut, tex, tex_hi = st_inputs
+    rows = 3
+    tracks = random_tracks(7, rows)
+    oracles.write_rocket_tracks(tmp_path, tracks)
+    assert refl.ref_st_init() == 0
+    assert refl.ref_st_rocket(str(tmp_path).encode()) == 0
+    refl.ref_st_textures(P(tex), P(tex_hi))
+    try:
+        for row in range(rows):
+            refl.ref_st_row(float(row))
+            t = 3.7 + 11.3 * row
+            for fid, eid in oracles.ST_IDS.items():
+                cls = _lib.ST_PARAMS[oracles.ST_STRUCT.get(fid, fid)]
+                vals = {f: refl.ref_st_get(tr.encode()) for f, tr in cls.TRACKS.items()}
+                for f, tr in cls.TRACKS.items():
+                    if tr in tracks:
+                        assert np.float32(vals[f]) == np.float32(tracks[tr][row]), (tr, row)
+                prm = cls(time=t, **vals).as_array()
+                rm, rg = aligned_empty(MW * MH), aligned_empty(MW * MH)
+                refl.ref_st_map(eid, P(rm), P(rg), t)
+                om, og = orc_map(orc, st_inputs, fid, prm, 1)
+                assert (om == rm).all(), (fid, row, int((om != rm).sum()))
+                if fid == "fdtunnel":
+                    assert (og == rg).all(), (fid, row)
+                rd, od = aligned_empty(W * H), aligned_empty(W * H)
+                refl.ref_st_draw(eid, P(rd), t, 1.0)
+                assert orc.orc_st_draw(eid, P(od), W, H, P(prm), P(lut), P(tex), P(tex_hi), 1, 8) == 0
+                assert (od == rd).all(), (fid, row, "draw", int((od != rd).sum()))
+    finally:
+        # back to the demo's tracks for any later user of the harness
+        if REF_TARGET.exists():
+            refl.ref_st_rocket(str(REF_TARGET).encode())
