@@ -257,6 +257,39 @@ def main() -> int:
 
         f3 = fr3
 
+        # ------------------- the reference's raymarched effects (1280x720 Draw)
+        # plain: one Draw per call; batched: all frames of an effect in one call
+        # (one map launch, then the per-frame blur / blit of its Draw function)
+        n_st = 64
+        st_tex = np.ascontiguousarray(wl.noise_texture(11, 10).reshape(-1))
+        st_tex_hi = np.ascontiguousarray(wl.noise_texture(12, 10).reshape(-1))
+        ctx.fdtunnel_create(st_tex, st_tex_hi)
+        fr_st = ctx.empty(n_st * W * H)
+        st_per = {}
+        for fx in ["plasma", "nautilus", "spikey", "spikey_close", "sinuses", "laura", "fdtunnel"]:
+            name = "spikey" if fx == "spikey_close" else fx
+            close = fx == "spikey_close"
+            frames = wl.st_frames(fx, n_st)
+            arr = ctx._frames(name, frames)[0]
+            singles = [ctx._frames(name, [f])[0] for f in frames]
+
+            def st_plain():
+                for k in range(n_st):
+                    ctx.st_draw(name, fr_st.view(k * W * H, (W * H,)), W, H, singles[k], close=close)
+
+            def st_batched():
+                ctx.st_draw(name, fr_st, W, H, arr, close=close)
+
+            st_plain()
+            st_batched()
+            mp, mb = T.run(st_plain), T.run(st_batched)
+            px = n_st * W * H
+            st_per[fx] = {"frames": n_st, "plain_mpx_s": round(px / mp / 1e3, 1),
+                          "batched_mpx_s": round(px / mb / 1e3, 1),
+                          "ms_per_frame_batched": round(mb / n_st, 4)}
+            log("shadertoy", fx, st_per[fx])
+        per["shadertoy_1280x720"] = st_per
+
         # The CPU leg is the only part of the bench that runs the oracle: the
         # timed CPU baseline and the FP64 reference render the precision form's
         # error is measured against.
@@ -301,6 +334,44 @@ def main() -> int:
                     px += cpu_render(items[k], nthreads)
                     k += 1
                 return px, k, time.perf_counter() - t_c
+
+            # the reference's own Draw functions of the raymarched effects, on
+            # the host, where the compiled reference harness is present (its
+            # parameters come from Rocket track files written for the frames);
+            # else the oracle's restatement of them
+            import tempfile
+            lut_h = np.zeros(2049, np.float32)
+            orc.orc_cos_lut(oracles.P(lut_h))
+            use_ref = oracles.ref_available()
+            if use_ref:
+                refl = oracles.ref()
+                refl.ref_st_init()
+                refl.ref_st_textures(oracles.P(st_tex), oracles.P(st_tex_hi))
+            for fx in st_per:
+                name = "spikey" if fx == "spikey_close" else fx
+                frames = wl.st_frames(fx, 8)
+                out_h = oracles.aligned_empty(W * H)
+                with tempfile.TemporaryDirectory() as td:
+                    if use_ref:
+                        cls = type(frames[0])
+                        oracles.write_rocket_tracks(Path(td), {
+                            tr: [getattr(f, fld) for f in frames] for fld, tr in cls.TRACKS.items()})
+                        refl.ref_st_rocket(td.encode())
+                    t_c = time.perf_counter()
+                    for k, f in enumerate(frames):
+                        if use_ref:
+                            refl.ref_st_row(float(k))
+                            refl.ref_st_draw(oracles.ST_IDS[fx], oracles.P(out_h), f.time, 1.0)
+                        else:
+                            pa = f.as_array()
+                            orc.orc_st_draw(oracles.ST_IDS[fx], oracles.P(out_h), W, H, oracles.P(pa),
+                                            oracles.P(lut_h), oracles.P(st_tex), oracles.P(st_tex_hi),
+                                            1, nthr)
+                    dt_c = time.perf_counter() - t_c
+                st_per[fx]["cpu_mpx_s"] = round(8 * W * H / dt_c / 1e6, 2)
+                st_per[fx]["cpu_kind"] = ("reference Draw (OpenMP, OMP_NUM_THREADS threads)"
+                                          if use_ref else f"oracle Draw ({nthr} threads)")
+                log("shadertoy cpu", fx, st_per[fx]["cpu_mpx_s"])
 
             px, k, dt = cpu_rate(nthr, 10.0)
             px1, k1, dt1 = cpu_rate(1, 5.0)

@@ -300,3 +300,38 @@ def make_request(item: BatchItem, dst_ptr: int, scene: SdfScene, path: Lissajous
         r.grid = grid_ptr
         r.radius = item.radius
     return r
+
+
+# ---- the reference's raymarched effects ----------------------------------
+
+# One parameter set per effect: the demo's own Rocket track values
+# (target/sync of the reference) at a row where the effect is on screen.
+ST_DEMO = {
+    "plasma": {"speed": 1.33, "hue": 24.0, "gamma": 2.0, "desaturation": 0.0},
+    "nautilus": {"roll": 0.0, "hue": 72.0, "speed": 3.14, "desaturation": 0.63,
+                 "blur": 32.2194},
+    "spikey": {"speed": 12.0, "roll": 1.1, "specular": 22.0, "desaturation": -1.0, "hue": 66.0,
+               "gamma": 0.3125, "distZOffs": 0.66, "closeXOffs": 0.314, "closeZOffs": 1.0,
+               "closeZOffsScale": 2.0, "closeNormalGrain": 1.0, "closeScale": 1.0,
+               "closeRim": 1.0},
+    "spikey_close": {"speed": 10.0, "roll": 4.0, "specular": 8.0, "desaturation": 0.0,
+                     "hue": 33.0, "gamma": 1.66, "distXOffs": 0.5, "distYOffs": -1.0,
+                     "distZOffs": 0.8, "closeXOffs": 0.314, "closeZOffs": 0.999548,
+                     "closeZOffsScale": 2.0, "closeNormalGrain": 0.999729, "closeScale": 0.88,
+                     "closeRim": 1.0, "closeAspectMul": 1.0},
+    "sinuses": {"specular": 30.0, "roll": -2.9997, "speed": -5.0, "offsX": 0.0,
+                "gamma": 2.2422, "hue": 50.3438, "desaturation": -7.8517},
+    "laura": {"speed": 3.14, "yaw": 0.5991, "pitch": 0.14, "roll": 0.01, "hue": 24.0,
+              "saturate": 3.9746},
+    "fdtunnel": {"boxy": 0.9999, "flowerScale": 0.2, "flowerFreq": 4.0, "flowerPhase": 0.4999,
+                 "speed": 5.28, "roll": 1.0, "pitch": 1.4, "radius": 0.8375, "mulU": 1024.0,
+                 "mulV": 64.0, "litTiles": 1.0, "litBlur": 3.0, "fog1": 0.0, "fog2": 64.0},
+}
+
+
+def st_frames(effect: str, n: int, t0: float = 10.0):
+    """n frames of an effect (`spikey_close` is the spike's close-up) with the
+    demo's parameters at t = t0 + k/60, as ctypes parameter structs"""
+    from . import _lib
+    cls = _lib.ST_PARAMS["spikey" if effect == "spikey_close" else effect]
+    return [cls(time=t0 + k / 60.0, **ST_DEMO[effect]) for k in range(n)]
