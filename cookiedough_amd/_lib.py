@@ -157,6 +157,24 @@ SIGNATURES = {
                                C.c_size_t, _p, C.POINTER(_p)],
     "ckd_graph_launch": [_p],
     "ckd_graph_destroy": [_p],
+    "ckd_mix32": [_p, _p, _p, _u, C.c_uint8],
+    "ckd_mixover32": [_p, _p, _p, _u],
+    "ckd_sub32": [_p, _p, _p, _u],
+    "ckd_excl32": [_p, _p, _p, _u],
+    "ckd_softlight32": [_p, _p, _p, _u],
+    "ckd_softlight32a": [_p, _p, _p, _u],
+    "ckd_softlight32aa": [_p, _p, _p, _u, _f],
+    "ckd_overlay32": [_p, _p, _p, _u],
+    "ckd_overlay32a": [_p, _p, _p, _u],
+    "ckd_darken32_50": [_p, _p, _p, _u],
+    "ckd_mulsrc32": [_p, _p, _p, _u],
+    "ckd_mulsrc32a": [_p, _p, _p, _u],
+    "ckd_mixsrc32": [_p, _p, _p, _u],
+    "ckd_mixsrc32s": [_p, _p, _p, _u, _u, _u],
+    "ckd_blitsrc32": [_p, _p, _p, _u, _u, _u],
+    "ckd_blitsrc32a": [_p, _p, _p, _u, _u, _u, _f],
+    "ckd_fade32": [_p, _p, _u, C.c_uint32, C.c_uint8],
+    "ckd_zoom32": [_p, _p, _p, _u, _u, _f],
     "ckd_fdtunnel_create": [_p, _p, _p],
     "ckd_fdtunnel_map": [_p, _p, _p, _u, _u, C.POINTER(FdTunnelParams), _u],
     "ckd_fdtunnel_draw": [_p, _p, _u, _u, C.POINTER(FdTunnelParams), _u],

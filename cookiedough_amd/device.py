@@ -155,6 +155,29 @@ class Context:
         fog = np.ascontiguousarray(fog, np.uint32)
         check(self.lib.ckd_landscape_create(self.h, hmap.ctypes.data, cmap.ctypes.data, fog.ctypes.data))
 
+    # ---- compositing blends (code/util.cpp), in place on dst
+    def blend(self, op, dst, src, n, *args):
+        """op: mix32 (args: alpha), mixover32, sub32, excl32, softlight32,
+        softlight32a, softlight32aa (alpha), overlay32, overlay32a,
+        darken32_50, mulsrc32, mulsrc32a, mixsrc32"""
+        check(getattr(self.lib, f"ckd_{op}")(self.h, ptr(dst), ptr(src), n, *args))
+
+    def mixsrc32s(self, dst, src, resX, resY, srcStride):
+        check(self.lib.ckd_mixsrc32s(self.h, ptr(dst), ptr(src), resX, resY, srcStride))
+
+    def blitsrc32(self, dst, src, destResX, srcResX, yRes, alpha=None):
+        if alpha is None:
+            check(self.lib.ckd_blitsrc32(self.h, ptr(dst), ptr(src), destResX, srcResX, yRes))
+        else:
+            check(self.lib.ckd_blitsrc32a(self.h, ptr(dst), ptr(src), destResX, srcResX, yRes,
+                                          alpha))
+
+    def fade32(self, dst, n, rgb, alpha):
+        check(self.lib.ckd_fade32(self.h, ptr(dst), n, rgb, alpha))
+
+    def zoom32(self, dst, src, W, H, scale):
+        check(self.lib.ckd_zoom32(self.h, ptr(dst), ptr(src), W, H, scale))
+
     # ---- the reference's raymarched effects (code/shadertoy.cpp)
     def fdtunnel_create(self, tex, tex_highlights):
         tex = np.ascontiguousarray(tex, np.uint32)

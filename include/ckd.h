@@ -295,6 +295,61 @@ CKD_API ckd_status ckd_batch_graph_create(ckd_ctx *ctx, const ckd_request *reque
 CKD_API ckd_status ckd_graph_launch(ckd_graph *graph);
 CKD_API void ckd_graph_destroy(ckd_graph *graph);
 
+/* ---- compositing blends (code/util.cpp:58-811) ---------------------------
+ * In place on dst, device buffers, per channel of 0xAARRGGBB pixels, bit for
+ * bit the reference's results (including the unsigned wrap-around of the
+ * scalar blends, whose wrapped bits reach the packed pixel). */
+/* Mix32, code/util.cpp:83-96: dst + alpha*(src-dst)/256, all four channels */
+CKD_API ckd_status ckd_mix32(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src,
+                             unsigned numPixels, uint8_t alpha);
+/* MixOver32, code/util.cpp:148-177 (alpha channel of the result 0) */
+CKD_API ckd_status ckd_mixover32(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src,
+                                 unsigned numPixels);
+/* Sub32, code/util.cpp:179-192: saturating subtract */
+CKD_API ckd_status ckd_sub32(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src,
+                             unsigned numPixels);
+/* Excl32, code/util.cpp:194-224: exclusion, destination alpha kept */
+CKD_API ckd_status ckd_excl32(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src,
+                              unsigned numPixels);
+/* SoftLight32 / SoftLight32A / SoftLight32AA, code/util.cpp:227-346 */
+CKD_API ckd_status ckd_softlight32(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src,
+                                   unsigned numPixels);
+CKD_API ckd_status ckd_softlight32a(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src,
+                                    unsigned numPixels);
+CKD_API ckd_status ckd_softlight32aa(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src,
+                                     unsigned numPixels, float alpha);
+/* Overlay32 / Overlay32A (the integer forms), code/util.cpp:434-515 */
+CKD_API ckd_status ckd_overlay32(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src,
+                                 unsigned numPixels);
+CKD_API ckd_status ckd_overlay32a(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src,
+                                  unsigned numPixels);
+/* Darken32_50, code/util.cpp:518-549 */
+CKD_API ckd_status ckd_darken32_50(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src,
+                                   unsigned numPixels);
+/* MulSrc32 / MulSrc32A, code/util.cpp:605-634 */
+CKD_API ckd_status ckd_mulsrc32(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src,
+                                unsigned numPixels);
+CKD_API ckd_status ckd_mulsrc32a(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src,
+                                 unsigned numPixels);
+/* MixSrc32 / MixSrc32S, code/util.cpp:636-676: blend by the source alpha;
+ * the S form reads source rows srcStride pixels apart */
+CKD_API ckd_status ckd_mixsrc32(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src,
+                                unsigned numPixels);
+CKD_API ckd_status ckd_mixsrc32s(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src,
+                                 unsigned resX, unsigned resY, unsigned srcStride);
+/* BlitSrc32 / BlitSrc32A, code/util.cpp:707-752 (alpha in [0, 255]) */
+CKD_API ckd_status ckd_blitsrc32(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src,
+                                 unsigned destResX, unsigned srcResX, unsigned yRes);
+CKD_API ckd_status ckd_blitsrc32a(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src,
+                                  unsigned destResX, unsigned srcResX, unsigned yRes,
+                                  float alpha);
+/* Fade32, code/util.cpp:798-811: towards the colour RGB by alpha */
+CKD_API ckd_status ckd_fade32(ckd_ctx *ctx, uint32_t *dst, unsigned numPixels, uint32_t RGB,
+                              uint8_t alpha);
+/* Zoom32, code/util.cpp:58-81: nearest-neighbour zoom in, 0 < scale <= 1 */
+CKD_API ckd_status ckd_zoom32(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, unsigned xRes,
+                              unsigned yRes, float scale);
+
 /* ---- the reference's raymarched effects (code/shadertoy.cpp) ------------
  *
  * Each effect renders an FX map of (resX/2+4) x (resY/2+4) pixels for an

@@ -193,6 +193,30 @@ uint32_t orc_topixel(const float c[4]);
 /* x86 rsqrtss of this CPU */
 float orc_rsqrtss(float x);
 
+/* ---- compositing blends (code/util.cpp:58-811) ------------------------- */
+/* pinned: the compiled reference, bit for bit, on random images */
+void orc_mix32(uint32_t *dst, const uint32_t *src, unsigned n, uint8_t alpha);
+void orc_mixover32(uint32_t *dst, const uint32_t *src, unsigned n);
+void orc_sub32(uint32_t *dst, const uint32_t *src, unsigned n);
+void orc_excl32(uint32_t *dst, const uint32_t *src, unsigned n);
+void orc_softlight32(uint32_t *dst, const uint32_t *src, unsigned n);
+void orc_softlight32a(uint32_t *dst, const uint32_t *src, unsigned n);
+void orc_softlight32aa(uint32_t *dst, const uint32_t *src, unsigned n, float alpha);
+void orc_overlay32(uint32_t *dst, const uint32_t *src, unsigned n);
+void orc_overlay32a(uint32_t *dst, const uint32_t *src, unsigned n);
+void orc_darken32_50(uint32_t *dst, const uint32_t *src, unsigned n);
+void orc_mulsrc32(uint32_t *dst, const uint32_t *src, unsigned n);
+void orc_mulsrc32a(uint32_t *dst, const uint32_t *src, unsigned n);
+void orc_mixsrc32(uint32_t *dst, const uint32_t *src, unsigned n);
+void orc_mixsrc32s(uint32_t *dst, const uint32_t *src, unsigned resX, unsigned resY,
+                   unsigned srcStride);
+void orc_blitsrc32(uint32_t *dst, const uint32_t *src, unsigned destResX, unsigned srcResX,
+                   unsigned yRes);
+void orc_blitsrc32a(uint32_t *dst, const uint32_t *src, unsigned destResX, unsigned srcResX,
+                    unsigned yRes, float alpha);
+void orc_fade32(uint32_t *dst, unsigned n, uint32_t RGB, uint8_t alpha);
+void orc_zoom32(uint32_t *dst, const uint32_t *src, unsigned xRes, unsigned yRes, float scale);
+
 #ifdef __cplusplus
 }
 #endif

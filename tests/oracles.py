@@ -65,6 +65,24 @@ _ORC = {
     "orc_exp_ps": (_f, [_f]),
     "orc_topixel": (C.c_uint32, [_p]),
     "orc_rsqrtss": (_f, [_f]),
+    "orc_mix32": (None, [_p, _p, _u, C.c_uint8]),
+    "orc_mixover32": (None, [_p, _p, _u]),
+    "orc_sub32": (None, [_p, _p, _u]),
+    "orc_excl32": (None, [_p, _p, _u]),
+    "orc_softlight32": (None, [_p, _p, _u]),
+    "orc_softlight32a": (None, [_p, _p, _u]),
+    "orc_softlight32aa": (None, [_p, _p, _u, _f]),
+    "orc_overlay32": (None, [_p, _p, _u]),
+    "orc_overlay32a": (None, [_p, _p, _u]),
+    "orc_darken32_50": (None, [_p, _p, _u]),
+    "orc_mulsrc32": (None, [_p, _p, _u]),
+    "orc_mulsrc32a": (None, [_p, _p, _u]),
+    "orc_mixsrc32": (None, [_p, _p, _u]),
+    "orc_mixsrc32s": (None, [_p, _p, _u, _u, _u]),
+    "orc_blitsrc32": (None, [_p, _p, _u, _u, _u]),
+    "orc_blitsrc32a": (None, [_p, _p, _u, _u, _u, _f]),
+    "orc_fade32": (None, [_p, _u, C.c_uint32, C.c_uint8]),
+    "orc_zoom32": (None, [_p, _p, _u, _u, _f]),
 }
 
 _REF = {
@@ -103,6 +121,24 @@ _REF = {
     "ref_st_draw": (None, [_i, _p, _f, _f]),
     "ref_st_gamma": (None, [_p, _f, _p]),
     "ref_st_topixel4": (None, [_p, _p]),
+    "ref_mix32": (None, [_p, _p, _u, C.c_uint8]),
+    "ref_mixover32": (None, [_p, _p, _u]),
+    "ref_sub32": (None, [_p, _p, _u]),
+    "ref_excl32": (None, [_p, _p, _u]),
+    "ref_softlight32": (None, [_p, _p, _u]),
+    "ref_softlight32a": (None, [_p, _p, _u]),
+    "ref_softlight32aa": (None, [_p, _p, _u, _f]),
+    "ref_overlay32": (None, [_p, _p, _u]),
+    "ref_overlay32a": (None, [_p, _p, _u]),
+    "ref_darken32_50": (None, [_p, _p, _u]),
+    "ref_mulsrc32": (None, [_p, _p, _u]),
+    "ref_mulsrc32a": (None, [_p, _p, _u]),
+    "ref_mixsrc32": (None, [_p, _p, _u]),
+    "ref_mixsrc32s": (None, [_p, _p, _u, _u, _u]),
+    "ref_blitsrc32": (None, [_p, _p, _u, _u, _u]),
+    "ref_blitsrc32a": (None, [_p, _p, _u, _u, _u, _f]),
+    "ref_fade32": (None, [_p, _u, C.c_uint32, C.c_uint8]),
+    "ref_zoom32": (None, [_p, _p, _u, _u, _f]),
 }
 
 # oracle / harness effect ids (oracle/ckd_oracle.h) of the effects' map
