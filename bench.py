@@ -268,6 +268,8 @@ def main() -> int:
         st_tex = np.ascontiguousarray(wl.noise_texture(11, 10).reshape(-1))
         st_tex_hi = np.ascontiguousarray(wl.noise_texture(12, 10).reshape(-1))
         ctx.fdtunnel_create(st_tex, st_tex_hi)
+        st_blur = wl.st_blur_maps(W, H)
+        ctx.spikey_blurmaps(st_blur, W, H)
         fr_st = ctx.empty(n_st * W * H)
         st_per = {}
         for fx in ["plasma", "nautilus", "spikey", "spikey_close", "sinuses", "laura", "fdtunnel"]:
@@ -351,6 +353,7 @@ def main() -> int:
                 refl = oracles.ref()
                 refl.ref_st_init()
                 refl.ref_st_textures(oracles.P(st_tex), oracles.P(st_tex_hi))
+                refl.ref_st_blurmaps(oracles.P(st_blur), oracles.P(st_blur[st_blur.size // 2:]))
             for fx in st_per:
                 name = "spikey" if fx == "spikey_close" else fx
                 frames = wl.st_frames(fx, 8)
@@ -370,7 +373,7 @@ def main() -> int:
                             pa = f.as_array()
                             orc.orc_st_draw(oracles.ST_IDS[fx], oracles.P(out_h), W, H, oracles.P(pa),
                                             oracles.P(lut_h), oracles.P(st_tex), oracles.P(st_tex_hi),
-                                            1, nthr)
+                                            oracles.P(st_blur), 1, nthr)
                     dt_c = time.perf_counter() - t_c
                 st_per[fx]["cpu_mpx_s"] = round(8 * W * H / dt_c / 1e6, 2)
                 st_per[fx]["cpu_kind"] = ("reference Draw (OpenMP, OMP_NUM_THREADS threads)"

@@ -176,6 +176,7 @@ SIGNATURES = {
     "ckd_fade32": [_p, _p, _u, C.c_uint32, C.c_uint8],
     "ckd_zoom32": [_p, _p, _p, _u, _u, _f],
     "ckd_fdtunnel_create": [_p, _p, _p],
+    "ckd_spikey_blurmaps": [_p, _p, _p, _u, _u],
     "ckd_fdtunnel_map": [_p, _p, _p, _u, _u, C.POINTER(FdTunnelParams), _u],
     "ckd_fdtunnel_draw": [_p, _p, _u, _u, C.POINTER(FdTunnelParams), _u],
     "ckd_spikey_map": [_p, _p, _u, _u, C.POINTER(SpikeyParams), _u, _i],

@@ -34,6 +34,10 @@ struct ckd_ctx {
     uint32_t *fdTex = nullptr, *fdTexHi = nullptr;
     uint32_t *stScratch = nullptr;
     size_t stScratchCap = 0;
+    // the close-up spike's two blur maps (Shadertoy_Create), for the output
+    // size they were given for
+    uint32_t *spikeBlurMaps = nullptr;
+    unsigned spikeBlurW = 0, spikeBlurH = 0;
 
     // landscape maps (Landscape_Create)
     uint8_t *lsHeight = nullptr;

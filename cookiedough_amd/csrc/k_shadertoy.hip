@@ -835,6 +835,13 @@ bool finite_params(const float *p, size_t n)
 
 namespace ckd {
 
+// compositing blends on device buffers (k_composite.hip)
+ckd_status comp_mix32(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, unsigned n, uint32_t alpha);
+ckd_status comp_softlight32aa(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, unsigned n,
+                              uint32_t iA);
+ckd_status comp_overlay32a(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, unsigned n);
+uint32_t softlight_alpha(float alpha);
+
 // Renders nFrames FX maps of one effect (frames: host array of per-frame
 // constants, already converted).
 static ckd_status st_launch(ckd_ctx *ctx, int fx, const StFrame *frames, unsigned nFrames,
@@ -953,6 +960,46 @@ ckd_status st_maps(ckd_ctx *ctx, int fx, const P *params, unsigned nFrames, uint
     return ckd::st_launch(ctx, fx, frames.data(), nFrames, maps, glows, resX, resY);
 }
 
+// clampf(lo, hi, v) = std::max(lo, std::min(hi, v)) (code/Std3DMath-stripped/Math.h:37-40)
+float clampf_(float lo, float hi, float v)
+{
+    const float m = (v < hi) ? v : hi;
+    return (lo < m) ? m : lo;
+}
+
+// the close-up's mix-blur layer on one FX map (code/shadertoy.cpp:665-701):
+// blend of the two blur maps (Mix32), box blurs in place, SoftLight32AA of
+// a blurred copy of the map by the blur map, Overlay32A back onto the map;
+// work = two FX-map scratch images
+ckd_status spikey_mix_blur(ckd_ctx *ctx, const ckd_spikey_params &p, uint32_t *map,
+                           unsigned mapW, unsigned mapH, uint32_t *blurMap, uint32_t *fx1)
+{
+    const float mbOpacity = clampf_(0.f, 1.f, p.closeMixBlurOpacity);
+    if (!(mbOpacity > 0.f)) return CKD_OK;
+    const unsigned n = mapW * mapH;
+    const size_t bytes = (size_t)n * sizeof(uint32_t);
+    const float mbMap = clampf_(0.f, 1.f, p.closeMixBlurMap);
+    const float mbBlur = clampf_(0.f, 100.f, p.closeMixBlur);
+    const float mbMapBlur = clampf_(0.f, 100.f, p.closeMixMapBlur);
+    const uint32_t *maps = ctx->spikeBlurMaps;
+    ckd_status s;
+    CKD_HIP(hipMemcpyAsync(blurMap, mbMap == 1.f ? maps + n : maps, bytes, hipMemcpyDeviceToDevice,
+                           ctx->stream));
+    if (mbMap != 0.f && mbMap != 1.f &&
+        (s = ckd::comp_mix32(ctx, blurMap, maps + n, n, (uint8_t)(unsigned)(mbMap * 255.f))) != CKD_OK)
+        return s;
+    CKD_HIP(hipMemcpyAsync(fx1, map, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    if (mbMapBlur >= 1.f &&
+        (s = ckd_blur07(ctx, blurMap, blurMap, mapW, mapH, box_blur_scale(mbMapBlur))) != CKD_OK)
+        return s;
+    if (mbBlur >= 1.f && (s = ckd_blur07(ctx, fx1, fx1, mapW, mapH, box_blur_scale(mbBlur))) != CKD_OK)
+        return s;
+    if ((s = ckd::comp_softlight32aa(ctx, fx1, blurMap, n,
+                                     ckd::softlight_alpha(tanhf(mbBlur + mbOpacity)))) != CKD_OK)
+        return s;
+    return ckd::comp_overlay32a(ctx, map, fx1, n);
+}
+
 // Map(s) into context scratch, then each frame's post-processing and
 // Fx_Blit_2x2 into dst, as the effect's Draw function.
 template <typename P>
@@ -962,15 +1009,20 @@ ckd_status st_draw(ckd_ctx *ctx, int fx, const P *params, unsigned nFrames, uint
     const unsigned mapW = resX / 2 + 4, mapH = resY / 2 + 4;
     const size_t mapPx = (size_t)mapW * mapH, outPx = (size_t)resX * resY;
     const bool glow = fx == FX_FDTUNNEL;
+    // maps, then glow maps (tunnel) or the mix-blur layer's two work images
     uint32_t *scratch = nullptr;
-    ckd_status s = ckd::st_scratch(ctx, mapPx * nFrames * (glow ? 2 : 1), &scratch);
+    ckd_status s = ckd::st_scratch(ctx, mapPx * nFrames * (glow ? 2 : 1) + 2 * mapPx, &scratch);
     if (s != CKD_OK) return s;
     uint32_t *maps = scratch, *glows = glow ? scratch + mapPx * nFrames : nullptr;
+    uint32_t *work = scratch + mapPx * nFrames * (glow ? 2 : 1);
     if ((s = st_maps(ctx, fx, params, nFrames, maps, glows, resX, resY, close)) != CKD_OK) return s;
     for (unsigned k = 0; k < nFrames; ++k) {
         uint32_t *map = maps + mapPx * k, *out = dst + outPx * k;
         if constexpr (std::is_same<P, ckd_spikey_params>::value) {
-            // Spikey_Draw, code/shadertoy.cpp:712-725
+            // Spikey_Draw, code/shadertoy.cpp:663-725
+            if (close && (s = spikey_mix_blur(ctx, params[k], map, mapW, mapH, work,
+                                              work + mapPx)) != CKD_OK)
+                return s;
             if (!close && params[k].distWarmup != 0.f &&
                 (s = ckd_blur07_h(ctx, map, map, mapW, mapH,
                                   box_blur_scale(1.f + params[k].distWarmup))) != CKD_OK)
@@ -1043,15 +1095,40 @@ ckd_status ckd_spikey_draw(ckd_ctx *ctx, uint32_t *dst, unsigned resX, unsigned 
 {
     ckd_status s = check_geometry("ckd_spikey_draw", resX, resY, nFrames, dst, frames);
     if (s != CKD_OK || (s = check_params("ckd_spikey_draw", frames, nFrames)) != CKD_OK) return s;
+    CKD_REQUIRE_CTX(ctx);
     if (close)
         for (unsigned k = 0; k < nFrames; ++k) {
-            const float o = frames[k].closeMixBlurOpacity;
-            // saturatef(o) > 0 (code/shadertoy.cpp:668-671)
-            CKD_REQUIRE(!(o > 0.f), "ckd_spikey_draw: frame %u: the close-up's mix-blur layer "
-                                    "(closeMixBlurOpacity > 0) is not supported", k);
+            // saturatef(opacity) > 0 turns the layer on (code/shadertoy.cpp:668-671)
+            CKD_REQUIRE(!(frames[k].closeMixBlurOpacity > 0.f) ||
+                            (ctx->spikeBlurMaps && ctx->spikeBlurW == resX && ctx->spikeBlurH == resY),
+                        "ckd_spikey_draw: frame %u: the close-up's mix-blur layer needs blur maps "
+                        "for %ux%u (ckd_spikey_blurmaps)", k, resX, resY);
         }
-    CKD_REQUIRE_CTX(ctx);
     return st_draw(ctx, FX_SPIKEY, frames, nFrames, dst, resX, resY, close != 0);
+}
+
+ckd_status ckd_spikey_blurmaps(ckd_ctx *ctx, const uint32_t *map1, const uint32_t *map2,
+                               unsigned resX, unsigned resY)
+{
+    CKD_REQUIRE(map1 && map2, "ckd_spikey_blurmaps: null map");
+    CKD_REQUIRE(resX >= 64 && resY >= 64 && resX <= 8192 && resY <= 8192 && resX % 8 == 0 &&
+                    resY % 2 == 0,
+                "ckd_spikey_blurmaps: output %ux%u: need 64..8192, width a multiple of 8, even height",
+                resX, resY);
+    CKD_REQUIRE_CTX(ctx);
+    const size_t px = (size_t)(resX / 2 + 4) * (resY / 2 + 4), bytes = px * sizeof(uint32_t);
+    if (ctx->spikeBlurMaps) {
+        CKD_HIP(hipStreamSynchronize(ctx->stream));
+        CKD_HIP(hipFree(ctx->spikeBlurMaps));
+        ctx->spikeBlurMaps = nullptr;
+    }
+    if (hipMalloc(&ctx->spikeBlurMaps, 2 * bytes) != hipSuccess)
+        return ckd::fail(CKD_ENOMEM, "ckd_spikey_blurmaps: cannot allocate the maps");
+    CKD_HIP(hipMemcpy(ctx->spikeBlurMaps, map1, bytes, hipMemcpyHostToDevice));
+    CKD_HIP(hipMemcpy(ctx->spikeBlurMaps + px, map2, bytes, hipMemcpyHostToDevice));
+    ctx->spikeBlurW = resX;
+    ctx->spikeBlurH = resY;
+    return CKD_OK;
 }
 
 ckd_status ckd_fdtunnel_create(ckd_ctx *ctx, const uint32_t *tex, const uint32_t *texHighlights)

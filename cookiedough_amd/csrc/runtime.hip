@@ -211,7 +211,7 @@ void ckd_destroy(ckd_ctx *ctx)
     void *bufs[] = {ctx->cosLut, ctx->lsHeight, ctx->lsColor, ctx->lsFog, ctx->tsHeight, ctx->tsColor,
                     ctx->tsFog, ctx->polarMap, ctx->polarInv, ctx->b26Scratch[0],
                     ctx->b26Scratch[1], ctx->staging, ctx->fdTex, ctx->fdTexHi,
-                    ctx->stScratch};
+                    ctx->stScratch, ctx->spikeBlurMaps};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete ctx;

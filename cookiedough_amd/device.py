@@ -186,6 +186,15 @@ class Context:
             raise ValueError("fdtunnel textures are 1024x1024")
         check(self.lib.ckd_fdtunnel_create(self.h, tex.ctypes.data, hi.ctypes.data))
 
+    def spikey_blurmaps(self, maps, resX=1280, resY=720):
+        """the close-up spike's two blur maps, back to back in `maps`"""
+        maps = np.ascontiguousarray(maps, np.uint32).reshape(-1)
+        n = (resX // 2 + 4) * (resY // 2 + 4)
+        if maps.size != 2 * n:
+            raise ValueError(f"need 2 x {n} pixels")
+        check(self.lib.ckd_spikey_blurmaps(self.h, maps.ctypes.data, maps[n:].ctypes.data, resX,
+                                           resY))
+
     @staticmethod
     def _frames(effect, frames):
         cls = _lib.ST_PARAMS[effect]

@@ -167,7 +167,7 @@ StTracks<22> s_spikeyTracks;
 StTracks<7> s_sinusesTracks;
 StTracks<6> s_lauraTracks;
 StTracks<14> s_tunnelTracks;
-uint32_t *s_spikeBlurMaps[2] = {nullptr, nullptr}; // loaded as the reference does
+uint32_t *s_spikeBlurMaps[2] = {nullptr, nullptr}; // the close-up's mix-blur layer
 
 // renders one frame on the GPU into the staging image and downloads it
 template <typename F> void st_draw(const char *name, uint32_t *pDest, F render)
@@ -335,7 +335,8 @@ bool Shadertoy_Create()
     s_spikeBlurMaps[0] = Image_Load32("assets/shadertoy/close-up-blur-map-1.png");
     s_spikeBlurMaps[1] = Image_Load32("assets/shadertoy/close-up-blur-map-2.png");
     if (!s_spikeBlurMaps[0] || !s_spikeBlurMaps[1]) return false;
-    if (ckd_fdtunnel_create(g_ctx, tex, texHi) != CKD_OK) {
+    if (ckd_fdtunnel_create(g_ctx, tex, texHi) != CKD_OK ||
+        ckd_spikey_blurmaps(g_ctx, s_spikeBlurMaps[0], s_spikeBlurMaps[1], kResX, kResY) != CKD_OK) {
         report("Shadertoy_Create");
         return false;
     }
@@ -365,9 +366,6 @@ void Nautilus_Draw(uint32_t *pDest, float time, float delta)
             [&](uint32_t *d) { return ckd_nautilus_draw(g_ctx, d, kResX, kResY, &p, 1); });
 }
 
-// The close-up's mix-blur layer (closeSpike:MixBlurOpacity > 0,
-// code/shadertoy.cpp:668-701) is not provided: such a call reports an
-// error through SetLastError and leaves pDest as it was.
 void Spikey_Draw(uint32_t *pDest, float time, float delta, bool close)
 {
     (void)delta;

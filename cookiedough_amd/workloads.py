@@ -335,3 +335,12 @@ def st_frames(effect: str, n: int, t0: float = 10.0):
     from . import _lib
     cls = _lib.ST_PARAMS["spikey" if effect == "spikey_close" else effect]
     return [cls(time=t0 + k / 60.0, **ST_DEMO[effect]) for k in range(n)]
+
+
+def st_blur_maps(resX: int = 1280, resY: int = 720) -> np.ndarray:
+    """the close-up spike's two blur maps for an output of resX x resY: two
+    smooth noise images of the FX map's size, back to back (uint32)"""
+    mw, mh = resX // 2 + 4, resY // 2 + 4
+    log2 = max(10, int(np.ceil(np.log2(max(mw, mh)))))
+    return np.concatenate([np.ascontiguousarray(noise_texture(13 + k, log2)[:mh, :mw]).reshape(-1)
+                           for k in range(2)])

@@ -401,9 +401,10 @@ CKD_API ckd_status ckd_nautilus_draw(ckd_ctx *ctx, uint32_t *dst, unsigned resX,
  * close = 0: RenderSpikeyMap_2x2_Distant (:525-598), or when distWarmup is
  *   non-zero _Distant_SpecularOnly (:600-659) followed by the in-place
  *   horizontal box blur of the map;
- * close = 1: RenderSpikeyMap_2x2_Close (:432-523).  Its mix-blur layer
- *   (closeMixBlurOpacity > 0) blends the reference's blur-map images and is
- *   not provided: ckd_spikey_draw returns CKD_EINVAL for it. */
+ * close = 1: RenderSpikeyMap_2x2_Close (:432-523), and in the Draw form,
+ *   when closeMixBlurOpacity > 0, the mix-blur layer (:665-701) over the two
+ *   blur maps given to ckd_spikey_blurmaps for the same output size
+ *   (CKD_EINVAL without them). */
 typedef struct ckd_spikey_params {
     float time;
     float speed;               /* spike:Speed               */
@@ -433,6 +434,10 @@ CKD_API ckd_status ckd_spikey_map(ckd_ctx *ctx, uint32_t *maps, unsigned resX, u
                                   const ckd_spikey_params *frames, unsigned nFrames, int close);
 CKD_API ckd_status ckd_spikey_draw(ckd_ctx *ctx, uint32_t *dst, unsigned resX, unsigned resY,
                                    const ckd_spikey_params *frames, unsigned nFrames, int close);
+/* the close-up's two blur maps (Shadertoy_Create, code/shadertoy.cpp:180-185),
+ * host memory, each the FX map's size for an output of resX x resY */
+CKD_API ckd_status ckd_spikey_blurmaps(ckd_ctx *ctx, const uint32_t *map1, const uint32_t *map2,
+                                       unsigned resX, unsigned resY);
 
 /* RenderSinMap_2x2 / Sinuses_Draw (code/shadertoy.cpp:862-988) */
 typedef struct ckd_sinuses_params {

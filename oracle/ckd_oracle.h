@@ -180,11 +180,12 @@ enum { ORC_ST_PLASMA = 0, ORC_ST_NAUTILUS = 1, ORC_ST_SPIKEY_DISTANT = 2,
 void orc_st_map(int effect, uint32_t *dst, uint32_t *glow, unsigned resX, unsigned resY,
                 const float *params, const float *lut, const uint32_t *tex,
                 const uint32_t *texHi, int rsqrtMode, int nthreads);
-/* the effect's Draw function into a resX x resY frame (0, or -1 when the
- * close-up's mix-blur layer is asked for) */
+/* the effect's Draw function into a resX x resY frame; blurMaps = the
+ * close-up's two blur maps (FX-map size each, back to back; NULL allowed
+ * when its mix-blur layer is off).  0, or -1 for the layer without maps. */
 int orc_st_draw(int effect, uint32_t *dst, unsigned resX, unsigned resY, const float *params,
-                const float *lut, const uint32_t *tex, const uint32_t *texHi, int rsqrtMode,
-                int nthreads);
+                const float *lut, const uint32_t *tex, const uint32_t *texHi,
+                const uint32_t *blurMaps, int rsqrtMode, int nthreads);
 /* one lane of exp_ps / log_ps (3rdparty/sse_mathfun.h) and ToPixel4 of one
  * pixel (code/shadertoy-util.h:136-146) */
 float orc_log_ps(float x);

@@ -890,34 +890,72 @@ static float box_blur_scale(float strength)
     return strength;
 }
 
+/* clampf(lo, hi, v), code/Std3DMath-stripped/Math.h:37-40 */
+static float clampf_(float lo, float hi, float v) { return std_max(lo, std_min(hi, v)); }
+
+/* the close-up's mix-blur layer, code/shadertoy.cpp:665-701: a blend of the
+ * two blur maps (Mix32), blurs (BoxBlur32 in place), SoftLight32AA of the
+ * blurred copy of the map by the blur map, Overlay32A back onto the map */
+static int spikey_mix_blur(uint32_t *map, unsigned mapW, unsigned mapH, const float *params,
+                           const uint32_t *blurMaps, int nthreads)
+{
+    const float mbOpacity = std_max(0.f, std_min(1.f, params[22]));
+    if (!(mbOpacity > 0.f)) return 0;
+    if (!blurMaps) return -1;
+    const size_t n = (size_t)mapW * mapH;
+    const float mbMap = clampf_(0.f, 1.f, params[19]);
+    const float mbBlur = clampf_(0.f, 100.f, params[20]);
+    const float mbMapBlur = clampf_(0.f, 100.f, params[21]);
+    uint32_t *blurMap = (uint32_t *)malloc(n * 4), *fx1 = (uint32_t *)malloc(n * 4);
+    if (!blurMap || !fx1) { free(blurMap); free(fx1); return -2; }
+    if (0.f == mbMap) memcpy(blurMap, blurMaps, n * 4);
+    else if (1.f == mbMap) memcpy(blurMap, blurMaps + n, n * 4);
+    else {
+        memcpy(blurMap, blurMaps, n * 4);
+        orc_mix32(blurMap, blurMaps + n, (unsigned)n, (uint8_t)(unsigned)(mbMap * 255.f));
+    }
+    memcpy(fx1, map, n * 4);
+    if (mbMapBlur >= 1.f) orc_blur07(blurMap, blurMap, mapW, mapH, box_blur_scale(mbMapBlur), nthreads);
+    if (mbBlur >= 1.f) orc_blur07(fx1, fx1, mapW, mapH, box_blur_scale(mbBlur), nthreads);
+    orc_softlight32aa(fx1, blurMap, (unsigned)n, tanhf(mbBlur + mbOpacity));
+    orc_overlay32a(map, fx1, (unsigned)n);
+    free(blurMap);
+    free(fx1);
+    return 0;
+}
+
 /* the effects' Draw functions: Plasma_Draw (code/shadertoy.cpp:276-280),
- * Nautilus_Draw (:385-396), Spikey_Draw (:661-727, close-up without the
- * mix-blur layer), Sinuses_Draw (:984-988), Laura_Draw (:1105-1109),
- * Tunnel_Draw (:840-861).  Returns -1 for the close-up's mix-blur layer
- * (closeMixBlurOpacity > 0), which needs the reference's blend maps. */
+ * Nautilus_Draw (:385-396), Spikey_Draw (:661-727), Sinuses_Draw
+ * (:984-988), Laura_Draw (:1105-1109), Tunnel_Draw (:840-861).  blurMaps:
+ * the close-up's two blur maps, back to back, each the FX map's size (may
+ * be NULL when the mix-blur layer is off; -1 when it is on without them). */
 int orc_st_draw(int effect, uint32_t *dst, unsigned resX, unsigned resY, const float *params,
-                const float *lut, const uint32_t *tex, const uint32_t *texHi, int rsqrtMode,
-                int nthreads)
+                const float *lut, const uint32_t *tex, const uint32_t *texHi,
+                const uint32_t *blurMaps, int rsqrtMode, int nthreads)
 {
     const unsigned mapW = resX / 2 + 4, mapH = resY / 2 + 4;
     const size_t n = (size_t)mapW * mapH;
-    if (effect == ORC_ST_SPIKEY_CLOSE && std_max(0.f, std_min(1.f, params[22])) > 0.f) return -1;
     uint32_t *map = (uint32_t *)calloc(n, 4), *glow = (uint32_t *)calloc(n, 4);
     if (!map || !glow) { free(map); free(glow); return -2; }
     orc_st_map(effect, map, glow, resX, resY, params, lut, tex, texHi, rsqrtMode, nthreads);
+    int rc = 0;
+    if (effect == ORC_ST_SPIKEY_CLOSE)
+        rc = spikey_mix_blur(map, mapW, mapH, params, blurMaps, nthreads);
     if (effect == ORC_ST_SPIKEY_DISTANT && params[10] != 0.f)
         orc_blur07_h(map, map, mapW, mapH, box_blur_scale(1.f + params[10]), nthreads);
     if (effect == ORC_ST_FDTUNNEL && 0 != (int)roundf(params[11])) {
-        const float litBlur = std_max(0.f, std_min(100.f, params[12]));
+        const float litBlur = clampf_(0.f, 100.f, params[12]);
         if (litBlur >= 1.f) orc_blur07(glow, glow, mapW, mapH, box_blur_scale(litBlur), nthreads);
         orc_add32(map, glow, (unsigned)n);
     }
-    orc_fx_blit_2x2(dst, map, resX, resY);
-    if (effect == ORC_ST_NAUTILUS) {
-        const float blur = box_blur_scale(params[5]);
-        if (0.f != blur) orc_blur07(dst, dst, resX, resY, blur, nthreads);
+    if (rc == 0) {
+        orc_fx_blit_2x2(dst, map, resX, resY);
+        if (effect == ORC_ST_NAUTILUS) {
+            const float blur = box_blur_scale(params[5]);
+            if (0.f != blur) orc_blur07(dst, dst, resX, resY, blur, nthreads);
+        }
     }
     free(map);
     free(glow);
-    return 0;
+    return rc;
 }

@@ -79,6 +79,16 @@ REF_API void ref_st_textures(uint32_t *tex, uint32_t *texHighlights)
     s_pFDTunnelTexHighlights = texHighlights;
 }
 
+/* the close-up's two blur maps (644x364 each) and the blend target
+ * Shadertoy_Create allocates (code/shadertoy.cpp:180-185) */
+REF_API void ref_st_blurmaps(uint32_t *map1, uint32_t *map2)
+{
+    s_pSpikeBlurMaps[0] = map1;
+    s_pSpikeBlurMaps[1] = map2;
+    if (nullptr == s_pSpikeBlurMap)
+        s_pSpikeBlurMap = static_cast<uint32_t*>(mallocAligned((1280*720)/2 * sizeof(uint32_t), kAlignTo));
+}
+
 /* one FX map (644x364) of an effect; the distant spike picks its renderer
  * as Spikey_Draw does (code/shadertoy.cpp:712-725) */
 REF_API void ref_st_map(int effect, uint32_t *dst, uint32_t *glow, float time)

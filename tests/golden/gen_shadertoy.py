@@ -15,7 +15,8 @@ reference's FX map (and of its Draw frame where the reference can draw it),
 a 32x32 crop of the map and the rsqrtss values of this CPU for a probe set:
 the reference's vNorm4 uses _mm_rsqrt_ps, whose table is CPU specific, so
 the hashes are only reproducible where the probe matches.  The tunnel's two
-1024x1024 textures are workloads.noise_texture(11, 10) and (12, 10).
+1024x1024 textures are workloads.noise_texture(11, 10) and (12, 10), the
+close-up's blur maps workloads.st_blur_maps(1280, 720).
 
 Run from the repository root:  python tests/golden/gen_shadertoy.py
 """
@@ -88,10 +89,13 @@ def main() -> None:
     tex = np.ascontiguousarray(wl.noise_texture(11, 10).reshape(-1))
     tex_hi = np.ascontiguousarray(wl.noise_texture(12, 10).reshape(-1))
     ref.ref_st_textures(P(tex), P(tex_hi))
+    blur = wl.st_blur_maps(W, H)
+    ref.ref_st_blurmaps(P(blur), P(blur[blur.size // 2:]))
 
     gold = {"generator": "tests/golden/gen_shadertoy.py", "resolution": [W, H], "map": [MW, MH],
             "crop": [[CROP[0].start, CROP[0].stop], [CROP[1].start, CROP[1].stop]],
             "textures": "workloads.noise_texture(11, 10), workloads.noise_texture(12, 10)",
+            "blur_maps": "workloads.st_blur_maps(1280, 720)",
             "rsqrtss_probe": {"x": PROBE, "y": [float(orc.orc_rsqrtss(x)) for x in PROBE]},
             "cases": []}
     arrays = {}
@@ -117,12 +121,9 @@ def main() -> None:
                 case["glow_sha256"] = sha(g)
                 case["oracle_rsqrtss_equal"] &= bool((og == g).all())
                 arrays[f"glow_crop_{k}"] = g.reshape(MH, MW)[CROP].copy()
-            # the close-up's mix-blur layer reads blend maps the harness
-            # cannot load: Draw only where it is off
-            if not (fid == "spikey_close" and vals["closeMixBlurOpacity"] > 0.0):
-                d = aligned_empty(W * H)
-                ref.ref_st_draw(oracles.ST_IDS[fid], P(d), t, 1.0)
-                case["draw_sha256"] = sha(d)
+            d = aligned_empty(W * H)
+            ref.ref_st_draw(oracles.ST_IDS[fid], P(d), t, 1.0)
+            case["draw_sha256"] = sha(d)
             gold["cases"].append(case)
             print(fid, row, "oracle(rsqrtss) == reference:", case["oracle_rsqrtss_equal"])
     (HERE / "shadertoy.json").write_text(json.dumps(gold, indent=1) + "\n")

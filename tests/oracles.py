@@ -60,7 +60,7 @@ _ORC = {
     "orc_lutsinf": (_f, [_p, _f]),
     "orc_tape_warp32": (None, [_p, _p, _u, _u, _f, _f, _p]),
     "orc_st_map": (None, [_i, _p, _p, _u, _u, _p, _p, _p, _p, _i, _i]),
-    "orc_st_draw": (_i, [_i, _p, _u, _u, _p, _p, _p, _p, _i, _i]),
+    "orc_st_draw": (_i, [_i, _p, _u, _u, _p, _p, _p, _p, _p, _i, _i]),
     "orc_log_ps": (_f, [_f]),
     "orc_exp_ps": (_f, [_f]),
     "orc_topixel": (C.c_uint32, [_p]),
@@ -117,6 +117,7 @@ _REF = {
     "ref_st_row": (None, [C.c_double]),
     "ref_st_get": (C.c_double, [C.c_char_p]),
     "ref_st_textures": (None, [_p, _p]),
+    "ref_st_blurmaps": (None, [_p, _p]),
     "ref_st_map": (None, [_i, _p, _p, _f]),
     "ref_st_draw": (None, [_i, _p, _f, _f]),
     "ref_st_gamma": (None, [_p, _f, _p]),

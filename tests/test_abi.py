@@ -116,9 +116,11 @@ def test_shadertoy_argument_validation_without_gpu():
     p.gamma = float("nan")
     assert lib.ckd_plasma_map(None, 8, 1280, 720, C.byref(p), 1) == CKD_EINVAL
     assert "finite" in _lib.last_error()
-    s = _lib.SpikeyParams(time=1.0, closeMixBlurOpacity=0.25)
+    s = _lib.SpikeyParams(time=1.0, closeMixBlurOpacity=float("inf"))
     assert lib.ckd_spikey_draw(None, 8, 1280, 720, C.byref(s), 1, 1) == CKD_EINVAL
-    assert "mix-blur" in _lib.last_error()
+    assert "finite" in _lib.last_error()
+    assert lib.ckd_spikey_blurmaps(None, None, 8, 1280, 720) == CKD_EINVAL
+    assert "null map" in _lib.last_error()
     t = _lib.FdTunnelParams(time=1.0)
     assert lib.ckd_fdtunnel_map(None, 8, None, 1280, 720, C.byref(t), 1) == CKD_EINVAL
     assert "glow" in _lib.last_error()

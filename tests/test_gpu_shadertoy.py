@@ -44,7 +44,9 @@ def st(ctx, orc):
     tex = np.ascontiguousarray(wl.noise_texture(11, 10).reshape(-1))
     tex_hi = np.ascontiguousarray(wl.noise_texture(12, 10).reshape(-1))
     ctx.fdtunnel_create(tex, tex_hi)
-    return lut, tex, tex_hi
+    blur = wl.st_blur_maps(W, H)
+    ctx.spikey_blurmaps(blur, W, H)
+    return lut, tex, tex_hi, blur
 
 
 def params_of(case):
@@ -54,7 +56,7 @@ def params_of(case):
 
 
 def orc_map(orc, st, eid, prm, W, H):
-    lut, tex, tex_hi = st
+    lut, tex, tex_hi, _ = st
     mw, mh = W // 2 + 4, H // 2 + 4
     m, g = aligned_empty(mw * mh), aligned_empty(mw * mh)
     p = prm.as_array()
@@ -93,7 +95,7 @@ def test_maps_match_oracle(ctx, orc, st, gold):
 
 
 def test_draws_match_oracle(ctx, orc, st, gold):
-    lut, tex, tex_hi = st
+    lut, tex, tex_hi, blur = st
     g, _ = gold
     for c in g["cases"]:
         if "draw_sha256" not in c:
@@ -107,7 +109,7 @@ def test_draws_match_oracle(ctx, orc, st, gold):
         od = aligned_empty(W * H)
         p = prm.as_array()
         assert orc.orc_st_draw(oracles.ST_IDS[c["effect"]], P(od), W, H, P(p), P(lut), P(tex),
-                               P(tex_hi), 0, 16) == 0
+                               P(tex_hi), P(blur), 0, 16) == 0
         check_close(d.numpy(), od, f"{c['effect']} row {c['row']} draw",
                     exact=c["effect"] == "plasma")
 
@@ -167,13 +169,33 @@ def test_other_resolutions_match_oracle(ctx, orc, st, gold, res):
             check_close(gg, og, f"{fid} {res} glow")
 
 
-def test_close_mix_blur_layer_rejected(ctx, st):
-    p = _lib.SpikeyParams(time=1.0, speed=1.0, specular=4.0, gamma=1.0, closeScale=1.0,
+def test_close_mix_blur_layer(ctx, orc, st):
+    """the close-up's mix-blur layer: needs blur maps for the output size;
+    with them, every branch (map blend 0 / 1 / between, blurs on / off)
+    against the oracle"""
+    lut, tex, tex_hi, blur = st
+    from cookiedough_amd.device import Context
+    p = _lib.SpikeyParams(time=3.0, speed=10.0, roll=4.0, specular=8.0, hue=33.0, gamma=1.66,
+                          closeXOffs=0.314, closeZOffs=0.5, closeZOffsScale=2.0,
+                          closeNormalGrain=0.33, closeScale=0.88, closeAspectMul=1.0,
                           closeMixBlurOpacity=0.5)
-    d = ctx.empty(W * H)
-    with pytest.raises(_lib.CkdError, match="mix-blur"):
-        ctx.st_draw("spikey", d, W, H, p, close=True)
-    # the map alone is fine
-    m = ctx.empty((W // 2 + 4) * (H // 2 + 4))
-    ctx.st_map("spikey", m, W, H, p, close=True)
-    ctx.sync()
+    fresh = Context(0)
+    try:
+        d = fresh.empty(W * H)
+        with pytest.raises(_lib.CkdError, match="blur maps"):
+            fresh.st_draw("spikey", d, W, H, p, close=True)
+    finally:
+        fresh.close()
+    for mbMap, mbBlur, mbMapBlur, opacity in [(0.0, 0.0, 0.0, 0.5), (1.0, 12.0, 0.0, 0.072),
+                                              (0.35, 30.0, 8.0, 1.0), (0.7, 0.5, 40.0, 3.0)]:
+        q = _lib.SpikeyParams.from_buffer_copy(p)
+        q.closeMixBlurMap, q.closeMixBlur, q.closeMixMapBlur = mbMap, mbBlur, mbMapBlur
+        q.closeMixBlurOpacity = opacity
+        d = ctx.empty(W * H)
+        ctx.st_draw("spikey", d, W, H, q, close=True)
+        ctx.sync()
+        od = aligned_empty(W * H)
+        a = q.as_array()
+        assert orc.orc_st_draw(oracles.ST_IDS["spikey_close"], P(od), W, H, P(a), P(lut), P(tex),
+                               P(tex_hi), P(blur), 0, 16) == 0
+        check_close(d.numpy(), od, f"mix-blur {mbMap} {mbBlur} {mbMapBlur} {opacity}")

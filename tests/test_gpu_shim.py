@@ -129,10 +129,11 @@ def st_assets(tmp_path_factory):
     tex_hi = np.ascontiguousarray(wl.noise_texture(12, 10).reshape(-1))
     (d / "nytrik-hextexture.png.bin").write_bytes(tex.tobytes())
     (d / "nytrik-hextexture-fx.png.bin").write_bytes(tex_hi.tobytes())
-    blur = np.zeros(644 * 364, np.uint32)
-    (d / "close-up-blur-map-1.png.bin").write_bytes(blur.tobytes())
-    (d / "close-up-blur-map-2.png.bin").write_bytes(blur.tobytes())
-    return d, tex, tex_hi
+    blur = wl.st_blur_maps(1280, 720)
+    n = blur.size // 2
+    (d / "close-up-blur-map-1.png.bin").write_bytes(blur[:n].tobytes())
+    (d / "close-up-blur-map-2.png.bin").write_bytes(blur[n:].tobytes())
+    return d, tex, tex_hi, blur
 
 
 def test_shadertoy_draws(orc, st_assets, tmp_path):
@@ -143,7 +144,7 @@ def test_shadertoy_draws(orc, st_assets, tmp_path):
     import oracles
     from cookiedough_amd import _lib
     from helpers import error_stats
-    d, tex, tex_hi = st_assets
+    d, tex, tex_hi, blur = st_assets
     lut = np.zeros(2049, np.float32)
     orc.orc_cos_lut(P(lut))
     gold = json.loads((Path(__file__).resolve().parent / "golden" / "shadertoy.json").read_text())
@@ -163,7 +164,7 @@ def test_shadertoy_draws(orc, st_assets, tmp_path):
         want = np.zeros(1280 * 720, np.uint32)
         p = np.asarray(c["params"], np.float32)
         assert orc.orc_st_draw(oracles.ST_IDS[c["effect"]], P(want), 1280, 720, P(p), P(lut),
-                               P(tex), P(tex_hi), 0, 8) == 0
+                               P(tex), P(tex_hi), P(blur), 0, 8) == 0
         es = error_stats(got, want)
         print(c["effect"], es, flush=True)
         assert es["max"] <= 2 and es["frac_diff"] <= 0.005, (c["effect"], es)

@@ -42,11 +42,11 @@ def st_inputs(orc):
     orc.orc_cos_lut(P(lut))
     tex = np.ascontiguousarray(wl.noise_texture(11, 10).reshape(-1))
     tex_hi = np.ascontiguousarray(wl.noise_texture(12, 10).reshape(-1))
-    return lut, tex, tex_hi
+    return lut, tex, tex_hi, wl.st_blur_maps(W, H)
 
 
 def orc_map(orc, st_inputs, effect, params, mode, W=W, H=H):
-    lut, tex, tex_hi = st_inputs
+    lut, tex, tex_hi, _ = st_inputs
     mw, mh = W // 2 + 4, H // 2 + 4
     m, g = aligned_empty(mw * mh), aligned_empty(mw * mh)
     p = np.ascontiguousarray(params, np.float32)
@@ -78,7 +78,7 @@ def test_oracle_matches_reference_hashes(orc, gold, st_inputs):
     g, _ = gold
     if not probe_matches(orc, g):
         pytest.skip("this CPU's rsqrtss table differs from the fixture's")
-    lut, tex, tex_hi = st_inputs
+    lut, tex, tex_hi, blur = st_inputs
     for c in g["cases"]:
         m, gl = orc_map(orc, st_inputs, c["effect"], c["params"], 1)
         assert sha(m) == c["map_sha256"], (c["effect"], c["row"])
@@ -88,7 +88,7 @@ def test_oracle_matches_reference_hashes(orc, gold, st_inputs):
             d = aligned_empty(W * H)
             p = np.ascontiguousarray(c["params"], np.float32)
             assert orc.orc_st_draw(oracles.ST_IDS[c["effect"]], P(d), W, H, P(p), P(lut), P(tex),
-                                   P(tex_hi), 1, 8) == 0
+                                   P(tex_hi), P(blur), 1, 8) == 0
             assert sha(d) == c["draw_sha256"], (c["effect"], c["row"], "draw")
 
 
@@ -207,6 +207,10 @@ def random_tracks(seed: int, rows: int) -> dict:
         "closeSpike:zOffs": u(-1, 1), "closeSpike:zOffsScale": u(0, 1),
         "closeSpike:NormalGrain": u(0.1, 1), "closeSpike:Scale": u(0.3, 1.3),
         "closeSpike:Rim": pick(0.0, 1.0), "closeSpike:AspectMul": pick(0.0, 1.0),
+        "closeSpike:MixBlurOpacity": [0.0, 0.4, 1.7][:rows] + u(0, 1)[3:],
+        "closeSpike:MixBlurMap": [0.0, 1.0, 0.35][:rows] + u(0, 1)[3:],
+        "closeSpike:MixBlur": [0.0, 7.0, 30.0][:rows] + u(0, 40)[3:],
+        "closeSpike:MixMapBlur": [0.0, 12.0, 0.5][:rows] + u(0, 40)[3:],
         "sinusesTunnel:Specular": u(1, 30), "sinusesTunnel:Roll": u(-3, 0),
         "sinusesTunnel:Speed": u(-5, 3), "sinusesTunnel:OffsX": u(0, 0.4),
         "sinusesTunnel:Gamma": u(2, 8), "sinusesTunnel:Hue": u(40, 55),
@@ -226,13 +230,15 @@ def test_oracle_vs_reference_through_rocket(orc, refl, st_inputs, tmp_path):
     """random track values, written as Rocket track files and read by the
     reference's own player: oracle (this CPU's rsqrtss) == reference, maps
     and Draw frames of every effect"""
-    lut, tex, tex_hi = st_inputs
+    lut, tex, tex_hi, blur = st_inputs
     rows = 3
     tracks = random_tracks(7, rows)
     oracles.write_rocket_tracks(tmp_path, tracks)
     assert refl.ref_st_init() == 0
     assert refl.ref_st_rocket(str(tmp_path).encode()) == 0
     refl.ref_st_textures(P(tex), P(tex_hi))
+    n = blur.size // 2
+    refl.ref_st_blurmaps(P(blur), P(blur[n:]))
     try:
         for row in range(rows):
             refl.ref_st_row(float(row))
@@ -252,7 +258,8 @@ def test_oracle_vs_reference_through_rocket(orc, refl, st_inputs, tmp_path):
                     assert (og == rg).all(), (fid, row)
                 rd, od = aligned_empty(W * H), aligned_empty(W * H)
                 refl.ref_st_draw(eid, P(rd), t, 1.0)
-                assert orc.orc_st_draw(eid, P(od), W, H, P(prm), P(lut), P(tex), P(tex_hi), 1, 8) == 0
+                assert orc.orc_st_draw(eid, P(od), W, H, P(prm), P(lut), P(tex), P(tex_hi), P(blur),
+                                       1, 8) == 0
                 assert (od == rd).all(), (fid, row, "draw", int((od != rd).sum()))
     finally:
         # back to the demo's tracks for any later user of the harness
