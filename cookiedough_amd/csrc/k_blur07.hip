@@ -169,85 +169,103 @@ __global__ __launch_bounds__(256) void k_blur07_v(uint32_t *__restrict__ dst,
 }
 
 // in place: the exact recurrence of one line (code/deprecated/boxblur.cpp:
-// 81-125 / 165-219 with pSrc == pDest).  Lane per line; the outputs the
-// subtract needs again `median` steps later live in an LDS ring of `ring`
-// entries per lane (ring > median, power of two).
-CKD_DEV void sat_add(uint32_t acc[4], uint32_t v, int k)
+// 81-125 / 165-219 with pSrc == pDest).  The four channels are independent
+// recurrences (u16 saturation per channel), so each line takes four lanes,
+// one per channel.  The outputs the subtract needs again `median` steps
+// later live in an LDS ring of `ring` bytes per lane (ring > median, power
+// of two), laid out [entry][lane] so a wave's accesses are contiguous.  The
+// input pixels are always ahead of the outputs, so they are read from
+// global memory; the loops are unrolled so those loads issue ahead of the
+// dependent chain.
+CKD_DEV uint32_t sat16(uint32_t s) { return s > 65535u ? 65535u : s; }
+
+CKD_DEV uint32_t div_byte(uint32_t acc, uint32_t div)
 {
-    const uint32_t s = acc[k] + v;
-    acc[k] = s > 65535u ? 65535u : s;
+    const uint32_t q = (acc * div) >> 16;
+    return q > 32767u ? 0u : (q > 255u ? 255u : q);
 }
 
-CKD_DEV void b07_add(uint32_t acc[4], uint32_t rem[4], uint32_t px, uint32_t shift)
+__global__ __launch_bounds__(256) void k_blur07_inplace(uint32_t *img, int nLines,
+                                                        int len, size_t lineStride,
+                                                        size_t pixStride, int ring, B07Params p)
 {
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint32_t v = (px >> (8 * k)) & 0xff;
-        sat_add(acc, rem[k], k);
-        rem[k] = v >> shift;
-        sat_add(acc, v - rem[k], k);
-    }
-}
-
-CKD_DEV void b07_sub(uint32_t acc[4], uint32_t rem[4], uint32_t px, uint32_t shift)
-{
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint32_t v = (px >> (8 * k)) & 0xff;
-        acc[k] = acc[k] > rem[k] ? acc[k] - rem[k] : 0;
-        rem[k] = v >> shift;
-        const uint32_t d = v - rem[k];
-        acc[k] = acc[k] > d ? acc[k] - d : 0;
-    }
-}
-
-CKD_DEV uint32_t b07_div(const uint32_t acc[4], uint32_t div)
-{
-    uint32_t px = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint32_t q = (acc[k] * div) >> 16;
-        px |= (q > 32767u ? 0u : (q > 255u ? 255u : q)) << (8 * k);
-    }
-    return px;
-}
-
-__global__ __launch_bounds__(64) void k_blur07_inplace(uint32_t *__restrict__ img, int nLines,
-                                                       int len, size_t lineStride,
-                                                       size_t pixStride, int ring, B07Params p)
-{
-    extern __shared__ uint32_t ringBuf[];
-    const int line = blockIdx.x * blockDim.x + threadIdx.x;
+    extern __shared__ uint8_t ringBuf[];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int line = t >> 2, k = t & 3;
     if (line >= nLines) return;
-    uint32_t *r = ringBuf + (size_t)threadIdx.x * ring;
+    uint8_t *r = ringBuf + threadIdx.x;
     const int rmask = ring - 1;
-    uint32_t *base = img + (size_t)line * lineStride;
-    uint32_t acc[4] = {0, 0, 0, 0}, addRem[4] = {0, 0, 0, 0}, subRem[4] = {0, 0, 0, 0};
+    // the input bytes read are always ahead of the output bytes written (a
+    // pixel is read as input before its output is stored), so the two
+    // views never touch a byte in conflicting order: restrict lets the
+    // loads issue ahead of the stores
+    const uint8_t *__restrict__ in = reinterpret_cast<const uint8_t *>(img + (size_t)line * lineStride) + k;
+    uint8_t *__restrict__ out = reinterpret_cast<uint8_t *>(img + (size_t)line * lineStride) + k;
+    const size_t step = pixStride * 4;
+    const uint32_t shift = p.shift;
+    uint32_t acc = 0, addRem = 0, subRem = 0;
     const int e = (int)p.edgeSpan, m = (int)p.median;
     const int full = len - (m + e);
     int addPos = 0, dest = 0, subPos = 0;
-
-    for (int i = 0; i < e; ++i) b07_add(acc, addRem, base[(size_t)(addPos++) * pixStride], p.shift);
+    auto add = [&](uint32_t v) {
+        acc = sat16(acc + addRem);
+        addRem = v >> shift;
+        acc = sat16(acc + (v - addRem));
+    };
+    auto sub = [&](uint32_t v) {
+        acc = acc > subRem ? acc - subRem : 0;
+        subRem = v >> shift;
+        const uint32_t d = v - subRem;
+        acc = acc > d ? acc - d : 0;
+    };
+    auto emit = [&](uint32_t o) {
+        r[(size_t)(dest & rmask) * 256] = (uint8_t)o;
+        out[(size_t)dest * step] = (uint8_t)o;
+        ++dest;
+    };
+#pragma unroll 8
+    for (int i = 0; i < e; ++i) add(in[(size_t)(addPos++) * step]);
+#pragma unroll 8
     for (int i = 0; i < m; ++i) {
-        b07_add(acc, addRem, base[(size_t)(addPos++) * pixStride], p.shift);
-        const uint32_t o = b07_div(acc, p.edgeDivs[i]);
-        r[dest & rmask] = o;
-        base[(size_t)(dest++) * pixStride] = o;
+        add(in[(size_t)(addPos++) * step]);
+        emit(div_byte(acc, p.edgeDivs[i]));
     }
-    for (int i = 0; i < full; ++i) {
-        b07_add(acc, addRem, base[(size_t)(addPos++) * pixStride], p.shift);
-        b07_sub(acc, subRem, r[(subPos++) & rmask], p.shift);
-        const uint32_t o = b07_div(acc, p.fullDiv);
-        r[dest & rmask] = o;
-        base[(size_t)(dest++) * pixStride] = o;
+    // the long middle run: eight inputs at a time, the next eight loaded
+    // before this eight's outputs are stored (program order keeps the loads
+    // ahead of the possibly-aliasing stores)
+    constexpr int kChunk = 8;
+    const int chunks = full / kChunk;
+    uint32_t nxt[kChunk];
+    if (chunks > 0) {
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) nxt[j] = in[(size_t)(addPos + j) * step];
     }
-    if (p.subEdges)
-        for (int k = 0; k < 4; ++k) sat_add(acc, addRem[k], k);
+    for (int c = 0; c < chunks; ++c) {
+        uint32_t cur[kChunk];
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) cur[j] = nxt[j];
+        addPos += kChunk;
+        if (c + 1 < chunks) {
+#pragma unroll
+            for (int j = 0; j < kChunk; ++j) nxt[j] = in[(size_t)(addPos + j) * step];
+        }
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+            add(cur[j]);
+            sub(r[(size_t)((subPos++) & rmask) * 256]);
+            emit(div_byte(acc, p.fullDiv));
+        }
+    }
+    for (int i = chunks * kChunk; i < full; ++i) {
+        add(in[(size_t)(addPos++) * step]);
+        sub(r[(size_t)((subPos++) & rmask) * 256]);
+        emit(div_byte(acc, p.fullDiv));
+    }
+    if (p.subEdges) acc = sat16(acc + addRem);
+#pragma unroll 8
     for (int i = e; i > 0; --i) {
-        b07_sub(acc, subRem, r[(subPos++) & rmask], p.shift);
-        const uint32_t o = b07_div(acc, p.edgeDivs[i - 1]);
-        r[dest & rmask] = o;
-        base[(size_t)(dest++) * pixStride] = o;
+        sub(r[(size_t)((subPos++) & rmask) * 256]);
+        emit(div_byte(acc, p.edgeDivs[i - 1]));
     }
 }
 
@@ -281,8 +299,8 @@ ckd_status launch_h(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, unsigned x
 {
     if (dst == src) {
         const int ring = ring_for(p.median);
-        const size_t lds = (size_t)ring * 64 * sizeof(uint32_t);
-        k_blur07_inplace<<<ckd_div_up(yRes, 64), 64, lds, ctx->stream>>>(
+        const size_t lds = (size_t)ring * 256;
+        k_blur07_inplace<<<ckd_div_up((size_t)yRes * 4, 256), 256, lds, ctx->stream>>>(
             dst, (int)yRes, (int)xRes, xRes, 1, ring, p);
         return ckd::check_launch("blur07_h (in place)");
     }
@@ -297,8 +315,8 @@ ckd_status launch_v(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, unsigned x
 {
     if (dst == src) {
         const int ring = ring_for(p.median);
-        const size_t lds = (size_t)ring * 64 * sizeof(uint32_t);
-        k_blur07_inplace<<<ckd_div_up(xRes, 64), 64, lds, ctx->stream>>>(
+        const size_t lds = (size_t)ring * 256;
+        k_blur07_inplace<<<ckd_div_up((size_t)xRes * 4, 256), 256, lds, ctx->stream>>>(
             dst, (int)xRes, (int)yRes, 1, xRes, ring, p);
         return ckd::check_launch("blur07_v (in place)");
     }
