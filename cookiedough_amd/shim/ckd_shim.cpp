@@ -525,3 +525,79 @@ void TapeWarp32(uint32_t *pDest, const uint32_t *pSrc, unsigned xRes, unsigned y
         return ckd_tape_warp32(g_ctx, d, s, xRes, yRes, strength, speed);
     });
 }
+
+// ---- compositing blends (code/util.cpp:58-811) ---------------------------
+
+#define CKD_SHIM_FLAT_BLEND(Name, fn)                                                           \
+    void Name(uint32_t *pDest, const uint32_t *pSrc, unsigned numPixels)                        \
+    {                                                                                           \
+        blit2(#Name, pDest, numPixels, pSrc, numPixels,                                         \
+              [&](uint32_t *d, uint32_t *s) { return fn(g_ctx, d, s, numPixels); });            \
+    }
+
+CKD_SHIM_FLAT_BLEND(MixOver32, ckd_mixover32)
+CKD_SHIM_FLAT_BLEND(Sub32, ckd_sub32)
+CKD_SHIM_FLAT_BLEND(Excl32, ckd_excl32)
+CKD_SHIM_FLAT_BLEND(SoftLight32, ckd_softlight32)
+CKD_SHIM_FLAT_BLEND(SoftLight32A, ckd_softlight32a)
+CKD_SHIM_FLAT_BLEND(Overlay32, ckd_overlay32)
+CKD_SHIM_FLAT_BLEND(Overlay32A, ckd_overlay32a)
+CKD_SHIM_FLAT_BLEND(Darken32_50, ckd_darken32_50)
+CKD_SHIM_FLAT_BLEND(MulSrc32, ckd_mulsrc32)
+CKD_SHIM_FLAT_BLEND(MulSrc32A, ckd_mulsrc32a)
+CKD_SHIM_FLAT_BLEND(MixSrc32, ckd_mixsrc32)
+
+#undef CKD_SHIM_FLAT_BLEND
+
+void Mix32(uint32_t *pDest, const uint32_t *pSrc, unsigned numPixels, uint8_t alpha)
+{
+    blit2("Mix32", pDest, numPixels, pSrc, numPixels,
+          [&](uint32_t *d, uint32_t *s) { return ckd_mix32(g_ctx, d, s, numPixels, alpha); });
+}
+
+void SoftLight32AA(uint32_t *pDest, const uint32_t *pSrc, unsigned numPixels, float alpha)
+{
+    blit2("SoftLight32AA", pDest, numPixels, pSrc, numPixels, [&](uint32_t *d, uint32_t *s) {
+        return ckd_softlight32aa(g_ctx, d, s, numPixels, alpha);
+    });
+}
+
+void MixSrc32S(uint32_t *pDest, const uint32_t *pSrc, unsigned destResX, unsigned destResY,
+               unsigned srcStride)
+{
+    // the source rows reach (destResY-1)*srcStride + destResX pixels
+    const size_t srcPixels = destResY ? (size_t)(destResY - 1) * srcStride + destResX : 0;
+    blit2("MixSrc32S", pDest, (size_t)destResX * destResY, pSrc, srcPixels,
+          [&](uint32_t *d, uint32_t *s) {
+              return ckd_mixsrc32s(g_ctx, d, s, destResX, destResY, srcStride);
+          });
+}
+
+void BlitSrc32(uint32_t *pDest, const uint32_t *pSrc, unsigned destResX, unsigned srcResX,
+               unsigned yRes)
+{
+    blit2("BlitSrc32", pDest, (size_t)destResX * yRes, pSrc, (size_t)srcResX * yRes,
+          [&](uint32_t *d, uint32_t *s) { return ckd_blitsrc32(g_ctx, d, s, destResX, srcResX, yRes); });
+}
+
+void BlitSrc32A(uint32_t *pDest, const uint32_t *pSrc, unsigned destResX, unsigned srcResX,
+                unsigned yRes, float alpha)
+{
+    blit2("BlitSrc32A", pDest, (size_t)destResX * yRes, pSrc, (size_t)srcResX * yRes,
+          [&](uint32_t *d, uint32_t *s) {
+              return ckd_blitsrc32a(g_ctx, d, s, destResX, srcResX, yRes, alpha);
+          });
+}
+
+void Fade32(uint32_t *pDest, unsigned int numPixels, uint32_t RGB, uint8_t alpha)
+{
+    filter("Fade32", pDest, pDest, numPixels, true,
+           [&](uint32_t *d, uint32_t *) { return ckd_fade32(g_ctx, d, numPixels, RGB, alpha); });
+}
+
+void Zoom32(uint32_t *pDest, const uint32_t *pSrc, unsigned xRes, unsigned yRes, float scale)
+{
+    filter("Zoom32", pDest, pSrc, (size_t)xRes * yRes, false, [&](uint32_t *d, uint32_t *s) {
+        return ckd_zoom32(g_ctx, d, s, xRes, yRes, scale);
+    });
+}

@@ -69,3 +69,26 @@ void BlitAdd32A(uint32_t *pDest, const uint32_t *pSrc, unsigned destResX, unsign
                 unsigned yRes, float alpha);
 void TapeWarp32(uint32_t *pDest, const uint32_t *pSrc, unsigned xRes, unsigned yRes,
                 float strength, float speed);
+
+// code/util.h:70-122, the compositing blends
+void Zoom32(uint32_t *pDest, const uint32_t *pSrc, unsigned xRes, unsigned yRes, float scale);
+void Mix32(uint32_t *pDest, const uint32_t *pSrc, unsigned numPixels, uint8_t alpha);
+void MixOver32(uint32_t *pDest, const uint32_t *pSrc, unsigned numPixels);
+void Sub32(uint32_t *pDest, const uint32_t *pSrc, unsigned numPixels);
+void Excl32(uint32_t *pDest, const uint32_t *pSrc, unsigned numPixels);
+void SoftLight32(uint32_t *pDest, const uint32_t *pSrc, unsigned numPixels);
+void SoftLight32A(uint32_t *pDest, const uint32_t *pSrc, unsigned numPixels);
+void SoftLight32AA(uint32_t *pDest, const uint32_t *pSrc, unsigned numPixels, float alpha);
+void Overlay32(uint32_t *pDest, const uint32_t *pSrc, unsigned numPixels);
+void Overlay32A(uint32_t *pDest, const uint32_t *pSrc, unsigned numPixels);
+void Darken32_50(uint32_t *pDest, const uint32_t *pSrc, unsigned numPixels);
+void MulSrc32(uint32_t *pDest, const uint32_t *pSrc, unsigned int numPixels);
+void MulSrc32A(uint32_t *pDest, const uint32_t *pSrc, unsigned int numPixels);
+void MixSrc32(uint32_t *pDest, const uint32_t *pSrc, unsigned int numPixels);
+void MixSrc32S(uint32_t *pDest, const uint32_t *pSrc, unsigned destResX, unsigned destResY,
+               unsigned srcStride);
+void BlitSrc32(uint32_t *pDest, const uint32_t *pSrc, unsigned destResX, unsigned srcResX,
+               unsigned yRes);
+void BlitSrc32A(uint32_t *pDest, const uint32_t *pSrc, unsigned destResX, unsigned srcResX,
+                unsigned yRes, float alpha);
+void Fade32(uint32_t *pDest, unsigned int numPixels, uint32_t RGB, uint8_t alpha);

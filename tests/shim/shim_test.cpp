@@ -10,6 +10,8 @@
 //   shim_test fxblit <in 644x364> <out>
 //   shim_test blur07 <in> <W> <H> <strength> <h|v|2d> <inplace 0|1> <out>
 //   shim_test tapewarp <in> <W> <H> <strength> <speed> <out>
+//   shim_test blend <op> <dst> <src> <n> <arg> <out>       (flat blends, Mix32/Fade32/
+//                                                         SoftLight32AA take <arg>)
 //   shim_test shadertoy <plasma|nautilus|spikey|spikey_close|sinuses|laura|tunnel>
 //             <time> <out>                            (1280x720 frame)
 #include <cstdio>
@@ -146,6 +148,29 @@ int main(int argc, char **argv)
         TapeWarp32(res.data(), in.data(), W, H, (float)atof(argv[5]), (float)atof(argv[6]));
         BoxBlur_Destroy();
         write_pixels(argv[7], res.data(), res.size());
+    } else if (cmd == "blend" && argc == 8) {
+        const std::string op = argv[2];
+        std::vector<uint32_t> d = read_pixels(argv[3]), s = read_pixels(argv[4]);
+        const unsigned n = (unsigned)atoi(argv[5]);
+        const float arg = (float)atof(argv[6]);
+        if (d.size() < n || s.size() < n || !BoxBlur_Create()) return 1;
+        if (op == "mix32") Mix32(d.data(), s.data(), n, (uint8_t)arg);
+        else if (op == "mixover32") MixOver32(d.data(), s.data(), n);
+        else if (op == "sub32") Sub32(d.data(), s.data(), n);
+        else if (op == "excl32") Excl32(d.data(), s.data(), n);
+        else if (op == "softlight32") SoftLight32(d.data(), s.data(), n);
+        else if (op == "softlight32a") SoftLight32A(d.data(), s.data(), n);
+        else if (op == "softlight32aa") SoftLight32AA(d.data(), s.data(), n, arg);
+        else if (op == "overlay32") Overlay32(d.data(), s.data(), n);
+        else if (op == "overlay32a") Overlay32A(d.data(), s.data(), n);
+        else if (op == "darken32_50") Darken32_50(d.data(), s.data(), n);
+        else if (op == "mulsrc32") MulSrc32(d.data(), s.data(), n);
+        else if (op == "mulsrc32a") MulSrc32A(d.data(), s.data(), n);
+        else if (op == "mixsrc32") MixSrc32(d.data(), s.data(), n);
+        else if (op == "fade32") Fade32(d.data(), n, s[0], (uint8_t)arg);
+        else return 2;
+        BoxBlur_Destroy();
+        write_pixels(argv[7], d.data(), n);
     } else if (cmd == "shadertoy" && argc == 5) {
         const std::string fx = argv[2];
         const float t = (float)atof(argv[3]);

@@ -171,3 +171,27 @@ def test_shadertoy_draws(orc, st_assets, tmp_path):
         if c["effect"] == "plasma":
             assert es["max"] == 0
     assert len(seen) == 7
+
+
+@pytest.mark.parametrize("op,arg", [("mix32", 77), ("mixover32", 0), ("sub32", 0), ("excl32", 0),
+                                    ("softlight32", 0), ("softlight32a", 0),
+                                    ("softlight32aa", 0.6), ("overlay32", 0), ("overlay32a", 0),
+                                    ("darken32_50", 0), ("mulsrc32", 0), ("mulsrc32a", 0),
+                                    ("mixsrc32", 0), ("fade32", 200)])
+def test_blends(orc, tmp_path, op, arg):
+    n = 1280 * 720
+    d, s = rand_img(61, n), rand_img(62, n)
+    (tmp_path / "d.bin").write_bytes(d.tobytes())
+    (tmp_path / "s.bin").write_bytes(s.tobytes())
+    out = tmp_path / "out.bin"
+    run(["blend", op, tmp_path / "d.bin", tmp_path / "s.bin", n, arg, out])
+    want = d.copy()
+    if op in ("mix32",):
+        orc.orc_mix32(P(want), P(s), n, int(arg))
+    elif op == "fade32":
+        orc.orc_fade32(P(want), n, int(s[0]), int(arg))
+    elif op == "softlight32aa":
+        orc.orc_softlight32aa(P(want), P(s), n, float(arg))
+    else:
+        getattr(orc, "orc_" + op)(P(want), P(s), n)
+    assert np.array_equal(np.fromfile(out, np.uint32), want), op
