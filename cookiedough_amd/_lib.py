@@ -95,6 +95,18 @@ FdTunnelParams = _params("FdTunnelParams", [
     ("litTiles", "tunnel:LitTiles"), ("litBlur", "tunnel:LitBlur"), ("fog1", "tunnel:Fog1"),
     ("fog2", "tunnel:Fog2")])
 
+# the voxel ball (code/ball.cpp) and the torus twister (code/torus-twister.cpp)
+BallParams = _params("BallParams", [
+    ("blur", "ball:Blur"), ("radius", "ball:Radius"), ("rayLength", "ball:RayLength"),
+    ("spikes", "ball:Spikes"), ("hasBeams", "ball:HasBeams"),
+    ("baseShapeIndex", "ball:BaseShapeIndex"), ("speed", "ball:Speed"),
+    ("beamAttenuation", "ball:BeamAttenuation"), ("beamAlphaMin", "ball:BeamAlphaMin"),
+    ("rotateOffsX", "ball:RotateOffsX"), ("rotateOffsY", "ball:RotateOffsY"),
+    ("beams1", "ball:Beams1"), ("beams2", "ball:Beams2"), ("beams3", "ball:Beams3"),
+    ("lowBeams", "ball:BallLowBeams")])
+TwisterParams = _params("TwisterParams", [
+    ("speed", "twister:Speed"), ("shearSpeed", "twister::ShearSpeed"), ("blur", "twister:Blur")])
+
 # effect name -> parameter struct; "spikey" takes a close flag
 ST_PARAMS = {"plasma": PlasmaParams, "nautilus": NautilusParams, "spikey": SpikeyParams,
              "sinuses": SinusesParams, "laura": LauraParams, "fdtunnel": FdTunnelParams}

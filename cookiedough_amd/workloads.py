@@ -344,3 +344,60 @@ def st_blur_maps(resX: int = 1280, resY: int = 720) -> np.ndarray:
     log2 = max(10, int(np.ceil(np.log2(max(mw, mh)))))
     return np.concatenate([np.ascontiguousarray(noise_texture(13 + k, log2)[:mh, :mw]).reshape(-1)
                            for k in range(2)])
+
+
+# ---- the voxel ball and the torus twister ----------------------------------
+
+def _height_map(seed: int, top: int) -> np.ndarray:
+    t = noise_texture(seed, 10)
+    return ((t & 0xff).astype(np.uint32) * top // 255).astype(np.uint8).reshape(-1)
+
+
+def _argb_noise(seed: int, n: int) -> np.ndarray:
+    return (splitmix64(seed, n) & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+
+
+def ball_maps(W: int = 1280, H: int = 720) -> dict:
+    """the voxel ball's inputs: five 1024^2 height maps (heights <= 160, so
+    no projected height leaves a 1280-pixel row at the largest radius), two
+    colour maps, three beam maps, an environment map (1024^2 ARGB), two
+    W x H backgrounds and a W x H halo"""
+    return {
+        "heightMaps": np.concatenate([_height_map(20 + i, 160) for i in range(5)]),
+        "colorMaps": np.concatenate([noise_texture(30 + i, 10).reshape(-1) for i in range(2)]),
+        "beamMaps": np.concatenate([noise_texture(32 + i, 10).reshape(-1) for i in range(3)]),
+        "envMap": np.ascontiguousarray(noise_texture(35, 10).reshape(-1)),
+        "backgrounds": np.concatenate([_argb_noise(36 + i, W * H) for i in range(2)]),
+        "halo": _argb_noise(38, W * H),
+    }
+
+
+def twister_maps(W: int = 1280, H: int = 720) -> dict:
+    """the torus twister's inputs: a 1024^2 height map, a 1024^2 colour map
+    and a W x H background"""
+    return {"heightMap": _height_map(40, 255),
+            "colorMap": np.ascontiguousarray(noise_texture(41, 10).reshape(-1)),
+            "background": _argb_noise(42, W * H)}
+
+
+# the demo's track values (target/sync of the reference) at rows where the
+# effects are on screen
+BALL_DEMO = [
+    {"blur": 10.0, "radius": 1800.0, "rayLength": 512.0, "spikes": 0.0, "hasBeams": 1.0,
+     "baseShapeIndex": 1.0, "speed": 3.14, "beamAttenuation": 32.0, "beamAlphaMin": 196.0,
+     "beams1": 0.5, "beams2": 0.66, "beams3": 1.0, "lowBeams": 196.0},
+    {"blur": 30.0, "radius": 1750.0, "rayLength": 512.0, "spikes": 200.0, "hasBeams": 0.0,
+     "baseShapeIndex": 4.0, "speed": 4.0, "beamAttenuation": 32.0, "beamAlphaMin": 196.0,
+     "rotateOffsX": 256.0, "rotateOffsY": -256.0, "lowBeams": 128.0},
+]
+TWISTER_DEMO = {"speed": 8.0, "shearSpeed": 0.19, "blur": 0.0}
+
+
+def ball_frames(n: int, demo: int = 0, t0: float = 10.0):
+    from . import _lib
+    return [_lib.BallParams(time=t0 + k / 60.0, **BALL_DEMO[demo]) for k in range(n)]
+
+
+def twister_frames(n: int, t0: float = 10.0):
+    from . import _lib
+    return [_lib.TwisterParams(time=t0 + k / 60.0, **TWISTER_DEMO) for k in range(n)]

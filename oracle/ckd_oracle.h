@@ -218,6 +218,25 @@ void orc_blitsrc32a(uint32_t *dst, const uint32_t *src, unsigned destResX, unsig
 void orc_fade32(uint32_t *dst, unsigned n, uint32_t RGB, uint8_t alpha);
 void orc_zoom32(uint32_t *dst, const uint32_t *src, unsigned xRes, unsigned yRes, float scale);
 
+/* ---- voxel ball, torus twister, Polar_BlitA (code/ball.cpp,
+ * code/torus-twister.cpp, code/polar.cpp:157-198) ----------------------- */
+/* pinned: the compiled reference's Ball_Draw / Twister_Draw / Polar_BlitA */
+void orc_polar_blit_a(uint32_t *dst, const uint32_t *src, const int *map, unsigned W,
+                      unsigned H, unsigned srcStride, int nthreads);
+void orc_twister_tables(uint32_t heightProj[512], uint32_t heightProjNorm[512]);
+void orc_twister_frame(uint32_t *target, unsigned W, unsigned H, const uint8_t *hmap,
+                       const uint32_t *cmap, float time, float speed, float shearSpeed,
+                       int nthreads);
+int orc_twister_draw(uint32_t *dst, uint32_t *target, unsigned W, unsigned H,
+                     const uint8_t *hmap, const uint32_t *cmap, const uint32_t *background,
+                     const int *polarMap, const float *p, int nthreads);
+void orc_ball_rays(unsigned H, float time, float rayLengthTrack, float rotateOffsX,
+                   float rotateOffsY, int *fromXY, int *dXY);
+int orc_ball_draw(uint32_t *dst, uint32_t *target, unsigned W, unsigned H,
+                  const uint8_t *heightMaps, const uint32_t *colorMaps, const uint32_t *beamMaps,
+                  const uint32_t *envMap, const uint32_t *backgrounds, const uint32_t *halo,
+                  const int *polarMap, const float *p, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

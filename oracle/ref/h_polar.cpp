@@ -18,3 +18,24 @@ REF_API void ref_polar_blit(uint32_t *dst, const uint32_t *src, int inverse)
 {
     Polar_Blit(dst, src, inverse != 0);
 }
+
+/* code/polar.cpp:157-198 (1280x720; the forward blit's 32-row tiles reach
+ * 16 rows past the image: dst needs 16 guard rows, and the forward map
+ * must be padded with ref_polar_pad_forward first) */
+REF_API void ref_polar_blit_a(uint32_t *dst, const uint32_t *src, int inverse)
+{
+    Polar_BlitA(dst, src, inverse != 0);
+}
+
+/* Replaces the forward map by a copy with 16 zero rows appended, so the
+ * forward blits' last tile row reads inside its allocation (the map values
+ * for the image itself are unchanged). */
+REF_API void ref_polar_pad_forward(void)
+{
+    const size_t rows = kTargetResY + 16;
+    int *padded = static_cast<int *>(mallocAligned(kTargetResX * rows * 2 * sizeof(int), kAlignTo));
+    memset(padded, 0, kTargetResX * rows * 2 * sizeof(int));
+    memcpy(padded, s_pMap, kTargetResX * kTargetResY * 2 * sizeof(int));
+    freeAligned(s_pMap);
+    s_pMap = padded;
+}

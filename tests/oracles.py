@@ -83,6 +83,12 @@ _ORC = {
     "orc_blitsrc32a": (None, [_p, _p, _u, _u, _u, _f]),
     "orc_fade32": (None, [_p, _u, C.c_uint32, C.c_uint8]),
     "orc_zoom32": (None, [_p, _p, _u, _u, _f]),
+    "orc_polar_blit_a": (None, [_p, _p, _p, _u, _u, _u, _i]),
+    "orc_twister_tables": (None, [_p, _p]),
+    "orc_twister_frame": (None, [_p, _u, _u, _p, _p, _f, _f, _f, _i]),
+    "orc_twister_draw": (_i, [_p, _p, _u, _u, _p, _p, _p, _p, _p, _i]),
+    "orc_ball_rays": (None, [_u, _f, _f, _f, _f, _p, _p]),
+    "orc_ball_draw": (_i, [_p, _p, _u, _u, _p, _p, _p, _p, _p, _p, _p, _p, _i]),
 }
 
 _REF = {
@@ -140,6 +146,16 @@ _REF = {
     "ref_blitsrc32a": (None, [_p, _p, _u, _u, _u, _f]),
     "ref_fade32": (None, [_p, _u, C.c_uint32, C.c_uint8]),
     "ref_zoom32": (None, [_p, _p, _u, _u, _f]),
+    "ref_polar_blit_a": (None, [_p, _p, _i]),
+    "ref_polar_pad_forward": (None, []),
+    "ref_shared_init": (_i, []),
+    "ref_shared_target": (_p, [_u]),
+    "ref_shared_clear": (None, [_u]),
+    "ref_ball_setup": (_i, [C.c_char_p, _p, _p, _p, _p, _p, _p]),
+    "ref_ball_draw": (None, [_p, _f, _f]),
+    "ref_twister_setup": (_i, [C.c_char_p, _p, _p, _p]),
+    "ref_twister_tables": (None, [_p, _p]),
+    "ref_twister_draw": (None, [_p, _f, _f]),
 }
 
 # oracle / harness effect ids (oracle/ckd_oracle.h) of the effects' map
