@@ -296,6 +296,39 @@ def main() -> int:
             log("shadertoy", fx, st_per[fx])
         per["shadertoy_1280x720"] = st_per
 
+        # ------------------- the voxel ball and the torus twister (1280x720 Draw)
+        # plain: one Draw per call; batched: all frames in one call (the
+        # per-frame tables are staged and the launches queued back to back)
+        n_v2 = 32
+        bm_v2, tm_v2 = wl.ball_maps(W, H), wl.twister_maps(W, H)
+        ctx.ball_create(bm_v2, W, H)
+        ctx.twister_create(tm_v2["heightMap"], tm_v2["colorMap"], tm_v2["background"], W, H)
+        tgt_v2 = ctx.empty(W * H)
+        ctx.memset32(tgt_v2, 0, W * H)
+        v2_sets = {"ball_beams": ("ball", wl.ball_frames(n_v2, 0)),
+                   "ball_envmap": ("ball", wl.ball_frames(n_v2, 1)),
+                   "twister": ("twister", wl.twister_frames(n_v2))}
+        v2_per = {}
+        for key, (kind, frames) in v2_sets.items():
+            draw = ctx.ball_draw if kind == "ball" else ctx.twister_draw
+
+            def v2_plain():
+                for k in range(n_v2):
+                    draw(fr_st.view(k * W * H, (W * H,)), tgt_v2, W, H, frames[k])
+
+            def v2_batched():
+                draw(fr_st, tgt_v2, W, H, frames)
+
+            v2_plain()
+            v2_batched()
+            mp, mb = T.run(v2_plain), T.run(v2_batched)
+            px = n_v2 * W * H
+            v2_per[key] = {"frames": n_v2, "plain_mpx_s": round(px / mp / 1e3, 1),
+                           "batched_mpx_s": round(px / mb / 1e3, 1),
+                           "ms_per_frame_batched": round(mb / n_v2, 4)}
+            log("voxel2", key, v2_per[key])
+        per["ball_twister_1280x720"] = v2_per
+
         # The CPU leg is the only part of the bench that runs the oracle: the
         # timed CPU baseline and the FP64 reference render the precision form's
         # error is measured against.
@@ -379,6 +412,56 @@ def main() -> int:
                 st_per[fx]["cpu_kind"] = ("reference Draw (OpenMP, OMP_NUM_THREADS threads)"
                                           if use_ref else f"oracle Draw ({nthr} threads)")
                 log("shadertoy cpu", fx, st_per[fx]["cpu_mpx_s"])
+
+            # the ball and the twister: the reference's Draw (tracks as above)
+            # where the harness is present, else the oracle's
+            m0_h = np.zeros(2 * W * H, np.int32)
+            orc.orc_polar_maps(oracles.P(m0_h), oracles.P(np.zeros(2 * W * H, np.int32)), W, H, W, H)
+            if use_ref:
+                refl.ref_shared_init()
+                refl.ref_polar_create()
+                refl.ref_polar_pad_forward()
+            for key, (kind, frames) in v2_sets.items():
+                frames = frames[:8]
+                cls = type(frames[0])
+                out_h = oracles.aligned_empty(W * (H + 16))
+                tgt_h = np.zeros(W * H, np.uint32)
+                with tempfile.TemporaryDirectory() as td:
+                    if use_ref:
+                        oracles.write_rocket_tracks(Path(td), {
+                            tr: [getattr(f, fld) for f in frames] for fld, tr in cls.TRACKS.items()})
+                        refl.ref_st_rocket(td.encode())
+                        if kind == "ball":
+                            refl.ref_ball_setup(td.encode(), oracles.P(bm_v2["heightMaps"]),
+                                                oracles.P(bm_v2["colorMaps"]), oracles.P(bm_v2["beamMaps"]),
+                                                oracles.P(bm_v2["envMap"]), oracles.P(bm_v2["backgrounds"]),
+                                                oracles.P(bm_v2["halo"]))
+                        else:
+                            refl.ref_twister_setup(td.encode(), oracles.P(tm_v2["heightMap"]),
+                                                   oracles.P(tm_v2["colorMap"]),
+                                                   oracles.P(tm_v2["background"]))
+                    t_c = time.perf_counter()
+                    for k, f in enumerate(frames):
+                        if use_ref:
+                            refl.ref_st_row(float(k))
+                            (refl.ref_ball_draw if kind == "ball" else refl.ref_twister_draw)(
+                                oracles.P(out_h), f.time, 1.0)
+                        elif kind == "ball":
+                            orc.orc_ball_draw(oracles.P(out_h), oracles.P(tgt_h), W, H,
+                                              oracles.P(bm_v2["heightMaps"]), oracles.P(bm_v2["colorMaps"]),
+                                              oracles.P(bm_v2["beamMaps"]), oracles.P(bm_v2["envMap"]),
+                                              oracles.P(bm_v2["backgrounds"]), oracles.P(bm_v2["halo"]),
+                                              oracles.P(m0_h), oracles.P(f.as_array()), nthr)
+                        else:
+                            orc.orc_twister_draw(oracles.P(out_h), oracles.P(tgt_h), W, H,
+                                                 oracles.P(tm_v2["heightMap"]), oracles.P(tm_v2["colorMap"]),
+                                                 oracles.P(tm_v2["background"]), oracles.P(m0_h),
+                                                 oracles.P(f.as_array()), nthr)
+                    dt_c = time.perf_counter() - t_c
+                v2_per[key]["cpu_mpx_s"] = round(len(frames) * W * H / dt_c / 1e6, 2)
+                v2_per[key]["cpu_kind"] = ("reference Draw (OpenMP, OMP_NUM_THREADS threads)"
+                                           if use_ref else f"oracle Draw ({nthr} threads)")
+                log("voxel2 cpu", key, v2_per[key]["cpu_mpx_s"])
 
             px, k, dt = cpu_rate(nthr, 10.0)
             px1, k1, dt1 = cpu_rate(1, 5.0)

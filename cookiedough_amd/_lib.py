@@ -187,6 +187,11 @@ SIGNATURES = {
     "ckd_blitsrc32a": [_p, _p, _p, _u, _u, _u, _f],
     "ckd_fade32": [_p, _p, _u, C.c_uint32, C.c_uint8],
     "ckd_zoom32": [_p, _p, _p, _u, _u, _f],
+    "ckd_polar_blit_a": [_p, _p, _p, _u, _u, _i],
+    "ckd_twister_create": [_p, _p, _p, _p, _u, _u],
+    "ckd_twister_draw": [_p, _p, _p, _u, _u, C.POINTER(TwisterParams), _u],
+    "ckd_ball_create": [_p, _p, _p, _p, _p, _p, _p, _u, _u],
+    "ckd_ball_draw": [_p, _p, _p, _u, _u, C.POINTER(BallParams), _u],
     "ckd_fdtunnel_create": [_p, _p, _p],
     "ckd_spikey_blurmaps": [_p, _p, _p, _u, _u],
     "ckd_fdtunnel_map": [_p, _p, _p, _u, _u, C.POINTER(FdTunnelParams), _u],

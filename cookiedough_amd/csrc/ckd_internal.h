@@ -39,6 +39,19 @@ struct ckd_ctx {
     uint32_t *spikeBlurMaps = nullptr;
     unsigned spikeBlurW = 0, spikeBlurH = 0;
 
+    // torus twister (Twister_Create): height map, colour map, the two
+    // 512-entry ray tables and the W x H background
+    uint8_t *twHeight = nullptr;
+    uint32_t *twColor = nullptr, *twTables = nullptr, *twBackground = nullptr;
+    unsigned twW = 0, twH = 0;
+    // voxel ball (Ball_Create): 5 height maps + their mix (bytes), then 2
+    // colour maps, 3 beam maps, the beam mix and the environment map; the
+    // two backgrounds and the halo (W x H each); the largest map height
+    uint8_t *ballMaps = nullptr;
+    uint32_t *ballImages = nullptr;
+    unsigned ballW = 0, ballH = 0;
+    uint32_t ballTopHeight = 0;
+
     // landscape maps (Landscape_Create)
     uint8_t *lsHeight = nullptr;
     uint32_t *lsColor = nullptr;

@@ -1,0 +1,560 @@
+// k_ball.hip -- the voxel ball (code/ball.cpp), the torus twister
+// (code/torus-twister.cpp) and Polar_BlitA (code/polar.cpp:157-198).
+//
+// Both casters march one ray per lane across a 1024^2 height map and draw
+// colored spans along a row of the render target (the ball: one ray per row
+// from its start; the twister: two per row, outwards from the centre).
+// Per-frame tables (height projections, lighting norms, ray directions and
+// origins) are the reference's FP32 host code, computed here on the host
+// in the same order and staged once per frame.  Spans stay inside their row:
+// the reference writes past the row end when a projected height exceeds it
+// (map heights above 1279*256/radius), and such frames are rejected.
+#include "ckd_internal.h"
+#include "ckd_pixel.h"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#pragma clang fp contract(off)
+
+namespace {
+
+constexpr float kPI = 3.1415926535897932384626433832795f;
+constexpr float kGoldenRatio = 1.61803398875f;
+constexpr float kGoldenAngle = 2.39996f;
+constexpr float kEpsilon = 1.1920928955078125e-07f;
+constexpr unsigned kTwRay = 512;     // code/torus-twister.cpp:29
+constexpr unsigned kBallMaxRay = 1024; // code/ball.cpp:56
+constexpr size_t kMapPx = 1024 * 1024;
+
+// ---- device: samplers and spans with 16-bit lanes -------------------------
+
+struct Uv {
+    unsigned u0, u1, v0, v1, fu, fv;
+};
+
+// bsamp_prepUVs (code/bilinear.h:10-31) on a 1024^2 map
+CKD_DEV Uv prep(int U, int V)
+{
+    Uv r;
+    r.u0 = (unsigned)(U >> 8) & 1023u;
+    r.u1 = ((unsigned)(U >> 8) + 1u) & 1023u;
+    r.v0 = ((unsigned)(V >> 8) & 1023u) << 10;
+    r.v1 = (((unsigned)(V >> 8) + 1u) & 1023u) << 10;
+    r.fu = (unsigned)U & 0xffu;
+    r.fv = (unsigned)V & 0xffu;
+    return r;
+}
+
+CKD_DEV uint32_t samp8(const uint8_t *__restrict__ m, const Uv &t)
+{
+    return ckd_bilerp8(m[t.u0 + t.v0], m[t.u1 + t.v0], m[t.u0 + t.v1], m[t.u1 + t.v1], t.fu, t.fv);
+}
+
+// bsamp32_16 (code/bilinear.h:58-83): four channel lanes (each <= 255)
+CKD_DEV void samp16(const uint32_t *__restrict__ m, const Uv &t, uint32_t c[4])
+{
+    const uint32_t p = ckd_bilerp32(m[t.u0 + t.v0], m[t.u1 + t.v0], m[t.u0 + t.v1], m[t.u1 + t.v1],
+                                    t.fu, t.fv);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) c[k] = (p >> (8 * k)) & 0xffu;
+}
+
+// _mm_adds_epu16
+CKD_DEV uint32_t adds16(uint32_t a, uint32_t b)
+{
+    const uint32_t s = a + b;
+    return s > 65535u ? 65535u : s;
+}
+
+// cspanISSE16 (code/cspan.h:47-78) with 16-bit lane colours A and B: the
+// lanes are zero-extended to 32 bits and delta = B - A taken modulo 2^32,
+// then split into the two 16-bit halves _mm_madd_epi16 multiplies
+CKD_DEV void span16(uint32_t *__restrict__ dst, int incr, uint32_t length, uint32_t drawLength,
+                    const uint32_t A[4], const uint32_t B[4])
+{
+    const uint32_t divisor = 65536u / length;
+    const uint32_t pre = length - drawLength;
+    const int64_t m = (int64_t)(int32_t)divisor * (int64_t)(int32_t)pre;
+    const uint32_t mlo = (uint32_t)m, mhi = (uint32_t)((uint64_t)m >> 32);
+    uint32_t acc[4], step[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t delta = B[k] - A[k];
+        step[k] = (uint32_t)ckd_madd16(delta, divisor);
+        acc[k] = (A[k] << 16) + (uint32_t)ckd_madd16(delta, (k & 1) ? mhi : mlo);
+    }
+    for (uint32_t i = 0; i < drawLength; ++i) {
+        *dst = ckd_span_pixel(acc);
+        dst += incr;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc[k] += step[k];
+    }
+}
+
+// ---- the twister ---------------------------------------------------------
+
+struct TwFrame {
+    int fromX, fromY; // per row, 24:8
+};
+
+// vtwister_ray (code/torus-twister.cpp:39-85) for both rays of every row:
+// lane 2*row+side; side 0 walks right from the centre, side 1 left
+__global__ __launch_bounds__(256) void k_twister(uint32_t *__restrict__ target, unsigned W,
+                                                 unsigned H, const TwFrame *__restrict__ rows,
+                                                 const uint32_t *__restrict__ tables,
+                                                 const uint8_t *__restrict__ hmap,
+                                                 const uint32_t *__restrict__ cmap)
+{
+    const unsigned lane = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned row = lane >> 1, side = lane & 1;
+    if (row >= H) return;
+    const uint32_t *hp = tables, *hn = tables + kTwRay;
+    const int dX = side ? -512 : 512, dir = side ? -1 : 1;
+    int curX = side ? (int)((uint32_t)rows[row].fromX - 512u) : rows[row].fromX;
+    const int curY = rows[row].fromY;
+    uint32_t *p = target + (size_t)row * W + (W >> 1) - side;
+    uint32_t lastColor[4], color[4];
+    samp16(cmap, prep(curX, curY), lastColor);
+    uint32_t lastHeight = 0, lastDrawn = 0;
+    for (unsigned i = 0; i < kTwRay; ++i) {
+        curX = (int)((uint32_t)curX - (uint32_t)dX);
+        const Uv t = prep(curX, curY);
+        const uint32_t h = samp8(hmap, t);
+        samp16(cmap, t, color);
+        const uint32_t diffuse = h * hn[i] >> 8;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) color[k] = adds16(color[k], diffuse & 0xffffu);
+        const uint32_t height = h * hp[i] >> 8;
+        if (height > lastDrawn) {
+            const uint32_t drawLength = height - lastDrawn;
+            span16(p, dir, height - lastHeight, drawLength, lastColor, color);
+            p += (ptrdiff_t)drawLength * dir;
+            lastDrawn = height;
+        }
+        lastHeight = height;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) lastColor[k] = color[k];
+    }
+}
+
+// ---- the ball ------------------------------------------------------------
+
+struct BallFrame {
+    unsigned rayLength, beamAtten, lowLight;
+    float beamAlphaMin;
+    int fromX, fromY;
+    uint32_t heightProj[kBallMaxRay];
+    int32_t norm[kBallMaxRay][3];
+};
+
+// vball_ray_beams / vball_ray_no_beams (code/ball.cpp:77-261), lane per row
+template <bool kBeams>
+__global__ __launch_bounds__(256) void k_ball(uint32_t *__restrict__ target, unsigned W,
+                                              unsigned H, const BallFrame *__restrict__ f,
+                                              const int2 *__restrict__ dXY,
+                                              const uint8_t *__restrict__ hmapMix,
+                                              const uint32_t *__restrict__ cmap,
+                                              const uint32_t *__restrict__ aux)
+{
+    const unsigned row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= H) return;
+    uint32_t *dst = target + (size_t)row * W;
+    const int dX = dXY[row].x, dY = dXY[row].y;
+    int curX = f->fromX, curY = f->fromY;
+    int envU = (1024 >> 1) << 8, envV = envU;
+    uint32_t lastColor[4], color[4], extra[4], beamAccum[4] = {0, 0, 0, 0};
+    samp16(cmap, prep(curX, curY), lastColor);
+    uint32_t lastHeight = 0, lastDrawn = 0;
+    const unsigned rayLength = f->rayLength;
+    for (unsigned i = 0; i < rayLength; ++i) {
+        curX = (int)((uint32_t)curX - (uint32_t)dX);
+        curY = (int)((uint32_t)curY - (uint32_t)dY);
+        const Uv t = prep(curX, curY);
+        const uint32_t h = samp8(hmapMix, t);
+        samp16(cmap, t, color);
+        if constexpr (kBeams) {
+            samp16(aux, t, extra); // beam map
+            const uint32_t hi = h * (uint32_t)f->norm[i][0] >> 8;
+            const uint32_t lo = h * (uint32_t)f->norm[i][1] >> 8;
+            const uint32_t lit = (hi + (((uint32_t)(int32_t)(lo - hi) * f->lowLight) >> 8)) & 0xffffu;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t b = ((extra[k] * f->beamAtten) & 0xffffu) >> 8;
+                beamAccum[k] = adds16(beamAccum[k], ((b * lit) & 0xffffu) >> 8);
+                color[k] = adds16(adds16(color[k], beamAccum[k]), lit);
+            }
+        } else {
+            envU = (int)((uint32_t)envU - ((uint32_t)dX << 1));
+            envV = (int)((uint32_t)envV - ((uint32_t)dY << 1));
+            samp16(aux, prep((int)((uint32_t)envU + h), (int)((uint32_t)envV + h)), extra); // env map
+            const uint32_t diffuse = h * (uint32_t)f->norm[i][2] >> 8;
+            const uint32_t full = (32u + diffuse) < 255u ? 32u + diffuse : 255u;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                color[k] = adds16(adds16(color[k], ((extra[k] * full) & 0xffffu) >> 8), diffuse & 0xffffu);
+        }
+        const uint32_t height = h * f->heightProj[i] >> 8;
+        if (height > lastDrawn) {
+            span16(dst + lastDrawn, 1, height - lastHeight, height - lastDrawn, lastColor, color);
+            lastDrawn = height;
+        }
+        lastHeight = height;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) lastColor[k] = color[k];
+    }
+    if constexpr (kBeams) {
+        // beam extrusion (code/ball.cpp:176-204)
+        uint32_t beamCol = 0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int v = (int)(int16_t)(uint16_t)beamAccum[k];
+            beamCol |= (uint32_t)(v < 0 ? 0 : (v > 255 ? 255 : v)) << (8 * k);
+        }
+        const uint32_t remainder = (W - 1) - lastDrawn;
+        const uint32_t bR = beamCol >> 16, bG = (beamCol >> 8) & 0xff, bB = beamCol & 0xff;
+        // unsigned(0.0722f*65536.f) etc.
+        const uint32_t lum = ((bR * 4731u) >> 16) + ((bG * 46871u) >> 16) + ((bB * 13932u) >> 16);
+        const float fLum = (float)lum;
+        const float alphaStep = 1.f / (float)(remainder - 1);
+        const float amin = f->beamAlphaMin;
+        float curStep = 0.f;
+        for (uint32_t k = 0; k < remainder; ++k) {
+            float s = curStep;
+            s = s * s * (3.f - 2.f * s);
+            const float a = amin + (fLum - amin) * s;
+            dst[lastDrawn++] = beamCol | ((uint32_t)a << 24);
+            curStep += alphaStep;
+        }
+    }
+}
+
+// ---- Polar_BlitA ------------------------------------------------------------
+
+// Fetch16 through the map (code/polar.cpp:95-106), then the mix by the
+// fetched alpha: d + a*(s-d)/256 per channel (code/polar.cpp:157-174)
+__global__ __launch_bounds__(256) void k_polar_blit_a(uint32_t *__restrict__ dst,
+                                                      const uint32_t *__restrict__ src,
+                                                      const int2 *__restrict__ map, unsigned W,
+                                                      unsigned H)
+{
+    const unsigned x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= W || y >= H) return;
+    const size_t i = (size_t)y * W + x;
+    const int2 uv = map[i];
+    const unsigned U0 = (unsigned)(uv.x >> 8), V0 = (unsigned)(uv.y >> 8) * W;
+    const uint32_t s = ckd_bilerp32(src[U0 + V0], src[U0 + 1 + V0], src[U0 + V0 + W],
+                                    src[U0 + 1 + V0 + W], (uint32_t)uv.x & 0xff,
+                                    (uint32_t)uv.y & 0xff);
+    const uint32_t a = s >> 24, d = dst[i];
+    uint32_t r = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t dk = (d >> (8 * k)) & 0xff, sk = (s >> (8 * k)) & 0xff;
+        r |= ((dk * (256 - a) + sk * a) >> 8) << (8 * k);
+    }
+    dst[i] = r;
+}
+
+// ---- host: the reference's FP32 set-up -------------------------------------
+
+// float -> int as x86 cvttss2si (NaN and out of range give INT_MIN), and
+// float -> unsigned through a 64-bit cvttss2si (the low 32 bits)
+int cvt_i32(float v) { return (v == v && v < 2147483648.f && v >= -2147483648.f) ? (int)v : INT32_MIN; }
+uint32_t cvt_u32(float v)
+{
+    return (v == v && v < 9.2233720368547758e18f && v >= -9.2233720368547758e18f)
+               ? (uint32_t)(int64_t)v
+               : 0u;
+}
+int fp24(float v) { return cvt_i32(v * 256.f); }
+int geti(float v) { return cvt_i32(roundf(v)); }
+int clampi_(int lo, int hi, int v) { return v < lo ? lo : (v > hi ? hi : v); }
+float smax(float a, float b) { return (a < b) ? b : a; }
+float smin(float a, float b) { return (b < a) ? b : a; }
+float clampf_(float lo, float hi, float v) { return smax(lo, smin(hi, v)); }
+float box_blur_scale(float s) { return s != 0.f ? smax(1.f, smin(100.f, s)) * 0.01f : s; }
+
+// vtwister_precalc (code/torus-twister.cpp:118-134)
+void twister_tables(uint32_t *hp, uint32_t *hn)
+{
+    for (unsigned i = 0; i < kTwRay; ++i) {
+        const float angle = kPI / (float)(kTwRay - 1) * (float)i;
+        hp[i] = cvt_u32(600.f * sinf(angle));
+        const float cosine = cosf(angle * 0.99f);
+        hn[i] = cosine > 0.f ? cvt_u32(255.f * powf(cosine, 2.f)) : 0u;
+    }
+}
+
+// vball_precalc (code/ball.cpp:263-286) and the vball set-up (:290-333)
+void ball_frame(const ckd_ball_params &p, unsigned H, BallFrame &f, std::vector<int2> &dXY)
+{
+    const float radius = clampf_(1.f, 1920.f, p.radius);
+    f.rayLength = (unsigned)clampi_(1, (int)kBallMaxRay, geti(p.rayLength));
+    const float angStepSin = kPI / (float)(f.rayLength - 1);
+    const float angStepCos = angStepSin * 0.99f;
+    for (unsigned i = 0; i < f.rayLength; ++i) {
+        f.heightProj[i] = cvt_u32(radius * sinf(angStepSin * (float)i));
+        const float cosine = cosf(angStepCos * (float)i);
+        if (cosine >= 0.f) {
+            f.norm[i][0] = cvt_i32(255.f * powf(cosine, kGoldenRatio));
+            f.norm[i][1] = cvt_i32(255.f * powf(cosine, kGoldenAngle));
+            f.norm[i][2] = cvt_i32(255.f * powf(cosine, kPI));
+        } else {
+            f.norm[i][0] = f.norm[i][1] = f.norm[i][2] = 0;
+        }
+    }
+    f.beamAtten = (unsigned)clampi_(0, 255, geti(p.beamAttenuation));
+    f.beamAlphaMin = clampf_(0.f, 255.f, p.beamAlphaMin);
+    f.lowLight = (unsigned)clampi_(0, 255, geti(p.lowBeams));
+    const float time = p.time * p.speed;
+    const float timeScale = (float)f.rayLength * (0.25f / 1024.f);
+    f.fromX = fp24(1024.f * sinf(time * timeScale) + 512.f + p.rotateOffsX);
+    f.fromY = fp24(1024.f * cosf(time * timeScale) + 512.f + p.rotateOffsY);
+    const float delta = (2.f * kPI) / (float)(H - 1);
+    dXY.resize(H);
+    for (unsigned r = 0; r < H; ++r) {
+        // voxel::calc_fandeltas (code/voxel-shared.h:18-31)
+        float X = cosf((float)r * delta), Y = sinf((float)r * delta);
+        if (fabsf(X + Y) > kEpsilon) {
+            const float length = 1.f / sqrtf(X * X + Y * Y);
+            X *= length;
+            Y *= length;
+        }
+        dXY[r] = make_int2(fp24(X), fp24(Y));
+    }
+}
+
+bool finite_all(const float *p, size_t n)
+{
+    for (size_t i = 0; i < n; ++i)
+        if (!(p[i] == p[i]) || p[i] > 3.0e38f || p[i] < -3.0e38f) return false;
+    return true;
+}
+
+ckd_status ensure_polar(ckd_ctx *ctx, unsigned W, unsigned H)
+{
+    if (ctx->polarMap && ctx->polarW == W && ctx->polarH == H) return CKD_OK;
+    return ckd_polar_create(ctx, W, H);
+}
+
+} // namespace
+
+namespace ckd {
+ckd_status comp_mix32(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, unsigned n, uint32_t alpha);
+ckd_status launch_memset32(ckd_ctx *ctx, uint32_t *dst, uint32_t value, size_t n);
+} // namespace ckd
+
+extern "C" {
+
+ckd_status ckd_polar_blit_a(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, unsigned W,
+                            unsigned H, int inverse)
+{
+    CKD_REQUIRE(dst && src, "ckd_polar_blit_a: null buffer");
+    CKD_REQUIRE(dst != src, "ckd_polar_blit_a: dst must differ from src");
+    CKD_REQUIRE_CTX(ctx);
+    CKD_REQUIRE(ctx->polarMap && ctx->polarW == W && ctx->polarH == H,
+                "ckd_polar_blit_a: no polar maps for %ux%u (ckd_polar_create)", W, H);
+    dim3 grid(ckd_div_up(W, 256), H);
+    k_polar_blit_a<<<grid, 256, 0, ctx->stream>>>(
+        dst, src, reinterpret_cast<const int2 *>(inverse ? ctx->polarInv : ctx->polarMap), W, H);
+    return ckd::check_launch("polar_blit_a");
+}
+
+ckd_status ckd_twister_create(ckd_ctx *ctx, const uint8_t *heightMap, const uint32_t *colorMap,
+                              const uint32_t *background, unsigned W, unsigned H)
+{
+    CKD_REQUIRE(heightMap && colorMap && background, "ckd_twister_create: null map");
+    // the spans reach 597 pixels from the centre (600 * 255/256)
+    CKD_REQUIRE(W >= 1200 && W <= 8192 && H >= 2 && H <= 8192 && (W % 2) == 0,
+                "ckd_twister_create: %ux%u: need an even width of 1200..8192 and 2..8192 rows", W, H);
+    CKD_REQUIRE_CTX(ctx);
+    ckd_status s = ensure_polar(ctx, W, H);
+    if (s != CKD_OK) return s;
+    if (!ctx->twHeight) {
+        if (hipMalloc(&ctx->twHeight, kMapPx) != hipSuccess ||
+            hipMalloc(&ctx->twColor, kMapPx * 4) != hipSuccess ||
+            hipMalloc(&ctx->twTables, 2 * kTwRay * sizeof(uint32_t)) != hipSuccess)
+            return ckd::fail(CKD_ENOMEM, "ckd_twister_create: cannot allocate the maps");
+    }
+    if (ctx->twBackground && (ctx->twW != W || ctx->twH != H)) {
+        CKD_HIP(hipFree(ctx->twBackground));
+        ctx->twBackground = nullptr;
+    }
+    if (!ctx->twBackground && hipMalloc(&ctx->twBackground, (size_t)W * H * 4) != hipSuccess)
+        return ckd::fail(CKD_ENOMEM, "ckd_twister_create: cannot allocate the background");
+    uint32_t tables[2 * kTwRay];
+    twister_tables(tables, tables + kTwRay);
+    CKD_HIP(hipMemcpy(ctx->twHeight, heightMap, kMapPx, hipMemcpyHostToDevice));
+    CKD_HIP(hipMemcpy(ctx->twColor, colorMap, kMapPx * 4, hipMemcpyHostToDevice));
+    CKD_HIP(hipMemcpy(ctx->twBackground, background, (size_t)W * H * 4, hipMemcpyHostToDevice));
+    CKD_HIP(hipMemcpy(ctx->twTables, tables, sizeof(tables), hipMemcpyHostToDevice));
+    ctx->twW = W;
+    ctx->twH = H;
+    return CKD_OK;
+}
+
+ckd_status ckd_twister_draw(ckd_ctx *ctx, uint32_t *dst, uint32_t *target, unsigned W, unsigned H,
+                            const ckd_twister_params *frames, unsigned nFrames)
+{
+    CKD_REQUIRE(dst && target && frames, "ckd_twister_draw: null buffer");
+    CKD_REQUIRE(dst != target, "ckd_twister_draw: dst must differ from target");
+    for (unsigned k = 0; k < nFrames; ++k)
+        CKD_REQUIRE(finite_all(&frames[k].time, sizeof(ckd_twister_params) / sizeof(float)),
+                    "ckd_twister_draw: frame %u: parameters must be finite", k);
+    CKD_REQUIRE_CTX(ctx);
+    CKD_REQUIRE(ctx->twHeight && ctx->twW == W && ctx->twH == H,
+                "ckd_twister_draw: no twister maps for %ux%u (ckd_twister_create)", W, H);
+    ckd_status s = ensure_polar(ctx, W, H);
+    if (s != CKD_OK) return s;
+    const size_t px = (size_t)W * H;
+    std::vector<TwFrame> rows(H);
+    for (unsigned k = 0; k < nFrames; ++k) {
+        // vtwister (code/torus-twister.cpp:89-116)
+        const ckd_twister_params &p = frames[k];
+        const float fMapSizeHH = (1024.f * 0.5f) - 0.5f, fMapSizeHHH = (1024.f * 0.25f) - 0.5f;
+        const float mapStepY = 1024.f / (float)(H - 1), shearStep = (2.f * kPI) / (float)(H - 1);
+        for (unsigned r = 0; r < H; ++r) {
+            const float shearAngle = (float)r * shearStep;
+            rows[r].fromX = fp24(fMapSizeHH + fMapSizeHHH * sinf(p.time * p.shearSpeed + shearAngle));
+            rows[r].fromY = fp24((float)r * mapStepY + p.time * p.speed);
+        }
+        void *dRows = nullptr;
+        if ((s = ckd::stage(ctx, rows.data(), sizeof(TwFrame) * H, &dRows)) != CKD_OK) return s;
+        // Twister_Draw (code/torus-twister.cpp:164-187)
+        if ((s = ckd::launch_memset32(ctx, target, 0, px)) != CKD_OK) return s;
+        k_twister<<<ckd_div_up((size_t)2 * H, 256), 256, 0, ctx->stream>>>(
+            target, W, H, static_cast<const TwFrame *>(dRows), ctx->twTables, ctx->twHeight,
+            ctx->twColor);
+        if ((s = ckd::check_launch("twister")) != CKD_OK) return s;
+        if (p.blur != 0.f &&
+            (s = ckd_blur07_h(ctx, target, target, W, H, box_blur_scale(p.blur))) != CKD_OK)
+            return s;
+        uint32_t *out = dst + px * k;
+        CKD_HIP(hipMemcpyAsync(out, ctx->twBackground, px * 4, hipMemcpyDeviceToDevice, ctx->stream));
+        if ((s = ckd_polar_blit_a(ctx, out, target, W, H, 0)) != CKD_OK) return s;
+    }
+    return CKD_OK;
+}
+
+ckd_status ckd_ball_create(ckd_ctx *ctx, const uint8_t *heightMaps5, const uint32_t *colorMaps2,
+                           const uint32_t *beamMaps3, const uint32_t *envMap,
+                           const uint32_t *backgrounds2, const uint32_t *halo, unsigned W,
+                           unsigned H)
+{
+    CKD_REQUIRE(heightMaps5 && colorMaps2 && beamMaps3 && envMap && backgrounds2 && halo,
+                "ckd_ball_create: null map");
+    CKD_REQUIRE(W >= 64 && W <= 8192 && H >= 2 && H <= 8192, "ckd_ball_create: bad size %ux%u", W, H);
+    CKD_REQUIRE_CTX(ctx);
+    ckd_status s = ensure_polar(ctx, W, H);
+    if (s != CKD_OK) return s;
+    // the largest map height bounds every projected height (radius <= 1920)
+    uint32_t top = 0;
+    for (size_t i = 0; i < 5 * kMapPx; ++i) top = heightMaps5[i] > top ? heightMaps5[i] : top;
+    if (!ctx->ballMaps) {
+        // heights 5 + mix, colour 2, beam 3 + mix, env
+        const size_t bytes = 6 * kMapPx + (2 + 4 + 1) * kMapPx * 4;
+        if (hipMalloc(&ctx->ballMaps, bytes) != hipSuccess)
+            return ckd::fail(CKD_ENOMEM, "ckd_ball_create: cannot allocate the maps");
+    }
+    if (ctx->ballImages && (ctx->ballW != W || ctx->ballH != H)) {
+        CKD_HIP(hipFree(ctx->ballImages));
+        ctx->ballImages = nullptr;
+    }
+    if (!ctx->ballImages && hipMalloc(&ctx->ballImages, (size_t)3 * W * H * 4) != hipSuccess)
+        return ckd::fail(CKD_ENOMEM, "ckd_ball_create: cannot allocate the backgrounds");
+    uint8_t *h = ctx->ballMaps;
+    uint32_t *c = reinterpret_cast<uint32_t *>(h + 6 * kMapPx);
+    CKD_HIP(hipMemcpy(h, heightMaps5, 5 * kMapPx, hipMemcpyHostToDevice));
+    CKD_HIP(hipMemcpy(c, colorMaps2, 2 * kMapPx * 4, hipMemcpyHostToDevice));
+    CKD_HIP(hipMemcpy(c + 2 * kMapPx, beamMaps3, 3 * kMapPx * 4, hipMemcpyHostToDevice));
+    CKD_HIP(hipMemcpy(c + 6 * kMapPx, envMap, kMapPx * 4, hipMemcpyHostToDevice));
+    CKD_HIP(hipMemcpy(ctx->ballImages, backgrounds2, (size_t)2 * W * H * 4, hipMemcpyHostToDevice));
+    CKD_HIP(hipMemcpy(ctx->ballImages + (size_t)2 * W * H, halo, (size_t)W * H * 4,
+                      hipMemcpyHostToDevice));
+    ctx->ballW = W;
+    ctx->ballH = H;
+    ctx->ballTopHeight = top;
+    return CKD_OK;
+}
+
+ckd_status ckd_ball_draw(ckd_ctx *ctx, uint32_t *dst, uint32_t *target, unsigned W, unsigned H,
+                         const ckd_ball_params *frames, unsigned nFrames)
+{
+    CKD_REQUIRE(dst && target && frames, "ckd_ball_draw: null buffer");
+    CKD_REQUIRE(dst != target, "ckd_ball_draw: dst must differ from target");
+    for (unsigned k = 0; k < nFrames; ++k)
+        CKD_REQUIRE(finite_all(&frames[k].time, sizeof(ckd_ball_params) / sizeof(float)),
+                    "ckd_ball_draw: frame %u: parameters must be finite", k);
+    CKD_REQUIRE_CTX(ctx);
+    CKD_REQUIRE(ctx->ballMaps && ctx->ballW == W && ctx->ballH == H,
+                "ckd_ball_draw: no ball maps for %ux%u (ckd_ball_create)", W, H);
+    for (unsigned k = 0; k < nFrames; ++k) {
+        // every span stays in its row: height <= top*radius/256 <= W-1
+        const float radius = clampf_(1.f, 1920.f, frames[k].radius);
+        CKD_REQUIRE((uint64_t)ctx->ballTopHeight * (uint64_t)radius <= (uint64_t)(W - 1) * 256u,
+                    "ckd_ball_draw: frame %u: radius %g with map heights up to %u reaches past a "
+                    "%u-pixel row", k, (double)radius, ctx->ballTopHeight, W);
+    }
+    ckd_status s = ensure_polar(ctx, W, H);
+    if (s != CKD_OK) return s;
+    const size_t px = (size_t)W * H;
+    uint8_t *heights = ctx->ballMaps, *hmapMix = heights + 5 * kMapPx;
+    uint32_t *cmaps = reinterpret_cast<uint32_t *>(heights + 6 * kMapPx);
+    uint32_t *beamMaps = cmaps + 2 * kMapPx, *beamMix = cmaps + 5 * kMapPx, *env = cmaps + 6 * kMapPx;
+    const uint32_t *backgrounds = ctx->ballImages, *halo = ctx->ballImages + 2 * px;
+    std::vector<uint8_t> host(sizeof(BallFrame) + sizeof(int2) * H);
+    std::vector<int2> dXY;
+    for (unsigned k = 0; k < nFrames; ++k) {
+        const ckd_ball_params &p = frames[k];
+        // Ball_Draw (code/ball.cpp:442-511): the height and beam map mixes
+        const bool beams = geti(p.hasBeams) != 0;
+        const unsigned base = (unsigned)clampi_(1, 4, geti(p.baseShapeIndex));
+        CKD_HIP(hipMemcpyAsync(hmapMix, heights + base * kMapPx, kMapPx, hipMemcpyDeviceToDevice,
+                               ctx->stream));
+        const uint8_t spikes = (uint8_t)geti(p.spikes);
+        if (spikes != 0 &&
+            (s = ckd::comp_mix32(ctx, reinterpret_cast<uint32_t *>(hmapMix),
+                                 reinterpret_cast<const uint32_t *>(heights), (unsigned)(kMapPx / 4),
+                                 spikes)) != CKD_OK)
+            return s;
+        if (beams) {
+            if ((s = ckd::launch_memset32(ctx, beamMix, 0, kMapPx)) != CKD_OK) return s;
+            const float a[3] = {p.beams1, p.beams2, p.beams3};
+            for (int m = 0; m < 3; ++m) {
+                const float sa = smax(0.f, smin(1.f, a[m])); // saturatef
+                if (sa > 0.f &&
+                    (s = ckd_blit_add32a(ctx, beamMix, beamMaps + m * kMapPx, 1024, 1024, 1024, sa)) != CKD_OK)
+                    return s;
+            }
+        }
+        // vball (code/ball.cpp:290-333)
+        BallFrame *f = reinterpret_cast<BallFrame *>(host.data());
+        ball_frame(p, H, *f, dXY);
+        std::memcpy(host.data() + sizeof(BallFrame), dXY.data(), sizeof(int2) * H);
+        void *dHost = nullptr;
+        if ((s = ckd::stage(ctx, host.data(), host.size(), &dHost)) != CKD_OK) return s;
+        const BallFrame *df = static_cast<const BallFrame *>(dHost);
+        const int2 *ddXY = reinterpret_cast<const int2 *>(static_cast<uint8_t *>(dHost) + sizeof(BallFrame));
+        if (!beams && (s = ckd::launch_memset32(ctx, target, 0, px)) != CKD_OK) return s;
+        if (beams)
+            k_ball<true><<<ckd_div_up(H, 256), 256, 0, ctx->stream>>>(target, W, H, df, ddXY, hmapMix,
+                                                                       cmaps, beamMix);
+        else
+            k_ball<false><<<ckd_div_up(H, 256), 256, 0, ctx->stream>>>(target, W, H, df, ddXY, hmapMix,
+                                                                        cmaps + kMapPx, env);
+        if ((s = ckd::check_launch("ball")) != CKD_OK) return s;
+        if (p.blur != 0.f &&
+            (s = ckd_blur07_h(ctx, target, target, W, H, box_blur_scale(p.blur))) != CKD_OK)
+            return s;
+        uint32_t *out = dst + px * k;
+        CKD_HIP(hipMemcpyAsync(out, backgrounds + (beams ? 0 : px), px * 4, hipMemcpyDeviceToDevice,
+                               ctx->stream));
+        if ((s = ckd_polar_blit_a(ctx, out, target, W, H, 0)) != CKD_OK) return s;
+        if (beams && (s = ckd_softlight32a(ctx, out, halo, (unsigned)px)) != CKD_OK) return s;
+    }
+    return CKD_OK;
+}
+
+} // extern "C"

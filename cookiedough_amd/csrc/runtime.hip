@@ -211,7 +211,8 @@ void ckd_destroy(ckd_ctx *ctx)
     void *bufs[] = {ctx->cosLut, ctx->lsHeight, ctx->lsColor, ctx->lsFog, ctx->tsHeight, ctx->tsColor,
                     ctx->tsFog, ctx->polarMap, ctx->polarInv, ctx->b26Scratch[0],
                     ctx->b26Scratch[1], ctx->staging, ctx->fdTex, ctx->fdTexHi,
-                    ctx->stScratch, ctx->spikeBlurMaps};
+                    ctx->stScratch, ctx->spikeBlurMaps, ctx->twHeight, ctx->twColor,
+                    ctx->twTables, ctx->twBackground, ctx->ballMaps, ctx->ballImages};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete ctx;

@@ -224,6 +224,49 @@ class Context:
         else:
             check(fn(self.h, ptr(dst), resX, resY, arr, n))
 
+    # ---- voxel ball, torus twister (code/ball.cpp, code/torus-twister.cpp)
+    def polar_blit_a(self, dst, src, W, H, inverse=False):
+        check(self.lib.ckd_polar_blit_a(self.h, ptr(dst), ptr(src), W, H, int(inverse)))
+
+    @staticmethod
+    def _host(a, dtype, n, what):
+        a = np.ascontiguousarray(a, dtype).reshape(-1)
+        if a.size != n:
+            raise ValueError(f"{what}: need {n} elements, got {a.size}")
+        return a
+
+    def twister_create(self, hmap, cmap, background, W=1280, H=720):
+        m = 1024 * 1024
+        hmap = self._host(hmap, np.uint8, m, "height map")
+        cmap = self._host(cmap, np.uint32, m, "colour map")
+        bg = self._host(background, np.uint32, W * H, "background")
+        check(self.lib.ckd_twister_create(self.h, hmap.ctypes.data, cmap.ctypes.data,
+                                          bg.ctypes.data, W, H))
+
+    def twister_draw(self, dst, target, W, H, frames):
+        frames = [frames] if isinstance(frames, _lib.TwisterParams) else list(frames)
+        arr = (_lib.TwisterParams * len(frames))(*frames)
+        check(self.lib.ckd_twister_draw(self.h, ptr(dst), ptr(target), W, H, arr, len(frames)))
+
+    def ball_create(self, maps: dict, W=1280, H=720):
+        """maps as workloads.ball_maps: heightMaps (5 x 1024^2 bytes),
+        colorMaps (2 x 1024^2), beamMaps (3 x 1024^2), envMap (1024^2),
+        backgrounds (2 x W*H), halo (W*H)"""
+        m = 1024 * 1024
+        h = self._host(maps["heightMaps"], np.uint8, 5 * m, "height maps")
+        c = self._host(maps["colorMaps"], np.uint32, 2 * m, "colour maps")
+        b = self._host(maps["beamMaps"], np.uint32, 3 * m, "beam maps")
+        e = self._host(maps["envMap"], np.uint32, m, "environment map")
+        bg = self._host(maps["backgrounds"], np.uint32, 2 * W * H, "backgrounds")
+        halo = self._host(maps["halo"], np.uint32, W * H, "halo")
+        check(self.lib.ckd_ball_create(self.h, h.ctypes.data, c.ctypes.data, b.ctypes.data,
+                                       e.ctypes.data, bg.ctypes.data, halo.ctypes.data, W, H))
+
+    def ball_draw(self, dst, target, W, H, frames):
+        frames = [frames] if isinstance(frames, _lib.BallParams) else list(frames)
+        arr = (_lib.BallParams * len(frames))(*frames)
+        check(self.lib.ckd_ball_draw(self.h, ptr(dst), ptr(target), W, H, arr, len(frames)))
+
     def landscape_draw(self, dst, W, H, cams):
         arr = (_lib.LandscapeCam * len(cams))(*cams)
         check(self.lib.ckd_landscape_draw(self.h, ptr(dst), W, H, arr, len(cams)))

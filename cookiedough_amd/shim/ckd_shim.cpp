@@ -169,6 +169,32 @@ StTracks<6> s_lauraTracks;
 StTracks<14> s_tunnelTracks;
 uint32_t *s_spikeBlurMaps[2] = {nullptr, nullptr}; // the close-up's mix-blur layer
 
+// ---- voxel ball and torus twister (code/ball.cpp:337-441,
+// code/torus-twister.cpp:136-158) ------------------------------------------
+const char *const kBallTracks[] = {
+    "ball:Blur", "ball:Radius", "ball:RayLength", "ball:Spikes", "ball:HasBeams",
+    "ball:BaseShapeIndex", "ball:Speed", "ball:BeamAttenuation", "ball:BeamAlphaMin",
+    "ball:RotateOffsX", "ball:RotateOffsY", "ball:Beams1", "ball:Beams2", "ball:Beams3",
+    "ball:BallLowBeams"};
+const char *const kTwisterTracks[] = {"twister:Speed", "twister::ShearSpeed", "twister:Blur"};
+
+bool s_ballReady = false, s_twReady = false;
+StTracks<15> s_ballTracks;
+StTracks<3> s_twisterTracks;
+uint32_t *s_ballBackground = nullptr; // Ball_GetBackground
+// the render target the ball and the twister draw into (the reference's
+// g_renderTarget[0]), cleared when allocated; the beam rays leave its last
+// column as it was
+DevImage g_target;
+
+uint32_t *render_target(size_t pixels)
+{
+    const bool fresh = g_target.cap < pixels;
+    uint32_t *t = g_target.get(pixels);
+    if (t && fresh && ckd_memset32(g_ctx, t, 0, pixels) != CKD_OK) return nullptr;
+    return t;
+}
+
 // renders one frame on the GPU into the staging image and downloads it
 template <typename F> void st_draw(const char *name, uint32_t *pDest, F render)
 {
@@ -600,4 +626,132 @@ void Zoom32(uint32_t *pDest, const uint32_t *pSrc, unsigned xRes, unsigned yRes,
     filter("Zoom32", pDest, pSrc, (size_t)xRes * yRes, false, [&](uint32_t *d, uint32_t *s) {
         return ckd_zoom32(g_ctx, d, s, xRes, yRes, scale);
     });
+}
+
+void Polar_BlitA(uint32_t *pDest, const uint32_t *pSrc, bool inverse)
+{
+    const size_t n = (size_t)kResX * kResY;
+    blit2("Polar_BlitA", pDest, n, pSrc, n, [&](uint32_t *d, uint32_t *s) {
+        return ckd_polar_blit_a(g_ctx, d, s, kResX, kResY, inverse ? 1 : 0);
+    });
+}
+
+// ---- voxel ball ---------------------------------------------------------
+
+bool Ball_Create()
+{
+    if (!acquire()) return false;
+    static const char *const kHeightMaps[5] = {"assets/ball/hmap_1_1k.jpg", "assets/ball/hmap_4_1k.jpg",
+                                               "assets/ball/hmap_2_1k.jpg", "assets/ball/hmap_3_1k.jpg",
+                                               "assets/ball/hmap_5_1k.jpg"};
+    static const char *const kColorMaps[2] = {"assets/ball/colormap_1k.jpg",
+                                              "assets/ball/colormap_2_1k.jpg"};
+    static const char *const kBeamMaps[3] = {"assets/ball/beammap_1k_1.jpg",
+                                             "assets/ball/beammap_1k_2.jpg",
+                                             "assets/ball/beammap_1k_3-2.jpg"};
+    static const char *const kBackgrounds[2] = {"assets/ball/nytrik-background_1280x720.png",
+                                                "assets/ball/nytrik-background-2-1280x720.png"};
+    constexpr size_t kMap = 1024 * 1024, kOut = (size_t)kResX * kResY;
+    std::string err;
+    uint8_t *heights = new uint8_t[5 * kMap];
+    uint32_t *images = new uint32_t[6 * kMap + 3 * kOut];
+    uint32_t *colors = images, *beams = images + 2 * kMap, *env = images + 5 * kMap;
+    uint32_t *bgs = images + 6 * kMap, *halo = bgs + 2 * kOut;
+    bool ok = true;
+    for (int i = 0; ok && i < 5; ++i) {
+        const uint8_t *m = Image_Load8(kHeightMaps[i]);
+        if ((ok = m != nullptr)) memcpy(heights + i * kMap, m, kMap);
+    }
+    for (int i = 0; ok && i < 2; ++i) {
+        const uint32_t *m = Image_Load32(kColorMaps[i]);
+        if ((ok = m != nullptr)) memcpy(colors + i * kMap, m, kMap * 4);
+    }
+    for (int i = 0; ok && i < 3; ++i) {
+        const uint32_t *m = Image_Load32(kBeamMaps[i]);
+        if ((ok = m != nullptr)) memcpy(beams + i * kMap, m, kMap * 4);
+    }
+    const uint32_t *e = ok ? Image_Load32("assets/ball/envmap3_1k.jpg") : nullptr;
+    if (ok && (ok = e != nullptr)) memcpy(env, e, kMap * 4);
+    uint32_t *bg[2] = {nullptr, nullptr};
+    for (int i = 0; ok && i < 2; ++i) {
+        bg[i] = Image_Load32(kBackgrounds[i]);
+        if ((ok = bg[i] != nullptr)) memcpy(bgs + i * kOut, bg[i], kOut * 4);
+    }
+    const uint32_t *h = ok ? Image_Load32("assets/ball/halo.png") : nullptr;
+    if (ok && (ok = h != nullptr)) memcpy(halo, h, kOut * 4);
+    if (ok && ckd_ball_create(g_ctx, heights, colors, beams, env, bgs, halo, kResX, kResY) != CKD_OK) {
+        report("Ball_Create");
+        ok = false;
+    }
+    delete[] heights;
+    delete[] images;
+    if (!ok) {
+        release();
+        return false;
+    }
+    s_ballBackground = bg[0];
+    s_ballTracks.add(kBallTracks);
+    s_ballReady = true;
+    return true;
+}
+
+void Ball_Destroy()
+{
+    if (s_ballReady) release();
+    s_ballReady = false;
+}
+
+void Ball_Draw(uint32_t *pDest, float time, float delta)
+{
+    (void)delta;
+    if (!s_ballReady) return;
+    const ckd_ball_params p = s_ballTracks.read<ckd_ball_params>(time);
+    const size_t n = (size_t)kResX * kResY;
+    uint32_t *a = g_a.get(n), *t = render_target(n);
+    if (!a || !t || ckd_ball_draw(g_ctx, a, t, kResX, kResY, &p, 1) != CKD_OK || !down(pDest, a, n))
+        report("Ball_Draw");
+}
+
+uint32_t *Ball_GetBackground() { return s_ballBackground; }
+
+bool Ball_HasBeams()
+{
+    // Rocket::geti: int(roundf(getf(track)))
+    return (int)roundf((float)Rocket::get(s_ballTracks.t[4])) != 0;
+}
+
+// ---- torus twister --------------------------------------------------------
+
+bool Twister_Create()
+{
+    if (!acquire()) return false;
+    const uint8_t *hm = Image_Load8("assets/twister/hmap_2_1k.jpg");
+    const uint32_t *cm = Image_Load32("assets/twister/colormap_1k.jpg");
+    const uint32_t *bg = (hm && cm) ? Image_Load32("assets/twister/nytrik-background_1280x720.png")
+                                    : nullptr;
+    if (!bg || ckd_twister_create(g_ctx, hm, cm, bg, kResX, kResY) != CKD_OK) {
+        if (bg) report("Twister_Create");
+        release();
+        return false;
+    }
+    s_twisterTracks.add(kTwisterTracks);
+    s_twReady = true;
+    return true;
+}
+
+void Twister_Destroy()
+{
+    if (s_twReady) release();
+    s_twReady = false;
+}
+
+void Twister_Draw(uint32_t *pDest, float time, float delta)
+{
+    (void)delta;
+    if (!s_twReady) return;
+    const ckd_twister_params p = s_twisterTracks.read<ckd_twister_params>(time);
+    const size_t n = (size_t)kResX * kResY;
+    uint32_t *a = g_a.get(n), *t = render_target(n);
+    if (!a || !t || ckd_twister_draw(g_ctx, a, t, kResX, kResY, &p, 1) != CKD_OK || !down(pDest, a, n))
+        report("Twister_Draw");
 }

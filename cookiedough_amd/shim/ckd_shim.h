@@ -2,8 +2,8 @@
 // implemented over the libckd C ABI (include/ckd.h).  Same names and
 // signatures as the reference headers, so code/demo.cpp links against this
 // shim in place of landscape.cpp, tunnelscape.cpp, polar.cpp,
-// fx-blitter.cpp, boxblur.cpp, deprecated/boxblur.cpp, shadertoy.cpp and
-// the blits of util.cpp.  Buffers are host memory, as in the reference (1280x720 frames,
+// fx-blitter.cpp, boxblur.cpp, deprecated/boxblur.cpp, shadertoy.cpp,
+// ball.cpp, torus-twister.cpp and the blits of util.cpp.  Buffers are host memory, as in the reference (1280x720 frames,
 // 644x364 FX maps); every call uploads its inputs, renders on the GPU and
 // downloads the result before returning.
 #pragma once
@@ -25,6 +25,20 @@ void Tunnelscape_Draw(uint32_t *pDest, float time, float delta);
 bool Polar_Create();
 void Polar_Destroy();
 void Polar_Blit(uint32_t *pDest, const uint32_t *pSrc, bool inverse = false);
+// code/polar.h:14
+void Polar_BlitA(uint32_t *pDest, const uint32_t *pSrc, bool inverse = false);
+
+// code/ball.h:7-13
+bool Ball_Create();
+void Ball_Destroy();
+void Ball_Draw(uint32_t *pDest, float time, float delta);
+uint32_t *Ball_GetBackground();
+bool Ball_HasBeams();
+
+// code/torus-twister.h:7-9
+bool Twister_Create();
+void Twister_Destroy();
+void Twister_Draw(uint32_t *pDest, float time, float delta);
 
 // code/fx-blitter.h:26-33
 constexpr size_t kShimNumFxMaps = 4;

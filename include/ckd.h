@@ -500,6 +500,70 @@ CKD_API ckd_status ckd_fdtunnel_map(ckd_ctx *ctx, uint32_t *maps, uint32_t *glow
 CKD_API ckd_status ckd_fdtunnel_draw(ckd_ctx *ctx, uint32_t *dst, unsigned resX, unsigned resY,
                                      const ckd_fdtunnel_params *frames, unsigned nFrames);
 
+/* ---- voxel ball, torus twister, Polar_BlitA ------------------------------ */
+
+/* Polar_BlitA (code/polar.cpp:157-198): dst = dst + a*(s - dst)/256 per
+ * channel, s the bilinear fetch of src through the polar map (inverse = 1:
+ * the inverse map) and a its alpha.  W x H must match ckd_polar_create. */
+CKD_API ckd_status ckd_polar_blit_a(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, unsigned W,
+                                    unsigned H, int inverse);
+
+/* Twister_Create / Twister_Draw (code/torus-twister.cpp:136-187).  _create
+ * copies the 1024x1024 height map (bytes), the 1024x1024 colour map and the
+ * W x H background from host memory (the reference's 1280 x 720; an even W
+ * of at least 1200, so that both 597-pixel rays of a row fit) and builds the
+ * polar maps for W x H.  _draw renders nFrames frames into dst
+ * (nFrames * W * H pixels): the twister into `target` (W x H, device
+ * scratch: the reference's g_renderTarget[0]), its optional box blur, then
+ * the background with Polar_BlitA of the target. */
+typedef struct ckd_twister_params {
+    float time;
+    float speed;      /* twister:Speed        */
+    float shearSpeed; /* twister::ShearSpeed  */
+    float blur;       /* twister:Blur         */
+} ckd_twister_params;
+CKD_API ckd_status ckd_twister_create(ckd_ctx *ctx, const uint8_t *heightMap,
+                                      const uint32_t *colorMap, const uint32_t *background,
+                                      unsigned W, unsigned H);
+CKD_API ckd_status ckd_twister_draw(ckd_ctx *ctx, uint32_t *dst, uint32_t *target, unsigned W,
+                                    unsigned H, const ckd_twister_params *frames,
+                                    unsigned nFrames);
+
+/* Ball_Create / Ball_Draw (code/ball.cpp:337-511).  _create copies, from
+ * host memory: five 1024x1024 height maps (bytes; map 0 is the spike map
+ * mixed over the base shape 1..4), two colour maps (beams, no beams), three
+ * beam maps, the environment map, two W x H backgrounds (beams, no beams)
+ * and the W x H halo, and builds the polar maps.  _draw renders nFrames
+ * frames into dst through `target` (W x H device scratch, the reference's
+ * g_renderTarget[0]; the beam rays leave its last column as it was).  A
+ * frame whose radius would project a map height past the end of a row
+ * (height * radius / 256 > W - 1, where the reference writes into the next
+ * row) is rejected. */
+typedef struct ckd_ball_params {
+    float time;
+    float blur;            /* ball:Blur             */
+    float radius;          /* ball:Radius           */
+    float rayLength;       /* ball:RayLength        */
+    float spikes;          /* ball:Spikes           */
+    float hasBeams;        /* ball:HasBeams         */
+    float baseShapeIndex;  /* ball:BaseShapeIndex   */
+    float speed;           /* ball:Speed            */
+    float beamAttenuation; /* ball:BeamAttenuation  */
+    float beamAlphaMin;    /* ball:BeamAlphaMin     */
+    float rotateOffsX;     /* ball:RotateOffsX      */
+    float rotateOffsY;     /* ball:RotateOffsY      */
+    float beams1;          /* ball:Beams1           */
+    float beams2;          /* ball:Beams2           */
+    float beams3;          /* ball:Beams3           */
+    float lowBeams;        /* ball:BallLowBeams     */
+} ckd_ball_params;
+CKD_API ckd_status ckd_ball_create(ckd_ctx *ctx, const uint8_t *heightMaps5,
+                                   const uint32_t *colorMaps2, const uint32_t *beamMaps3,
+                                   const uint32_t *envMap, const uint32_t *backgrounds2,
+                                   const uint32_t *halo, unsigned W, unsigned H);
+CKD_API ckd_status ckd_ball_draw(ckd_ctx *ctx, uint32_t *dst, uint32_t *target, unsigned W,
+                                 unsigned H, const ckd_ball_params *frames, unsigned nFrames);
+
 #ifdef __cplusplus
 }
 #endif

@@ -14,6 +14,10 @@
 //                                                         SoftLight32AA take <arg>)
 //   shim_test shadertoy <plasma|nautilus|spikey|spikey_close|sinuses|laura|tunnel>
 //             <time> <out>                            (1280x720 frame)
+//   shim_test ball <time> <frames> <out>              (frames successive 1280x720
+//                                                     frames at the same time)
+//   shim_test twister <time> <out>
+//   shim_test polar_blit_a <dst> <src> <inverse 0|1> <out>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -185,6 +189,26 @@ int main(int argc, char **argv)
         else return 2;
         Shadertoy_Destroy();
         write_pixels(argv[4], out.data(), n);
+    } else if (cmd == "ball" && argc == 5) {
+        const int frames = atoi(argv[3]);
+        if (frames < 1 || !Ball_Create()) return 1;
+        std::vector<uint32_t> all((size_t)frames * n);
+        for (int k = 0; k < frames; ++k) Ball_Draw(all.data() + (size_t)k * n, (float)atof(argv[2]), 1.f);
+        if (Ball_GetBackground() == nullptr) return 1;
+        printf("has beams: %d\n", Ball_HasBeams() ? 1 : 0);
+        Ball_Destroy();
+        write_pixels(argv[4], all.data(), all.size());
+    } else if (cmd == "twister" && argc == 4) {
+        if (!Twister_Create()) return 1;
+        Twister_Draw(out.data(), (float)atof(argv[2]), 1.f);
+        Twister_Destroy();
+        write_pixels(argv[3], out.data(), n);
+    } else if (cmd == "polar_blit_a" && argc == 6) {
+        std::vector<uint32_t> d = read_pixels(argv[2]), s = read_pixels(argv[3]);
+        if (d.size() != n || s.size() != n || !Polar_Create()) return 1;
+        Polar_BlitA(d.data(), s.data(), atoi(argv[4]) != 0);
+        Polar_Destroy();
+        write_pixels(argv[5], d.data(), n);
     } else {
         return 2;
     }

@@ -129,10 +129,32 @@ def test_shadertoy_argument_validation_without_gpu():
 def test_shadertoy_param_structs_match_the_header():
     """field order of each ctypes struct == the C struct in include/ckd.h"""
     text = (ROOT / "include" / "ckd.h").read_text()
-    for name, cls in _lib.ST_PARAMS.items():
+    structs = dict(_lib.ST_PARAMS, ball=_lib.BallParams, twister=_lib.TwisterParams)
+    for name, cls in structs.items():
         body = re.search(r"typedef struct ckd_%s_params \{(.*?)\} ckd_%s_params;" % (name, name),
                          text, re.S).group(1)
         fields = re.findall(r"float (\w+);", body)
         assert fields == [f for f, _ in cls._fields_], name
         tracks = re.findall(r"/\* ([\w:]+)\s*\*/", body)
         assert tracks == list(cls.TRACKS.values()), name
+
+
+def test_voxel2_argument_validation_without_gpu():
+    lib = _lib.load()
+    b = _lib.BallParams(time=1.0, radius=float("nan"))
+    assert lib.ckd_ball_draw(None, 8, 16, 1280, 720, C.byref(b), 1) == CKD_EINVAL
+    assert "finite" in _lib.last_error()
+    assert lib.ckd_ball_draw(None, 8, 8, 1280, 720, C.byref(b), 1) == CKD_EINVAL
+    assert "differ" in _lib.last_error()
+    b.radius = 1800.0
+    assert lib.ckd_ball_draw(None, 8, 16, 1280, 720, C.byref(b), 1) == CKD_EINVAL
+    assert "null context" in _lib.last_error()
+    assert lib.ckd_ball_create(None, 8, 8, 8, 8, 8, None, 1280, 720) == CKD_EINVAL
+    assert "null map" in _lib.last_error()
+    t = _lib.TwisterParams(time=1.0, speed=float("inf"))
+    assert lib.ckd_twister_draw(None, 8, 16, 1280, 720, C.byref(t), 1) == CKD_EINVAL
+    assert "finite" in _lib.last_error()
+    assert lib.ckd_twister_create(None, 8, 8, 8, 1000, 720) == CKD_EINVAL
+    assert "width" in _lib.last_error()
+    assert lib.ckd_polar_blit_a(None, 8, 8, 1280, 720, 0) == CKD_EINVAL
+    assert "differ" in _lib.last_error()

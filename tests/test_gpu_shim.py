@@ -195,3 +195,85 @@ def test_blends(orc, tmp_path, op, arg):
     else:
         getattr(orc, "orc_" + op)(P(want), P(s), n)
     assert np.array_equal(np.fromfile(out, np.uint32), want), op
+
+
+@pytest.fixture(scope="module")
+def v2_assets(tmp_path_factory, orc):
+    d = tmp_path_factory.mktemp("v2_assets")
+    bm, tm = wl.ball_maps(1280, 720), wl.twister_maps(1280, 720)
+    mp = 1024 * 1024
+    names = ["hmap_1_1k.jpg", "hmap_4_1k.jpg", "hmap_2_1k.jpg", "hmap_3_1k.jpg", "hmap_5_1k.jpg"]
+    for i, nm in enumerate(names):
+        (d / f"{nm}.bin").write_bytes(bm["heightMaps"][i * mp:(i + 1) * mp].tobytes())
+    for i, nm in enumerate(["colormap_1k.jpg", "colormap_2_1k.jpg"]):
+        (d / f"{nm}.bin").write_bytes(bm["colorMaps"][i * mp:(i + 1) * mp].tobytes())
+    for i, nm in enumerate(["beammap_1k_1.jpg", "beammap_1k_2.jpg", "beammap_1k_3-2.jpg"]):
+        (d / f"{nm}.bin").write_bytes(bm["beamMaps"][i * mp:(i + 1) * mp].tobytes())
+    (d / "envmap3_1k.jpg.bin").write_bytes(bm["envMap"].tobytes())
+    n = 1280 * 720
+    for i, nm in enumerate(["nytrik-background_1280x720.png", "nytrik-background-2-1280x720.png"]):
+        (d / f"{nm}.bin").write_bytes(bm["backgrounds"][i * n:(i + 1) * n].tobytes())
+    (d / "halo.png.bin").write_bytes(bm["halo"].tobytes())
+    # the twister's own assets (asset names without folders: its colour map
+    # and background share the ball's file names, so they get a folder)
+    tw = d / "twister"
+    tw.mkdir()
+    (tw / "hmap_2_1k.jpg.bin").write_bytes(tm["heightMap"].tobytes())
+    (tw / "colormap_1k.jpg.bin").write_bytes(tm["colorMap"].tobytes())
+    (tw / "nytrik-background_1280x720.png.bin").write_bytes(tm["background"].tobytes())
+    m0 = np.zeros(2 * n, np.int32)
+    orc.orc_polar_maps(P(m0), P(np.zeros(2 * n, np.int32)), 1280, 720, 1280, 720)
+    return d, bm, tm, m0
+
+
+def _tracks(prm):
+    return ";".join(f"{tr}={repr(float(getattr(prm, f)))}" for f, tr in type(prm).TRACKS.items())
+
+
+@pytest.mark.parametrize("demo", [0, 1])
+def test_ball_draws(orc, v2_assets, tmp_path, demo):
+    """Ball_Create + two Ball_Draw calls (the render target carries over, as
+    g_renderTarget[0] does) against the oracle's Draw, bit-exact"""
+    d, bm, _, m0 = v2_assets
+    prm = wl.ball_frames(1, demo, t0=12.25)[0]
+    out = tmp_path / "ball.bin"
+    r = run(["ball", repr(float(prm.time)), 2, out],
+            env={"CKD_SHIM_ASSETS": str(d), "CKD_SHIM_TRACKS": _tracks(prm)})
+    assert f"has beams: {int(prm.hasBeams)}" in r.stdout
+    got = np.fromfile(out, np.uint32).reshape(2, -1)
+    target = np.zeros(1280 * 720, np.uint32)
+    for k in range(2):
+        want = np.zeros(1280 * 720, np.uint32)
+        assert orc.orc_ball_draw(P(want), P(target), 1280, 720, P(bm["heightMaps"]),
+                                 P(bm["colorMaps"]), P(bm["beamMaps"]), P(bm["envMap"]),
+                                 P(bm["backgrounds"]), P(bm["halo"]), P(m0),
+                                 P(prm.as_array()), 8) == 0
+        assert np.array_equal(got[k], want), (demo, k)
+
+
+def test_twister_draw(orc, v2_assets, tmp_path):
+    d, _, tm, m0 = v2_assets
+    prm = wl.twister_frames(1, t0=7.5)[0]
+    prm.blur = 9.0
+    out = tmp_path / "tw.bin"
+    run(["twister", repr(float(prm.time)), out],
+        env={"CKD_SHIM_ASSETS": str(d / "twister"), "CKD_SHIM_TRACKS": _tracks(prm)})
+    want, tgt = np.zeros(1280 * 720, np.uint32), np.zeros(1280 * 720, np.uint32)
+    assert orc.orc_twister_draw(P(want), P(tgt), 1280, 720, P(tm["heightMap"]), P(tm["colorMap"]),
+                                P(tm["background"]), P(m0), P(prm.as_array()), 8) == 0
+    assert np.array_equal(np.fromfile(out, np.uint32), want)
+
+
+@pytest.mark.parametrize("inverse", [0, 1])
+def test_polar_blit_a(orc, tmp_path, inverse):
+    n = 1280 * 720
+    dst, src = rand_img(74, n), rand_img(75, n)
+    dst.tofile(tmp_path / "d.bin")
+    src.tofile(tmp_path / "s.bin")
+    out = tmp_path / "out.bin"
+    run(["polar_blit_a", tmp_path / "d.bin", tmp_path / "s.bin", inverse, out])
+    m0, m1 = np.zeros(2 * n, np.int32), np.zeros(2 * n, np.int32)
+    orc.orc_polar_maps(P(m0), P(m1), 1280, 720, 1280, 720)
+    want = dst.copy()
+    orc.orc_polar_blit_a(P(want), P(src), P(m1 if inverse else m0), 1280, 720, 1280, 8)
+    assert np.array_equal(np.fromfile(out, np.uint32), want)
