@@ -93,6 +93,77 @@ CKD_DEV void span16(uint32_t *__restrict__ dst, int incr, uint32_t length, uint3
     }
 }
 
+// ---- one wave per ray --------------------------------------------------------
+//
+// The reference marches a ray step by step: step i samples the maps at
+// from - (i+1)*d, projects a height H_i, and when H_i exceeds the furthest
+// pixel drawn so far (D_i = max(0, H_0..H_{i-1})) draws the span
+// [D_i, H_i) from the previous step's colour to its own, with length
+// H_i - H_{i-1}.  Only D_i (a running maximum) and, with beams, the
+// saturating beam sum (min(65535, prefix sum) for non-negative terms) carry
+// from step to step, so a 64-lane wave takes 64 steps at once: every lane
+// samples its own step, wave prefix scans give D_i and the beam sums, the
+// previous step's height and colour come from the neighbouring lane, and
+// lane 63's values carry into the next 64 steps.  Every span is the one the
+// sequential loop draws, pixel for pixel.
+
+constexpr unsigned kWave = 64;
+
+CKD_DEV uint32_t wave_incl_max(uint32_t v, unsigned lane)
+{
+#pragma unroll
+    for (unsigned off = 1; off < kWave; off <<= 1) {
+        const uint32_t o = __shfl_up(v, off, kWave);
+        if (lane >= off) v = o > v ? o : v;
+    }
+    return v;
+}
+
+CKD_DEV uint32_t wave_incl_sum(uint32_t v, unsigned lane)
+{
+#pragma unroll
+    for (unsigned off = 1; off < kWave; off <<= 1) {
+        const uint32_t o = __shfl_up(v, off, kWave);
+        if (lane >= off) v += o;
+    }
+    return v;
+}
+
+// the state one chunk of 64 steps hands to the next
+struct RayCarry {
+    uint32_t drawn = 0;      // D: furthest pixel drawn
+    uint32_t height = 0;     // the last step's height
+    uint32_t colLo, colHi;   // the last step's colour, two 16-bit lanes each
+};
+
+// draws this lane's span and advances the carry; `last` is the lane of the
+// chunk's last step
+CKD_DEV void chunk_spans(uint32_t *__restrict__ base, int dir, unsigned lane, unsigned last,
+                         bool active, uint32_t height, const uint32_t color[4], RayCarry &c)
+{
+    const uint32_t lo = color[0] | (color[1] << 16), hi = color[2] | (color[3] << 16);
+    uint32_t pLo = __shfl_up(lo, 1, kWave), pHi = __shfl_up(hi, 1, kWave);
+    uint32_t pH = __shfl_up(height, 1, kWave);
+    const uint32_t incl = wave_incl_max(height, lane);
+    uint32_t D = __shfl_up(incl, 1, kWave);
+    if (lane == 0) {
+        pLo = c.colLo;
+        pHi = c.colHi;
+        pH = c.height;
+        D = 0;
+    }
+    D = D > c.drawn ? D : c.drawn;
+    if (active && height > D) {
+        const uint32_t A[4] = {pLo & 0xffffu, pLo >> 16, pHi & 0xffffu, pHi >> 16};
+        span16(base + (ptrdiff_t)dir * (ptrdiff_t)D, dir, height - pH, height - D, A, color);
+    }
+    const uint32_t top = __shfl(incl, kWave - 1, kWave);
+    c.drawn = top > c.drawn ? top : c.drawn;
+    c.height = __shfl(height, last, kWave);
+    c.colLo = __shfl(lo, last, kWave);
+    c.colHi = __shfl(hi, last, kWave);
+}
+
 // ---- the twister ---------------------------------------------------------
 
 struct TwFrame {
@@ -100,42 +171,38 @@ struct TwFrame {
 };
 
 // vtwister_ray (code/torus-twister.cpp:39-85) for both rays of every row:
-// lane 2*row+side; side 0 walks right from the centre, side 1 left
+// wave 2*row+side; side 0 walks right from the centre, side 1 left
 __global__ __launch_bounds__(256) void k_twister(uint32_t *__restrict__ target, unsigned W,
                                                  unsigned H, const TwFrame *__restrict__ rows,
                                                  const uint32_t *__restrict__ tables,
                                                  const uint8_t *__restrict__ hmap,
                                                  const uint32_t *__restrict__ cmap)
 {
-    const unsigned lane = blockIdx.x * blockDim.x + threadIdx.x;
-    const unsigned row = lane >> 1, side = lane & 1;
-    if (row >= H) return;
+    const unsigned lane = threadIdx.x & (kWave - 1);
+    const unsigned ray = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (ray >= 2 * H) return; // whole waves
+    const unsigned row = ray >> 1, side = ray & 1;
     const uint32_t *hp = tables, *hn = tables + kTwRay;
-    const int dX = side ? -512 : 512, dir = side ? -1 : 1;
-    int curX = side ? (int)((uint32_t)rows[row].fromX - 512u) : rows[row].fromX;
+    const uint32_t dX = side ? (uint32_t)-512 : 512u;
+    const int dir = side ? -1 : 1;
+    const int fromX = side ? (int)((uint32_t)rows[row].fromX - 512u) : rows[row].fromX;
     const int curY = rows[row].fromY;
-    uint32_t *p = target + (size_t)row * W + (W >> 1) - side;
-    uint32_t lastColor[4], color[4];
-    samp16(cmap, prep(curX, curY), lastColor);
-    uint32_t lastHeight = 0, lastDrawn = 0;
-    for (unsigned i = 0; i < kTwRay; ++i) {
-        curX = (int)((uint32_t)curX - (uint32_t)dX);
-        const Uv t = prep(curX, curY);
+    uint32_t *base = target + (size_t)row * W + (W >> 1) - side;
+    uint32_t c0[4];
+    samp16(cmap, prep(fromX, curY), c0);
+    RayCarry c;
+    c.colLo = c0[0] | (c0[1] << 16);
+    c.colHi = c0[2] | (c0[3] << 16);
+    for (unsigned i0 = 0; i0 < kTwRay; i0 += kWave) {
+        const unsigned i = i0 + lane;
+        const Uv t = prep((int)((uint32_t)fromX - (i + 1) * dX), curY);
         const uint32_t h = samp8(hmap, t);
+        uint32_t color[4];
         samp16(cmap, t, color);
-        const uint32_t diffuse = h * hn[i] >> 8;
+        const uint32_t diffuse = (h * hn[i] >> 8) & 0xffffu;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) color[k] = adds16(color[k], diffuse & 0xffffu);
-        const uint32_t height = h * hp[i] >> 8;
-        if (height > lastDrawn) {
-            const uint32_t drawLength = height - lastDrawn;
-            span16(p, dir, height - lastHeight, drawLength, lastColor, color);
-            p += (ptrdiff_t)drawLength * dir;
-            lastDrawn = height;
-        }
-        lastHeight = height;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) lastColor[k] = color[k];
+        for (int k = 0; k < 4; ++k) color[k] = adds16(color[k], diffuse);
+        chunk_spans(base, dir, lane, kWave - 1, true, h * hp[i] >> 8, color, c);
     }
 }
 
@@ -149,7 +216,7 @@ struct BallFrame {
     int32_t norm[kBallMaxRay][3];
 };
 
-// vball_ray_beams / vball_ray_no_beams (code/ball.cpp:77-261), lane per row
+// vball_ray_beams / vball_ray_no_beams (code/ball.cpp:77-261), wave per row
 template <bool kBeams>
 __global__ __launch_bounds__(256) void k_ball(uint32_t *__restrict__ target, unsigned W,
                                               unsigned H, const BallFrame *__restrict__ f,
@@ -158,74 +225,100 @@ __global__ __launch_bounds__(256) void k_ball(uint32_t *__restrict__ target, uns
                                               const uint32_t *__restrict__ cmap,
                                               const uint32_t *__restrict__ aux)
 {
-    const unsigned row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= H) return;
+    const unsigned lane = threadIdx.x & (kWave - 1);
+    const unsigned row = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (row >= H) return; // whole waves
     uint32_t *dst = target + (size_t)row * W;
-    const int dX = dXY[row].x, dY = dXY[row].y;
-    int curX = f->fromX, curY = f->fromY;
-    int envU = (1024 >> 1) << 8, envV = envU;
-    uint32_t lastColor[4], color[4], extra[4], beamAccum[4] = {0, 0, 0, 0};
-    samp16(cmap, prep(curX, curY), lastColor);
-    uint32_t lastHeight = 0, lastDrawn = 0;
-    const unsigned rayLength = f->rayLength;
-    for (unsigned i = 0; i < rayLength; ++i) {
-        curX = (int)((uint32_t)curX - (uint32_t)dX);
-        curY = (int)((uint32_t)curY - (uint32_t)dY);
-        const Uv t = prep(curX, curY);
-        const uint32_t h = samp8(hmapMix, t);
-        samp16(cmap, t, color);
+    const uint32_t dX = (uint32_t)dXY[row].x, dY = (uint32_t)dXY[row].y;
+    const uint32_t fromX = (uint32_t)f->fromX, fromY = (uint32_t)f->fromY;
+    const uint32_t env0 = (1024u >> 1) << 8;
+    const unsigned L = f->rayLength;
+    uint32_t c0[4];
+    samp16(cmap, prep((int)fromX, (int)fromY), c0);
+    RayCarry c;
+    c.colLo = c0[0] | (c0[1] << 16);
+    c.colHi = c0[2] | (c0[3] << 16);
+    uint32_t beamSum[4] = {0, 0, 0, 0}; // unsaturated; at most 1024*255
+    for (unsigned i0 = 0; i0 < L; i0 += kWave) {
+        const unsigned i = i0 + lane;
+        const bool active = i < L;
+        uint32_t height = 0, color[4] = {0, 0, 0, 0}, extra[4], bl[4] = {0, 0, 0, 0}, lit = 0;
+        if (active) {
+            const uint32_t n = i + 1;
+            const Uv t = prep((int)(fromX - n * dX), (int)(fromY - n * dY));
+            const uint32_t h = samp8(hmapMix, t);
+            samp16(cmap, t, color);
+            if constexpr (kBeams) {
+                samp16(aux, t, extra); // beam map
+                const uint32_t hi = h * (uint32_t)f->norm[i][0] >> 8;
+                const uint32_t lo = h * (uint32_t)f->norm[i][1] >> 8;
+                lit = (hi + (((uint32_t)(int32_t)(lo - hi) * f->lowLight) >> 8)) & 0xffffu;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const uint32_t b = ((extra[k] * f->beamAtten) & 0xffffu) >> 8;
+                    bl[k] = ((b * lit) & 0xffffu) >> 8;
+                }
+            } else {
+                const uint32_t eu = env0 - n * (dX << 1), ev = env0 - n * (dY << 1);
+                samp16(aux, prep((int)(eu + h), (int)(ev + h)), extra); // environment map
+                const uint32_t diffuse = h * (uint32_t)f->norm[i][2] >> 8;
+                const uint32_t full = (32u + diffuse) < 255u ? 32u + diffuse : 255u;
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    color[k] = adds16(adds16(color[k], ((extra[k] * full) & 0xffffu) >> 8),
+                                      diffuse & 0xffffu);
+            }
+            height = h * f->heightProj[i] >> 8;
+        }
         if constexpr (kBeams) {
-            samp16(aux, t, extra); // beam map
-            const uint32_t hi = h * (uint32_t)f->norm[i][0] >> 8;
-            const uint32_t lo = h * (uint32_t)f->norm[i][1] >> 8;
-            const uint32_t lit = (hi + (((uint32_t)(int32_t)(lo - hi) * f->lowLight) >> 8)) & 0xffffu;
+            // a chunk's sums stay below 64*255, so two channels share a register
+            const uint32_t s01 = wave_incl_sum(bl[0] | (bl[1] << 16), lane);
+            const uint32_t s23 = wave_incl_sum(bl[2] | (bl[3] << 16), lane);
+            const uint32_t part[4] = {s01 & 0xffffu, s01 >> 16, s23 & 0xffffu, s23 >> 16};
+            const uint32_t t01 = __shfl(s01, kWave - 1, kWave), t23 = __shfl(s23, kWave - 1, kWave);
+            const uint32_t tot[4] = {t01 & 0xffffu, t01 >> 16, t23 & 0xffffu, t23 >> 16};
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const uint32_t b = ((extra[k] * f->beamAtten) & 0xffffu) >> 8;
-                beamAccum[k] = adds16(beamAccum[k], ((b * lit) & 0xffffu) >> 8);
-                color[k] = adds16(adds16(color[k], beamAccum[k]), lit);
+                const uint32_t s = beamSum[k] + part[k];
+                const uint32_t acc = s < 65535u ? s : 65535u;
+                color[k] = adds16(adds16(color[k], acc), lit);
+                beamSum[k] += tot[k];
             }
-        } else {
-            envU = (int)((uint32_t)envU - ((uint32_t)dX << 1));
-            envV = (int)((uint32_t)envV - ((uint32_t)dY << 1));
-            samp16(aux, prep((int)((uint32_t)envU + h), (int)((uint32_t)envV + h)), extra); // env map
-            const uint32_t diffuse = h * (uint32_t)f->norm[i][2] >> 8;
-            const uint32_t full = (32u + diffuse) < 255u ? 32u + diffuse : 255u;
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-                color[k] = adds16(adds16(color[k], ((extra[k] * full) & 0xffffu) >> 8), diffuse & 0xffffu);
         }
-        const uint32_t height = h * f->heightProj[i] >> 8;
-        if (height > lastDrawn) {
-            span16(dst + lastDrawn, 1, height - lastHeight, height - lastDrawn, lastColor, color);
-            lastDrawn = height;
-        }
-        lastHeight = height;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) lastColor[k] = color[k];
+        const unsigned last = (L - i0 - 1) < (kWave - 1) ? (L - i0 - 1) : (kWave - 1);
+        chunk_spans(dst, 1, lane, last, active, height, color, c);
     }
     if constexpr (kBeams) {
-        // beam extrusion (code/ball.cpp:176-204)
+        // beam extrusion (code/ball.cpp:176-204): packus of the 16-bit sums
         uint32_t beamCol = 0;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const int v = (int)(int16_t)(uint16_t)beamAccum[k];
-            beamCol |= (uint32_t)(v < 0 ? 0 : (v > 255 ? 255 : v)) << (8 * k);
+            const uint32_t v = beamSum[k] < 65535u ? beamSum[k] : 65535u;
+            beamCol |= (v > 32767u ? 0u : (v > 255u ? 255u : v)) << (8 * k);
         }
+        const uint32_t lastDrawn = c.drawn;
         const uint32_t remainder = (W - 1) - lastDrawn;
         const uint32_t bR = beamCol >> 16, bG = (beamCol >> 8) & 0xff, bB = beamCol & 0xff;
-        // unsigned(0.0722f*65536.f) etc.
+        // unsigned(0.0722f*65536.f), unsigned(0.7152f*65536.f), unsigned(0.2126f*65536.f)
         const uint32_t lum = ((bR * 4731u) >> 16) + ((bG * 46871u) >> 16) + ((bB * 13932u) >> 16);
         const float fLum = (float)lum;
         const float alphaStep = 1.f / (float)(remainder - 1);
         const float amin = f->beamAlphaMin;
+        // the alpha ramp accumulates curStep += alphaStep pixel by pixel; every
+        // lane runs the same chain and keeps the value of its own pixel
         float curStep = 0.f;
-        for (uint32_t k = 0; k < remainder; ++k) {
-            float s = curStep;
-            s = s * s * (3.f - 2.f * s);
-            const float a = amin + (fLum - amin) * s;
-            dst[lastDrawn++] = beamCol | ((uint32_t)a << 24);
-            curStep += alphaStep;
+        for (uint32_t k0 = 0; k0 < remainder; k0 += kWave) {
+            float mine = 0.f;
+            for (unsigned j = 0; j < kWave; ++j) {
+                mine = (j == lane) ? curStep : mine;
+                curStep += alphaStep;
+            }
+            const uint32_t k = k0 + lane;
+            if (k < remainder) {
+                const float q = mine * mine * (3.f - 2.f * mine);
+                const float a = amin + (fLum - amin) * q;
+                dst[lastDrawn + k] = beamCol | ((uint32_t)a << 24);
+            }
         }
     }
 }
@@ -424,7 +517,7 @@ ckd_status ckd_twister_draw(ckd_ctx *ctx, uint32_t *dst, uint32_t *target, unsig
         if ((s = ckd::stage(ctx, rows.data(), sizeof(TwFrame) * H, &dRows)) != CKD_OK) return s;
         // Twister_Draw (code/torus-twister.cpp:164-187)
         if ((s = ckd::launch_memset32(ctx, target, 0, px)) != CKD_OK) return s;
-        k_twister<<<ckd_div_up((size_t)2 * H, 256), 256, 0, ctx->stream>>>(
+        k_twister<<<ckd_div_up((size_t)2 * H, 256 / kWave), 256, 0, ctx->stream>>>(
             target, W, H, static_cast<const TwFrame *>(dRows), ctx->twTables, ctx->twHeight,
             ctx->twColor);
         if ((s = ckd::check_launch("twister")) != CKD_OK) return s;
@@ -539,10 +632,10 @@ ckd_status ckd_ball_draw(ckd_ctx *ctx, uint32_t *dst, uint32_t *target, unsigned
         const int2 *ddXY = reinterpret_cast<const int2 *>(static_cast<uint8_t *>(dHost) + sizeof(BallFrame));
         if (!beams && (s = ckd::launch_memset32(ctx, target, 0, px)) != CKD_OK) return s;
         if (beams)
-            k_ball<true><<<ckd_div_up(H, 256), 256, 0, ctx->stream>>>(target, W, H, df, ddXY, hmapMix,
+            k_ball<true><<<ckd_div_up(H, 256 / kWave), 256, 0, ctx->stream>>>(target, W, H, df, ddXY, hmapMix,
                                                                        cmaps, beamMix);
         else
-            k_ball<false><<<ckd_div_up(H, 256), 256, 0, ctx->stream>>>(target, W, H, df, ddXY, hmapMix,
+            k_ball<false><<<ckd_div_up(H, 256 / kWave), 256, 0, ctx->stream>>>(target, W, H, df, ddXY, hmapMix,
                                                                         cmaps + kMapPx, env);
         if ((s = ckd::check_launch("ball")) != CKD_OK) return s;
         if (p.blur != 0.f &&

@@ -15,7 +15,9 @@
 //
 // In place (dst == src) the reference's subtract reads pixels it has
 // already overwritten, which makes every line a sequential recurrence with
-// u16 saturation; those kernels run one lane per line and replay it.
+// u16 saturation; those kernels run one lane per line and channel and
+// replay it, on lines staged in LDS (k_blur07_lines) or, for lines too long
+// for LDS, reading the image directly (k_blur07_inplace).
 #include "ckd_internal.h"
 #include "ckd_pixel.h"
 
@@ -172,16 +174,25 @@ __global__ __launch_bounds__(256) void k_blur07_v(uint32_t *__restrict__ dst,
 // 81-125 / 165-219 with pSrc == pDest).  The four channels are independent
 // recurrences (u16 saturation per channel), so each line takes four lanes,
 // one per channel.  The outputs the subtract needs again `median` steps
-// later live in an LDS ring of `ring` bytes per lane (ring > median, power
-// of two), laid out [entry][lane] so a wave's accesses are contiguous.  The
-// input pixels are always ahead of the outputs, so they are read from
-// global memory; the loops are unrolled so those loads issue ahead of the
-// dependent chain.
+// later live in an LDS ring of `ring` bytes per lane (ring > median, a
+// power of two of at least 4), laid out [entry/4][lane][entry%4] so four
+// consecutive entries of a lane are one dword.  The input pixels are always
+// ahead of the outputs, so they are read from global memory, a chunk of
+// eight ahead.  In the middle run a chunk of eight steps also reads its
+// eight ring entries up front: they were written at least median - 7 >= 1
+// steps before the chunk starts (median >= 8), so the LDS latency is paid
+// once per chunk instead of once per step.
+//
+// Per step the reference adds (p - p>>shift) + the previous step's p>>shift
+// with two saturating u16 adds, and subtracts the same split of the
+// outgoing pixel with two adds clamped at 0; both terms are >= 0, so each
+// pair equals one saturating add / clamped subtract of their sum.
 CKD_DEV uint32_t sat16(uint32_t s) { return s > 65535u ? 65535u : s; }
 
 CKD_DEV uint32_t div_byte(uint32_t acc, uint32_t div)
 {
-    const uint32_t q = (acc * div) >> 16;
+    // acc and div are below 2^16: the 24-bit multiply is exact
+    const uint32_t q = __umul24(acc, div) >> 16;
     return q > 32767u ? 0u : (q > 255u ? 255u : q);
 }
 
@@ -193,8 +204,11 @@ __global__ __launch_bounds__(256) void k_blur07_inplace(uint32_t *img, int nLine
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int line = t >> 2, k = t & 3;
     if (line >= nLines) return;
-    uint8_t *r = ringBuf + threadIdx.x;
+    uint8_t *lane = ringBuf + threadIdx.x * 4;
     const int rmask = ring - 1;
+    auto slot = [&](int entry) -> uint8_t & {
+        return lane[(size_t)((entry & rmask) >> 2) * 1024 + (entry & 3)];
+    };
     // the input bytes read are always ahead of the output bytes written (a
     // pixel is read as input before its output is stored), so the two
     // views never touch a byte in conflicting order: restrict lets the
@@ -208,65 +222,274 @@ __global__ __launch_bounds__(256) void k_blur07_inplace(uint32_t *img, int nLine
     const int full = len - (m + e);
     int addPos = 0, dest = 0, subPos = 0;
     auto add = [&](uint32_t v) {
-        acc = sat16(acc + addRem);
-        addRem = v >> shift;
-        acc = sat16(acc + (v - addRem));
+        const uint32_t rem = v >> shift;
+        const uint32_t a = addRem + (v - rem);
+        addRem = rem;
+        acc = sat16(acc + a);
     };
     auto sub = [&](uint32_t v) {
-        acc = acc > subRem ? acc - subRem : 0;
-        subRem = v >> shift;
-        const uint32_t d = v - subRem;
-        acc = acc > d ? acc - d : 0;
+        const uint32_t rem = v >> shift;
+        const uint32_t s = subRem + (v - rem);
+        subRem = rem;
+        acc = acc > s ? acc - s : 0u;
     };
+    const uint8_t *inp = in;  // in + addPos * step
+    uint8_t *outp = out;      // out + dest * step
     auto emit = [&](uint32_t o) {
-        r[(size_t)(dest & rmask) * 256] = (uint8_t)o;
-        out[(size_t)dest * step] = (uint8_t)o;
+        slot(dest) = (uint8_t)o;
+        *outp = (uint8_t)o;
+        outp += step;
         ++dest;
     };
+    auto next_in = [&]() {
+        const uint32_t v = *inp;
+        inp += step;
+        ++addPos;
+        return v;
+    };
 #pragma unroll 8
-    for (int i = 0; i < e; ++i) add(in[(size_t)(addPos++) * step]);
+    for (int i = 0; i < e; ++i) add(next_in());
 #pragma unroll 8
     for (int i = 0; i < m; ++i) {
-        add(in[(size_t)(addPos++) * step]);
+        add(next_in());
         emit(div_byte(acc, p.edgeDivs[i]));
     }
     // the long middle run: eight inputs at a time, the next eight loaded
     // before this eight's outputs are stored (program order keeps the loads
     // ahead of the possibly-aliasing stores)
     constexpr int kChunk = 8;
-    const int chunks = full / kChunk;
+    const int chunks = m >= kChunk ? full / kChunk : 0;
     uint32_t nxt[kChunk];
     if (chunks > 0) {
 #pragma unroll
-        for (int j = 0; j < kChunk; ++j) nxt[j] = in[(size_t)(addPos + j) * step];
+        for (int j = 0; j < kChunk; ++j) nxt[j] = inp[(size_t)j * step];
     }
     for (int c = 0; c < chunks; ++c) {
         uint32_t cur[kChunk];
 #pragma unroll
         for (int j = 0; j < kChunk; ++j) cur[j] = nxt[j];
         addPos += kChunk;
+        inp += kChunk * step;
         if (c + 1 < chunks) {
 #pragma unroll
-            for (int j = 0; j < kChunk; ++j) nxt[j] = in[(size_t)(addPos + j) * step];
+            for (int j = 0; j < kChunk; ++j) nxt[j] = inp[(size_t)j * step];
         }
+        // subPos is a multiple of 8 here (the middle run starts at 0)
+        const uint32_t w0 = *reinterpret_cast<const uint32_t *>(&slot(subPos));
+        const uint32_t w1 = *reinterpret_cast<const uint32_t *>(&slot(subPos + 4));
+        subPos += kChunk;
 #pragma unroll
         for (int j = 0; j < kChunk; ++j) {
             add(cur[j]);
-            sub(r[(size_t)((subPos++) & rmask) * 256]);
+            sub(((j < 4 ? w0 : w1) >> (8 * (j & 3))) & 0xffu);
             emit(div_byte(acc, p.fullDiv));
         }
     }
     for (int i = chunks * kChunk; i < full; ++i) {
-        add(in[(size_t)(addPos++) * step]);
-        sub(r[(size_t)((subPos++) & rmask) * 256]);
+        add(next_in());
+        sub(slot(subPos++));
         emit(div_byte(acc, p.fullDiv));
     }
     if (p.subEdges) acc = sat16(acc + addRem);
 #pragma unroll 8
     for (int i = e; i > 0; --i) {
-        sub(r[(size_t)((subPos++) & rmask) * 256]);
+        sub(slot(subPos++));
         emit(div_byte(acc, p.edgeDivs[i - 1]));
     }
+}
+
+// in place with the lines staged in LDS: a one-wave block copies NL lines
+// into LDS (one dword per pixel, each line padded by a dword so the NL
+// lines fall in different banks), runs the recurrence there -- lane
+// (line, channel) reads and overwrites byte `channel` of its line's pixels,
+// exactly as the reference's loop reads and overwrites the image -- and
+// writes the lines back.  No global access is on the recurrence's path.
+// Every phase takes K steps at a time with their LDS reads issued first:
+// inputs are always read before their position is overwritten, and the
+// outgoing pixel of step j was written at step j - median, so K <= median
+// keeps a chunk's reads clear of its own writes.
+// One channel of one line.  Odd spans have shift 8, so the split of each
+// pixel into p - p>>shift and p>>shift is (p, 0): a step is one saturating
+// u16 add and one u16 subtract clamped at 0.  Even spans carry the
+// half-pixel remainders between steps.
+template <bool kEven> struct B07Lane {
+    uint8_t *px; // byte `channel` of pixel 0; pixels are 4 bytes apart
+    int addPos, subPos, dest;
+    uint32_t acc, addRem, subRem;
+    CKD_DEV void add(uint32_t v)
+    {
+        uint32_t a = v;
+        if constexpr (kEven) {
+            const uint32_t rem = v >> 1;
+            a = addRem + (v - rem);
+            addRem = rem;
+        }
+        acc = __builtin_elementwise_add_sat((uint16_t)acc, (uint16_t)a);
+    }
+    CKD_DEV void sub(uint32_t v)
+    {
+        uint32_t d = v;
+        if constexpr (kEven) {
+            const uint32_t rem = v >> 1;
+            d = subRem + (v - rem);
+            subRem = rem;
+        }
+        acc = __builtin_elementwise_sub_sat((uint16_t)acc, (uint16_t)d);
+    }
+    // Div + packus_epi16 (code/deprecated/boxblur.cpp:39-42)
+    CKD_DEV uint8_t out(uint32_t div) const
+    {
+        const int16_t q = (int16_t)(__umul24(acc, div) >> 16);
+        return (uint8_t)__builtin_elementwise_min(__builtin_elementwise_max(q, (int16_t)0),
+                                                  (int16_t)255);
+    }
+};
+
+template <int K, bool kAdd, bool kSub, bool kEmit, bool kEven, typename Div>
+CKD_DEV void b07_run(B07Lane<kEven> &s, int n, Div div)
+{
+    int i = 0;
+    for (; i + K <= n; i += K) {
+        uint32_t vin[K], vsub[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            if constexpr (kAdd) vin[j] = s.px[(s.addPos + j) * 4];
+            if constexpr (kSub) vsub[j] = s.px[(s.subPos + j) * 4];
+        }
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            if constexpr (kAdd) s.add(vin[j]);
+            if constexpr (kSub) s.sub(vsub[j]);
+            if constexpr (kEmit) s.px[(s.dest + j) * 4] = s.out(div(i + j));
+        }
+        if constexpr (kAdd) s.addPos += K;
+        if constexpr (kSub) s.subPos += K;
+        if constexpr (kEmit) s.dest += K;
+    }
+    for (; i < n; ++i) {
+        if constexpr (kAdd) s.add(s.px[(s.addPos++) * 4]);
+        if constexpr (kSub) s.sub(s.px[(s.subPos++) * 4]);
+        if constexpr (kEmit) s.px[(s.dest++) * 4] = s.out(div(i));
+    }
+}
+
+// the phases that subtract, with the largest chunk the median allows
+template <bool kAdd, bool kEven, typename Div>
+CKD_DEV void b07_run_sub(B07Lane<kEven> &s, int n, int median, Div div)
+{
+    if (median >= 8) b07_run<8, kAdd, true, true>(s, n, div);
+    else if (median >= 4) b07_run<4, kAdd, true, true>(s, n, div);
+    else if (median >= 2) b07_run<2, kAdd, true, true>(s, n, div);
+    else b07_run<1, kAdd, true, true>(s, n, div);
+}
+
+// the whole line (code/deprecated/boxblur.cpp:81-125)
+template <bool kEven>
+CKD_DEV void b07_lds_line(uint8_t *px, int len, const B07Params &p, const uint16_t *edgeDivs)
+{
+    B07Lane<kEven> s;
+    s.px = px;
+    s.addPos = s.subPos = s.dest = 0;
+    s.acc = s.addRem = s.subRem = 0;
+    const int e = (int)p.edgeSpan, m = (int)p.median;
+    const int full = len - (m + e);
+    const uint32_t fullDiv = p.fullDiv;
+    b07_run<8, true, false, false>(s, e, [](int) { return 0u; });
+    b07_run<8, true, false, true>(s, m, [&](int i) { return (uint32_t)edgeDivs[i]; });
+    b07_run_sub<true>(s, full, m, [&](int) { return fullDiv; });
+    if constexpr (kEven) s.acc = __builtin_elementwise_add_sat((uint16_t)s.acc, (uint16_t)s.addRem);
+    b07_run_sub<false>(s, e, m, [&](int i) { return (uint32_t)edgeDivs[e - 1 - i]; });
+}
+
+// lines of `len` pixels: line l, pixel x at img[l*lineStride + x*pixStride]
+constexpr int kLdsBatch = 16;
+
+template <bool kStore>
+CKD_DEV void b07_stage(uint32_t *img, uint32_t *lds, int line0, int nl, int len, int stride,
+                       size_t lineStride, size_t pixStride, bool vec4)
+{
+    const int tid = threadIdx.x;
+    if (pixStride == 1 && vec4) {
+        // rows, 16-byte accesses along each row
+        const int perLine = len >> 2, total = nl * perLine;
+        for (int base = 0; base < total; base += 64 * kLdsBatch) {
+            uint4 r[kLdsBatch];
+#pragma unroll
+            for (int b = 0; b < kLdsBatch; ++b) {
+                const int idx = base + b * 64 + tid;
+                if (idx < total) {
+                    const int l = idx / perLine, q = idx - l * perLine;
+                    uint4 *g = reinterpret_cast<uint4 *>(img + (size_t)(line0 + l) * lineStride) + q;
+                    uint32_t *d = lds + l * stride + 4 * q;
+                    if constexpr (kStore) *g = make_uint4(d[0], d[1], d[2], d[3]);
+                    else r[b] = *g;
+                }
+            }
+            if constexpr (!kStore) {
+#pragma unroll
+                for (int b = 0; b < kLdsBatch; ++b) {
+                    const int idx = base + b * 64 + tid;
+                    if (idx < total) {
+                        const int l = idx / perLine, q = idx - l * perLine;
+                        uint32_t *d = lds + l * stride + 4 * q;
+                        d[0] = r[b].x;
+                        d[1] = r[b].y;
+                        d[2] = r[b].z;
+                        d[3] = r[b].w;
+                    }
+                }
+            }
+        }
+        return;
+    }
+    // one pixel per access; lines fastest across lanes (coalesced for columns)
+    const int total = nl * len;
+    for (int base = 0; base < total; base += 64 * kLdsBatch) {
+        uint32_t r[kLdsBatch];
+#pragma unroll
+        for (int b = 0; b < kLdsBatch; ++b) {
+            const int idx = base + b * 64 + tid;
+            if (idx < total) {
+                const int x = idx / nl, l = idx - x * nl;
+                uint32_t *g = img + (size_t)(line0 + l) * lineStride + (size_t)x * pixStride;
+                if constexpr (kStore) *g = lds[l * stride + x];
+                else r[b] = *g;
+            }
+        }
+        if constexpr (!kStore) {
+#pragma unroll
+            for (int b = 0; b < kLdsBatch; ++b) {
+                const int idx = base + b * 64 + tid;
+                if (idx < total) {
+                    const int x = idx / nl, l = idx - x * nl;
+                    lds[l * stride + x] = r[b];
+                }
+            }
+        }
+    }
+}
+
+template <int NL>
+__global__ __launch_bounds__(64) void k_blur07_lines(uint32_t *img, int nLines, int len,
+                                                     size_t lineStride, size_t pixStride,
+                                                     int vec4, B07Params p)
+{
+    extern __shared__ uint32_t lineBuf[];
+    __shared__ uint16_t edgeDivs[256]; // read per step in the edge phases
+    const int stride = len + 1;
+    const int line0 = blockIdx.x * NL;
+    const int nl = min(NL, nLines - line0);
+    for (int i = threadIdx.x; i < 256; i += 64) edgeDivs[i] = p.edgeDivs[i];
+    b07_stage<false>(img, lineBuf, line0, nl, len, stride, lineStride, pixStride, vec4 != 0);
+    __syncthreads();
+    const int l = threadIdx.x >> 2, k = threadIdx.x & 3;
+    if (l < nl) {
+        uint8_t *px = reinterpret_cast<uint8_t *>(lineBuf + l * stride) + k;
+        if (p.subEdges) b07_lds_line<true>(px, len, p, edgeDivs);
+        else b07_lds_line<false>(px, len, p, edgeDivs);
+    }
+    __syncthreads();
+    b07_stage<true>(img, lineBuf, line0, nl, len, stride, lineStride, pixStride, vec4 != 0);
 }
 
 ckd_status validate(const char *fn, const uint32_t *dst, const uint32_t *src, unsigned xRes,
@@ -283,9 +506,10 @@ ckd_status validate(const char *fn, const uint32_t *dst, const uint32_t *src, un
     return CKD_OK;
 }
 
+// power of two > median, at least 4 (the ring holds whole dwords per lane)
 int ring_for(unsigned median)
 {
-    int r = 1;
+    int r = 4;
     while (r <= (int)median) r <<= 1;
     return r;
 }
@@ -294,16 +518,45 @@ int ring_for(unsigned median)
 
 namespace {
 
+// lines per block of the LDS-staged kernel: the most of 16/8/4/2/1 whose
+// padded lines fit in 64 KiB; 0 when a single line does not fit
+int lines_per_block(int len)
+{
+    for (int nl = 16; nl >= 1; nl >>= 1)
+        if ((size_t)nl * (size_t)(len + 1) * 4 <= 65536) return nl;
+    return 0;
+}
+
+ckd_status launch_inplace(ckd_ctx *ctx, uint32_t *img, int nLines, int len, size_t lineStride,
+                          size_t pixStride, const B07Params &p, const char *what)
+{
+    const int nl = lines_per_block(len);
+    if (nl == 0) {
+        // lines longer than 16383 pixels: the recurrence reads the image
+        const int ring = ring_for(p.median);
+        const size_t lds = (size_t)ring * 256;
+        k_blur07_inplace<<<ckd_div_up((size_t)nLines * 4, 256), 256, lds, ctx->stream>>>(
+            img, nLines, len, lineStride, pixStride, ring, p);
+        return ckd::check_launch(what);
+    }
+    const size_t lds = (size_t)nl * (size_t)(len + 1) * 4;
+    const int vec4 = pixStride == 1 && (len % 4) == 0 && (lineStride % 4) == 0 &&
+                     ((uintptr_t)img % 16) == 0;
+    const unsigned grid = ckd_div_up((size_t)nLines, (size_t)nl);
+    switch (nl) {
+    case 16: k_blur07_lines<16><<<grid, 64, lds, ctx->stream>>>(img, nLines, len, lineStride, pixStride, vec4, p); break;
+    case 8: k_blur07_lines<8><<<grid, 64, lds, ctx->stream>>>(img, nLines, len, lineStride, pixStride, vec4, p); break;
+    case 4: k_blur07_lines<4><<<grid, 64, lds, ctx->stream>>>(img, nLines, len, lineStride, pixStride, vec4, p); break;
+    case 2: k_blur07_lines<2><<<grid, 64, lds, ctx->stream>>>(img, nLines, len, lineStride, pixStride, vec4, p); break;
+    default: k_blur07_lines<1><<<grid, 64, lds, ctx->stream>>>(img, nLines, len, lineStride, pixStride, vec4, p); break;
+    }
+    return ckd::check_launch(what);
+}
+
 ckd_status launch_h(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, unsigned xRes,
                     unsigned yRes, const B07Params &p)
 {
-    if (dst == src) {
-        const int ring = ring_for(p.median);
-        const size_t lds = (size_t)ring * 256;
-        k_blur07_inplace<<<ckd_div_up((size_t)yRes * 4, 256), 256, lds, ctx->stream>>>(
-            dst, (int)yRes, (int)xRes, xRes, 1, ring, p);
-        return ckd::check_launch("blur07_h (in place)");
-    }
+    if (dst == src) return launch_inplace(ctx, dst, (int)yRes, (int)xRes, xRes, 1, p, "blur07_h (in place)");
     if (xRes > (unsigned)kHMaxW) return ckd::fail(CKD_EINVAL, "blur07_h: xRes %u > %d", xRes, kHMaxW);
     const size_t lds = (size_t)(xRes + 1) * sizeof(uint64_t) + (size_t)xRes * sizeof(uint32_t);
     k_blur07_h<<<yRes, kHBlock, lds, ctx->stream>>>(dst, src, (int)xRes, p);
@@ -313,13 +566,7 @@ ckd_status launch_h(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, unsigned x
 ckd_status launch_v(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, unsigned xRes,
                     unsigned yRes, const B07Params &p)
 {
-    if (dst == src) {
-        const int ring = ring_for(p.median);
-        const size_t lds = (size_t)ring * 256;
-        k_blur07_inplace<<<ckd_div_up((size_t)xRes * 4, 256), 256, lds, ctx->stream>>>(
-            dst, (int)xRes, (int)yRes, 1, xRes, ring, p);
-        return ckd::check_launch("blur07_v (in place)");
-    }
+    if (dst == src) return launch_inplace(ctx, dst, (int)xRes, (int)yRes, 1, xRes, p, "blur07_v (in place)");
     dim3 grid(ckd_div_up(xRes, 64), ckd_div_up(yRes, kVSeg));
     k_blur07_v<<<grid, 64, 0, ctx->stream>>>(dst, src, (int)xRes, (int)yRes, p);
     return ckd::check_launch("blur07_v");
