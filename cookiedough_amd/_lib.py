@@ -110,6 +110,8 @@ TwisterParams = _params("TwisterParams", [
 # effect name -> parameter struct; "spikey" takes a close flag
 ST_PARAMS = {"plasma": PlasmaParams, "nautilus": NautilusParams, "spikey": SpikeyParams,
              "sinuses": SinusesParams, "laura": LauraParams, "fdtunnel": FdTunnelParams}
+# every effect whose parameters are Rocket tracks (ckd_effect_tracks)
+TRACK_PARAMS = dict(ST_PARAMS, ball=BallParams, twister=TwisterParams)
 
 class Request(C.Structure):
     _fields_ = [("effect", C.c_int), ("W", C.c_uint), ("H", C.c_uint),
@@ -188,6 +190,13 @@ SIGNATURES = {
     "ckd_fade32": [_p, _p, _u, C.c_uint32, C.c_uint8],
     "ckd_zoom32": [_p, _p, _p, _u, _u, _f],
     "ckd_polar_blit_a": [_p, _p, _p, _u, _u, _i],
+    "ckd_sync_open": [C.c_char_p, C.POINTER(_p)],
+    "ckd_sync_close": [_p],
+    "ckd_sync_get": [_p, C.c_int, C.c_double],
+    "ckd_sync_row": [C.c_double],
+    "ckd_sync_track": [_p, C.c_char_p, C.POINTER(C.c_int)],
+    "ckd_sync_params": [_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_float), _u, _p],
+    "ckd_effect_tracks": [C.c_char_p, C.POINTER(C.POINTER(C.c_char_p)), C.POINTER(C.c_uint)],
     "ckd_twister_create": [_p, _p, _p, _p, _u, _u],
     "ckd_twister_draw": [_p, _p, _p, _u, _u, C.POINTER(TwisterParams), _u],
     "ckd_ball_create": [_p, _p, _p, _p, _p, _p, _p, _u, _u],
@@ -207,7 +216,8 @@ SIGNATURES = {
     "ckd_laura_map": [_p, _p, _u, _u, C.POINTER(LauraParams), _u],
     "ckd_laura_draw": [_p, _p, _u, _u, C.POINTER(LauraParams), _u],
 }
-_RESTYPES = {"ckd_destroy": None, "ckd_last_error": C.c_char_p, "ckd_graph_destroy": None}
+_RESTYPES = {"ckd_destroy": None, "ckd_last_error": C.c_char_p, "ckd_graph_destroy": None,
+             "ckd_sync_close": None, "ckd_sync_get": C.c_double, "ckd_sync_row": C.c_double}
 
 _lib = None
 

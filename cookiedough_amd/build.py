@@ -21,7 +21,7 @@ LIB = PKG / "libckd.so"
 
 SOURCES = ["runtime.hip", "k_blit.hip", "k_blur07.hip", "k_blur26.hip", "k_remap.hip", "k_tunnel.hip",
            "k_voxel.hip", "k_sdf.hip", "k_batch.hip", "k_shadertoy.hip",
-           "k_composite.hip", "k_ball.hip"]
+           "k_composite.hip", "k_ball.hip", "sync_tracks.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -116,41 +116,24 @@ float clampf(float lo, float hi, float v) { return std::fmax(lo, std::fmin(hi, v
 bool s_tsReady = false;
 SyncTrack trackStarsStepU, trackStarsStepV, trackStarsSpeed, trackStarsBlur;
 
-// ---- raymarched effects (code/shadertoy.cpp:25-172) ----------------------
-// Each effect's tracks in the order of its ckd_*_params fields after
-// `time`; a Draw call reads them into the struct.
-const char *const kPlasmaTracks[] = {"plasma:Speed", "plasma:Hue", "plasma:Gamma",
-                                     "plasma:Desaturation"};
-const char *const kNautilusTracks[] = {"nautilus:Roll", "nautilus:Hue", "nautilus:Speed",
-                                       "nautilus:Desat", "nautilus:Blur"};
-const char *const kSpikeyTracks[] = {
-    "spike:Speed", "spike:Roll", "spike:Specular", "spike:Desaturation", "spike:Hue",
-    "spike:Gamma", "distSpike:xOffs", "distSpike:yOffs", "distSpike:zOffs", "distSpike:Warmup",
-    "closeSpike:xOffs", "closeSpike:yOffs", "closeSpike:zOffs", "closeSpike:zOffsScale",
-    "closeSpike:NormalGrain", "closeSpike:Scale", "closeSpike:Rim", "closeSpike:AspectMul",
-    "closeSpike:MixBlurMap", "closeSpike:MixBlur", "closeSpike:MixMapBlur",
-    "closeSpike:MixBlurOpacity"};
-const char *const kSinusesTracks[] = {"sinusesTunnel:Specular", "sinusesTunnel:Roll",
-                                      "sinusesTunnel:Speed", "sinusesTunnel:OffsX",
-                                      "sinusesTunnel:Gamma", "sinusesTunnel:Hue",
-                                      "sinusesTunnel:Desaturation"};
-const char *const kLauraTracks[] = {"laura:Speed", "laura:Yaw", "laura:Pitch",
-                                    "laura:Roll", "laura:Hue", "laura:Saturate"};
-const char *const kTunnelTracks[] = {
-    "tunnel:Boxy", "tunnel:FlowerScale", "tunnel:FlowerFreq", "tunnel:FlowerPhase",
-    "tunnel:Speed", "tunnel:Roll", "tunnel:Pitch", "tunnel:Radius", "tunnel:MulU",
-    "tunnel:MulV", "tunnel:LitTiles", "tunnel:LitBlur", "tunnel:Fog1", "tunnel:Fog2"};
-
-template <size_t N> struct StTracks {
+// ---- raymarched effects, ball, twister ------------------------------------
+// Each effect's Rocket tracks, in the order of its ckd_*_params fields after
+// `time`, come from the library (ckd_effect_tracks); a Draw call reads them
+// into the struct.
+template <typename P> struct FxTracks {
+    static constexpr size_t N = sizeof(P) / sizeof(float) - 1;
     SyncTrack t[N] = {};
-    void add(const char *const (&names)[N])
+    bool add(const char *effect)
     {
+        const char *const *names = nullptr;
+        unsigned n = 0;
+        if (ckd_effect_tracks(effect, &names, &n) != CKD_OK || n != N) return false;
         for (size_t i = 0; i < N; ++i) t[i] = Rocket::AddTrack(names[i]);
+        return true;
     }
-    // time, then the track values, into a ckd_*_params struct of floats
-    template <typename P> P read(float time) const
+    // time, then the track values (Rocket::getf), into the struct of floats
+    P read(float time) const
     {
-        static_assert(sizeof(P) == (N + 1) * sizeof(float), "parameter struct layout");
         float v[N + 1];
         v[0] = time;
         for (size_t i = 0; i < N; ++i) v[i + 1] = (float)Rocket::get(t[i]);
@@ -161,26 +144,19 @@ template <size_t N> struct StTracks {
 };
 
 bool s_stReady = false;
-StTracks<4> s_plasmaTracks;
-StTracks<5> s_nautilusTracks;
-StTracks<22> s_spikeyTracks;
-StTracks<7> s_sinusesTracks;
-StTracks<6> s_lauraTracks;
-StTracks<14> s_tunnelTracks;
+FxTracks<ckd_plasma_params> s_plasmaTracks;
+FxTracks<ckd_nautilus_params> s_nautilusTracks;
+FxTracks<ckd_spikey_params> s_spikeyTracks;
+FxTracks<ckd_sinuses_params> s_sinusesTracks;
+FxTracks<ckd_laura_params> s_lauraTracks;
+FxTracks<ckd_fdtunnel_params> s_tunnelTracks;
 uint32_t *s_spikeBlurMaps[2] = {nullptr, nullptr}; // the close-up's mix-blur layer
 
 // ---- voxel ball and torus twister (code/ball.cpp:337-441,
 // code/torus-twister.cpp:136-158) ------------------------------------------
-const char *const kBallTracks[] = {
-    "ball:Blur", "ball:Radius", "ball:RayLength", "ball:Spikes", "ball:HasBeams",
-    "ball:BaseShapeIndex", "ball:Speed", "ball:BeamAttenuation", "ball:BeamAlphaMin",
-    "ball:RotateOffsX", "ball:RotateOffsY", "ball:Beams1", "ball:Beams2", "ball:Beams3",
-    "ball:BallLowBeams"};
-const char *const kTwisterTracks[] = {"twister:Speed", "twister::ShearSpeed", "twister:Blur"};
-
 bool s_ballReady = false, s_twReady = false;
-StTracks<15> s_ballTracks;
-StTracks<3> s_twisterTracks;
+FxTracks<ckd_ball_params> s_ballTracks;
+FxTracks<ckd_twister_params> s_twisterTracks;
 uint32_t *s_ballBackground = nullptr; // Ball_GetBackground
 // the render target the ball and the twister draw into (the reference's
 // g_renderTarget[0]), cleared when allocated; the beam rays leave its last
@@ -347,12 +323,13 @@ void Tunnelscape_Draw(uint32_t *pDest, float time, float delta)
 bool Shadertoy_Create()
 {
     if (!acquire()) return false;
-    s_plasmaTracks.add(kPlasmaTracks);
-    s_nautilusTracks.add(kNautilusTracks);
-    s_spikeyTracks.add(kSpikeyTracks);
-    s_sinusesTracks.add(kSinusesTracks);
-    s_lauraTracks.add(kLauraTracks);
-    s_tunnelTracks.add(kTunnelTracks);
+    if (!s_plasmaTracks.add("plasma") || !s_nautilusTracks.add("nautilus") ||
+        !s_spikeyTracks.add("spikey") || !s_sinusesTracks.add("sinuses") ||
+        !s_lauraTracks.add("laura") || !s_tunnelTracks.add("fdtunnel")) {
+        report("Shadertoy_Create (tracks)");
+        release();
+        return false;
+    }
     // the free-directional tunnel's 1024x1024 textures and the close-up's
     // blur maps (code/shadertoy.cpp:174-185)
     const uint32_t *tex = Image_Load32("assets/shadertoy/nytrik-hextexture.png");
@@ -379,7 +356,7 @@ void Shadertoy_Destroy()
 void Plasma_Draw(uint32_t *pDest, float time, float delta)
 {
     (void)delta;
-    const ckd_plasma_params p = s_plasmaTracks.read<ckd_plasma_params>(time);
+    const ckd_plasma_params p = s_plasmaTracks.read(time);
     st_draw("Plasma_Draw", pDest,
             [&](uint32_t *d) { return ckd_plasma_draw(g_ctx, d, kResX, kResY, &p, 1); });
 }
@@ -387,7 +364,7 @@ void Plasma_Draw(uint32_t *pDest, float time, float delta)
 void Nautilus_Draw(uint32_t *pDest, float time, float delta)
 {
     (void)delta;
-    const ckd_nautilus_params p = s_nautilusTracks.read<ckd_nautilus_params>(time);
+    const ckd_nautilus_params p = s_nautilusTracks.read(time);
     st_draw("Nautilus_Draw", pDest,
             [&](uint32_t *d) { return ckd_nautilus_draw(g_ctx, d, kResX, kResY, &p, 1); });
 }
@@ -395,7 +372,7 @@ void Nautilus_Draw(uint32_t *pDest, float time, float delta)
 void Spikey_Draw(uint32_t *pDest, float time, float delta, bool close)
 {
     (void)delta;
-    const ckd_spikey_params p = s_spikeyTracks.read<ckd_spikey_params>(time);
+    const ckd_spikey_params p = s_spikeyTracks.read(time);
     st_draw("Spikey_Draw", pDest, [&](uint32_t *d) {
         return ckd_spikey_draw(g_ctx, d, kResX, kResY, &p, 1, close ? 1 : 0);
     });
@@ -404,7 +381,7 @@ void Spikey_Draw(uint32_t *pDest, float time, float delta, bool close)
 void Sinuses_Draw(uint32_t *pDest, float time, float delta)
 {
     (void)delta;
-    const ckd_sinuses_params p = s_sinusesTracks.read<ckd_sinuses_params>(time);
+    const ckd_sinuses_params p = s_sinusesTracks.read(time);
     st_draw("Sinuses_Draw", pDest,
             [&](uint32_t *d) { return ckd_sinuses_draw(g_ctx, d, kResX, kResY, &p, 1); });
 }
@@ -412,7 +389,7 @@ void Sinuses_Draw(uint32_t *pDest, float time, float delta)
 void Laura_Draw(uint32_t *pDest, float time, float delta)
 {
     (void)delta;
-    const ckd_laura_params p = s_lauraTracks.read<ckd_laura_params>(time);
+    const ckd_laura_params p = s_lauraTracks.read(time);
     st_draw("Laura_Draw", pDest,
             [&](uint32_t *d) { return ckd_laura_draw(g_ctx, d, kResX, kResY, &p, 1); });
 }
@@ -420,7 +397,7 @@ void Laura_Draw(uint32_t *pDest, float time, float delta)
 void Tunnel_Draw(uint32_t *pDest, float time, float delta)
 {
     (void)delta;
-    const ckd_fdtunnel_params p = s_tunnelTracks.read<ckd_fdtunnel_params>(time);
+    const ckd_fdtunnel_params p = s_tunnelTracks.read(time);
     st_draw("Tunnel_Draw", pDest,
             [&](uint32_t *d) { return ckd_fdtunnel_draw(g_ctx, d, kResX, kResY, &p, 1); });
 }
@@ -689,8 +666,12 @@ bool Ball_Create()
         release();
         return false;
     }
+    if (!s_ballTracks.add("ball")) {
+        report("Ball_Create (tracks)");
+        release();
+        return false;
+    }
     s_ballBackground = bg[0];
-    s_ballTracks.add(kBallTracks);
     s_ballReady = true;
     return true;
 }
@@ -705,7 +686,7 @@ void Ball_Draw(uint32_t *pDest, float time, float delta)
 {
     (void)delta;
     if (!s_ballReady) return;
-    const ckd_ball_params p = s_ballTracks.read<ckd_ball_params>(time);
+    const ckd_ball_params p = s_ballTracks.read(time);
     const size_t n = (size_t)kResX * kResY;
     uint32_t *a = g_a.get(n), *t = render_target(n);
     if (!a || !t || ckd_ball_draw(g_ctx, a, t, kResX, kResY, &p, 1) != CKD_OK || !down(pDest, a, n))
@@ -734,7 +715,11 @@ bool Twister_Create()
         release();
         return false;
     }
-    s_twisterTracks.add(kTwisterTracks);
+    if (!s_twisterTracks.add("twister")) {
+        report("Twister_Create (tracks)");
+        release();
+        return false;
+    }
     s_twReady = true;
     return true;
 }
@@ -749,7 +734,7 @@ void Twister_Draw(uint32_t *pDest, float time, float delta)
 {
     (void)delta;
     if (!s_twReady) return;
-    const ckd_twister_params p = s_twisterTracks.read<ckd_twister_params>(time);
+    const ckd_twister_params p = s_twisterTracks.read(time);
     const size_t n = (size_t)kResX * kResY;
     uint32_t *a = g_a.get(n), *t = render_target(n);
     if (!a || !t || ckd_twister_draw(g_ctx, a, t, kResX, kResY, &p, 1) != CKD_OK || !down(pDest, a, n))

@@ -564,6 +564,37 @@ CKD_API ckd_status ckd_ball_create(ckd_ctx *ctx, const uint8_t *heightMaps5,
 CKD_API ckd_status ckd_ball_draw(ckd_ctx *ctx, uint32_t *dst, uint32_t *target, unsigned W,
                                  unsigned H, const ckd_ball_params *frames, unsigned nFrames);
 
+/* ---- GNU Rocket track playback (host only) ------------------------------- */
+
+/* A demo's sync tracks, read as the Rocket player reads them
+ * (3rdparty/rocket-stripped/lib/device.c:41-75,309-331): track `name` is
+ * <dir>/sync/_<name with every char outside [A-Za-z0-9._/] as -XX>.track; a
+ * missing file is a track without keys.  Replaces sync_create_device +
+ * sync_get_track (code/rocket.cpp:26-46) for callers that drive the ABI's
+ * parameter structs from the production tracks. */
+typedef struct ckd_sync ckd_sync;
+CKD_API ckd_status ckd_sync_open(const char *dir, ckd_sync **out);
+CKD_API void ckd_sync_close(ckd_sync *sync);
+/* loads (once) and returns the index of track `name`; CKD_EINVAL for a file
+ * that is not a track file */
+CKD_API ckd_status ckd_sync_track(ckd_sync *sync, const char *name, int *index);
+/* sync_get_val (3rdparty/rocket-stripped/lib/track.c:29-60): the track's
+ * value at a fractional row, in double */
+CKD_API double ckd_sync_get(const ckd_sync *sync, int index, double row);
+/* Audio_Rocket_Sync (code/audio.cpp:18,159-179): the row at a time in
+ * seconds (170 BPM as 174, 16 rows a beat) */
+CKD_API double ckd_sync_row(double seconds);
+/* the Rocket track names of an effect's parameter struct, in field order
+ * after `time`: "plasma", "nautilus", "spikey", "sinuses", "laura",
+ * "fdtunnel", "ball", "twister" */
+CKD_API ckd_status ckd_effect_tracks(const char *effect, const char *const **names,
+                                     unsigned *count);
+/* fills nFrames parameter structs of `effect` (ckd_<effect>_params,
+ * contiguous): time = times[k], every other field (float) of its track at
+ * rows[k] (Rocket::getf, code/rocket.h:23-25) */
+CKD_API ckd_status ckd_sync_params(ckd_sync *sync, const char *effect, const double *rows,
+                                   const float *times, unsigned nFrames, void *params);
+
 #ifdef __cplusplus
 }
 #endif

@@ -102,6 +102,7 @@ _REF = {
     "ref_tscape_ray": (None, [_p, _p, _p, _p, _i, _i, _i, _i]),
     "ref_polar_maps": (None, [_p, _p, _u, _u, _u, _u]),
     "ref_polar_create": (_i, []),
+    "ref_sync_value": (C.c_double, [C.c_char_p, C.c_char_p, C.c_double]),
     "ref_polar_destroy": (None, []),
     "ref_polar_blit": (None, [_p, _p, _i]),
     "ref_blur07_h": (None, [_p, _p, _u, _u, _f]),
@@ -165,18 +166,29 @@ ST_IDS = {"plasma": 0, "nautilus": 1, "spikey": 2, "spikey_close": 3, "sinuses":
 ST_STRUCT = {"spikey_close": "spikey"}  # id name -> parameter struct name
 
 
+def track_file_name(name: str) -> str:
+    """_<name>.track with every char outside [A-Za-z0-9._/] as -XX
+    (3rdparty/rocket-stripped/lib/device.c:26-75)"""
+    return "_" + "".join(ch if (ch.isascii() and (ch.isalnum() or ch in "._/")) else
+                         "".join(f"-{b:02X}" for b in ch.encode()) for ch in name) + ".track"
+
+
+def write_track_keys(root: Path, name: str, keys) -> None:
+    """one track file with keys [(row, value, type)], rows ascending"""
+    import struct
+    d = Path(root) / "sync"
+    d.mkdir(parents=True, exist_ok=True)
+    blob = struct.pack("<i", len(keys)) + b"".join(struct.pack("<ifb", r, float(v), t)
+                                                   for r, v, t in keys)
+    (d / track_file_name(name)).write_bytes(blob)
+
+
 def write_rocket_tracks(root: Path, values: dict) -> None:
     """Writes GNU Rocket player track files (root/sync/_<name>.track, the
     layout 3rdparty/rocket-stripped/lib/device.c reads): for each track
     name, one step key per row 0..K-1 holding values[name][row]."""
-    import struct
-    d = Path(root) / "sync"
-    d.mkdir(parents=True, exist_ok=True)
     for name, vals in values.items():
-        enc = "".join(ch if (ch.isalnum() or ch in "._/") else f"-{ord(ch):02X}" for ch in name)
-        blob = struct.pack("<i", len(vals)) + b"".join(
-            struct.pack("<ifb", row, float(v), 0) for row, v in enumerate(vals))
-        (d / f"_{enc}.track").write_bytes(blob)
+        write_track_keys(root, name, [(row, v, 0) for row, v in enumerate(vals)])
 
 
 def _bind(path: Path, table: dict) -> C.CDLL:

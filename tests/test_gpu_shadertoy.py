@@ -199,3 +199,32 @@ def test_close_mix_blur_layer(ctx, orc, st):
         assert orc.orc_st_draw(oracles.ST_IDS["spikey_close"], P(od), W, H, P(a), P(lut), P(tex),
                                P(tex_hi), P(blur), 0, 16) == 0
         check_close(d.numpy(), od, f"mix-blur {mbMap} {mbBlur} {mbMapBlur} {opacity}")
+
+
+def test_draws_from_rocket_tracks(ctx, orc, st, tmp_path):
+    """the library's track player fills the effects' parameter structs from
+    interpolated keys; the Draw frames equal the oracle's on them"""
+    import oracles
+    from cookiedough_amd.sync import Sync
+    lut, tex, tex_hi, blur = st
+    rng = np.random.default_rng(13)
+    for fx in ["plasma", "laura", "sinuses"]:
+        cls = _lib.ST_PARAMS[fx]
+        for fld, tr in cls.TRACKS.items():
+            base = float(getattr(wl.st_frames(fx, 1)[0], fld))
+            keys = [(r, base * float(rng.uniform(0.8, 1.2)), ty) for r, ty in [(0, 1), (32, 2), (64, 3)]]
+            oracles.write_track_keys(tmp_path, tr, keys)
+    rows = [5.5, 40.125, 70.0]
+    with Sync(tmp_path) as s:
+        for fx in ["plasma", "laura", "sinuses"]:
+            frames = s.params(fx, rows, [r / 8.0 for r in rows])
+            d = ctx.empty(len(rows) * W * H)
+            ctx.st_draw(fx, d, W, H, frames)
+            ctx.sync()
+            got = d.numpy().reshape(len(rows), -1)
+            for k in range(len(rows)):
+                od = aligned_empty(W * H)
+                p = frames[k].as_array()
+                assert orc.orc_st_draw(oracles.ST_IDS[fx], P(od), W, H, P(p), P(lut), P(tex),
+                                       P(tex_hi), P(blur), 0, 16) == 0
+                check_close(got[k], od, f"{fx} tracks row {rows[k]}", exact=fx == "plasma")

@@ -225,3 +225,42 @@ def test_ball_rejects_rows_overflow(ctx, maps):
             c.ball_draw(d, t, 1280, 720, prm)
     finally:
         c.close()
+
+
+def test_frames_from_rocket_tracks(ctx, orc, maps, tmp_path):
+    """the library's track player drives the Draw calls: interpolated keys
+    (linear, smooth, ramp, step) read at fractional rows, ball and twister
+    drawn alternately into one render target as the demo does, bit-exact
+    against the oracle on the same parameters"""
+    import oracles
+    from cookiedough_amd.sync import Sync
+    bm, tm, (m0, _) = maps
+    rng = np.random.default_rng(12)
+    ranges = {"radius": (700.0, 1900.0), "rayLength": (300.0, 1024.0), "spikes": (0.0, 255.0),
+              "hasBeams": (0.0, 1.0), "baseShapeIndex": (1.0, 4.0), "speed": (-4.0, 4.0),
+              "beams1": (0.0, 1.0), "beams2": (0.0, 1.0), "beams3": (0.0, 1.0),
+              "shearSpeed": (-0.3, 0.3)}
+    for cls in (_lib.BallParams, _lib.TwisterParams):
+        for fld, tr in cls.TRACKS.items():
+            lo, hi = ranges.get(fld, (0.0, 60.0))
+            keys = [(r, float(rng.uniform(lo, hi)), ty) for r, ty in [(0, 1), (40, 2), (80, 3), (120, 0)]]
+            oracles.write_track_keys(tmp_path, tr, keys)
+    rows = [3.5, 44.25, 79.75, 150.0]
+    times = [Sync.row(r) / 46.4 for r in rows]
+    with Sync(tmp_path) as s:
+        bp, tp = s.params("ball", rows, times), s.params("twister", rows, times)
+    t = zeroed(ctx, W * H)
+    host_t = np.zeros(W * H, np.uint32)
+    for k in range(len(rows)):
+        print("tracks row", rows[k], flush=True)
+        d = ctx.empty(W * H)
+        ctx.ball_draw(d, t, W, H, bp[k])
+        ctx.sync()
+        assert np.array_equal(d.numpy(), orc_ball(orc, bm, m0, bp[k], W, H, host_t)), ("ball", k)
+        ctx.twister_draw(d, t, W, H, tp[k])
+        ctx.sync()
+        want, host_t = aligned_empty(W * H), aligned_empty(W * H)
+        assert orc.orc_twister_draw(P(want), P(host_t), W, H, P(tm["heightMap"]), P(tm["colorMap"]),
+                                    P(tm["background"]), P(m0), P(tp[k].as_array()), 16) == 0
+        assert np.array_equal(d.numpy(), want), ("twister", k)
+        assert np.array_equal(t.numpy(), host_t), ("target", k)
