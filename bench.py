@@ -18,6 +18,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import re
 import sys
 import time
 from pathlib import Path
@@ -73,6 +74,9 @@ def main() -> int:
                     help="steps,chunk,blocksPerCU,vSeg: fixed launch shapes instead of autotuning")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the batched headline (for kernel profiles)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-MiB device buffer to another, 4 times: the known-size "
+                         "traffic that scales the HBM byte counters of a profile")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -84,6 +88,17 @@ def main() -> int:
     from cookiedough_amd.device import Context
     ctx = Context(0)
     T = Timer(ctx)
+
+    if args.calibrate_copy:
+        n = args.calibrate_copy * (1 << 20) // 4
+        a, b = ctx.empty(n), ctx.empty(n)
+        ctx.memset32(a, 0x01020304, n)
+        for _ in range(4):
+            ctx.memcpy_d2d(b, a, 4 * n)
+        ctx.sync()
+        print(json.dumps({"copy_bytes": 4 * n, "copies": 4}))
+        ctx.close()
+        return 0
 
     # ---------------------------------------------------------------- inputs
     t_in = time.time()
@@ -157,9 +172,17 @@ def main() -> int:
     steps_k = int(pcount.numpy()[0]) - st_zero
     flops_k = sdf_flops(steps_k, psdf["px"], 8)
     achieved = flops_k / (ms_k * 1e-3) / 1e12
+    # HBM bytes per launch of this kernel on the 512-request batch, measured
+    # with the FETCH_SIZE / WRITE_SIZE counters scaled by a copy of known size
+    traffic = None
+    prof = Path(__file__).resolve().parent / "profiles" / "k_sdf_persistent_batch512_traffic.md"
+    if args.requests == 512 and prof.exists():
+        m = re.search(r"traffic_bytes_per_launch = (\d+)", prof.read_text())
+        traffic = int(m.group(1)) if m else None
     roofline = {"bound": "valu", "kernel": "k_sdf_persistent<fp16 shading, 8 spheres>",
                 "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
+                "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-scaled)",
                 "launches": launches, "avg_launch_ms": round(ms_k / max(1, launches), 4),
                 "algorithmic": "per launch: sum(steps)*(F_sdf+7) + px*(6*F_sdf+40), "
                                "F_sdf = 11*8+2 = 90 FLOP"}

@@ -104,6 +104,9 @@ class Context:
         return ms.value, n.value
 
     # ---- entry points --------------------------------------------------------
+    def memcpy_d2d(self, dst, src, nbytes):
+        check(self.lib.ckd_memcpy_d2d(self.h, ptr(dst), ptr(src), nbytes))
+
     def memset32(self, dst, value, n):
         check(self.lib.ckd_memset32(self.h, ptr(dst), value, n))
 
