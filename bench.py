@@ -74,6 +74,9 @@ def main() -> int:
                     help="steps,chunk,blocksPerCU,vSeg: fixed launch shapes instead of autotuning")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the batched headline (for kernel profiles)")
+    ap.add_argument("--sections", default="classic,c3,shadertoy,voxel2",
+                    help="per-effect sections to run: classic (SDF c0, voxel c1, tunnel c2), "
+                         "c3, shadertoy, voxel2 (ball and twister)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer to another, 4 times: the known-size "
                          "traffic that scales the HBM byte counters of a profile")
@@ -189,168 +192,173 @@ def main() -> int:
     log("roofline", roofline)
 
     per, es, cpu = {}, None, None
+    sections = set(args.sections.split(","))
     if not args.headline_only:
         # ----------------------------------------------------- per effect (c0-c2)
         W, H = 1280, 720
+        if "classic" in sections:
 
-        # SDF c0: 64 frames at t=k/60
-        cams = [wl.sdf_camera(scene8, k / 60) for k in range(64)]
-        fr_sdf = ctx.empty(64 * W * H)
+            # SDF c0: 64 frames at t=k/60
+            cams = [wl.sdf_camera(scene8, k / 60) for k in range(64)]
+            fr_sdf = ctx.empty(64 * W * H)
 
-        def sdf_plain():
-            for k, c in enumerate(cams):
-                ctx.sdf_render(fr_sdf.view(k * W * H, (W * H,)), W, H, scene8, c, CKD_SHADE_FP32)
+            def sdf_plain():
+                for k, c in enumerate(cams):
+                    ctx.sdf_render(fr_sdf.view(k * W * H, (W * H,)), W, H, scene8, c, CKD_SHADE_FP32)
 
-        sdf_reqs = []
-        for k in range(64):
-            r = wl.make_request(wl.BatchItem(CKD_FX_SDF, W, H, k / 60), fr_sdf.ptr + 4 * k * W * H,
-                                scene8, path, None)
-            sdf_reqs.append(r)
-        sdf_reqs = ctx.request_array(sdf_reqs)
+            sdf_reqs = []
+            for k in range(64):
+                r = wl.make_request(wl.BatchItem(CKD_FX_SDF, W, H, k / 60), fr_sdf.ptr + 4 * k * W * H,
+                                    scene8, path, None)
+                sdf_reqs.append(r)
+            sdf_reqs = ctx.request_array(sdf_reqs)
 
-        def sdf_batched():
-            ctx.render_batch(sdf_reqs, scene8, CKD_SHADE_FP16, None, 9, None, 0, pcount)
+            def sdf_batched():
+                ctx.render_batch(sdf_reqs, scene8, CKD_SHADE_FP16, None, 9, None, 0, pcount)
     
-        # voxel c1: 120 frames on the Lissajous path
-        lcams = [wl.landscape_cam(path, k / 60) for k in range(120)]
-        fr_vox = ctx.empty(120 * W * H)
+            # voxel c1: 120 frames on the Lissajous path
+            lcams = [wl.landscape_cam(path, k / 60) for k in range(120)]
+            fr_vox = ctx.empty(120 * W * H)
 
-        def vox_plain():
-            for k, c in enumerate(lcams):
-                ctx.landscape_draw(fr_vox.view(k * W * H, (W * H,)), W, H, [c])
+            def vox_plain():
+                for k, c in enumerate(lcams):
+                    ctx.landscape_draw(fr_vox.view(k * W * H, (W * H,)), W, H, [c])
 
-        def vox_batched():
-            ctx.landscape_draw(fr_vox, W, H, lcams)
+            def vox_batched():
+                ctx.landscape_draw(fr_vox, W, H, lcams)
 
-        # tunnel c2: 240 frames, radius U{1..32}
-        radii = wl.tunnel_radii(0, 240)
-        tgrids = [ctx.upload(wl.tunnel_grid(W, H, k / 60)) for k in range(240)]
-        fr_tun = ctx.empty(240 * W * H)
-        t0 = ctx.empty(W * H)
-        tun_tmp = ctx.empty(240 * W * H)
+            # tunnel c2: 240 frames, radius U{1..32}
+            radii = wl.tunnel_radii(0, 240)
+            tgrids = [ctx.upload(wl.tunnel_grid(W, H, k / 60)) for k in range(240)]
+            fr_tun = ctx.empty(240 * W * H)
+            t0 = ctx.empty(W * H)
+            tun_tmp = ctx.empty(240 * W * H)
 
-        def tun_plain():
+            def tun_plain():
+                for k in range(240):
+                    ctx.tunnel_frame(fr_tun.view(k * W * H, (W * H,)), t0, W, H, tgrids[k], d_tex, 9,
+                                     int(radii[k]))
+
+            tun_reqs = []
             for k in range(240):
-                ctx.tunnel_frame(fr_tun.view(k * W * H, (W * H,)), t0, W, H, tgrids[k], d_tex, 9,
-                                 int(radii[k]))
+                it = wl.BatchItem(CKD_FX_TUNNEL, W, H, k / 60, int(radii[k]))
+                tun_reqs.append(wl.make_request(it, fr_tun.ptr + 4 * k * W * H, scene8, path, tgrids[k].ptr))
+            tun_reqs = ctx.request_array(tun_reqs)
 
-        tun_reqs = []
-        for k in range(240):
-            it = wl.BatchItem(CKD_FX_TUNNEL, W, H, k / 60, int(radii[k]))
-            tun_reqs.append(wl.make_request(it, fr_tun.ptr + 4 * k * W * H, scene8, path, tgrids[k].ptr))
-        tun_reqs = ctx.request_array(tun_reqs)
+            def tun_batched():
+                ctx.render_batch(tun_reqs, scene8, CKD_SHADE_FP16, d_tex, 9, tun_tmp, 240 * W * H)
 
-        def tun_batched():
-            ctx.render_batch(tun_reqs, scene8, CKD_SHADE_FP16, d_tex, 9, tun_tmp, 240 * W * H)
+            for name, frames, plain, batched in [("sdf_c0", 64, sdf_plain, sdf_batched),
+                                                 ("voxel_c1", 120, vox_plain, vox_batched),
+                                                 ("tunnel_c2", 240, tun_plain, tun_batched)]:
+                plain()
+                batched()
+                mp = T.run(plain)
+                mb = T.run(batched)
+                px = frames * W * H
+                per[name] = {"frames": frames, "plain_fp32_mpx_s": round(px / mp / 1e3, 1),
+                             "batched_fp16_mpx_s": round(px / mb / 1e3, 1),
+                             "speedup": round(mp / mb, 3)}
+                log(name, per[name])
 
-        for name, frames, plain, batched in [("sdf_c0", 64, sdf_plain, sdf_batched),
-                                             ("voxel_c1", 120, vox_plain, vox_batched),
-                                             ("tunnel_c2", 240, tun_plain, tun_batched)]:
-            plain()
-            batched()
-            mp = T.run(plain)
-            mb = T.run(batched)
-            px = frames * W * H
-            per[name] = {"frames": frames, "plain_fp32_mpx_s": round(px / mp / 1e3, 1),
-                         "batched_fp16_mpx_s": round(px / mb / 1e3, 1),
-                         "speedup": round(mp / mb, 3)}
-            log(name, per[name])
+        if "c3" in sections:
+            # --------------------------------------- c3 precision form (2560x1440)
+            W3, H3 = 2560, 1440
+            n3 = 64
+            fr3 = ctx.empty(n3 * W3 * H3)
+            cams3 = [wl.sdf_camera(scene32, k / 60) for k in range(n3)]
 
-        # --------------------------------------- c3 precision form (2560x1440)
-        W3, H3 = 2560, 1440
-        n3 = 64
-        fr3 = ctx.empty(n3 * W3 * H3)
-        cams3 = [wl.sdf_camera(scene32, k / 60) for k in range(n3)]
+            def c3_plain():
+                for k, c in enumerate(cams3):
+                    ctx.sdf_render(fr3.view(k * W3 * H3, (W3 * H3,)), W3, H3, scene32, c, CKD_SHADE_FP32)
 
-        def c3_plain():
-            for k, c in enumerate(cams3):
-                ctx.sdf_render(fr3.view(k * W3 * H3, (W3 * H3,)), W3, H3, scene32, c, CKD_SHADE_FP32)
+            reqs3 = [wl.make_request(wl.BatchItem(CKD_FX_SDF, W3, H3, k / 60), fr3.ptr + 4 * k * W3 * H3,
+                                     scene32, path, None) for k in range(n3)]
+            reqs3 = ctx.request_array(reqs3)
 
-        reqs3 = [wl.make_request(wl.BatchItem(CKD_FX_SDF, W3, H3, k / 60), fr3.ptr + 4 * k * W3 * H3,
-                                 scene32, path, None) for k in range(n3)]
-        reqs3 = ctx.request_array(reqs3)
+            def c3_batched():
+                ctx.render_batch(reqs3, scene32, CKD_SHADE_FP16, None, 9, None, 0)
 
-        def c3_batched():
-            ctx.render_batch(reqs3, scene32, CKD_SHADE_FP16, None, 9, None, 0)
+            c3_plain()
+            c3_batched()
+            mp, mb = T.run(c3_plain), T.run(c3_batched)
+            px3 = n3 * W3 * H3
+            per["sdf_c3_precision"] = {"frames": n3, "plain_fp32_mpx_s": round(px3 / mp / 1e3, 1),
+                                       "batched_fp16_mpx_s": round(px3 / mb / 1e3, 1),
+                                       "speedup": round(mp / mb, 3)}
+            log("sdf_c3", per["sdf_c3_precision"])
 
-        c3_plain()
-        c3_batched()
-        mp, mb = T.run(c3_plain), T.run(c3_batched)
-        px3 = n3 * W3 * H3
-        per["sdf_c3_precision"] = {"frames": n3, "plain_fp32_mpx_s": round(px3 / mp / 1e3, 1),
-                                   "batched_fp16_mpx_s": round(px3 / mb / 1e3, 1),
-                                   "speedup": round(mp / mb, 3)}
-        log("sdf_c3", per["sdf_c3_precision"])
 
-        f3 = fr3
+        if "shadertoy" in sections:
+            # ------------------- the reference's raymarched effects (1280x720 Draw)
+            # plain: one Draw per call; batched: all frames of an effect in one call
+            # (one map launch, then the per-frame blur / blit of its Draw function)
+            n_st = 64
+            st_tex = np.ascontiguousarray(wl.noise_texture(11, 10).reshape(-1))
+            st_tex_hi = np.ascontiguousarray(wl.noise_texture(12, 10).reshape(-1))
+            ctx.fdtunnel_create(st_tex, st_tex_hi)
+            st_blur = wl.st_blur_maps(W, H)
+            ctx.spikey_blurmaps(st_blur, W, H)
+            fr_st = ctx.empty(n_st * W * H)
+            st_per = {}
+            for fx in ["plasma", "nautilus", "spikey", "spikey_close", "sinuses", "laura", "fdtunnel"]:
+                name = "spikey" if fx == "spikey_close" else fx
+                close = fx == "spikey_close"
+                frames = wl.st_frames(fx, n_st)
+                arr = ctx._frames(name, frames)[0]
+                singles = [ctx._frames(name, [f])[0] for f in frames]
 
-        # ------------------- the reference's raymarched effects (1280x720 Draw)
-        # plain: one Draw per call; batched: all frames of an effect in one call
-        # (one map launch, then the per-frame blur / blit of its Draw function)
-        n_st = 64
-        st_tex = np.ascontiguousarray(wl.noise_texture(11, 10).reshape(-1))
-        st_tex_hi = np.ascontiguousarray(wl.noise_texture(12, 10).reshape(-1))
-        ctx.fdtunnel_create(st_tex, st_tex_hi)
-        st_blur = wl.st_blur_maps(W, H)
-        ctx.spikey_blurmaps(st_blur, W, H)
-        fr_st = ctx.empty(n_st * W * H)
-        st_per = {}
-        for fx in ["plasma", "nautilus", "spikey", "spikey_close", "sinuses", "laura", "fdtunnel"]:
-            name = "spikey" if fx == "spikey_close" else fx
-            close = fx == "spikey_close"
-            frames = wl.st_frames(fx, n_st)
-            arr = ctx._frames(name, frames)[0]
-            singles = [ctx._frames(name, [f])[0] for f in frames]
+                def st_plain():
+                    for k in range(n_st):
+                        ctx.st_draw(name, fr_st.view(k * W * H, (W * H,)), W, H, singles[k], close=close)
 
-            def st_plain():
-                for k in range(n_st):
-                    ctx.st_draw(name, fr_st.view(k * W * H, (W * H,)), W, H, singles[k], close=close)
+                def st_batched():
+                    ctx.st_draw(name, fr_st, W, H, arr, close=close)
 
-            def st_batched():
-                ctx.st_draw(name, fr_st, W, H, arr, close=close)
+                st_plain()
+                st_batched()
+                mp, mb = T.run(st_plain), T.run(st_batched)
+                px = n_st * W * H
+                st_per[fx] = {"frames": n_st, "plain_mpx_s": round(px / mp / 1e3, 1),
+                              "batched_mpx_s": round(px / mb / 1e3, 1),
+                              "ms_per_frame_batched": round(mb / n_st, 4)}
+                log("shadertoy", fx, st_per[fx])
+            per["shadertoy_1280x720"] = st_per
 
-            st_plain()
-            st_batched()
-            mp, mb = T.run(st_plain), T.run(st_batched)
-            px = n_st * W * H
-            st_per[fx] = {"frames": n_st, "plain_mpx_s": round(px / mp / 1e3, 1),
-                          "batched_mpx_s": round(px / mb / 1e3, 1),
-                          "ms_per_frame_batched": round(mb / n_st, 4)}
-            log("shadertoy", fx, st_per[fx])
-        per["shadertoy_1280x720"] = st_per
+        if "voxel2" in sections:
+            # ------------------- the voxel ball and the torus twister (1280x720 Draw)
+            # plain: one Draw per call; batched: all frames in one call (the
+            # per-frame tables are staged and the launches queued back to back)
+            n_v2 = 32
+            fr_v2 = ctx.empty(n_v2 * W * H)
+            bm_v2, tm_v2 = wl.ball_maps(W, H), wl.twister_maps(W, H)
+            ctx.ball_create(bm_v2, W, H)
+            ctx.twister_create(tm_v2["heightMap"], tm_v2["colorMap"], tm_v2["background"], W, H)
+            tgt_v2 = ctx.empty(W * H)
+            ctx.memset32(tgt_v2, 0, W * H)
+            v2_sets = {"ball_beams": ("ball", wl.ball_frames(n_v2, 0)),
+                       "ball_envmap": ("ball", wl.ball_frames(n_v2, 1)),
+                       "twister": ("twister", wl.twister_frames(n_v2))}
+            v2_per = {}
+            for key, (kind, frames) in v2_sets.items():
+                draw = ctx.ball_draw if kind == "ball" else ctx.twister_draw
 
-        # ------------------- the voxel ball and the torus twister (1280x720 Draw)
-        # plain: one Draw per call; batched: all frames in one call (the
-        # per-frame tables are staged and the launches queued back to back)
-        n_v2 = 32
-        bm_v2, tm_v2 = wl.ball_maps(W, H), wl.twister_maps(W, H)
-        ctx.ball_create(bm_v2, W, H)
-        ctx.twister_create(tm_v2["heightMap"], tm_v2["colorMap"], tm_v2["background"], W, H)
-        tgt_v2 = ctx.empty(W * H)
-        ctx.memset32(tgt_v2, 0, W * H)
-        v2_sets = {"ball_beams": ("ball", wl.ball_frames(n_v2, 0)),
-                   "ball_envmap": ("ball", wl.ball_frames(n_v2, 1)),
-                   "twister": ("twister", wl.twister_frames(n_v2))}
-        v2_per = {}
-        for key, (kind, frames) in v2_sets.items():
-            draw = ctx.ball_draw if kind == "ball" else ctx.twister_draw
+                def v2_plain():
+                    for k in range(n_v2):
+                        draw(fr_v2.view(k * W * H, (W * H,)), tgt_v2, W, H, frames[k])
 
-            def v2_plain():
-                for k in range(n_v2):
-                    draw(fr_st.view(k * W * H, (W * H,)), tgt_v2, W, H, frames[k])
+                def v2_batched():
+                    draw(fr_v2, tgt_v2, W, H, frames)
 
-            def v2_batched():
-                draw(fr_st, tgt_v2, W, H, frames)
-
-            v2_plain()
-            v2_batched()
-            mp, mb = T.run(v2_plain), T.run(v2_batched)
-            px = n_v2 * W * H
-            v2_per[key] = {"frames": n_v2, "plain_mpx_s": round(px / mp / 1e3, 1),
-                           "batched_mpx_s": round(px / mb / 1e3, 1),
-                           "ms_per_frame_batched": round(mb / n_v2, 4)}
-            log("voxel2", key, v2_per[key])
-        per["ball_twister_1280x720"] = v2_per
+                v2_plain()
+                v2_batched()
+                mp, mb = T.run(v2_plain), T.run(v2_batched)
+                px = n_v2 * W * H
+                v2_per[key] = {"frames": n_v2, "plain_mpx_s": round(px / mp / 1e3, 1),
+                               "batched_mpx_s": round(px / mb / 1e3, 1),
+                               "ms_per_frame_batched": round(mb / n_v2, 4)}
+                log("voxel2", key, v2_per[key])
+            per["ball_twister_1280x720"] = v2_per
 
         # The CPU leg is the only part of the bench that runs the oracle: the
         # timed CPU baseline and the FP64 reference render the precision form's
@@ -360,17 +368,18 @@ def main() -> int:
             from helpers import error_stats  # noqa: E402
             orc = oracles.oracle()
             nthr = min(16, os.cpu_count() or 1)
-            # precision form: FP16 shading vs the FP64 render on a row sample
-            errs = []
-            c3_batched()
-            for k in [0, 63]:
-                img = fr3.view(k * W3 * H3, (W3 * H3,)).numpy().reshape(H3, W3)
-                rows = np.arange(7, H3, 36)
-                ref = np.zeros(len(rows) * W3, np.uint32)
-                orc.orc_sdf_rows(oracles.P(ref), W3, H3, C.byref(scene32), k / 60, 64, 7, 36, None, nthr)
-                errs.append((img[rows].reshape(-1), ref))
-            es = error_stats(np.concatenate([e[0] for e in errs]), np.concatenate([e[1] for e in errs]))
-            log("precision", es)
+            if "c3" in sections:
+                # precision form: FP16 shading vs the FP64 render on a row sample
+                errs = []
+                c3_batched()
+                for k in [0, 63]:
+                    img = fr3.view(k * W3 * H3, (W3 * H3,)).numpy().reshape(H3, W3)
+                    rows = np.arange(7, H3, 36)
+                    ref = np.zeros(len(rows) * W3, np.uint32)
+                    orc.orc_sdf_rows(oracles.P(ref), W3, H3, C.byref(scene32), k / 60, 64, 7, 36, None, nthr)
+                    errs.append((img[rows].reshape(-1), ref))
+                es = error_stats(np.concatenate([e[0] for e in errs]), np.concatenate([e[1] for e in errs]))
+                log("precision", es)
             # bounded sample of the batched workload: its requests in order, as
             # many as fit in about 10 s of CPU time, rendered by the oracle
             def cpu_render(it, nthreads):
@@ -405,86 +414,88 @@ def main() -> int:
             lut_h = np.zeros(2049, np.float32)
             orc.orc_cos_lut(oracles.P(lut_h))
             use_ref = oracles.ref_available()
-            if use_ref:
-                refl = oracles.ref()
+            refl = oracles.ref() if use_ref else None
+            if use_ref and "shadertoy" in sections:
                 refl.ref_st_init()
                 refl.ref_st_textures(oracles.P(st_tex), oracles.P(st_tex_hi))
                 refl.ref_st_blurmaps(oracles.P(st_blur), oracles.P(st_blur[st_blur.size // 2:]))
-            for fx in st_per:
-                name = "spikey" if fx == "spikey_close" else fx
-                frames = wl.st_frames(fx, 8)
-                out_h = oracles.aligned_empty(W * H)
-                with tempfile.TemporaryDirectory() as td:
-                    if use_ref:
-                        cls = type(frames[0])
-                        oracles.write_rocket_tracks(Path(td), {
-                            tr: [getattr(f, fld) for f in frames] for fld, tr in cls.TRACKS.items()})
-                        refl.ref_st_rocket(td.encode())
-                    t_c = time.perf_counter()
-                    for k, f in enumerate(frames):
+            if "shadertoy" in sections:
+                for fx in st_per:
+                    name = "spikey" if fx == "spikey_close" else fx
+                    frames = wl.st_frames(fx, 8)
+                    out_h = oracles.aligned_empty(W * H)
+                    with tempfile.TemporaryDirectory() as td:
                         if use_ref:
-                            refl.ref_st_row(float(k))
-                            refl.ref_st_draw(oracles.ST_IDS[fx], oracles.P(out_h), f.time, 1.0)
-                        else:
-                            pa = f.as_array()
-                            orc.orc_st_draw(oracles.ST_IDS[fx], oracles.P(out_h), W, H, oracles.P(pa),
-                                            oracles.P(lut_h), oracles.P(st_tex), oracles.P(st_tex_hi),
-                                            oracles.P(st_blur), 1, nthr)
-                    dt_c = time.perf_counter() - t_c
-                st_per[fx]["cpu_mpx_s"] = round(8 * W * H / dt_c / 1e6, 2)
-                st_per[fx]["cpu_kind"] = ("reference Draw (OpenMP, OMP_NUM_THREADS threads)"
-                                          if use_ref else f"oracle Draw ({nthr} threads)")
-                log("shadertoy cpu", fx, st_per[fx]["cpu_mpx_s"])
+                            cls = type(frames[0])
+                            oracles.write_rocket_tracks(Path(td), {
+                                tr: [getattr(f, fld) for f in frames] for fld, tr in cls.TRACKS.items()})
+                            refl.ref_st_rocket(td.encode())
+                        t_c = time.perf_counter()
+                        for k, f in enumerate(frames):
+                            if use_ref:
+                                refl.ref_st_row(float(k))
+                                refl.ref_st_draw(oracles.ST_IDS[fx], oracles.P(out_h), f.time, 1.0)
+                            else:
+                                pa = f.as_array()
+                                orc.orc_st_draw(oracles.ST_IDS[fx], oracles.P(out_h), W, H, oracles.P(pa),
+                                                oracles.P(lut_h), oracles.P(st_tex), oracles.P(st_tex_hi),
+                                                oracles.P(st_blur), 1, nthr)
+                        dt_c = time.perf_counter() - t_c
+                    st_per[fx]["cpu_mpx_s"] = round(8 * W * H / dt_c / 1e6, 2)
+                    st_per[fx]["cpu_kind"] = ("reference Draw (OpenMP, OMP_NUM_THREADS threads)"
+                                              if use_ref else f"oracle Draw ({nthr} threads)")
+                    log("shadertoy cpu", fx, st_per[fx]["cpu_mpx_s"])
 
-            # the ball and the twister: the reference's Draw (tracks as above)
-            # where the harness is present, else the oracle's
-            m0_h = np.zeros(2 * W * H, np.int32)
-            orc.orc_polar_maps(oracles.P(m0_h), oracles.P(np.zeros(2 * W * H, np.int32)), W, H, W, H)
-            if use_ref:
-                refl.ref_shared_init()
-                refl.ref_polar_create()
-                refl.ref_polar_pad_forward()
-            for key, (kind, frames) in v2_sets.items():
-                frames = frames[:8]
-                cls = type(frames[0])
-                out_h = oracles.aligned_empty(W * (H + 16))
-                tgt_h = np.zeros(W * H, np.uint32)
-                with tempfile.TemporaryDirectory() as td:
-                    if use_ref:
-                        oracles.write_rocket_tracks(Path(td), {
-                            tr: [getattr(f, fld) for f in frames] for fld, tr in cls.TRACKS.items()})
-                        refl.ref_st_rocket(td.encode())
-                        if kind == "ball":
-                            refl.ref_ball_setup(td.encode(), oracles.P(bm_v2["heightMaps"]),
-                                                oracles.P(bm_v2["colorMaps"]), oracles.P(bm_v2["beamMaps"]),
-                                                oracles.P(bm_v2["envMap"]), oracles.P(bm_v2["backgrounds"]),
-                                                oracles.P(bm_v2["halo"]))
-                        else:
-                            refl.ref_twister_setup(td.encode(), oracles.P(tm_v2["heightMap"]),
-                                                   oracles.P(tm_v2["colorMap"]),
-                                                   oracles.P(tm_v2["background"]))
-                    t_c = time.perf_counter()
-                    for k, f in enumerate(frames):
+            if "voxel2" in sections:
+                # the ball and the twister: the reference's Draw (tracks as above)
+                # where the harness is present, else the oracle's
+                m0_h = np.zeros(2 * W * H, np.int32)
+                orc.orc_polar_maps(oracles.P(m0_h), oracles.P(np.zeros(2 * W * H, np.int32)), W, H, W, H)
+                if use_ref:
+                    refl.ref_shared_init()
+                    refl.ref_polar_create()
+                    refl.ref_polar_pad_forward()
+                for key, (kind, frames) in v2_sets.items():
+                    frames = frames[:8]
+                    cls = type(frames[0])
+                    out_h = oracles.aligned_empty(W * (H + 16))
+                    tgt_h = np.zeros(W * H, np.uint32)
+                    with tempfile.TemporaryDirectory() as td:
                         if use_ref:
-                            refl.ref_st_row(float(k))
-                            (refl.ref_ball_draw if kind == "ball" else refl.ref_twister_draw)(
-                                oracles.P(out_h), f.time, 1.0)
-                        elif kind == "ball":
-                            orc.orc_ball_draw(oracles.P(out_h), oracles.P(tgt_h), W, H,
-                                              oracles.P(bm_v2["heightMaps"]), oracles.P(bm_v2["colorMaps"]),
-                                              oracles.P(bm_v2["beamMaps"]), oracles.P(bm_v2["envMap"]),
-                                              oracles.P(bm_v2["backgrounds"]), oracles.P(bm_v2["halo"]),
-                                              oracles.P(m0_h), oracles.P(f.as_array()), nthr)
-                        else:
-                            orc.orc_twister_draw(oracles.P(out_h), oracles.P(tgt_h), W, H,
-                                                 oracles.P(tm_v2["heightMap"]), oracles.P(tm_v2["colorMap"]),
-                                                 oracles.P(tm_v2["background"]), oracles.P(m0_h),
-                                                 oracles.P(f.as_array()), nthr)
-                    dt_c = time.perf_counter() - t_c
-                v2_per[key]["cpu_mpx_s"] = round(len(frames) * W * H / dt_c / 1e6, 2)
-                v2_per[key]["cpu_kind"] = ("reference Draw (OpenMP, OMP_NUM_THREADS threads)"
-                                           if use_ref else f"oracle Draw ({nthr} threads)")
-                log("voxel2 cpu", key, v2_per[key]["cpu_mpx_s"])
+                            oracles.write_rocket_tracks(Path(td), {
+                                tr: [getattr(f, fld) for f in frames] for fld, tr in cls.TRACKS.items()})
+                            refl.ref_st_rocket(td.encode())
+                            if kind == "ball":
+                                refl.ref_ball_setup(td.encode(), oracles.P(bm_v2["heightMaps"]),
+                                                    oracles.P(bm_v2["colorMaps"]), oracles.P(bm_v2["beamMaps"]),
+                                                    oracles.P(bm_v2["envMap"]), oracles.P(bm_v2["backgrounds"]),
+                                                    oracles.P(bm_v2["halo"]))
+                            else:
+                                refl.ref_twister_setup(td.encode(), oracles.P(tm_v2["heightMap"]),
+                                                       oracles.P(tm_v2["colorMap"]),
+                                                       oracles.P(tm_v2["background"]))
+                        t_c = time.perf_counter()
+                        for k, f in enumerate(frames):
+                            if use_ref:
+                                refl.ref_st_row(float(k))
+                                (refl.ref_ball_draw if kind == "ball" else refl.ref_twister_draw)(
+                                    oracles.P(out_h), f.time, 1.0)
+                            elif kind == "ball":
+                                orc.orc_ball_draw(oracles.P(out_h), oracles.P(tgt_h), W, H,
+                                                  oracles.P(bm_v2["heightMaps"]), oracles.P(bm_v2["colorMaps"]),
+                                                  oracles.P(bm_v2["beamMaps"]), oracles.P(bm_v2["envMap"]),
+                                                  oracles.P(bm_v2["backgrounds"]), oracles.P(bm_v2["halo"]),
+                                                  oracles.P(m0_h), oracles.P(f.as_array()), nthr)
+                            else:
+                                orc.orc_twister_draw(oracles.P(out_h), oracles.P(tgt_h), W, H,
+                                                     oracles.P(tm_v2["heightMap"]), oracles.P(tm_v2["colorMap"]),
+                                                     oracles.P(tm_v2["background"]), oracles.P(m0_h),
+                                                     oracles.P(f.as_array()), nthr)
+                        dt_c = time.perf_counter() - t_c
+                    v2_per[key]["cpu_mpx_s"] = round(len(frames) * W * H / dt_c / 1e6, 2)
+                    v2_per[key]["cpu_kind"] = ("reference Draw (OpenMP, OMP_NUM_THREADS threads)"
+                                               if use_ref else f"oracle Draw ({nthr} threads)")
+                    log("voxel2 cpu", key, v2_per[key]["cpu_mpx_s"])
 
             px, k, dt = cpu_rate(nthr, 10.0)
             px1, k1, dt1 = cpu_rate(1, 5.0)
