@@ -64,10 +64,15 @@ struct ckd_ctx {
     uint32_t *tsFog = nullptr;
     uint32_t tsFog0 = 0;
 
-    // polar maps (Polar_Create), interleaved (U,V) 24:8 per pixel
-    int32_t *polarMap = nullptr;
-    int32_t *polarInv = nullptr;
-    unsigned polarW = 0, polarH = 0;
+    // polar maps (Polar_Create), interleaved (U,V) 24:8 per pixel, one set
+    // per image size (the reference keeps 1280x720 and the FX map's 644x364)
+    struct PolarMaps {
+        int32_t *map = nullptr, *inv = nullptr;
+        unsigned W = 0, H = 0;
+    };
+    static constexpr int kPolarSlots = 4;
+    PolarMaps polar[kPolarSlots];
+    int polarNext = 0; // slot the next new size takes
 
     // 2026 blur scratch (BoxBlur_Create): two images plus guard pixels
     uint32_t *b26Scratch[2] = {nullptr, nullptr};
@@ -100,6 +105,9 @@ ckd_status fail(ckd_status s, const char *fmt, ...)
 
 // hipGetLastError after a launch: CKD_EDEVICE with the kernel name.
 ckd_status check_launch(const char *what);
+
+// The polar maps for W x H (ckd_polar_create), or nullptr.
+const ckd_ctx::PolarMaps *polar_maps(const ckd_ctx *ctx, unsigned W, unsigned H);
 
 // Device buffer of at least `bytes` owned by the context for per-call
 // parameters; enqueues the host->device copy on the context stream.

@@ -428,7 +428,7 @@ bool finite_all(const float *p, size_t n)
 
 ckd_status ensure_polar(ckd_ctx *ctx, unsigned W, unsigned H)
 {
-    if (ctx->polarMap && ctx->polarW == W && ctx->polarH == H) return CKD_OK;
+    if (ckd::polar_maps(ctx, W, H)) return CKD_OK;
     return ckd_polar_create(ctx, W, H);
 }
 
@@ -447,11 +447,13 @@ ckd_status ckd_polar_blit_a(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, un
     CKD_REQUIRE(dst && src, "ckd_polar_blit_a: null buffer");
     CKD_REQUIRE(dst != src, "ckd_polar_blit_a: dst must differ from src");
     CKD_REQUIRE_CTX(ctx);
-    CKD_REQUIRE(ctx->polarMap && ctx->polarW == W && ctx->polarH == H,
-                "ckd_polar_blit_a: no polar maps for %ux%u (ckd_polar_create)", W, H);
+    const ckd_ctx::PolarMaps *pm = ckd::polar_maps(ctx, W, H);
+    if (!pm)
+        return ckd::fail(CKD_ESTATE, "ckd_polar_blit_a: no polar maps for %ux%u (ckd_polar_create)",
+                         W, H);
     dim3 grid(ckd_div_up(W, 256), H);
     k_polar_blit_a<<<grid, 256, 0, ctx->stream>>>(
-        dst, src, reinterpret_cast<const int2 *>(inverse ? ctx->polarInv : ctx->polarMap), W, H);
+        dst, src, reinterpret_cast<const int2 *>(inverse ? pm->inv : pm->map), W, H);
     return ckd::check_launch("polar_blit_a");
 }
 

@@ -86,6 +86,13 @@ __global__ __launch_bounds__(256) void k_grid_remap(uint32_t *__restrict__ dst, 
 } // namespace
 
 namespace ckd {
+const ckd_ctx::PolarMaps *polar_maps(const ckd_ctx *ctx, unsigned W, unsigned H)
+{
+    for (const ckd_ctx::PolarMaps &pm : ctx->polar)
+        if (pm.map && pm.W == W && pm.H == H) return &pm;
+    return nullptr;
+}
+
 void tunnel_job_geometry(TunJob &J, uint32_t *dst, uint32_t *tmp, const int32_t *grid, unsigned W,
                          unsigned H, int radius);
 uint32_t tunnel_vblocks(unsigned W, unsigned H, int seg);
@@ -117,22 +124,31 @@ ckd_status ckd_polar_create(ckd_ctx *ctx, unsigned W, unsigned H)
     const size_t n = (size_t)W * H * 2;
     std::vector<int32_t> map(n), inv(n);
     calculate_maps(map.data(), inv.data(), W, H, W, H);
-    if (ctx->polarMap) {
-        CKD_HIP(hipStreamSynchronize(ctx->stream));
-        (void)hipFree(ctx->polarMap);
-        (void)hipFree(ctx->polarInv);
-        ctx->polarMap = ctx->polarInv = nullptr;
+    // the size's own slot, else the oldest one
+    int slot = -1;
+    for (int i = 0; i < ckd_ctx::kPolarSlots; ++i)
+        if (ctx->polar[i].map && ctx->polar[i].W == W && ctx->polar[i].H == H) slot = i;
+    if (slot < 0) {
+        slot = ctx->polarNext;
+        ctx->polarNext = (ctx->polarNext + 1) % ckd_ctx::kPolarSlots;
     }
-    if (hipMalloc(&ctx->polarMap, n * 4) != hipSuccess ||
-        hipMalloc(&ctx->polarInv, n * 4) != hipSuccess) {
-        if (ctx->polarMap) (void)hipFree(ctx->polarMap);
-        ctx->polarMap = ctx->polarInv = nullptr;
+    ckd_ctx::PolarMaps &pm = ctx->polar[slot];
+    if (pm.map) {
+        // queued blits may still read the maps being replaced
+        CKD_HIP(hipStreamSynchronize(ctx->stream));
+        (void)hipFree(pm.map);
+        (void)hipFree(pm.inv);
+        pm = ckd_ctx::PolarMaps();
+    }
+    if (hipMalloc(&pm.map, n * 4) != hipSuccess || hipMalloc(&pm.inv, n * 4) != hipSuccess) {
+        if (pm.map) (void)hipFree(pm.map);
+        pm = ckd_ctx::PolarMaps();
         return ckd::fail(CKD_ENOMEM, "ckd_polar_create: allocation failed");
     }
-    CKD_HIP(hipMemcpy(ctx->polarMap, map.data(), n * 4, hipMemcpyHostToDevice));
-    CKD_HIP(hipMemcpy(ctx->polarInv, inv.data(), n * 4, hipMemcpyHostToDevice));
-    ctx->polarW = W;
-    ctx->polarH = H;
+    CKD_HIP(hipMemcpy(pm.map, map.data(), n * 4, hipMemcpyHostToDevice));
+    CKD_HIP(hipMemcpy(pm.inv, inv.data(), n * 4, hipMemcpyHostToDevice));
+    pm.W = W;
+    pm.H = H;
     return CKD_OK;
 }
 
@@ -142,12 +158,13 @@ ckd_status ckd_polar_blit(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, unsi
     CKD_REQUIRE(dst && src, "ckd_polar_blit: null buffer");
     CKD_REQUIRE(dst != src, "ckd_polar_blit: dst must differ from src");
     CKD_REQUIRE_CTX(ctx);
-    if (!ctx->polarMap || ctx->polarW != W || ctx->polarH != H)
+    const ckd_ctx::PolarMaps *pm = ckd::polar_maps(ctx, W, H);
+    if (!pm)
         return ckd::fail(CKD_ESTATE, "ckd_polar_blit: no polar maps for %ux%u (ckd_polar_create)",
                          W, H);
     dim3 g(ckd_div_up(W, 256), H);
     k_polar_blit<<<g, 256, 0, ctx->stream>>>(
-        dst, src, reinterpret_cast<const int2 *>(inverse ? ctx->polarInv : ctx->polarMap), W, H);
+        dst, src, reinterpret_cast<const int2 *>(inverse ? pm->inv : pm->map), W, H);
     return ckd::check_launch("polar_blit");
 }
 

@@ -208,8 +208,12 @@ void ckd_destroy(ckd_ctx *ctx)
     if (ctx->forkEv) (void)hipEventDestroy(ctx->forkEv);
     if (ctx->joinEv) (void)hipEventDestroy(ctx->joinEv);
     if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
+    for (auto &pm : ctx->polar) {
+        if (pm.map) (void)hipFree(pm.map);
+        if (pm.inv) (void)hipFree(pm.inv);
+    }
     void *bufs[] = {ctx->cosLut, ctx->lsHeight, ctx->lsColor, ctx->lsFog, ctx->tsHeight, ctx->tsColor,
-                    ctx->tsFog, ctx->polarMap, ctx->polarInv, ctx->b26Scratch[0],
+                    ctx->tsFog, ctx->b26Scratch[0],
                     ctx->b26Scratch[1], ctx->staging, ctx->fdTex, ctx->fdTexHi,
                     ctx->stScratch, ctx->spikeBlurMaps, ctx->twHeight, ctx->twColor,
                     ctx->twTables, ctx->twBackground, ctx->ballMaps, ctx->ballImages};

@@ -405,7 +405,9 @@ void Tunnel_Draw(uint32_t *pDest, float time, float delta)
 bool Polar_Create()
 {
     if (!acquire()) return false;
-    if (ckd_polar_create(g_ctx, kResX, kResY) != CKD_OK) {
+    // the output's maps and the FX map's (code/polar.cpp:61-70)
+    if (ckd_polar_create(g_ctx, kResX, kResY) != CKD_OK ||
+        ckd_polar_create(g_ctx, kFxMapResX, kFxMapResY) != CKD_OK) {
         report("Polar_Create");
         return false;
     }
@@ -603,6 +605,15 @@ void Zoom32(uint32_t *pDest, const uint32_t *pSrc, unsigned xRes, unsigned yRes,
     filter("Zoom32", pDest, pSrc, (size_t)xRes * yRes, false, [&](uint32_t *d, uint32_t *s) {
         return ckd_zoom32(g_ctx, d, s, xRes, yRes, scale);
     });
+}
+
+// Polar_Blit at the FX map's size, 644x364 (code/polar.cpp:200-218)
+void Polar_Blit_2x2(uint32_t *pDest, const uint32_t *pSrc, bool inverse)
+{
+    filter("Polar_Blit_2x2", pDest, pSrc, (size_t)kFxMapResX * kFxMapResY, false,
+           [&](uint32_t *d, uint32_t *s) {
+               return ckd_polar_blit(g_ctx, d, s, kFxMapResX, kFxMapResY, inverse ? 1 : 0);
+           });
 }
 
 void Polar_BlitA(uint32_t *pDest, const uint32_t *pSrc, bool inverse)

@@ -25,8 +25,9 @@ void Tunnelscape_Draw(uint32_t *pDest, float time, float delta);
 bool Polar_Create();
 void Polar_Destroy();
 void Polar_Blit(uint32_t *pDest, const uint32_t *pSrc, bool inverse = false);
-// code/polar.h:14
+// code/polar.h:14-17
 void Polar_BlitA(uint32_t *pDest, const uint32_t *pSrc, bool inverse = false);
+void Polar_Blit_2x2(uint32_t *pDest, const uint32_t *pSrc, bool inverse = false);
 
 // code/ball.h:7-13
 bool Ball_Create();

@@ -144,8 +144,12 @@ CKD_API ckd_status ckd_blur26(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src,
 /* ---- polar UV-map remap, code/polar.cpp ----------------------------------
  * ckd_polar_create computes the 24:8 maps of CalculateMaps
  * (code/polar.cpp:18-59) on the host in FP32 for a W x H target and uploads
- * them (Polar_Create, code/polar.cpp:61-72).  Polar_Blit
- * (code/polar.cpp:135-154) then remaps src (W x H) into dst (W x H). */
+ * them (Polar_Create, code/polar.cpp:61-72, which makes them for the
+ * 1280 x 720 output and the 644 x 364 FX map).  The context keeps the maps
+ * of up to four sizes; a fifth replaces the oldest.  Polar_Blit
+ * (code/polar.cpp:135-154) then remaps src (W x H) into dst (W x H);
+ * Polar_Blit_2x2 (code/polar.cpp:200-218) is the same call at the FX
+ * map's size. */
 CKD_API ckd_status ckd_polar_create(ckd_ctx *ctx, unsigned W, unsigned H);
 CKD_API ckd_status ckd_polar_blit(ckd_ctx *ctx, uint32_t *dst,
                                   const uint32_t *src, unsigned W, unsigned H,

@@ -39,3 +39,27 @@ REF_API void ref_polar_pad_forward(void)
     freeAligned(s_pMap);
     s_pMap = padded;
 }
+
+/* code/polar.cpp:200-218 (644x364).  Its 64-row (inverse 32-row) tiles of
+ * 64 (32) pixels run past the image's last column into the next row and
+ * past its last row: dst needs 24 guard rows, and the maps must be padded
+ * with ref_polar_pad_2x2 first.  The pixels inside the image are those of a
+ * per-pixel blit with the 644x364 maps. */
+REF_API void ref_polar_blit_2x2(uint32_t *dst, const uint32_t *src, int inverse)
+{
+    Polar_Blit_2x2(dst, src, inverse != 0);
+}
+
+/* Replaces both FX-map maps by copies with 24 zero rows appended. */
+REF_API void ref_polar_pad_2x2(void)
+{
+    const size_t rows = kFxMapResY + 24;
+    int **maps[2] = {&s_pMap2x2, &s_pInvMap2x2};
+    for (int **m : maps) {
+        int *padded = static_cast<int *>(mallocAligned(kFxMapResX * rows * 2 * sizeof(int), kAlignTo));
+        memset(padded, 0, kFxMapResX * rows * 2 * sizeof(int));
+        memcpy(padded, *m, kFxMapSize * 2 * sizeof(int));
+        freeAligned(*m);
+        *m = padded;
+    }
+}

@@ -102,6 +102,8 @@ _REF = {
     "ref_tscape_ray": (None, [_p, _p, _p, _p, _i, _i, _i, _i]),
     "ref_polar_maps": (None, [_p, _p, _u, _u, _u, _u]),
     "ref_polar_create": (_i, []),
+    "ref_polar_blit_2x2": (None, [_p, _p, _i]),
+    "ref_polar_pad_2x2": (None, []),
     "ref_sync_value": (C.c_double, [C.c_char_p, C.c_char_p, C.c_double]),
     "ref_polar_destroy": (None, []),
     "ref_polar_blit": (None, [_p, _p, _i]),

@@ -18,6 +18,7 @@
 //                                                     frames at the same time)
 //   shim_test twister <time> <out>
 //   shim_test polar_blit_a <dst> <src> <inverse 0|1> <out>
+//   shim_test polar2x2 <in 644x364> <inverse 0|1> <out>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -203,6 +204,13 @@ int main(int argc, char **argv)
         Twister_Draw(out.data(), (float)atof(argv[2]), 1.f);
         Twister_Destroy();
         write_pixels(argv[3], out.data(), n);
+    } else if (cmd == "polar2x2" && argc == 5) {
+        std::vector<uint32_t> in = read_pixels(argv[2]);
+        if (in.size() != 644 * 364 || !Polar_Create()) return 1;
+        std::vector<uint32_t> res(in.size());
+        Polar_Blit_2x2(res.data(), in.data(), atoi(argv[3]) != 0);
+        Polar_Destroy();
+        write_pixels(argv[4], res.data(), res.size());
     } else if (cmd == "polar_blit_a" && argc == 6) {
         std::vector<uint32_t> d = read_pixels(argv[2]), s = read_pixels(argv[3]);
         if (d.size() != n || s.size() != n || !Polar_Create()) return 1;

@@ -156,6 +156,35 @@ def test_polar(ctx, orc, W, H):
         assert np.array_equal(d_dst.numpy(), ref), inv
 
 
+def test_polar_maps_of_several_sizes(orc):
+    """a context keeps the maps of every size it was given (the output's
+    and the FX map's, for Polar_Blit and Polar_Blit_2x2), up to four; a
+    fifth size replaces the oldest"""
+    from cookiedough_amd.device import Context
+    c = Context(0)
+    try:
+        sizes = [(1280, 720), (644, 364), (320, 180), (1920, 1080)]
+        for W, H in sizes:
+            c.polar_create(W, H)
+        for W, H in sizes + [(644, 364)]:
+            _say("polar kept", W, H)
+            src = rand_img(51, W * H)
+            m0, m1 = np.zeros(W * H * 2, np.int32), np.zeros(W * H * 2, np.int32)
+            orc.orc_polar_maps(P(m0), P(m1), W, H, W, H)
+            d_src, d_dst = c.upload(src), c.empty(W * H)
+            for inv, m in [(0, m0), (1, m1)]:
+                c.polar_blit(d_dst, d_src, W, H, inv)
+                ref = np.zeros(W * H, np.uint32)
+                orc.orc_polar_blit(P(ref), P(src), P(m), W, H, W, 8)
+                assert np.array_equal(d_dst.numpy(), ref), (W, H, inv)
+        c.polar_create(640, 360)  # replaces the oldest, 1280 x 720
+        with pytest.raises(CkdError, match="no polar maps for 1280x720"):
+            c.polar_blit(c.empty(1280 * 720), c.empty(1280 * 720), 1280, 720, 0)
+        c.sync()
+    finally:
+        c.close()
+
+
 @pytest.mark.parametrize("W,H", [(1280, 720), (320, 180), (1920, 1080)])
 def test_grid_remap(ctx, orc, W, H):
     _say("grid remap", W, H)

@@ -277,3 +277,17 @@ def test_polar_blit_a(orc, tmp_path, inverse):
     want = dst.copy()
     orc.orc_polar_blit_a(P(want), P(src), P(m1 if inverse else m0), 1280, 720, 1280, 8)
     assert np.array_equal(np.fromfile(out, np.uint32), want)
+
+
+@pytest.mark.parametrize("inverse", [0, 1])
+def test_polar_blit_2x2(orc, tmp_path, inverse):
+    W, H = 644, 364
+    src = rand_img(76, W * H)
+    src.tofile(tmp_path / "in.bin")
+    out = tmp_path / "out.bin"
+    run(["polar2x2", tmp_path / "in.bin", inverse, out])
+    m0, m1 = np.zeros(2 * W * H, np.int32), np.zeros(2 * W * H, np.int32)
+    orc.orc_polar_maps(P(m0), P(m1), W, H, W, H)
+    want = np.zeros(W * H, np.uint32)
+    orc.orc_polar_blit(P(want), P(src), P(m1 if inverse else m0), W, H, W, 8)
+    assert np.array_equal(np.fromfile(out, np.uint32), want)

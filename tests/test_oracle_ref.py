@@ -173,6 +173,27 @@ def test_polar_blit(orc, refl):
     refl.ref_polar_destroy()
 
 
+def test_polar_blit_2x2(orc, refl):
+    """Polar_Blit_2x2 (code/polar.cpp:200-218) at the FX map's 644x364 with
+    its own maps: inside the image, the per-pixel blit of the oracle with
+    orc_polar_maps(644, 364).  Its tiles overrun the image (guard rows and
+    padded maps in the harness)."""
+    W, H = 644, 364
+    assert refl.ref_polar_create() == 1
+    refl.ref_polar_pad_2x2()
+    src = aligned_empty(W * H)
+    src[:] = np.random.default_rng(5).integers(0, 2**32, W * H, dtype=np.uint64).astype(np.uint32)
+    m0, m1 = np.zeros(W * H * 2, np.int32), np.zeros(W * H * 2, np.int32)
+    orc.orc_polar_maps(P(m0), P(m1), W, H, W, H)
+    for inv, m in [(0, m0), (1, m1)]:
+        r = aligned_empty(W * (H + 24))
+        o = np.zeros(W * H, np.uint32)
+        refl.ref_polar_blit_2x2(P(r), P(src), inv)
+        orc.orc_polar_blit(P(o), P(src), P(m), W, H, W, 4)
+        assert np.array_equal(r[:W * H], o), inv
+    refl.ref_polar_destroy()
+
+
 BLUR07_SPANS = [1, 2, 3, 4, 5, 6, 7, 8, 9, 16, 17, 31, 32, 33, 64, 65, 100, 128, 129, 254, 255]
 
 
