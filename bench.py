@@ -124,6 +124,7 @@ def main() -> int:
         grids.append(g)
         reqs.append(wl.make_request(it, out.ptr + 4 * off, scene8, path, g.ptr if g else None))
         off += it.W * it.H
+    sdf_only = ctx.request_array([r for r, it in zip(reqs, items) if it.effect == CKD_FX_SDF])
     reqs = ctx.request_array(reqs)
     log(f"inputs ready in {time.time() - t_in:.1f}s: {len(items)} requests, {total_px / 1e6:.1f} Mpx")
 
@@ -189,6 +190,25 @@ def main() -> int:
                 "launches": launches, "avg_launch_ms": round(ms_k / max(1, launches), 4),
                 "algorithmic": "per launch: sum(steps)*(F_sdf+7) + px*(6*F_sdf+40), "
                                "F_sdf = 11*8+2 = 90 FLOP"}
+    # the same kernel alone: the batch's SDF requests only, at their own
+    # autotuned launch shape (in the batch it shares the GPU with the voxel
+    # and tunnel branch, which is what the figures above include)
+    if sdf_px_batch:
+        headline_tuning = ctx.get_tuning()
+        ctx.autotune_batch(sdf_only, scene8, CKD_SHADE_FP16, d_tex, 9, tmp, tmp_px)
+        ctx.kernel_timing(True)
+        st0 = int(pcount.numpy()[0])
+        for _ in range(args.warmup + args.steps):
+            ctx.render_batch(sdf_only, scene8, CKD_SHADE_FP16, d_tex, 9, tmp, tmp_px, pcount)
+        ctx.sync()
+        ms_a, launches_a = ctx.kernel_timing_read(1)
+        ctx.kernel_timing(False)
+        n_a = args.warmup + args.steps
+        ach_a = sdf_flops(int(pcount.numpy()[0]) - st0, sdf_px_batch * n_a, 8) / (ms_a * 1e-3) / 1e12
+        roofline["standalone"] = {"achieved": round(ach_a, 2), "frac": round(ach_a / PEAK_FP32_TFLOPS, 4),
+                                  "avg_launch_ms": round(ms_a / max(1, launches_a), 4),
+                                  "tuning": ctx.get_tuning().as_dict()}
+        ctx.set_tuning(headline_tuning)
     log("roofline", roofline)
 
     per, es, cpu = {}, None, None

@@ -18,7 +18,7 @@
 namespace ckd {
 ckd_status launch_sdf_persistent(ckd_ctx *ctx, uint32_t *queueCounter, uint32_t totalChunks,
                                  const SdfJob *jobs, int nJobs, const ckd_sdf_scene &sc,
-                                 int shading, unsigned long long *stepCount);
+                                 int shading, unsigned long long *stepCount, bool helper);
 void voxel_job_geometry(VoxJob &J, unsigned W, unsigned H, const ckd_landscape_cam &c);
 ckd_status launch_landscape_jobs(ckd_ctx *ctx, const VoxJob *jobs, int nJobs, uint32_t totalBlocks);
 void tunnel_job_geometry(TunJob &J, uint32_t *dst, uint32_t *tmp, const int32_t *grid, unsigned W,
@@ -157,7 +157,7 @@ ckd_status launch_plan(ckd_ctx *ctx, const Plan &P, unsigned char *dev, const ck
     if (P.nSdf > 0) {
         s = ckd::launch_sdf_persistent(ctx, reinterpret_cast<uint32_t *>(dev), P.chunks,
                                        reinterpret_cast<const SdfJob *>(dev + P.oSdf), P.nSdf,
-                                       *scene, shading, stepCount);
+                                       *scene, shading, stepCount, false);
         if (s != CKD_OK) return s;
     }
     {
@@ -172,6 +172,13 @@ ckd_status launch_plan(ckd_ctx *ctx, const Plan &P, unsigned char *dev, const ck
             memset(&none, 0, sizeof(none));
             s = ckd::launch_tunnel_jobs(ctx, reinterpret_cast<const TunJob *>(dev + P.oTun), P.nTun,
                                         none, P.tunRows, P.tunV, P.tunMaxW, tex, texLog2);
+            if (s != CKD_OK) return s;
+        }
+        if (fork) {
+            // the voxel / tunnel work is done: its CUs join the SDF march
+            s = ckd::launch_sdf_persistent(ctx, reinterpret_cast<uint32_t *>(dev), P.chunks,
+                                           reinterpret_cast<const SdfJob *>(dev + P.oSdf), P.nSdf,
+                                           *scene, shading, stepCount, true);
             if (s != CKD_OK) return s;
         }
     }
@@ -204,7 +211,7 @@ ckd_status valid_tuning(const ckd_tuning *t)
                 t->sdfChunk);
     CKD_REQUIRE(t->sdfBlocksPerCU >= 1 && t->sdfBlocksPerCU <= 32,
                 "tuning: sdfBlocksPerCU %d outside 1..32", t->sdfBlocksPerCU);
-    CKD_REQUIRE(t->tunnelVSeg >= 8 && t->tunnelVSeg <= 128, "tuning: tunnelVSeg %d outside 8..128",
+    CKD_REQUIRE(t->tunnelVSeg >= 8 && t->tunnelVSeg <= 512, "tuning: tunnelVSeg %d outside 8..512",
                 t->tunnelVSeg);
     return CKD_OK;
 }
@@ -252,7 +259,10 @@ ckd_status ckd_autotune_batch(ckd_ctx *ctx, const ckd_request *requests, unsigne
     if (s != CKD_OK) return s;
     CKD_REQUIRE_CTX(ctx);
     if (n == 0) return CKD_OK;
-    const int steps[] = {1, 2, 4}, chunks[] = {16, 32}, bpc[] = {4, 8, 16}, segs[] = {16, 32, 64};
+    // fewer persistent SDF blocks per CU leave room for the voxel / tunnel
+    // branch on the second stream, which otherwise waits for them to drain
+    const int steps[] = {1, 2, 4}, chunks[] = {16, 32}, bpc[] = {2, 3, 4, 5, 6, 8},
+              segs[] = {16, 32, 64, 128, 256};
     const ckd_tuning saved = ctx->tuning;
     const bool timing = ctx->timing;
     ctx->timing = false;

@@ -492,6 +492,21 @@ ckd_status launch_sdf(ckd_ctx *ctx, uint32_t *dst, unsigned W, unsigned H,
     return check_launch("sdf");
 }
 
+// blocks of one persistent kernel variant that fit on a CU at once
+template <bool kHalf, int kMaxSph>
+int persistent_blocks_per_cu(int steps)
+{
+    int n = 0;
+    hipError_t e;
+    if (steps <= 1)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_sdf_persistent<kHalf, kMaxSph, 1>, 256, 0);
+    else if (steps == 2)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_sdf_persistent<kHalf, kMaxSph, 2>, 256, 0);
+    else
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_sdf_persistent<kHalf, kMaxSph, 4>, 256, 0);
+    return e == hipSuccess ? n : 0;
+}
+
 template <bool kHalf, int kMaxSph>
 void launch_persistent_steps(int steps, unsigned blocks, hipStream_t st, uint32_t *queueCounter,
                              uint32_t totalChunks, const SdfJob *jobs, int nJobs,
@@ -507,19 +522,30 @@ void launch_persistent_steps(int steps, unsigned blocks, hipStream_t st, uint32_
 
 ckd_status launch_sdf_persistent(ckd_ctx *ctx, uint32_t *queueCounter, uint32_t totalChunks,
                                  const SdfJob *jobs, int nJobs, const ckd_sdf_scene &sc,
-                                 int shading, unsigned long long *stepCount)
+                                 int shading, unsigned long long *stepCount, bool helper)
 {
     if (totalChunks == 0) return CKD_OK;
-    // persistent: enough 4-wave blocks to fill every CU several times over;
-    // surplus blocks find the queue empty and exit
-    unsigned blocks = (unsigned)ctx->cuCount * (unsigned)ctx->tuning.sdfBlocksPerCU;
-    const unsigned useful = (totalChunks + kWavesPerBlock - 1) / kWavesPerBlock;
-    if (blocks > useful) blocks = useful;
-    if (blocks == 0) blocks = 1;
     const bool big = sc.nSpheres > 8;
     const int steps = ctx->tuning.sdfStepsPerRefill;
     const uint32_t chunk = (uint32_t)ctx->tuning.sdfChunk;
-    KernelTimer kt(ctx, CKD_KT_SDF_PERSISTENT);
+    // persistent: sdfBlocksPerCU 4-wave blocks per CU; surplus blocks find
+    // the queue empty and exit.  A helper launch (the batch's second stream,
+    // once its voxel and tunnel work is done) adds the blocks that fill each
+    // CU up to the kernel's occupancy and drains the same queue.
+    int perCU = ctx->tuning.sdfBlocksPerCU;
+    if (helper) {
+        const int fit = shading == CKD_SHADE_FP16
+                            ? (big ? persistent_blocks_per_cu<true, 32>(steps) : persistent_blocks_per_cu<true, 8>(steps))
+                            : (big ? persistent_blocks_per_cu<false, 32>(steps) : persistent_blocks_per_cu<false, 8>(steps));
+        perCU = fit - perCU;
+        if (perCU <= 0) return CKD_OK;
+    }
+    unsigned blocks = (unsigned)ctx->cuCount * (unsigned)perCU;
+    const unsigned useful = (totalChunks + kWavesPerBlock - 1) / kWavesPerBlock;
+    if (blocks > useful) blocks = useful;
+    if (blocks == 0) blocks = 1;
+    // only the main launch is timed: it spans the whole march of the batch
+    KernelTimer kt(ctx, helper ? -1 : CKD_KT_SDF_PERSISTENT);
     if (shading == CKD_SHADE_FP16) {
         if (big) launch_persistent_steps<true, 32>(steps, blocks, ctx->stream, queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk);
         else launch_persistent_steps<true, 8>(steps, blocks, ctx->stream, queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk);

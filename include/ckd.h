@@ -254,7 +254,7 @@ typedef struct ckd_tuning {
     int sdfStepsPerRefill; /* march steps between lane refills: 1, 2 or 4   */
     int sdfChunk;          /* side of the square pixel chunks: 16 or 32     */
     int sdfBlocksPerCU;    /* persistent 4-wave blocks per CU: 1..32         */
-    int tunnelVSeg;        /* rows per lane in the vertical blur: 8..128     */
+    int tunnelVSeg;        /* rows per lane in the vertical blur: 8..512     */
 } ckd_tuning;
 CKD_API ckd_status ckd_get_tuning(ckd_ctx *ctx, ckd_tuning *out);
 CKD_API ckd_status ckd_set_tuning(ckd_ctx *ctx, const ckd_tuning *t);
