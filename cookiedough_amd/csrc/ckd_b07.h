@@ -1,0 +1,31 @@
+// ckd_b07.h -- the 2007 box blur's parameters (code/deprecated/boxblur.cpp:
+// 52-76) and its batched in-place horizontal form, for the effects whose
+// Draw functions blur their render target in place.
+#pragma once
+
+#include "ckd_internal.h"
+
+namespace ckd {
+
+struct B07Params {
+    uint32_t span, edgeSpan, median, shift, subEdges;
+    uint32_t fullDiv;
+    uint16_t edgeDivs[256];
+};
+
+// one image of a batched in-place horizontal blur
+struct B07Job {
+    uint32_t *img;
+    B07Params p;
+};
+
+// the job of HorizontalBoxBlur32(img, img, W, H, strength); the checks of
+// ckd_blur07_h
+ckd_status b07_h_job(uint32_t *img, unsigned W, unsigned H, float strength, B07Job &job);
+
+// the in-place horizontal blurs of n jobs already in device memory, all on
+// W x H images (W <= 16383), one launch; vec4: every image 16-byte aligned
+ckd_status blur07_h_inplace_jobs(ckd_ctx *ctx, const B07Job *devJobs, unsigned n, unsigned W,
+                                 unsigned H, bool vec4);
+
+} // namespace ckd

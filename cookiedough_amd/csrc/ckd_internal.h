@@ -51,6 +51,9 @@ struct ckd_ctx {
     uint32_t *ballImages = nullptr;
     unsigned ballW = 0, ballH = 0;
     uint32_t ballTopHeight = 0;
+    // batched ball / twister Draw: per-frame render targets and map mixes
+    uint8_t *v2Scratch = nullptr;
+    size_t v2Cap = 0;
 
     // landscape maps (Landscape_Create)
     uint8_t *lsHeight = nullptr;

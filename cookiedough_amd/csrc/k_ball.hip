@@ -11,6 +11,8 @@
 // (map heights above 1279*256/radius), and such frames are rejected.
 #include "ckd_internal.h"
 #include "ckd_pixel.h"
+#include "ckd_blend.h"
+#include "ckd_b07.h"
 
 #include <cmath>
 #include <cstring>
@@ -171,8 +173,10 @@ struct TwFrame {
 };
 
 // vtwister_ray (code/torus-twister.cpp:39-85) for both rays of every row:
-// wave 2*row+side; side 0 walks right from the centre, side 1 left
-__global__ __launch_bounds__(256) void k_twister(uint32_t *__restrict__ target, unsigned W,
+// wave 2*row+side; side 0 walks right from the centre, side 1 left.
+// blockIdx.y is the frame: its rows and its render target (W x H, frames
+// back to back from `targets`)
+__global__ __launch_bounds__(256) void k_twister(uint32_t *__restrict__ targets, unsigned W,
                                                  unsigned H, const TwFrame *__restrict__ rows,
                                                  const uint32_t *__restrict__ tables,
                                                  const uint8_t *__restrict__ hmap,
@@ -181,6 +185,8 @@ __global__ __launch_bounds__(256) void k_twister(uint32_t *__restrict__ target, 
     const unsigned lane = threadIdx.x & (kWave - 1);
     const unsigned ray = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     if (ray >= 2 * H) return; // whole waves
+    uint32_t *target = targets + (size_t)blockIdx.y * W * H;
+    rows += (size_t)blockIdx.y * H;
     const unsigned row = ray >> 1, side = ray & 1;
     const uint32_t *hp = tables, *hn = tables + kTwRay;
     const uint32_t dX = side ? (uint32_t)-512 : 512u;
@@ -216,19 +222,34 @@ struct BallFrame {
     int32_t norm[kBallMaxRay][3];
 };
 
-// vball_ray_beams / vball_ray_no_beams (code/ball.cpp:77-261), wave per row
+// one frame of a batched Draw: its render target, map mixes and the index
+// of its tables (BallFrame, fan deltas) in the staged batch
+struct BallJob {
+    uint32_t *target;
+    const uint8_t *hmapMix;
+    const uint32_t *cmap;
+    const uint32_t *aux; // the beam map mix, or the environment map
+    uint32_t frame;
+};
+
+// vball_ray_beams / vball_ray_no_beams (code/ball.cpp:77-261), wave per row;
+// blockIdx.y picks the job.  Without beams the pixels past the ray's last
+// span are cleared (the reference clears the whole target first).
 template <bool kBeams>
-__global__ __launch_bounds__(256) void k_ball(uint32_t *__restrict__ target, unsigned W,
-                                              unsigned H, const BallFrame *__restrict__ f,
-                                              const int2 *__restrict__ dXY,
-                                              const uint8_t *__restrict__ hmapMix,
-                                              const uint32_t *__restrict__ cmap,
-                                              const uint32_t *__restrict__ aux)
+__global__ __launch_bounds__(256) void k_ball(const BallJob *__restrict__ jobs, unsigned W,
+                                              unsigned H, const BallFrame *__restrict__ frames,
+                                              const int2 *__restrict__ fanDeltas)
 {
     const unsigned lane = threadIdx.x & (kWave - 1);
     const unsigned row = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     if (row >= H) return; // whole waves
-    uint32_t *dst = target + (size_t)row * W;
+    const BallJob &J = jobs[blockIdx.y];
+    const BallFrame *__restrict__ f = frames + J.frame;
+    const int2 *__restrict__ dXY = fanDeltas + (size_t)J.frame * H;
+    const uint8_t *__restrict__ hmapMix = J.hmapMix;
+    const uint32_t *__restrict__ cmap = J.cmap;
+    const uint32_t *__restrict__ aux = J.aux;
+    uint32_t *dst = J.target + (size_t)row * W;
     const uint32_t dX = (uint32_t)dXY[row].x, dY = (uint32_t)dXY[row].y;
     const uint32_t fromX = (uint32_t)f->fromX, fromY = (uint32_t)f->fromY;
     const uint32_t env0 = (1024u >> 1) << 8;
@@ -287,6 +308,9 @@ __global__ __launch_bounds__(256) void k_ball(uint32_t *__restrict__ target, uns
         }
         const unsigned last = (L - i0 - 1) < (kWave - 1) ? (L - i0 - 1) : (kWave - 1);
         chunk_spans(dst, 1, lane, last, active, height, color, c);
+    }
+    if constexpr (!kBeams) {
+        for (uint32_t x = c.drawn + lane; x < W; x += kWave) dst[x] = 0u;
     }
     if constexpr (kBeams) {
         // beam extrusion (code/ball.cpp:176-204): packus of the 16-bit sums
@@ -348,6 +372,94 @@ __global__ __launch_bounds__(256) void k_polar_blit_a(uint32_t *__restrict__ dst
         r |= ((dk * (256 - a) + sk * a) >> 8) << (8 * k);
     }
     dst[i] = r;
+}
+
+// ---- batched Draw stages ----------------------------------------------------
+
+// the per-frame map mixes of Ball_Draw (code/ball.cpp:448-472): the height
+// map mix (Mix32 of the spike map over the base shape, four bytes per
+// pixel) and, with beams, the beam map mix (memset32, then BlitAdd32A of
+// each beam map with a non-zero alpha)
+struct BallMix {
+    const uint32_t *base; // height map 1..4 as 32-bit quads
+    uint32_t *hmapMix;
+    uint32_t spikes;      // Mix32 alpha; 0: copy
+    uint32_t *beamMix;    // null without beams
+    uint32_t lanes[3];    // 0x01010101 * unsigned(alpha*255) per beam map; 0: skipped
+};
+
+__global__ __launch_bounds__(256) void k_ball_mix(const BallMix *__restrict__ mixes,
+                                                  const uint32_t *__restrict__ spikeMap,
+                                                  const uint32_t *__restrict__ beamMaps,
+                                                  unsigned mapPx)
+{
+    const BallMix &M = mixes[blockIdx.y];
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= mapPx) return;
+    if (i < mapPx / 4) {
+        const uint32_t b = M.base[i];
+        M.hmapMix[i] = M.spikes ? blend_px<OP_MIX>(b, spikeMap[i], M.spikes, 0) : b;
+    }
+    if (M.beamMix) {
+        uint32_t acc = 0;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            if (!M.lanes[m]) continue;
+            const uint32_t s = beamMaps[(size_t)m * mapPx + i];
+            uint32_t t = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                t |= ((((s >> (8 * k)) & 0xff) * ((M.lanes[m] >> (8 * k)) & 0xff)) >> 8) << (8 * k);
+            acc = ckd_adds_u8x4(acc, t);
+        }
+        M.beamMix[i] = acc;
+    }
+}
+
+// the end of Ball_Draw / Twister_Draw: the background, then Polar_BlitA of
+// the render target (the forward map), then SoftLight32A of the halo when
+// halo != 0 (code/ball.cpp:489-502, code/torus-twister.cpp:177-183)
+struct ComposeJob {
+    const uint32_t *target;
+    uint32_t *out;
+    const uint32_t *background;
+    uint32_t halo;
+};
+
+__global__ __launch_bounds__(256) void k_compose(const ComposeJob *__restrict__ jobs,
+                                                 const int2 *__restrict__ map,
+                                                 const uint32_t *__restrict__ halo, unsigned W,
+                                                 unsigned H)
+{
+    const ComposeJob &J = jobs[blockIdx.z];
+    const unsigned x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= W || y >= H) return;
+    const size_t i = (size_t)y * W + x;
+    const uint32_t *src = J.target;
+    const int2 uv = map[i];
+    const unsigned U0 = (unsigned)(uv.x >> 8), V0 = (unsigned)(uv.y >> 8) * W;
+    const uint32_t s = ckd_bilerp32(src[U0 + V0], src[U0 + 1 + V0], src[U0 + V0 + W],
+                                    src[U0 + 1 + V0 + W], (uint32_t)uv.x & 0xff,
+                                    (uint32_t)uv.y & 0xff);
+    const uint32_t a = s >> 24, d = J.background[i];
+    uint32_t r = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t dk = (d >> (8 * k)) & 0xff, sk = (s >> (8 * k)) & 0xff;
+        r |= ((dk * (256 - a) + sk * a) >> 8) << (8 * k);
+    }
+    if (J.halo) r = blend_px<OP_SOFTLIGHT_A>(r, halo[i], 0, 0);
+    J.out[i] = r;
+}
+
+// dst[y*dstStride] = src[y*srcStride] for y < H: the render target's last
+// column, which the beam rays leave as it was
+__global__ __launch_bounds__(256) void k_copy_column(uint32_t *__restrict__ dst, size_t dstStride,
+                                                     const uint32_t *__restrict__ src,
+                                                     size_t srcStride, unsigned H)
+{
+    const unsigned y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y < H) dst[(size_t)y * dstStride] = src[(size_t)y * srcStride];
 }
 
 // ---- host: the reference's FP32 set-up -------------------------------------
@@ -432,10 +544,58 @@ ckd_status ensure_polar(ckd_ctx *ctx, unsigned W, unsigned H)
     return ckd_polar_create(ctx, W, H);
 }
 
+// frames a batched Draw renders together (each with its own render target)
+constexpr unsigned kGroup = 32;
+
+// the context's scratch for batched Draws, grown on demand
+ckd_status v2_scratch(ckd_ctx *ctx, size_t bytes, uint32_t *&out)
+{
+    if (bytes > ctx->v2Cap) {
+        // queued work may still use the old buffer
+        CKD_HIP(hipStreamSynchronize(ctx->stream));
+        if (ctx->v2Scratch) (void)hipFree(ctx->v2Scratch);
+        ctx->v2Scratch = nullptr;
+        ctx->v2Cap = 0;
+        if (hipMalloc(&ctx->v2Scratch, bytes) != hipSuccess) {
+            ctx->v2Scratch = nullptr;
+            return ckd::fail(CKD_ENOMEM, "batched Draw: cannot allocate %zu bytes of scratch", bytes);
+        }
+        ctx->v2Cap = bytes;
+    }
+    out = reinterpret_cast<uint32_t *>(ctx->v2Scratch);
+    return CKD_OK;
+}
+
+// the host image of one group's tables, staged with one copy: sections at
+// 256-byte offsets
+struct Blob {
+    std::vector<uint8_t> bytes;
+    void clear() { bytes.clear(); }
+    size_t add(size_t n)
+    {
+        const size_t o = (bytes.size() + 255) & ~(size_t)255;
+        bytes.resize(o + n);
+        return o;
+    }
+    size_t put(const void *src, size_t n)
+    {
+        const size_t o = add(n);
+        if (n) memcpy(bytes.data() + o, src, n);
+        return o;
+    }
+    template <typename T> T *at(size_t o) { return reinterpret_cast<T *>(bytes.data() + o); }
+    ckd_status stage(ckd_ctx *ctx, uint8_t *&dev)
+    {
+        void *d = nullptr;
+        ckd_status s = ckd::stage(ctx, bytes.data(), bytes.size(), &d);
+        dev = static_cast<uint8_t *>(d);
+        return s;
+    }
+};
+
 } // namespace
 
 namespace ckd {
-ckd_status comp_mix32(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, unsigned n, uint32_t alpha);
 ckd_status launch_memset32(ckd_ctx *ctx, uint32_t *dst, uint32_t value, size_t n);
 } // namespace ckd
 
@@ -501,35 +661,62 @@ ckd_status ckd_twister_draw(ckd_ctx *ctx, uint32_t *dst, uint32_t *target, unsig
     CKD_REQUIRE_CTX(ctx);
     CKD_REQUIRE(ctx->twHeight && ctx->twW == W && ctx->twH == H,
                 "ckd_twister_draw: no twister maps for %ux%u (ckd_twister_create)", W, H);
+    if (nFrames == 0) return CKD_OK;
     ckd_status s = ensure_polar(ctx, W, H);
     if (s != CKD_OK) return s;
+    const int2 *map = reinterpret_cast<const int2 *>(ckd::polar_maps(ctx, W, H)->map);
     const size_t px = (size_t)W * H;
-    std::vector<TwFrame> rows(H);
-    for (unsigned k = 0; k < nFrames; ++k) {
-        // vtwister (code/torus-twister.cpp:89-116)
-        const ckd_twister_params &p = frames[k];
-        const float fMapSizeHH = (1024.f * 0.5f) - 0.5f, fMapSizeHHH = (1024.f * 0.25f) - 0.5f;
-        const float mapStepY = 1024.f / (float)(H - 1), shearStep = (2.f * kPI) / (float)(H - 1);
-        for (unsigned r = 0; r < H; ++r) {
-            const float shearAngle = (float)r * shearStep;
-            rows[r].fromX = fp24(fMapSizeHH + fMapSizeHHH * sinf(p.time * p.shearSpeed + shearAngle));
-            rows[r].fromY = fp24((float)r * mapStepY + p.time * p.speed);
+    const unsigned G = nFrames < kGroup ? nFrames : kGroup;
+    uint32_t *tg = nullptr;
+    if ((s = v2_scratch(ctx, G * px * 4, tg)) != CKD_OK) return s;
+    Blob blob;
+    unsigned g = 0;
+    for (unsigned f0 = 0; f0 < nFrames; f0 += g) {
+        g = nFrames - f0 < G ? nFrames - f0 : G;
+        // Twister_Draw (code/torus-twister.cpp:164-187) of g frames, each in
+        // its own render target: the staged tables, then one launch per stage
+        blob.clear();
+        const size_t oRows = blob.add(sizeof(TwFrame) * H * g);
+        std::vector<ckd::B07Job> blurs;
+        std::vector<ComposeJob> comp(g);
+        for (unsigned j = 0; j < g; ++j) {
+            // vtwister (code/torus-twister.cpp:89-116)
+            const ckd_twister_params &p = frames[f0 + j];
+            TwFrame *rows = blob.at<TwFrame>(oRows) + (size_t)j * H;
+            const float fMapSizeHH = (1024.f * 0.5f) - 0.5f, fMapSizeHHH = (1024.f * 0.25f) - 0.5f;
+            const float mapStepY = 1024.f / (float)(H - 1), shearStep = (2.f * kPI) / (float)(H - 1);
+            for (unsigned r = 0; r < H; ++r) {
+                const float shearAngle = (float)r * shearStep;
+                rows[r].fromX = fp24(fMapSizeHH + fMapSizeHHH * sinf(p.time * p.shearSpeed + shearAngle));
+                rows[r].fromY = fp24((float)r * mapStepY + p.time * p.speed);
+            }
+            if (p.blur != 0.f) {
+                ckd::B07Job job;
+                if ((s = ckd::b07_h_job(tg + j * px, W, H, box_blur_scale(p.blur), job)) != CKD_OK) return s;
+                blurs.push_back(job);
+            }
+            comp[j] = ComposeJob{tg + j * px, dst + (f0 + j) * px, ctx->twBackground, 0u};
         }
-        void *dRows = nullptr;
-        if ((s = ckd::stage(ctx, rows.data(), sizeof(TwFrame) * H, &dRows)) != CKD_OK) return s;
-        // Twister_Draw (code/torus-twister.cpp:164-187)
-        if ((s = ckd::launch_memset32(ctx, target, 0, px)) != CKD_OK) return s;
-        k_twister<<<ckd_div_up((size_t)2 * H, 256 / kWave), 256, 0, ctx->stream>>>(
-            target, W, H, static_cast<const TwFrame *>(dRows), ctx->twTables, ctx->twHeight,
+        const size_t oBlur = blob.put(blurs.data(), sizeof(ckd::B07Job) * blurs.size());
+        const size_t oComp = blob.put(comp.data(), sizeof(ComposeJob) * g);
+        uint8_t *dev = nullptr;
+        if ((s = blob.stage(ctx, dev)) != CKD_OK) return s;
+        if ((s = ckd::launch_memset32(ctx, tg, 0, g * px)) != CKD_OK) return s;
+        k_twister<<<dim3(ckd_div_up((size_t)2 * H, 256 / kWave), g), 256, 0, ctx->stream>>>(
+            tg, W, H, reinterpret_cast<const TwFrame *>(dev + oRows), ctx->twTables, ctx->twHeight,
             ctx->twColor);
         if ((s = ckd::check_launch("twister")) != CKD_OK) return s;
-        if (p.blur != 0.f &&
-            (s = ckd_blur07_h(ctx, target, target, W, H, box_blur_scale(p.blur))) != CKD_OK)
+        if (!blurs.empty() &&
+            (s = ckd::blur07_h_inplace_jobs(ctx, reinterpret_cast<const ckd::B07Job *>(dev + oBlur),
+                                            (unsigned)blurs.size(), W, H, px % 4 == 0)) != CKD_OK)
             return s;
-        uint32_t *out = dst + px * k;
-        CKD_HIP(hipMemcpyAsync(out, ctx->twBackground, px * 4, hipMemcpyDeviceToDevice, ctx->stream));
-        if ((s = ckd_polar_blit_a(ctx, out, target, W, H, 0)) != CKD_OK) return s;
+        k_compose<<<dim3(ckd_div_up(W, 256), H, g), 256, 0, ctx->stream>>>(
+            reinterpret_cast<const ComposeJob *>(dev + oComp), map, nullptr, W, H);
+        if ((s = ckd::check_launch("twister compose")) != CKD_OK) return s;
     }
+    // the render target ends as the last frame left it
+    CKD_HIP(hipMemcpyAsync(target, tg + (size_t)(g - 1) * px, px * 4, hipMemcpyDeviceToDevice,
+                           ctx->stream));
     return CKD_OK;
 }
 
@@ -592,63 +779,135 @@ ckd_status ckd_ball_draw(ckd_ctx *ctx, uint32_t *dst, uint32_t *target, unsigned
                     "ckd_ball_draw: frame %u: radius %g with map heights up to %u reaches past a "
                     "%u-pixel row", k, (double)radius, ctx->ballTopHeight, W);
     }
+    if (nFrames == 0) return CKD_OK;
     ckd_status s = ensure_polar(ctx, W, H);
     if (s != CKD_OK) return s;
+    const int2 *map = reinterpret_cast<const int2 *>(ckd::polar_maps(ctx, W, H)->map);
     const size_t px = (size_t)W * H;
-    uint8_t *heights = ctx->ballMaps, *hmapMix = heights + 5 * kMapPx;
-    uint32_t *cmaps = reinterpret_cast<uint32_t *>(heights + 6 * kMapPx);
-    uint32_t *beamMaps = cmaps + 2 * kMapPx, *beamMix = cmaps + 5 * kMapPx, *env = cmaps + 6 * kMapPx;
+    const uint8_t *heights = ctx->ballMaps;
+    const uint32_t *cmaps = reinterpret_cast<const uint32_t *>(heights + 6 * kMapPx);
+    const uint32_t *beamMaps = cmaps + 2 * kMapPx, *env = cmaps + 6 * kMapPx;
     const uint32_t *backgrounds = ctx->ballImages, *halo = ctx->ballImages + 2 * px;
-    std::vector<uint8_t> host(sizeof(BallFrame) + sizeof(int2) * H);
+    // scratch per group of G frames: render targets, height map mixes, beam
+    // map mixes, and the carried last column
+    const unsigned G = nFrames < kGroup ? nFrames : kGroup;
+    const size_t tgBytes = G * px * 4, hmBytes = G * kMapPx, bmBytes = G * kMapPx * 4;
+    uint32_t *tg = nullptr;
+    if ((s = v2_scratch(ctx, tgBytes + hmBytes + bmBytes + (size_t)H * 4, tg)) != CKD_OK) return s;
+    uint8_t *hm = reinterpret_cast<uint8_t *>(tg) + tgBytes;
+    uint32_t *bm = reinterpret_cast<uint32_t *>(hm + hmBytes);
+    uint32_t *carry = bm + G * kMapPx;
+    // the column the first frame with beams keeps: the caller's target's
+    const unsigned colBlocks = ckd_div_up(H, 256);
+    k_copy_column<<<colBlocks, 256, 0, ctx->stream>>>(carry, 1, target + (W - 1), W, H);
+    if ((s = ckd::check_launch("ball column")) != CKD_OK) return s;
+    Blob blob;
     std::vector<int2> dXY;
-    for (unsigned k = 0; k < nFrames; ++k) {
-        const ckd_ball_params &p = frames[k];
-        // Ball_Draw (code/ball.cpp:442-511): the height and beam map mixes
-        const bool beams = geti(p.hasBeams) != 0;
-        const unsigned base = (unsigned)clampi_(1, 4, geti(p.baseShapeIndex));
-        CKD_HIP(hipMemcpyAsync(hmapMix, heights + base * kMapPx, kMapPx, hipMemcpyDeviceToDevice,
-                               ctx->stream));
-        const uint8_t spikes = (uint8_t)geti(p.spikes);
-        if (spikes != 0 &&
-            (s = ckd::comp_mix32(ctx, reinterpret_cast<uint32_t *>(hmapMix),
-                                 reinterpret_cast<const uint32_t *>(heights), (unsigned)(kMapPx / 4),
-                                 spikes)) != CKD_OK)
-            return s;
-        if (beams) {
-            if ((s = ckd::launch_memset32(ctx, beamMix, 0, kMapPx)) != CKD_OK) return s;
+    unsigned g = 0;
+    for (unsigned f0 = 0; f0 < nFrames; f0 += g) {
+        g = nFrames - f0 < G ? nFrames - f0 : G;
+        // Ball_Draw (code/ball.cpp:442-511) of g frames, each in its own
+        // render target
+        blob.clear();
+        const size_t oFrames = blob.add(sizeof(BallFrame) * g);
+        const size_t oFans = blob.add(sizeof(int2) * H * g);
+        std::vector<BallMix> mixes(g);
+        std::vector<BallJob> jobsBeams, jobsEnv;
+        std::vector<ckd::B07Job> blurEnv;
+        std::vector<ckd::B07Job> blurBeams; // per frame with beams, in frame order
+        std::vector<int> beamsBlur(g, -1);  // index into blurBeams
+        std::vector<ComposeJob> comp(g);
+        std::vector<bool> beams(g);
+        for (unsigned j = 0; j < g; ++j) {
+            const ckd_ball_params &p = frames[f0 + j];
+            beams[j] = geti(p.hasBeams) != 0;
+            const unsigned base = (unsigned)clampi_(1, 4, geti(p.baseShapeIndex));
+            BallMix &M = mixes[j];
+            M.base = reinterpret_cast<const uint32_t *>(heights + base * kMapPx);
+            M.hmapMix = reinterpret_cast<uint32_t *>(hm + j * kMapPx);
+            M.spikes = (uint8_t)geti(p.spikes);
+            M.beamMix = beams[j] ? bm + j * kMapPx : nullptr;
             const float a[3] = {p.beams1, p.beams2, p.beams3};
             for (int m = 0; m < 3; ++m) {
                 const float sa = smax(0.f, smin(1.f, a[m])); // saturatef
-                if (sa > 0.f &&
-                    (s = ckd_blit_add32a(ctx, beamMix, beamMaps + m * kMapPx, 1024, 1024, 1024, sa)) != CKD_OK)
-                    return s;
+                M.lanes[m] = sa > 0.f ? 0x01010101u * (uint32_t)(sa * 255.f) : 0u;
             }
+            // vball (code/ball.cpp:290-333)
+            ball_frame(p, H, blob.at<BallFrame>(oFrames)[j], dXY);
+            memcpy(blob.at<int2>(oFans) + (size_t)j * H, dXY.data(), sizeof(int2) * H);
+            uint32_t *t = tg + j * px;
+            const BallJob job{t, hm + j * kMapPx, beams[j] ? cmaps : cmaps + kMapPx,
+                              beams[j] ? bm + j * kMapPx : env, j};
+            (beams[j] ? jobsBeams : jobsEnv).push_back(job);
+            if (p.blur != 0.f) {
+                ckd::B07Job bj;
+                if ((s = ckd::b07_h_job(t, W, H, box_blur_scale(p.blur), bj)) != CKD_OK) return s;
+                if (beams[j]) {
+                    beamsBlur[j] = (int)blurBeams.size();
+                    blurBeams.push_back(bj);
+                } else {
+                    blurEnv.push_back(bj);
+                }
+            }
+            comp[j] = ComposeJob{t, dst + (f0 + j) * px, backgrounds + (beams[j] ? 0 : px),
+                                 beams[j] ? 1u : 0u};
         }
-        // vball (code/ball.cpp:290-333)
-        BallFrame *f = reinterpret_cast<BallFrame *>(host.data());
-        ball_frame(p, H, *f, dXY);
-        std::memcpy(host.data() + sizeof(BallFrame), dXY.data(), sizeof(int2) * H);
-        void *dHost = nullptr;
-        if ((s = ckd::stage(ctx, host.data(), host.size(), &dHost)) != CKD_OK) return s;
-        const BallFrame *df = static_cast<const BallFrame *>(dHost);
-        const int2 *ddXY = reinterpret_cast<const int2 *>(static_cast<uint8_t *>(dHost) + sizeof(BallFrame));
-        if (!beams && (s = ckd::launch_memset32(ctx, target, 0, px)) != CKD_OK) return s;
-        if (beams)
-            k_ball<true><<<ckd_div_up(H, 256 / kWave), 256, 0, ctx->stream>>>(target, W, H, df, ddXY, hmapMix,
-                                                                       cmaps, beamMix);
-        else
-            k_ball<false><<<ckd_div_up(H, 256 / kWave), 256, 0, ctx->stream>>>(target, W, H, df, ddXY, hmapMix,
-                                                                        cmaps + kMapPx, env);
-        if ((s = ckd::check_launch("ball")) != CKD_OK) return s;
-        if (p.blur != 0.f &&
-            (s = ckd_blur07_h(ctx, target, target, W, H, box_blur_scale(p.blur))) != CKD_OK)
+        const size_t oMix = blob.put(mixes.data(), sizeof(BallMix) * g);
+        const size_t oJB = blob.put(jobsBeams.data(), sizeof(BallJob) * jobsBeams.size());
+        const size_t oJE = blob.put(jobsEnv.data(), sizeof(BallJob) * jobsEnv.size());
+        const size_t oBE = blob.put(blurEnv.data(), sizeof(ckd::B07Job) * blurEnv.size());
+        const size_t oBB = blob.put(blurBeams.data(), sizeof(ckd::B07Job) * blurBeams.size());
+        const size_t oComp = blob.put(comp.data(), sizeof(ComposeJob) * g);
+        uint8_t *dev = nullptr;
+        if ((s = blob.stage(ctx, dev)) != CKD_OK) return s;
+        const BallFrame *dFrames = reinterpret_cast<const BallFrame *>(dev + oFrames);
+        const int2 *dFans = reinterpret_cast<const int2 *>(dev + oFans);
+        k_ball_mix<<<dim3(ckd_div_up(kMapPx, 256), g), 256, 0, ctx->stream>>>(
+            reinterpret_cast<const BallMix *>(dev + oMix),
+            reinterpret_cast<const uint32_t *>(heights), beamMaps, (unsigned)kMapPx);
+        if ((s = ckd::check_launch("ball mix")) != CKD_OK) return s;
+        const unsigned rowBlocks = ckd_div_up(H, 256 / kWave);
+        if (!jobsBeams.empty()) {
+            k_ball<true><<<dim3(rowBlocks, (unsigned)jobsBeams.size()), 256, 0, ctx->stream>>>(
+                reinterpret_cast<const BallJob *>(dev + oJB), W, H, dFrames, dFans);
+            if ((s = ckd::check_launch("ball")) != CKD_OK) return s;
+        }
+        if (!jobsEnv.empty()) {
+            k_ball<false><<<dim3(rowBlocks, (unsigned)jobsEnv.size()), 256, 0, ctx->stream>>>(
+                reinterpret_cast<const BallJob *>(dev + oJE), W, H, dFrames, dFans);
+            if ((s = ckd::check_launch("ball")) != CKD_OK) return s;
+        }
+        const bool vec4 = px % 4 == 0;
+        // frames without beams cleared their targets: their blurs go together
+        if (!blurEnv.empty() &&
+            (s = ckd::blur07_h_inplace_jobs(ctx, reinterpret_cast<const ckd::B07Job *>(dev + oBE),
+                                            (unsigned)blurEnv.size(), W, H, vec4)) != CKD_OK)
             return s;
-        uint32_t *out = dst + px * k;
-        CKD_HIP(hipMemcpyAsync(out, backgrounds + (beams ? 0 : px), px * 4, hipMemcpyDeviceToDevice,
-                               ctx->stream));
-        if ((s = ckd_polar_blit_a(ctx, out, target, W, H, 0)) != CKD_OK) return s;
-        if (beams && (s = ckd_softlight32a(ctx, out, halo, (unsigned)px)) != CKD_OK) return s;
+        // a frame with beams keeps its predecessor's last column (after that
+        // frame's blur), so these go in frame order
+        for (unsigned j = 0; j < g; ++j) {
+            if (!beams[j]) continue;
+            const uint32_t *src = j == 0 ? carry : tg + (j - 1) * px + (W - 1);
+            k_copy_column<<<colBlocks, 256, 0, ctx->stream>>>(tg + j * px + (W - 1), W, src,
+                                                             j == 0 ? 1 : W, H);
+            if ((s = ckd::check_launch("ball column")) != CKD_OK) return s;
+            if (beamsBlur[j] >= 0 &&
+                (s = ckd::blur07_h_inplace_jobs(
+                     ctx, reinterpret_cast<const ckd::B07Job *>(dev + oBB) + beamsBlur[j], 1, W, H,
+                     vec4)) != CKD_OK)
+                return s;
+        }
+        k_compose<<<dim3(ckd_div_up(W, 256), H, g), 256, 0, ctx->stream>>>(
+            reinterpret_cast<const ComposeJob *>(dev + oComp), map, halo, W, H);
+        if ((s = ckd::check_launch("ball compose")) != CKD_OK) return s;
+        // the next group's first frame keeps this group's last column
+        k_copy_column<<<colBlocks, 256, 0, ctx->stream>>>(carry, 1, tg + (g - 1) * px + (W - 1), W,
+                                                         H);
+        if ((s = ckd::check_launch("ball column")) != CKD_OK) return s;
     }
+    // the render target ends as the last frame left it
+    CKD_HIP(hipMemcpyAsync(target, tg + (size_t)(g - 1) * px, px * 4, hipMemcpyDeviceToDevice,
+                           ctx->stream));
     return CKD_OK;
 }
 

@@ -20,14 +20,14 @@
 // for LDS, reading the image directly (k_blur07_inplace).
 #include "ckd_internal.h"
 #include "ckd_pixel.h"
+#include "ckd_b07.h"
+
+#include <vector>
 
 namespace {
 
-struct B07Params {
-    uint32_t span, edgeSpan, median, shift, subEdges;
-    uint32_t fullDiv;
-    uint16_t edgeDivs[256];
-};
+using ckd::B07Job;
+using ckd::B07Params;
 
 // code/deprecated/boxblur.cpp:11-14 (stored through _mm_set1_epi16)
 uint16_t weight_to_div(unsigned weight) { return (uint16_t)(((65536u * 256u) / weight) >> 4); }
@@ -469,18 +469,17 @@ CKD_DEV void b07_stage(uint32_t *img, uint32_t *lds, int line0, int nl, int len,
     }
 }
 
+// one block: NL lines of img, staged, blurred in place, written back
 template <int NL>
-__global__ __launch_bounds__(64) void k_blur07_lines(uint32_t *img, int nLines, int len,
-                                                     size_t lineStride, size_t pixStride,
-                                                     int vec4, B07Params p)
+CKD_DEV void b07_lines_block(uint32_t *img, int nLines, int len, size_t lineStride,
+                             size_t pixStride, bool vec4, const B07Params &p, uint32_t *lineBuf,
+                             uint16_t *edgeDivs)
 {
-    extern __shared__ uint32_t lineBuf[];
-    __shared__ uint16_t edgeDivs[256]; // read per step in the edge phases
     const int stride = len + 1;
     const int line0 = blockIdx.x * NL;
     const int nl = min(NL, nLines - line0);
     for (int i = threadIdx.x; i < 256; i += 64) edgeDivs[i] = p.edgeDivs[i];
-    b07_stage<false>(img, lineBuf, line0, nl, len, stride, lineStride, pixStride, vec4 != 0);
+    b07_stage<false>(img, lineBuf, line0, nl, len, stride, lineStride, pixStride, vec4);
     __syncthreads();
     const int l = threadIdx.x >> 2, k = threadIdx.x & 3;
     if (l < nl) {
@@ -489,7 +488,29 @@ __global__ __launch_bounds__(64) void k_blur07_lines(uint32_t *img, int nLines, 
         else b07_lds_line<false>(px, len, p, edgeDivs);
     }
     __syncthreads();
-    b07_stage<true>(img, lineBuf, line0, nl, len, stride, lineStride, pixStride, vec4 != 0);
+    b07_stage<true>(img, lineBuf, line0, nl, len, stride, lineStride, pixStride, vec4);
+}
+
+template <int NL>
+__global__ __launch_bounds__(64) void k_blur07_lines(uint32_t *img, int nLines, int len,
+                                                     size_t lineStride, size_t pixStride,
+                                                     int vec4, B07Params p)
+{
+    extern __shared__ uint32_t lineBuf[];
+    __shared__ uint16_t edgeDivs[256]; // read per step in the edge phases
+    b07_lines_block<NL>(img, nLines, len, lineStride, pixStride, vec4 != 0, p, lineBuf, edgeDivs);
+}
+
+// the in-place horizontal blur of several images of one size, each with
+// its own strength: blockIdx.y picks the image
+template <int NL>
+__global__ __launch_bounds__(64) void k_blur07_lines_jobs(const B07Job *__restrict__ jobs,
+                                                          int nLines, int len, int vec4)
+{
+    extern __shared__ uint32_t lineBuf[];
+    __shared__ uint16_t edgeDivs[256];
+    const B07Job &J = jobs[blockIdx.y];
+    b07_lines_block<NL>(J.img, nLines, len, (size_t)len, 1, vec4 != 0, J.p, lineBuf, edgeDivs);
 }
 
 ckd_status validate(const char *fn, const uint32_t *dst, const uint32_t *src, unsigned xRes,
@@ -575,6 +596,31 @@ ckd_status launch_v(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, unsigned x
 } // namespace
 
 namespace ckd {
+
+ckd_status b07_h_job(uint32_t *img, unsigned W, unsigned H, float strength, B07Job &job)
+{
+    job.img = img;
+    return validate("blur07_h (in place)", img, img, W, H, strength, W, job.p);
+}
+
+ckd_status blur07_h_inplace_jobs(ckd_ctx *ctx, const B07Job *devJobs, unsigned n, unsigned W,
+                                 unsigned H, bool vec4)
+{
+    if (n == 0) return CKD_OK;
+    const int nl = lines_per_block((int)W);
+    if (nl == 0) return fail(CKD_EINVAL, "blur07_h (in place, batch): width %u above 16383", W);
+    const size_t lds = (size_t)nl * (size_t)(W + 1) * 4;
+    const int v4 = vec4 && (W % 4) == 0;
+    const dim3 grid(ckd_div_up((size_t)H, (size_t)nl), n);
+    switch (nl) {
+    case 16: k_blur07_lines_jobs<16><<<grid, 64, lds, ctx->stream>>>(devJobs, (int)H, (int)W, v4); break;
+    case 8: k_blur07_lines_jobs<8><<<grid, 64, lds, ctx->stream>>>(devJobs, (int)H, (int)W, v4); break;
+    case 4: k_blur07_lines_jobs<4><<<grid, 64, lds, ctx->stream>>>(devJobs, (int)H, (int)W, v4); break;
+    case 2: k_blur07_lines_jobs<2><<<grid, 64, lds, ctx->stream>>>(devJobs, (int)H, (int)W, v4); break;
+    default: k_blur07_lines_jobs<1><<<grid, 64, lds, ctx->stream>>>(devJobs, (int)H, (int)W, v4); break;
+    }
+    return check_launch("blur07_h (in place, batch)");
+}
 
 // used by the tunnel pipeline (span 2r+1, out of place)
 ckd_status blur07_hv_oop(ckd_ctx *ctx, uint32_t *tmp, uint32_t *dst, const uint32_t *src,

@@ -216,7 +216,7 @@ void ckd_destroy(ckd_ctx *ctx)
                     ctx->tsFog, ctx->b26Scratch[0],
                     ctx->b26Scratch[1], ctx->staging, ctx->fdTex, ctx->fdTexHi,
                     ctx->stScratch, ctx->spikeBlurMaps, ctx->twHeight, ctx->twColor,
-                    ctx->twTables, ctx->twBackground, ctx->ballMaps, ctx->ballImages};
+                    ctx->twTables, ctx->twBackground, ctx->ballMaps, ctx->ballImages, ctx->v2Scratch};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete ctx;
