@@ -214,12 +214,18 @@ __global__ __launch_bounds__(256) void k_twister(uint32_t *__restrict__ targets,
 
 // ---- the ball ------------------------------------------------------------
 
+// the per-step tables of one radius and ray length (vball_precalc)
+struct BallRays {
+    uint32_t heightProj[kBallMaxRay];
+    int32_t norm[kBallMaxRay][3];
+};
+
+// one frame's constants; `rays` indexes its BallRays in the staged batch
 struct BallFrame {
     unsigned rayLength, beamAtten, lowLight;
     float beamAlphaMin;
     int fromX, fromY;
-    uint32_t heightProj[kBallMaxRay];
-    int32_t norm[kBallMaxRay][3];
+    unsigned rays;
 };
 
 // one frame of a batched Draw: its render target, map mixes and the index
@@ -238,14 +244,15 @@ struct BallJob {
 template <bool kBeams>
 __global__ __launch_bounds__(256) void k_ball(const BallJob *__restrict__ jobs, unsigned W,
                                               unsigned H, const BallFrame *__restrict__ frames,
-                                              const int2 *__restrict__ fanDeltas)
+                                              const BallRays *__restrict__ rays,
+                                              const int2 *__restrict__ dXY)
 {
     const unsigned lane = threadIdx.x & (kWave - 1);
     const unsigned row = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     if (row >= H) return; // whole waves
     const BallJob &J = jobs[blockIdx.y];
     const BallFrame *__restrict__ f = frames + J.frame;
-    const int2 *__restrict__ dXY = fanDeltas + (size_t)J.frame * H;
+    const BallRays *__restrict__ R = rays + f->rays;
     const uint8_t *__restrict__ hmapMix = J.hmapMix;
     const uint32_t *__restrict__ cmap = J.cmap;
     const uint32_t *__restrict__ aux = J.aux;
@@ -271,8 +278,8 @@ __global__ __launch_bounds__(256) void k_ball(const BallJob *__restrict__ jobs, 
             samp16(cmap, t, color);
             if constexpr (kBeams) {
                 samp16(aux, t, extra); // beam map
-                const uint32_t hi = h * (uint32_t)f->norm[i][0] >> 8;
-                const uint32_t lo = h * (uint32_t)f->norm[i][1] >> 8;
+                const uint32_t hi = h * (uint32_t)R->norm[i][0] >> 8;
+                const uint32_t lo = h * (uint32_t)R->norm[i][1] >> 8;
                 lit = (hi + (((uint32_t)(int32_t)(lo - hi) * f->lowLight) >> 8)) & 0xffffu;
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
@@ -282,14 +289,14 @@ __global__ __launch_bounds__(256) void k_ball(const BallJob *__restrict__ jobs, 
             } else {
                 const uint32_t eu = env0 - n * (dX << 1), ev = env0 - n * (dY << 1);
                 samp16(aux, prep((int)(eu + h), (int)(ev + h)), extra); // environment map
-                const uint32_t diffuse = h * (uint32_t)f->norm[i][2] >> 8;
+                const uint32_t diffuse = h * (uint32_t)R->norm[i][2] >> 8;
                 const uint32_t full = (32u + diffuse) < 255u ? 32u + diffuse : 255u;
 #pragma unroll
                 for (int k = 0; k < 4; ++k)
                     color[k] = adds16(adds16(color[k], ((extra[k] * full) & 0xffffu) >> 8),
                                       diffuse & 0xffffu);
             }
-            height = h * f->heightProj[i] >> 8;
+            height = h * R->heightProj[i] >> 8;
         }
         if constexpr (kBeams) {
             // a chunk's sums stay below 64*255, so two channels share a register
@@ -492,24 +499,30 @@ void twister_tables(uint32_t *hp, uint32_t *hn)
     }
 }
 
-// vball_precalc (code/ball.cpp:263-286) and the vball set-up (:290-333)
-void ball_frame(const ckd_ball_params &p, unsigned H, BallFrame &f, std::vector<int2> &dXY)
+// vball_precalc (code/ball.cpp:263-286) for a clamped radius and ray length
+void ball_rays(float radius, unsigned rayLength, BallRays &t)
 {
-    const float radius = clampf_(1.f, 1920.f, p.radius);
-    f.rayLength = (unsigned)clampi_(1, (int)kBallMaxRay, geti(p.rayLength));
-    const float angStepSin = kPI / (float)(f.rayLength - 1);
+    const float angStepSin = kPI / (float)(rayLength - 1);
     const float angStepCos = angStepSin * 0.99f;
-    for (unsigned i = 0; i < f.rayLength; ++i) {
-        f.heightProj[i] = cvt_u32(radius * sinf(angStepSin * (float)i));
+    for (unsigned i = 0; i < rayLength; ++i) {
+        t.heightProj[i] = cvt_u32(radius * sinf(angStepSin * (float)i));
         const float cosine = cosf(angStepCos * (float)i);
         if (cosine >= 0.f) {
-            f.norm[i][0] = cvt_i32(255.f * powf(cosine, kGoldenRatio));
-            f.norm[i][1] = cvt_i32(255.f * powf(cosine, kGoldenAngle));
-            f.norm[i][2] = cvt_i32(255.f * powf(cosine, kPI));
+            t.norm[i][0] = cvt_i32(255.f * powf(cosine, kGoldenRatio));
+            t.norm[i][1] = cvt_i32(255.f * powf(cosine, kGoldenAngle));
+            t.norm[i][2] = cvt_i32(255.f * powf(cosine, kPI));
         } else {
-            f.norm[i][0] = f.norm[i][1] = f.norm[i][2] = 0;
+            t.norm[i][0] = t.norm[i][1] = t.norm[i][2] = 0;
         }
     }
+}
+
+// the per-frame set-up of vball (code/ball.cpp:290-333) and Ball_Draw's
+// clamped tracks; the radius comes back for the frame's BallRays
+void ball_frame(const ckd_ball_params &p, BallFrame &f, float &radius)
+{
+    radius = clampf_(1.f, 1920.f, p.radius);
+    f.rayLength = (unsigned)clampi_(1, (int)kBallMaxRay, geti(p.rayLength));
     f.beamAtten = (unsigned)clampi_(0, 255, geti(p.beamAttenuation));
     f.beamAlphaMin = clampf_(0.f, 255.f, p.beamAlphaMin);
     f.lowLight = (unsigned)clampi_(0, 255, geti(p.lowBeams));
@@ -517,10 +530,15 @@ void ball_frame(const ckd_ball_params &p, unsigned H, BallFrame &f, std::vector<
     const float timeScale = (float)f.rayLength * (0.25f / 1024.f);
     f.fromX = fp24(1024.f * sinf(time * timeScale) + 512.f + p.rotateOffsX);
     f.fromY = fp24(1024.f * cosf(time * timeScale) + 512.f + p.rotateOffsY);
+}
+
+// the fan of ray directions over H rows: voxel::calc_fandeltas
+// (code/voxel-shared.h:18-31) at angle row * 2pi/(H-1), as 24:8
+void ball_fans(unsigned H, std::vector<int2> &dXY)
+{
     const float delta = (2.f * kPI) / (float)(H - 1);
     dXY.resize(H);
     for (unsigned r = 0; r < H; ++r) {
-        // voxel::calc_fandeltas (code/voxel-shared.h:18-31)
         float X = cosf((float)r * delta), Y = sinf((float)r * delta);
         if (fabsf(X + Y) > kEpsilon) {
             const float length = 1.f / sqrtf(X * X + Y * Y);
@@ -744,8 +762,15 @@ ckd_status ckd_ball_create(ckd_ctx *ctx, const uint8_t *heightMaps5, const uint3
         CKD_HIP(hipFree(ctx->ballImages));
         ctx->ballImages = nullptr;
     }
-    if (!ctx->ballImages && hipMalloc(&ctx->ballImages, (size_t)3 * W * H * 4) != hipSuccess)
+    // two backgrounds and the halo, then the fan deltas of the H rays
+    const size_t fanOffset = ((size_t)3 * W * H * 4 + 255) & ~(size_t)255;
+    if (!ctx->ballImages &&
+        hipMalloc(&ctx->ballImages, fanOffset + (size_t)H * sizeof(int2)) != hipSuccess)
         return ckd::fail(CKD_ENOMEM, "ckd_ball_create: cannot allocate the backgrounds");
+    std::vector<int2> fans;
+    ball_fans(H, fans);
+    CKD_HIP(hipMemcpy(reinterpret_cast<uint8_t *>(ctx->ballImages) + fanOffset, fans.data(),
+                      (size_t)H * sizeof(int2), hipMemcpyHostToDevice));
     uint8_t *h = ctx->ballMaps;
     uint32_t *c = reinterpret_cast<uint32_t *>(h + 6 * kMapPx);
     CKD_HIP(hipMemcpy(h, heightMaps5, 5 * kMapPx, hipMemcpyHostToDevice));
@@ -788,6 +813,8 @@ ckd_status ckd_ball_draw(ckd_ctx *ctx, uint32_t *dst, uint32_t *target, unsigned
     const uint32_t *cmaps = reinterpret_cast<const uint32_t *>(heights + 6 * kMapPx);
     const uint32_t *beamMaps = cmaps + 2 * kMapPx, *env = cmaps + 6 * kMapPx;
     const uint32_t *backgrounds = ctx->ballImages, *halo = ctx->ballImages + 2 * px;
+    const int2 *fans = reinterpret_cast<const int2 *>(reinterpret_cast<const uint8_t *>(ctx->ballImages) +
+                                                      ((3 * px * 4 + 255) & ~(size_t)255));
     // scratch per group of G frames: render targets, height map mixes, beam
     // map mixes, and the carried last column
     const unsigned G = nFrames < kGroup ? nFrames : kGroup;
@@ -802,15 +829,14 @@ ckd_status ckd_ball_draw(ckd_ctx *ctx, uint32_t *dst, uint32_t *target, unsigned
     k_copy_column<<<colBlocks, 256, 0, ctx->stream>>>(carry, 1, target + (W - 1), W, H);
     if ((s = ckd::check_launch("ball column")) != CKD_OK) return s;
     Blob blob;
-    std::vector<int2> dXY;
     unsigned g = 0;
     for (unsigned f0 = 0; f0 < nFrames; f0 += g) {
         g = nFrames - f0 < G ? nFrames - f0 : G;
         // Ball_Draw (code/ball.cpp:442-511) of g frames, each in its own
         // render target
         blob.clear();
-        const size_t oFrames = blob.add(sizeof(BallFrame) * g);
-        const size_t oFans = blob.add(sizeof(int2) * H * g);
+        std::vector<BallFrame> fr(g);
+        std::vector<std::pair<float, unsigned>> rayKeys; // (radius, ray length) per BallRays
         std::vector<BallMix> mixes(g);
         std::vector<BallJob> jobsBeams, jobsEnv;
         std::vector<ckd::B07Job> blurEnv;
@@ -832,9 +858,16 @@ ckd_status ckd_ball_draw(ckd_ctx *ctx, uint32_t *dst, uint32_t *target, unsigned
                 const float sa = smax(0.f, smin(1.f, a[m])); // saturatef
                 M.lanes[m] = sa > 0.f ? 0x01010101u * (uint32_t)(sa * 255.f) : 0u;
             }
-            // vball (code/ball.cpp:290-333)
-            ball_frame(p, H, blob.at<BallFrame>(oFrames)[j], dXY);
-            memcpy(blob.at<int2>(oFans) + (size_t)j * H, dXY.data(), sizeof(int2) * H);
+            // vball (code/ball.cpp:290-333); one BallRays per distinct radius
+            // and ray length of the group
+            float radius = 0.f;
+            ball_frame(p, fr[j], radius);
+            unsigned r = 0;
+            while (r < rayKeys.size() &&
+                   !(rayKeys[r].first == radius && rayKeys[r].second == fr[j].rayLength))
+                ++r;
+            if (r == rayKeys.size()) rayKeys.emplace_back(radius, fr[j].rayLength);
+            fr[j].rays = r;
             uint32_t *t = tg + j * px;
             const BallJob job{t, hm + j * kMapPx, beams[j] ? cmaps : cmaps + kMapPx,
                               beams[j] ? bm + j * kMapPx : env, j};
@@ -852,6 +885,10 @@ ckd_status ckd_ball_draw(ckd_ctx *ctx, uint32_t *dst, uint32_t *target, unsigned
             comp[j] = ComposeJob{t, dst + (f0 + j) * px, backgrounds + (beams[j] ? 0 : px),
                                  beams[j] ? 1u : 0u};
         }
+        const size_t oFrames = blob.put(fr.data(), sizeof(BallFrame) * g);
+        const size_t oRays = blob.add(sizeof(BallRays) * rayKeys.size());
+        for (size_t r = 0; r < rayKeys.size(); ++r)
+            ball_rays(rayKeys[r].first, rayKeys[r].second, blob.at<BallRays>(oRays)[r]);
         const size_t oMix = blob.put(mixes.data(), sizeof(BallMix) * g);
         const size_t oJB = blob.put(jobsBeams.data(), sizeof(BallJob) * jobsBeams.size());
         const size_t oJE = blob.put(jobsEnv.data(), sizeof(BallJob) * jobsEnv.size());
@@ -861,7 +898,7 @@ ckd_status ckd_ball_draw(ckd_ctx *ctx, uint32_t *dst, uint32_t *target, unsigned
         uint8_t *dev = nullptr;
         if ((s = blob.stage(ctx, dev)) != CKD_OK) return s;
         const BallFrame *dFrames = reinterpret_cast<const BallFrame *>(dev + oFrames);
-        const int2 *dFans = reinterpret_cast<const int2 *>(dev + oFans);
+        const BallRays *dRays = reinterpret_cast<const BallRays *>(dev + oRays);
         k_ball_mix<<<dim3(ckd_div_up(kMapPx, 256), g), 256, 0, ctx->stream>>>(
             reinterpret_cast<const BallMix *>(dev + oMix),
             reinterpret_cast<const uint32_t *>(heights), beamMaps, (unsigned)kMapPx);
@@ -869,12 +906,12 @@ ckd_status ckd_ball_draw(ckd_ctx *ctx, uint32_t *dst, uint32_t *target, unsigned
         const unsigned rowBlocks = ckd_div_up(H, 256 / kWave);
         if (!jobsBeams.empty()) {
             k_ball<true><<<dim3(rowBlocks, (unsigned)jobsBeams.size()), 256, 0, ctx->stream>>>(
-                reinterpret_cast<const BallJob *>(dev + oJB), W, H, dFrames, dFans);
+                reinterpret_cast<const BallJob *>(dev + oJB), W, H, dFrames, dRays, fans);
             if ((s = ckd::check_launch("ball")) != CKD_OK) return s;
         }
         if (!jobsEnv.empty()) {
             k_ball<false><<<dim3(rowBlocks, (unsigned)jobsEnv.size()), 256, 0, ctx->stream>>>(
-                reinterpret_cast<const BallJob *>(dev + oJE), W, H, dFrames, dFans);
+                reinterpret_cast<const BallJob *>(dev + oJE), W, H, dFrames, dRays, fans);
             if ((s = ckd::check_launch("ball")) != CKD_OK) return s;
         }
         const bool vec4 = px % 4 == 0;
