@@ -24,8 +24,25 @@ struct B07Job {
 ckd_status b07_h_job(uint32_t *img, unsigned W, unsigned H, float strength, B07Job &job);
 
 // the in-place horizontal blurs of n jobs already in device memory, all on
-// W x H images (W <= 16383), one launch; vec4: every image 16-byte aligned
+// W x H images (W <= 16383), one launch; vec4: every image 16-byte aligned.
+// With splitStates (12 * 4 * H bytes per job) each line stops just before
+// its last pixel is added (every job needs W > span) and leaves its state
+// there for blur07_h_chain.
 ckd_status blur07_h_inplace_jobs(ckd_ctx *ctx, const B07Job *devJobs, unsigned n, unsigned W,
-                                 unsigned H, bool vec4);
+                                 unsigned H, bool vec4, uint32_t *splitStates = nullptr);
+
+// one image of a chain whose last column carries from image to image
+struct B07Chain {
+    uint32_t *img;
+    const uint32_t *state; // mode 2: its split blur's states
+    uint32_t mode;         // 0: its last column is final; 1: takes the carried
+                           // column; 2: takes it, then its split blur ends
+    B07Params p;           // mode 2
+};
+
+// walks n chained images (device array) in order; carry (H pixels) holds
+// the column coming in and receives the one going out
+ckd_status blur07_h_chain(ckd_ctx *ctx, const B07Chain *devChain, unsigned n, uint32_t *carry,
+                          unsigned W, unsigned H);
 
 } // namespace ckd

@@ -816,14 +816,18 @@ ckd_status ckd_ball_draw(ckd_ctx *ctx, uint32_t *dst, uint32_t *target, unsigned
     const int2 *fans = reinterpret_cast<const int2 *>(reinterpret_cast<const uint8_t *>(ctx->ballImages) +
                                                       ((3 * px * 4 + 255) & ~(size_t)255));
     // scratch per group of G frames: render targets, height map mixes, beam
-    // map mixes, and the carried last column
+    // map mixes, the split blurs' lane states, and the carried last column
     const unsigned G = nFrames < kGroup ? nFrames : kGroup;
     const size_t tgBytes = G * px * 4, hmBytes = G * kMapPx, bmBytes = G * kMapPx * 4;
+    const size_t stWords = (size_t)H * 12; // per frame: 3 words per line and channel
     uint32_t *tg = nullptr;
-    if ((s = v2_scratch(ctx, tgBytes + hmBytes + bmBytes + (size_t)H * 4, tg)) != CKD_OK) return s;
+    if ((s = v2_scratch(ctx, tgBytes + hmBytes + bmBytes + G * stWords * 4 + (size_t)H * 4, tg)) !=
+        CKD_OK)
+        return s;
     uint8_t *hm = reinterpret_cast<uint8_t *>(tg) + tgBytes;
     uint32_t *bm = reinterpret_cast<uint32_t *>(hm + hmBytes);
-    uint32_t *carry = bm + G * kMapPx;
+    uint32_t *states = bm + G * kMapPx;
+    uint32_t *carry = states + G * stWords;
     // the column the first frame with beams keeps: the caller's target's
     const unsigned colBlocks = ckd_div_up(H, 256);
     k_copy_column<<<colBlocks, 256, 0, ctx->stream>>>(carry, 1, target + (W - 1), W, H);
@@ -895,6 +899,29 @@ ckd_status ckd_ball_draw(ckd_ctx *ctx, uint32_t *dst, uint32_t *target, unsigned
         const size_t oBE = blob.put(blurEnv.data(), sizeof(ckd::B07Job) * blurEnv.size());
         const size_t oBB = blob.put(blurBeams.data(), sizeof(ckd::B07Job) * blurBeams.size());
         const size_t oComp = blob.put(comp.data(), sizeof(ComposeJob) * g);
+        // the carried column runs through the frames in order.  When every
+        // blur of a frame with beams spans less than the row, those blurs
+        // run together up to the step that adds the last column and one
+        // chain launch finishes them in frame order; else each waits for
+        // its predecessor.
+        bool split = true;
+        for (const ckd::B07Job &b : blurBeams) split = split && b.p.span < W;
+        size_t oChain = 0;
+        if (split) {
+            std::vector<ckd::B07Chain> chain(g);
+            for (unsigned j = 0; j < g; ++j) {
+                ckd::B07Chain &c = chain[j];
+                c.img = tg + j * px;
+                c.state = nullptr;
+                c.mode = beams[j] ? 1u : 0u;
+                if (beamsBlur[j] >= 0) {
+                    c.mode = 2;
+                    c.state = states + beamsBlur[j] * stWords;
+                    c.p = blurBeams[beamsBlur[j]].p;
+                }
+            }
+            oChain = blob.put(chain.data(), sizeof(ckd::B07Chain) * g);
+        }
         uint8_t *dev = nullptr;
         if ((s = blob.stage(ctx, dev)) != CKD_OK) return s;
         const BallFrame *dFrames = reinterpret_cast<const BallFrame *>(dev + oFrames);
@@ -920,9 +947,17 @@ ckd_status ckd_ball_draw(ckd_ctx *ctx, uint32_t *dst, uint32_t *target, unsigned
             (s = ckd::blur07_h_inplace_jobs(ctx, reinterpret_cast<const ckd::B07Job *>(dev + oBE),
                                             (unsigned)blurEnv.size(), W, H, vec4)) != CKD_OK)
             return s;
+        if (split) {
+            if ((s = ckd::blur07_h_inplace_jobs(ctx, reinterpret_cast<const ckd::B07Job *>(dev + oBB),
+                                                (unsigned)blurBeams.size(), W, H, vec4, states)) !=
+                    CKD_OK ||
+                (s = ckd::blur07_h_chain(ctx, reinterpret_cast<const ckd::B07Chain *>(dev + oChain),
+                                         g, carry, W, H)) != CKD_OK)
+                return s;
+        }
         // a frame with beams keeps its predecessor's last column (after that
         // frame's blur), so these go in frame order
-        for (unsigned j = 0; j < g; ++j) {
+        for (unsigned j = 0; j < g && !split; ++j) {
             if (!beams[j]) continue;
             const uint32_t *src = j == 0 ? carry : tg + (j - 1) * px + (W - 1);
             k_copy_column<<<colBlocks, 256, 0, ctx->stream>>>(tg + j * px + (W - 1), W, src,
@@ -937,10 +972,13 @@ ckd_status ckd_ball_draw(ckd_ctx *ctx, uint32_t *dst, uint32_t *target, unsigned
         k_compose<<<dim3(ckd_div_up(W, 256), H, g), 256, 0, ctx->stream>>>(
             reinterpret_cast<const ComposeJob *>(dev + oComp), map, halo, W, H);
         if ((s = ckd::check_launch("ball compose")) != CKD_OK) return s;
-        // the next group's first frame keeps this group's last column
-        k_copy_column<<<colBlocks, 256, 0, ctx->stream>>>(carry, 1, tg + (g - 1) * px + (W - 1), W,
-                                                         H);
-        if ((s = ckd::check_launch("ball column")) != CKD_OK) return s;
+        // the next group's first frame keeps this group's last column (the
+        // chain launch left it in carry)
+        if (!split) {
+            k_copy_column<<<colBlocks, 256, 0, ctx->stream>>>(carry, 1, tg + (g - 1) * px + (W - 1),
+                                                             W, H);
+            if ((s = ckd::check_launch("ball column")) != CKD_OK) return s;
+        }
     }
     // the render target ends as the last frame left it
     CKD_HIP(hipMemcpyAsync(target, tg + (size_t)(g - 1) * px, px * 4, hipMemcpyDeviceToDevice,

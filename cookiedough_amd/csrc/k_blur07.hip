@@ -383,9 +383,13 @@ CKD_DEV void b07_run_sub(B07Lane<kEven> &s, int n, int median, Div div)
     else b07_run<1, kAdd, true, true>(s, n, div);
 }
 
-// the whole line (code/deprecated/boxblur.cpp:81-125)
-template <bool kEven>
-CKD_DEV void b07_lds_line(uint8_t *px, int len, const B07Params &p, const uint16_t *edgeDivs)
+// the whole line (code/deprecated/boxblur.cpp:81-125).  kSplit: stop just
+// before the step that adds the line's last pixel (the middle run's last
+// step; the caller guarantees full >= 1) and save the lane's state to
+// state[0..2]; b07_tail finishes the line later.
+template <bool kEven, bool kSplit = false>
+CKD_DEV void b07_lds_line(uint8_t *px, int len, const B07Params &p, const uint16_t *edgeDivs,
+                          uint32_t *state = nullptr)
 {
     B07Lane<kEven> s;
     s.px = px;
@@ -396,6 +400,13 @@ CKD_DEV void b07_lds_line(uint8_t *px, int len, const B07Params &p, const uint16
     const uint32_t fullDiv = p.fullDiv;
     b07_run<8, true, false, false>(s, e, [](int) { return 0u; });
     b07_run<8, true, false, true>(s, m, [&](int i) { return (uint32_t)edgeDivs[i]; });
+    if constexpr (kSplit) {
+        b07_run_sub<true>(s, full - 1, m, [&](int) { return fullDiv; });
+        state[0] = s.acc;
+        state[1] = s.addRem;
+        state[2] = s.subRem;
+        return;
+    }
     b07_run_sub<true>(s, full, m, [&](int) { return fullDiv; });
     if constexpr (kEven) s.acc = __builtin_elementwise_add_sat((uint16_t)s.acc, (uint16_t)s.addRem);
     b07_run_sub<false>(s, e, m, [&](int i) { return (uint32_t)edgeDivs[e - 1 - i]; });
@@ -469,11 +480,13 @@ CKD_DEV void b07_stage(uint32_t *img, uint32_t *lds, int line0, int nl, int len,
     }
 }
 
-// one block: NL lines of img, staged, blurred in place, written back
-template <int NL>
+// one block: NL lines of img, staged, blurred in place, written back; with
+// kSplit each lane stops before the last pixel's add and saves its state at
+// state + ((line * 4) + channel) * 3
+template <int NL, bool kSplit = false>
 CKD_DEV void b07_lines_block(uint32_t *img, int nLines, int len, size_t lineStride,
                              size_t pixStride, bool vec4, const B07Params &p, uint32_t *lineBuf,
-                             uint16_t *edgeDivs)
+                             uint16_t *edgeDivs, uint32_t *state = nullptr)
 {
     const int stride = len + 1;
     const int line0 = blockIdx.x * NL;
@@ -484,8 +497,14 @@ CKD_DEV void b07_lines_block(uint32_t *img, int nLines, int len, size_t lineStri
     const int l = threadIdx.x >> 2, k = threadIdx.x & 3;
     if (l < nl) {
         uint8_t *px = reinterpret_cast<uint8_t *>(lineBuf + l * stride) + k;
-        if (p.subEdges) b07_lds_line<true>(px, len, p, edgeDivs);
-        else b07_lds_line<false>(px, len, p, edgeDivs);
+        if constexpr (kSplit) {
+            uint32_t *st = state + ((size_t)(line0 + l) * 4 + k) * 3;
+            if (p.subEdges) b07_lds_line<true, true>(px, len, p, edgeDivs, st);
+            else b07_lds_line<false, true>(px, len, p, edgeDivs, st);
+        } else {
+            if (p.subEdges) b07_lds_line<true>(px, len, p, edgeDivs);
+            else b07_lds_line<false>(px, len, p, edgeDivs);
+        }
     }
     __syncthreads();
     b07_stage<true>(img, lineBuf, line0, nl, len, stride, lineStride, pixStride, vec4);
@@ -503,14 +522,96 @@ __global__ __launch_bounds__(64) void k_blur07_lines(uint32_t *img, int nLines, 
 
 // the in-place horizontal blur of several images of one size, each with
 // its own strength: blockIdx.y picks the image
-template <int NL>
+template <int NL, bool kSplit>
 __global__ __launch_bounds__(64) void k_blur07_lines_jobs(const B07Job *__restrict__ jobs,
-                                                          int nLines, int len, int vec4)
+                                                          int nLines, int len, int vec4,
+                                                          uint32_t *__restrict__ states)
 {
     extern __shared__ uint32_t lineBuf[];
     __shared__ uint16_t edgeDivs[256];
     const B07Job &J = jobs[blockIdx.y];
-    b07_lines_block<NL>(J.img, nLines, len, (size_t)len, 1, vec4 != 0, J.p, lineBuf, edgeDivs);
+    b07_lines_block<NL, kSplit>(J.img, nLines, len, (size_t)len, 1, vec4 != 0, J.p, lineBuf,
+                                edgeDivs, kSplit ? states + (size_t)blockIdx.y * nLines * 12 : nullptr);
+}
+
+// the chain of a batch of render targets whose last column carries from one
+// image to the next (the ball's frames with beams): one lane per line and
+// channel walks the images in order.  mode 0: the image's last column is
+// final and becomes the carry; mode 1: the image takes the carry as its last
+// column; mode 2: the line's split blur ends with the carry as its last
+// pixel (the middle run's last step, then the end phase) and its blurred
+// last pixel becomes the carry.  carry (one pixel per line) holds the
+// incoming column and receives the outgoing one.
+//
+// The tail's subtracted values are the split blur's outputs at
+// full-1 .. full-1+e (all written before the tail, except, for even spans,
+// the last: the tail's own first output), so they are read into the lane's
+// LDS slots first and only the recurrence itself waits on the carry.
+constexpr int kChainLanes = 64;
+
+template <bool kEven>
+CKD_DEV uint32_t b07_chain_tail(uint8_t *px, int len, const B07Params &p, const uint32_t *state,
+                                uint32_t c, uint8_t *slots)
+{
+    const int e = (int)p.edgeSpan, m = (int)p.median;
+    const int full = len - (m + e);
+    const int nPre = kEven ? e : e + 1;
+    for (int i = 0; i < nPre; ++i) slots[i * kChainLanes] = px[(full - 1 + i) * 4];
+    B07Lane<kEven> s;
+    s.acc = state[0];
+    s.addRem = state[1];
+    s.subRem = state[2];
+    s.add(c);
+    s.sub(slots[0]);
+    const uint8_t o0 = s.out(p.fullDiv);
+    px[(m + full - 1) * 4] = o0;
+    uint32_t last = o0;
+    if constexpr (kEven) s.acc = __builtin_elementwise_add_sat((uint16_t)s.acc, (uint16_t)s.addRem);
+    uint8_t *out = px + (m + full) * 4;
+    int i = 0;
+    for (; i + 8 <= e; i += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = (kEven && i + j == e - 1) ? o0 : slots[(i + j + 1) * kChainLanes];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            s.sub(v[j]);
+            last = s.out(p.edgeDivs[e - 1 - (i + j)]);
+            out[(i + j) * 4] = (uint8_t)last;
+        }
+    }
+    for (; i < e; ++i) {
+        s.sub((kEven && i == e - 1) ? (uint32_t)o0 : (uint32_t)slots[(i + 1) * kChainLanes]);
+        last = s.out(p.edgeDivs[e - 1 - i]);
+        out[i * 4] = (uint8_t)last;
+    }
+    return last;
+}
+
+__global__ __launch_bounds__(kChainLanes) void k_blur07_chain(const ckd::B07Chain *__restrict__ chain,
+                                                              int n, uint32_t *__restrict__ carry,
+                                                              int nLines, int len)
+{
+    __shared__ uint8_t slots[256 * kChainLanes];
+    const int t = blockIdx.x * kChainLanes + threadIdx.x;
+    const int line = t >> 2, k = t & 3;
+    if (line >= nLines) return;
+    uint8_t *cp = reinterpret_cast<uint8_t *>(carry + line) + k;
+    uint32_t c = *cp;
+    for (int f = 0; f < n; ++f) {
+        const ckd::B07Chain &J = chain[f];
+        uint8_t *px = reinterpret_cast<uint8_t *>(J.img + (size_t)line * len) + k;
+        if (J.mode == 0) {
+            c = px[(size_t)(len - 1) * 4];
+        } else if (J.mode == 1) {
+            px[(size_t)(len - 1) * 4] = (uint8_t)c;
+        } else {
+            const uint32_t *st = J.state + ((size_t)line * 4 + k) * 3;
+            c = J.p.subEdges ? b07_chain_tail<true>(px, len, J.p, st, c, slots + threadIdx.x)
+                             : b07_chain_tail<false>(px, len, J.p, st, c, slots + threadIdx.x);
+        }
+    }
+    *cp = (uint8_t)c;
 }
 
 ckd_status validate(const char *fn, const uint32_t *dst, const uint32_t *src, unsigned xRes,
@@ -603,8 +704,21 @@ ckd_status b07_h_job(uint32_t *img, unsigned W, unsigned H, float strength, B07J
     return validate("blur07_h (in place)", img, img, W, H, strength, W, job.p);
 }
 
+template <bool kSplit>
+void launch_lines_jobs(int nl, dim3 grid, size_t lds, hipStream_t st, const B07Job *devJobs,
+                       unsigned W, unsigned H, int v4, uint32_t *states)
+{
+    switch (nl) {
+    case 16: k_blur07_lines_jobs<16, kSplit><<<grid, 64, lds, st>>>(devJobs, (int)H, (int)W, v4, states); break;
+    case 8: k_blur07_lines_jobs<8, kSplit><<<grid, 64, lds, st>>>(devJobs, (int)H, (int)W, v4, states); break;
+    case 4: k_blur07_lines_jobs<4, kSplit><<<grid, 64, lds, st>>>(devJobs, (int)H, (int)W, v4, states); break;
+    case 2: k_blur07_lines_jobs<2, kSplit><<<grid, 64, lds, st>>>(devJobs, (int)H, (int)W, v4, states); break;
+    default: k_blur07_lines_jobs<1, kSplit><<<grid, 64, lds, st>>>(devJobs, (int)H, (int)W, v4, states); break;
+    }
+}
+
 ckd_status blur07_h_inplace_jobs(ckd_ctx *ctx, const B07Job *devJobs, unsigned n, unsigned W,
-                                 unsigned H, bool vec4)
+                                 unsigned H, bool vec4, uint32_t *splitStates)
 {
     if (n == 0) return CKD_OK;
     const int nl = lines_per_block((int)W);
@@ -612,14 +726,18 @@ ckd_status blur07_h_inplace_jobs(ckd_ctx *ctx, const B07Job *devJobs, unsigned n
     const size_t lds = (size_t)nl * (size_t)(W + 1) * 4;
     const int v4 = vec4 && (W % 4) == 0;
     const dim3 grid(ckd_div_up((size_t)H, (size_t)nl), n);
-    switch (nl) {
-    case 16: k_blur07_lines_jobs<16><<<grid, 64, lds, ctx->stream>>>(devJobs, (int)H, (int)W, v4); break;
-    case 8: k_blur07_lines_jobs<8><<<grid, 64, lds, ctx->stream>>>(devJobs, (int)H, (int)W, v4); break;
-    case 4: k_blur07_lines_jobs<4><<<grid, 64, lds, ctx->stream>>>(devJobs, (int)H, (int)W, v4); break;
-    case 2: k_blur07_lines_jobs<2><<<grid, 64, lds, ctx->stream>>>(devJobs, (int)H, (int)W, v4); break;
-    default: k_blur07_lines_jobs<1><<<grid, 64, lds, ctx->stream>>>(devJobs, (int)H, (int)W, v4); break;
-    }
+    if (splitStates) launch_lines_jobs<true>(nl, grid, lds, ctx->stream, devJobs, W, H, v4, splitStates);
+    else launch_lines_jobs<false>(nl, grid, lds, ctx->stream, devJobs, W, H, v4, nullptr);
     return check_launch("blur07_h (in place, batch)");
+}
+
+ckd_status blur07_h_chain(ckd_ctx *ctx, const B07Chain *devChain, unsigned n, uint32_t *carry,
+                          unsigned W, unsigned H)
+{
+    if (n == 0) return CKD_OK;
+    k_blur07_chain<<<ckd_div_up((size_t)H * 4, kChainLanes), kChainLanes, 0, ctx->stream>>>(
+        devChain, (int)n, carry, (int)H, (int)W);
+    return check_launch("blur07_h chain");
 }
 
 // used by the tunnel pipeline (span 2r+1, out of place)
