@@ -23,6 +23,16 @@ struct B07Job {
 // ckd_blur07_h
 ckd_status b07_h_job(uint32_t *img, unsigned W, unsigned H, float strength, B07Job &job);
 
+// a job for the in-place 2D blur (BoxBlur32 with pDest == pSrc) of a W x H
+// image: the span must fit both passes
+ckd_status b07_job(uint32_t *img, unsigned W, unsigned H, float strength, B07Job &job);
+
+// the in-place blurs of n jobs already in device memory, all on W x H
+// images (lines up to 16383 pixels), one launch: along the rows, or along
+// the columns with `columns`; vec4: every image 16-byte aligned (rows only)
+ckd_status blur07_inplace_jobs(ckd_ctx *ctx, const B07Job *devJobs, unsigned n, unsigned W,
+                               unsigned H, bool columns, bool vec4, uint32_t *splitStates = nullptr);
+
 // the in-place horizontal blurs of n jobs already in device memory, all on
 // W x H images (W <= 16383), one launch; vec4: every image 16-byte aligned.
 // With splitStates (12 * 4 * H bytes per job) each line stops just before

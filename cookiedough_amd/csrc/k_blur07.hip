@@ -520,17 +520,19 @@ __global__ __launch_bounds__(64) void k_blur07_lines(uint32_t *img, int nLines, 
     b07_lines_block<NL>(img, nLines, len, lineStride, pixStride, vec4 != 0, p, lineBuf, edgeDivs);
 }
 
-// the in-place horizontal blur of several images of one size, each with
-// its own strength: blockIdx.y picks the image
+// the in-place blur of several images of one size, each with its own
+// strength: blockIdx.y picks the image.  Rows (lineStride len, pixStride 1)
+// or columns (lineStride 1, pixStride nLines).
 template <int NL, bool kSplit>
 __global__ __launch_bounds__(64) void k_blur07_lines_jobs(const B07Job *__restrict__ jobs,
-                                                          int nLines, int len, int vec4,
-                                                          uint32_t *__restrict__ states)
+                                                          int nLines, int len, int columns,
+                                                          int vec4, uint32_t *__restrict__ states)
 {
     extern __shared__ uint32_t lineBuf[];
     __shared__ uint16_t edgeDivs[256];
     const B07Job &J = jobs[blockIdx.y];
-    b07_lines_block<NL, kSplit>(J.img, nLines, len, (size_t)len, 1, vec4 != 0, J.p, lineBuf,
+    const size_t lineStride = columns ? 1 : (size_t)len, pixStride = columns ? (size_t)nLines : 1;
+    b07_lines_block<NL, kSplit>(J.img, nLines, len, lineStride, pixStride, vec4 != 0, J.p, lineBuf,
                                 edgeDivs, kSplit ? states + (size_t)blockIdx.y * nLines * 12 : nullptr);
 }
 
@@ -704,31 +706,48 @@ ckd_status b07_h_job(uint32_t *img, unsigned W, unsigned H, float strength, B07J
     return validate("blur07_h (in place)", img, img, W, H, strength, W, job.p);
 }
 
+ckd_status b07_job(uint32_t *img, unsigned W, unsigned H, float strength, B07Job &job)
+{
+    job.img = img;
+    return validate("blur07 (in place)", img, img, W, H, strength, W < H ? W : H, job.p);
+}
+
 template <bool kSplit>
 void launch_lines_jobs(int nl, dim3 grid, size_t lds, hipStream_t st, const B07Job *devJobs,
-                       unsigned W, unsigned H, int v4, uint32_t *states)
+                       int nLines, int len, int columns, int v4, uint32_t *states)
 {
     switch (nl) {
-    case 16: k_blur07_lines_jobs<16, kSplit><<<grid, 64, lds, st>>>(devJobs, (int)H, (int)W, v4, states); break;
-    case 8: k_blur07_lines_jobs<8, kSplit><<<grid, 64, lds, st>>>(devJobs, (int)H, (int)W, v4, states); break;
-    case 4: k_blur07_lines_jobs<4, kSplit><<<grid, 64, lds, st>>>(devJobs, (int)H, (int)W, v4, states); break;
-    case 2: k_blur07_lines_jobs<2, kSplit><<<grid, 64, lds, st>>>(devJobs, (int)H, (int)W, v4, states); break;
-    default: k_blur07_lines_jobs<1, kSplit><<<grid, 64, lds, st>>>(devJobs, (int)H, (int)W, v4, states); break;
+    case 16: k_blur07_lines_jobs<16, kSplit><<<grid, 64, lds, st>>>(devJobs, nLines, len, columns, v4, states); break;
+    case 8: k_blur07_lines_jobs<8, kSplit><<<grid, 64, lds, st>>>(devJobs, nLines, len, columns, v4, states); break;
+    case 4: k_blur07_lines_jobs<4, kSplit><<<grid, 64, lds, st>>>(devJobs, nLines, len, columns, v4, states); break;
+    case 2: k_blur07_lines_jobs<2, kSplit><<<grid, 64, lds, st>>>(devJobs, nLines, len, columns, v4, states); break;
+    default: k_blur07_lines_jobs<1, kSplit><<<grid, 64, lds, st>>>(devJobs, nLines, len, columns, v4, states); break;
     }
+}
+
+ckd_status blur07_inplace_jobs(ckd_ctx *ctx, const B07Job *devJobs, unsigned n, unsigned W,
+                               unsigned H, bool columns, bool vec4, uint32_t *splitStates)
+{
+    if (n == 0) return CKD_OK;
+    const unsigned len = columns ? H : W, nLines = columns ? W : H;
+    const int nl = lines_per_block((int)len);
+    if (nl == 0) return fail(CKD_EINVAL, "blur07 (in place, batch): lines of %u pixels, above 16383", len);
+    CKD_REQUIRE(!(splitStates && columns), "blur07 (in place, batch): split blurs run along rows");
+    const size_t lds = (size_t)nl * (size_t)(len + 1) * 4;
+    const int v4 = vec4 && !columns && (W % 4) == 0;
+    const dim3 grid(ckd_div_up((size_t)nLines, (size_t)nl), n);
+    if (splitStates)
+        launch_lines_jobs<true>(nl, grid, lds, ctx->stream, devJobs, (int)nLines, (int)len, 0, v4, splitStates);
+    else
+        launch_lines_jobs<false>(nl, grid, lds, ctx->stream, devJobs, (int)nLines, (int)len, columns ? 1 : 0, v4,
+                                 nullptr);
+    return check_launch(columns ? "blur07_v (in place, batch)" : "blur07_h (in place, batch)");
 }
 
 ckd_status blur07_h_inplace_jobs(ckd_ctx *ctx, const B07Job *devJobs, unsigned n, unsigned W,
                                  unsigned H, bool vec4, uint32_t *splitStates)
 {
-    if (n == 0) return CKD_OK;
-    const int nl = lines_per_block((int)W);
-    if (nl == 0) return fail(CKD_EINVAL, "blur07_h (in place, batch): width %u above 16383", W);
-    const size_t lds = (size_t)nl * (size_t)(W + 1) * 4;
-    const int v4 = vec4 && (W % 4) == 0;
-    const dim3 grid(ckd_div_up((size_t)H, (size_t)nl), n);
-    if (splitStates) launch_lines_jobs<true>(nl, grid, lds, ctx->stream, devJobs, W, H, v4, splitStates);
-    else launch_lines_jobs<false>(nl, grid, lds, ctx->stream, devJobs, W, H, v4, nullptr);
-    return check_launch("blur07_h (in place, batch)");
+    return blur07_inplace_jobs(ctx, devJobs, n, W, H, false, vec4, splitStates);
 }
 
 ckd_status blur07_h_chain(ckd_ctx *ctx, const B07Chain *devChain, unsigned n, uint32_t *carry,

@@ -19,6 +19,7 @@
 // log(FLT_MIN), and 255*c >= 32767.5 wraps to 0 in the packus pair.  The
 // remaining libm calls (powf, expf, atan2f) are the device library's.
 #include "ckd_internal.h"
+#include "ckd_b07.h"
 #include "ckd_pixel.h"
 
 #include <cmath>
@@ -1016,6 +1017,35 @@ ckd_status st_draw(ckd_ctx *ctx, int fx, const P *params, unsigned nFrames, uint
     uint32_t *maps = scratch, *glows = glow ? scratch + mapPx * nFrames : nullptr;
     uint32_t *work = scratch + mapPx * nFrames * (glow ? 2 : 1);
     if ((s = st_maps(ctx, fx, params, nFrames, maps, glows, resX, resY, close)) != CKD_OK) return s;
+    // the frames' in-place blurs (independent per frame) run as one batch:
+    // every row pass in one launch, then every column pass
+    std::vector<ckd::B07Job> blurs;
+    auto run_blurs = [&](unsigned W, unsigned H) -> ckd_status {
+        if (blurs.empty()) return CKD_OK;
+        bool vec4 = true;
+        for (const ckd::B07Job &j : blurs) vec4 = vec4 && (reinterpret_cast<uintptr_t>(j.img) & 15) == 0;
+        void *dev = nullptr;
+        ckd_status r = ckd::stage(ctx, blurs.data(), blurs.size() * sizeof(ckd::B07Job), &dev);
+        const ckd::B07Job *jobs = static_cast<const ckd::B07Job *>(dev);
+        const unsigned n = (unsigned)blurs.size();
+        if (r == CKD_OK) r = ckd::blur07_inplace_jobs(ctx, jobs, n, W, H, false, vec4);
+        if (r == CKD_OK) r = ckd::blur07_inplace_jobs(ctx, jobs, n, W, H, true, false);
+        return r;
+    };
+    if constexpr (std::is_same<P, ckd_fdtunnel_params>::value) {
+        // Tunnel_Draw, code/shadertoy.cpp:842-853: the glow map's BoxBlur32
+        for (unsigned k = 0; k < nFrames; ++k) {
+            const float lb = params[k].litBlur;
+            const float litBlur = (0.f < ((lb < 100.f) ? lb : 100.f)) ? ((lb < 100.f) ? lb : 100.f) : 0.f;
+            if (0 != (int)roundf(params[k].litTiles) && litBlur >= 1.f) {
+                blurs.emplace_back();
+                if ((s = ckd::b07_job(glows + mapPx * k, mapW, mapH, box_blur_scale(litBlur),
+                                      blurs.back())) != CKD_OK)
+                    return s;
+            }
+        }
+        if ((s = run_blurs(mapW, mapH)) != CKD_OK) return s;
+    }
     for (unsigned k = 0; k < nFrames; ++k) {
         uint32_t *map = maps + mapPx * k, *out = dst + outPx * k;
         if constexpr (std::is_same<P, ckd_spikey_params>::value) {
@@ -1030,23 +1060,23 @@ ckd_status st_draw(ckd_ctx *ctx, int fx, const P *params, unsigned nFrames, uint
         }
         if constexpr (std::is_same<P, ckd_fdtunnel_params>::value) {
             // Tunnel_Draw, code/shadertoy.cpp:842-853
-            if (0 != (int)roundf(params[k].litTiles)) {
-                const float lb = params[k].litBlur;
-                const float litBlur = (0.f < ((lb < 100.f) ? lb : 100.f)) ? ((lb < 100.f) ? lb : 100.f) : 0.f;
-                uint32_t *g = glows + mapPx * k;
-                if (litBlur >= 1.f &&
-                    (s = ckd_blur07(ctx, g, g, mapW, mapH, box_blur_scale(litBlur))) != CKD_OK)
-                    return s;
-                if ((s = ckd_add32(ctx, map, g, (unsigned)mapPx)) != CKD_OK) return s;
-            }
+            // (the glow map was blurred above)
+            if (0 != (int)roundf(params[k].litTiles) &&
+                (s = ckd_add32(ctx, map, glows + mapPx * k, (unsigned)mapPx)) != CKD_OK)
+                return s;
         }
         if ((s = ckd_fx_blit_2x2(ctx, out, map, resX, resY)) != CKD_OK) return s;
         if constexpr (std::is_same<P, ckd_nautilus_params>::value) {
-            // Nautilus_Draw, code/shadertoy.cpp:385-396
+            // Nautilus_Draw, code/shadertoy.cpp:385-396: BoxBlur32 of the
+            // output in place, run for every frame below
             const float blur = box_blur_scale(params[k].blur);
-            if (0.f != blur && (s = ckd_blur07(ctx, out, out, resX, resY, blur)) != CKD_OK) return s;
+            if (0.f != blur) {
+                blurs.emplace_back();
+                if ((s = ckd::b07_job(out, resX, resY, blur, blurs.back())) != CKD_OK) return s;
+            }
         }
     }
+    if constexpr (std::is_same<P, ckd_nautilus_params>::value) return run_blurs(resX, resY);
     return CKD_OK;
 }
 
