@@ -43,7 +43,7 @@ class SdfScene(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("sdfStepsPerRefill", C.c_int), ("sdfChunk", C.c_int),
-                ("sdfBlocksPerCU", C.c_int), ("tunnelVSeg", C.c_int)]
+                ("sdfBlocksPerCU", C.c_int), ("tunnelVSeg", C.c_int), ("blurKernel", C.c_int)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -149,6 +149,7 @@ SIGNATURES = {
     "ckd_blur07_h": [_p, _p, _p, _u, _u, _f],
     "ckd_blur07_v": [_p, _p, _p, _u, _u, _f],
     "ckd_blur07": [_p, _p, _p, _u, _u, _f],
+    "ckd_blur07_inplace_batch": [_p, _p, _p, _u, _u, _u, _i],
     "ckd_blur26_h": [_p, _p, _p, _u, _u, _f, _f, _u],
     "ckd_blur26_v": [_p, _p, _p, _u, _u, _f, _f, _u],
     "ckd_blur26": [_p, _p, _p, _u, _u, _f, _f, _u],

@@ -27,19 +27,24 @@ ckd_status b07_h_job(uint32_t *img, unsigned W, unsigned H, float strength, B07J
 // image: the span must fit both passes
 ckd_status b07_job(uint32_t *img, unsigned W, unsigned H, float strength, B07Job &job);
 
-// the in-place blurs of n jobs already in device memory, all on W x H
-// images (lines up to 16383 pixels), one launch: along the rows, or along
-// the columns with `columns`; vec4: every image 16-byte aligned (rows only)
-ckd_status blur07_inplace_jobs(ckd_ctx *ctx, const B07Job *devJobs, unsigned n, unsigned W,
-                               unsigned H, bool columns, bool vec4, uint32_t *splitStates = nullptr);
+// the in-place blurs of n jobs, all on W x H images, one launch: along the
+// rows, or along the columns with `columns`.  devJobs is the device copy of
+// hostJobs (the host copy picks the kernel and its ring length); vec4: every
+// image 16-byte aligned (rows only).  The kernel is the context tuning's
+// blurKernel: staged lines (lines up to 16383 pixels), a ring per lane, or
+// (0) the ring once the staged lines would need more than one round of blocks.
+ckd_status blur07_inplace_jobs(ckd_ctx *ctx, const B07Job *hostJobs, const B07Job *devJobs,
+                               unsigned n, unsigned W, unsigned H, bool columns, bool vec4,
+                               uint32_t *splitStates = nullptr);
 
 // the in-place horizontal blurs of n jobs already in device memory, all on
 // W x H images (W <= 16383), one launch; vec4: every image 16-byte aligned.
 // With splitStates (12 * 4 * H bytes per job) each line stops just before
 // its last pixel is added (every job needs W > span) and leaves its state
 // there for blur07_h_chain.
-ckd_status blur07_h_inplace_jobs(ckd_ctx *ctx, const B07Job *devJobs, unsigned n, unsigned W,
-                                 unsigned H, bool vec4, uint32_t *splitStates = nullptr);
+ckd_status blur07_h_inplace_jobs(ckd_ctx *ctx, const B07Job *hostJobs, const B07Job *devJobs,
+                                 unsigned n, unsigned W, unsigned H, bool vec4,
+                                 uint32_t *splitStates = nullptr);
 
 // one image of a chain whose last column carries from image to image
 struct B07Chain {

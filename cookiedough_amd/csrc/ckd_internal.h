@@ -86,7 +86,7 @@ struct ckd_ctx {
     size_t stagingCap = 0;
 
     // launch-shape tuning (ckd_autotune_batch / ckd_set_tuning)
-    ckd_tuning tuning = {2, 32, 8, 32};
+    ckd_tuning tuning = {2, 32, 8, 32, 0};
 
     // per-kernel HIP-event timing (ckd_kernel_timing)
     bool timing = false;

@@ -725,7 +725,7 @@ ckd_status ckd_twister_draw(ckd_ctx *ctx, uint32_t *dst, uint32_t *target, unsig
             ctx->twColor);
         if ((s = ckd::check_launch("twister")) != CKD_OK) return s;
         if (!blurs.empty() &&
-            (s = ckd::blur07_h_inplace_jobs(ctx, reinterpret_cast<const ckd::B07Job *>(dev + oBlur),
+            (s = ckd::blur07_h_inplace_jobs(ctx, blurs.data(), reinterpret_cast<const ckd::B07Job *>(dev + oBlur),
                                             (unsigned)blurs.size(), W, H, px % 4 == 0)) != CKD_OK)
             return s;
         k_compose<<<dim3(ckd_div_up(W, 256), H, g), 256, 0, ctx->stream>>>(
@@ -944,11 +944,11 @@ ckd_status ckd_ball_draw(ckd_ctx *ctx, uint32_t *dst, uint32_t *target, unsigned
         const bool vec4 = px % 4 == 0;
         // frames without beams cleared their targets: their blurs go together
         if (!blurEnv.empty() &&
-            (s = ckd::blur07_h_inplace_jobs(ctx, reinterpret_cast<const ckd::B07Job *>(dev + oBE),
+            (s = ckd::blur07_h_inplace_jobs(ctx, blurEnv.data(), reinterpret_cast<const ckd::B07Job *>(dev + oBE),
                                             (unsigned)blurEnv.size(), W, H, vec4)) != CKD_OK)
             return s;
         if (split) {
-            if ((s = ckd::blur07_h_inplace_jobs(ctx, reinterpret_cast<const ckd::B07Job *>(dev + oBB),
+            if ((s = ckd::blur07_h_inplace_jobs(ctx, blurBeams.data(), reinterpret_cast<const ckd::B07Job *>(dev + oBB),
                                                 (unsigned)blurBeams.size(), W, H, vec4, states)) !=
                     CKD_OK ||
                 (s = ckd::blur07_h_chain(ctx, reinterpret_cast<const ckd::B07Chain *>(dev + oChain),
@@ -965,8 +965,8 @@ ckd_status ckd_ball_draw(ckd_ctx *ctx, uint32_t *dst, uint32_t *target, unsigned
             if ((s = ckd::check_launch("ball column")) != CKD_OK) return s;
             if (beamsBlur[j] >= 0 &&
                 (s = ckd::blur07_h_inplace_jobs(
-                     ctx, reinterpret_cast<const ckd::B07Job *>(dev + oBB) + beamsBlur[j], 1, W, H,
-                     vec4)) != CKD_OK)
+                     ctx, &blurBeams[beamsBlur[j]],
+                     reinterpret_cast<const ckd::B07Job *>(dev + oBB) + beamsBlur[j], 1, W, H, vec4)) != CKD_OK)
                 return s;
         }
         k_compose<<<dim3(ckd_div_up(W, 256), H, g), 256, 0, ctx->stream>>>(

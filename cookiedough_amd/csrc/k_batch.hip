@@ -213,6 +213,8 @@ ckd_status valid_tuning(const ckd_tuning *t)
                 "tuning: sdfBlocksPerCU %d outside 1..32", t->sdfBlocksPerCU);
     CKD_REQUIRE(t->tunnelVSeg >= 8 && t->tunnelVSeg <= 512, "tuning: tunnelVSeg %d outside 8..512",
                 t->tunnelVSeg);
+    CKD_REQUIRE(t->blurKernel >= 0 && t->blurKernel <= 2, "tuning: blurKernel %d not in {0,1,2}",
+                t->blurKernel);
     return CKD_OK;
 }
 
@@ -273,7 +275,7 @@ ckd_status ckd_autotune_batch(ckd_ctx *ctx, const ckd_request *requests, unsigne
         for (int ch : chunks)
             for (int bp : bpc)
                 for (int sg : segs) {
-                    ctx->tuning = ckd_tuning{st, ch, bp, sg};
+                    ctx->tuning = ckd_tuning{st, ch, bp, sg, saved.blurKernel};
                     // warm once, then time the second run
                     s = run_eager(ctx, requests, n, scene, shading, tex, texLog2, tmp, nullptr);
                     if (s == CKD_OK) {

@@ -16,8 +16,9 @@
 // In place (dst == src) the reference's subtract reads pixels it has
 // already overwritten, which makes every line a sequential recurrence with
 // u16 saturation; those kernels run one lane per line and channel and
-// replay it, on lines staged in LDS (k_blur07_lines) or, for lines too long
-// for LDS, reading the image directly (k_blur07_inplace).
+// replay it, on lines staged in LDS (k_blur07_lines) or with an LDS ring of
+// the recent outputs per lane, reading the image directly
+// (k_blur07_ring_*).
 #include "ckd_internal.h"
 #include "ckd_pixel.h"
 #include "ckd_b07.h"
@@ -170,133 +171,6 @@ __global__ __launch_bounds__(256) void k_blur07_v(uint32_t *__restrict__ dst,
     }
 }
 
-// in place: the exact recurrence of one line (code/deprecated/boxblur.cpp:
-// 81-125 / 165-219 with pSrc == pDest).  The four channels are independent
-// recurrences (u16 saturation per channel), so each line takes four lanes,
-// one per channel.  The outputs the subtract needs again `median` steps
-// later live in an LDS ring of `ring` bytes per lane (ring > median, a
-// power of two of at least 4), laid out [entry/4][lane][entry%4] so four
-// consecutive entries of a lane are one dword.  The input pixels are always
-// ahead of the outputs, so they are read from global memory, a chunk of
-// eight ahead.  In the middle run a chunk of eight steps also reads its
-// eight ring entries up front: they were written at least median - 7 >= 1
-// steps before the chunk starts (median >= 8), so the LDS latency is paid
-// once per chunk instead of once per step.
-//
-// Per step the reference adds (p - p>>shift) + the previous step's p>>shift
-// with two saturating u16 adds, and subtracts the same split of the
-// outgoing pixel with two adds clamped at 0; both terms are >= 0, so each
-// pair equals one saturating add / clamped subtract of their sum.
-CKD_DEV uint32_t sat16(uint32_t s) { return s > 65535u ? 65535u : s; }
-
-CKD_DEV uint32_t div_byte(uint32_t acc, uint32_t div)
-{
-    // acc and div are below 2^16: the 24-bit multiply is exact
-    const uint32_t q = __umul24(acc, div) >> 16;
-    return q > 32767u ? 0u : (q > 255u ? 255u : q);
-}
-
-__global__ __launch_bounds__(256) void k_blur07_inplace(uint32_t *img, int nLines,
-                                                        int len, size_t lineStride,
-                                                        size_t pixStride, int ring, B07Params p)
-{
-    extern __shared__ uint8_t ringBuf[];
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int line = t >> 2, k = t & 3;
-    if (line >= nLines) return;
-    uint8_t *lane = ringBuf + threadIdx.x * 4;
-    const int rmask = ring - 1;
-    auto slot = [&](int entry) -> uint8_t & {
-        return lane[(size_t)((entry & rmask) >> 2) * 1024 + (entry & 3)];
-    };
-    // the input bytes read are always ahead of the output bytes written (a
-    // pixel is read as input before its output is stored), so the two
-    // views never touch a byte in conflicting order: restrict lets the
-    // loads issue ahead of the stores
-    const uint8_t *__restrict__ in = reinterpret_cast<const uint8_t *>(img + (size_t)line * lineStride) + k;
-    uint8_t *__restrict__ out = reinterpret_cast<uint8_t *>(img + (size_t)line * lineStride) + k;
-    const size_t step = pixStride * 4;
-    const uint32_t shift = p.shift;
-    uint32_t acc = 0, addRem = 0, subRem = 0;
-    const int e = (int)p.edgeSpan, m = (int)p.median;
-    const int full = len - (m + e);
-    int addPos = 0, dest = 0, subPos = 0;
-    auto add = [&](uint32_t v) {
-        const uint32_t rem = v >> shift;
-        const uint32_t a = addRem + (v - rem);
-        addRem = rem;
-        acc = sat16(acc + a);
-    };
-    auto sub = [&](uint32_t v) {
-        const uint32_t rem = v >> shift;
-        const uint32_t s = subRem + (v - rem);
-        subRem = rem;
-        acc = acc > s ? acc - s : 0u;
-    };
-    const uint8_t *inp = in;  // in + addPos * step
-    uint8_t *outp = out;      // out + dest * step
-    auto emit = [&](uint32_t o) {
-        slot(dest) = (uint8_t)o;
-        *outp = (uint8_t)o;
-        outp += step;
-        ++dest;
-    };
-    auto next_in = [&]() {
-        const uint32_t v = *inp;
-        inp += step;
-        ++addPos;
-        return v;
-    };
-#pragma unroll 8
-    for (int i = 0; i < e; ++i) add(next_in());
-#pragma unroll 8
-    for (int i = 0; i < m; ++i) {
-        add(next_in());
-        emit(div_byte(acc, p.edgeDivs[i]));
-    }
-    // the long middle run: eight inputs at a time, the next eight loaded
-    // before this eight's outputs are stored (program order keeps the loads
-    // ahead of the possibly-aliasing stores)
-    constexpr int kChunk = 8;
-    const int chunks = m >= kChunk ? full / kChunk : 0;
-    uint32_t nxt[kChunk];
-    if (chunks > 0) {
-#pragma unroll
-        for (int j = 0; j < kChunk; ++j) nxt[j] = inp[(size_t)j * step];
-    }
-    for (int c = 0; c < chunks; ++c) {
-        uint32_t cur[kChunk];
-#pragma unroll
-        for (int j = 0; j < kChunk; ++j) cur[j] = nxt[j];
-        addPos += kChunk;
-        inp += kChunk * step;
-        if (c + 1 < chunks) {
-#pragma unroll
-            for (int j = 0; j < kChunk; ++j) nxt[j] = inp[(size_t)j * step];
-        }
-        // subPos is a multiple of 8 here (the middle run starts at 0)
-        const uint32_t w0 = *reinterpret_cast<const uint32_t *>(&slot(subPos));
-        const uint32_t w1 = *reinterpret_cast<const uint32_t *>(&slot(subPos + 4));
-        subPos += kChunk;
-#pragma unroll
-        for (int j = 0; j < kChunk; ++j) {
-            add(cur[j]);
-            sub(((j < 4 ? w0 : w1) >> (8 * (j & 3))) & 0xffu);
-            emit(div_byte(acc, p.fullDiv));
-        }
-    }
-    for (int i = chunks * kChunk; i < full; ++i) {
-        add(next_in());
-        sub(slot(subPos++));
-        emit(div_byte(acc, p.fullDiv));
-    }
-    if (p.subEdges) acc = sat16(acc + addRem);
-#pragma unroll 8
-    for (int i = e; i > 0; --i) {
-        sub(slot(subPos++));
-        emit(div_byte(acc, p.edgeDivs[i - 1]));
-    }
-}
 
 // in place with the lines staged in LDS: a one-wave block copies NL lines
 // into LDS (one dword per pixel, each line padded by a dword so the NL
@@ -536,6 +410,160 @@ __global__ __launch_bounds__(64) void k_blur07_lines_jobs(const B07Job *__restri
                                 edgeDivs, kSplit ? states + (size_t)blockIdx.y * nLines * 12 : nullptr);
 }
 
+// The same in-place blur over a job array with an LDS ring per lane instead
+// of whole lines in LDS, so a batch's lines are all resident at once: one
+// lane per line and channel (64 lines per 256-lane block, lines of
+// blockIdx.y's image).  The recurrence adds position p at step p and emits
+// dest = p - edge; the subtract reads the output written `median` steps
+// earlier, which the ring keeps (ringLen >= median + 8 entries per lane,
+// a power of two, laid out [entry/4][lane][entry%4]).  Every input pixel is
+// read before its own output is stored (dest <= p), so inputs come straight
+// from the image, loaded 24..31 steps ahead in four rotating register
+// chunks of eight; outputs are stored to the image as they are made.
+// kSplit: stop before the line's last pixel is added and save the lane's
+// state, as k_blur07_lines_jobs does.
+// the image in the global address space, addressed by 32-bit byte offsets
+// (images stay below 4 GiB), so loads and stores are global_* instructions
+// whose completion the LDS ring's waits do not include
+typedef __attribute__((address_space(1))) uint8_t global_u8;
+
+template <int K, bool kEven, bool kSplit>
+CKD_DEV void b07_ring_line(global_u8 *img, uint32_t laneOff, uint32_t step, int len,
+                           const B07Params &p, const uint16_t *divs, uint8_t *lane, int rmask,
+                           uint32_t *state)
+{
+    auto slot = [&](int entry) -> uint8_t & {
+        return lane[(size_t)((entry & rmask) >> 2) * 1024 + (entry & 3)];
+    };
+    const int e = (int)p.edgeSpan, m = (int)p.median, em = e + m;
+    const int nAdd = kSplit ? len - 1 : len;
+    const uint32_t fullDiv = p.fullDiv;
+    B07Lane<kEven> s;
+    s.acc = s.addRem = s.subRem = 0;
+    auto emit = [&](int d, uint32_t div) {
+        const uint8_t o = s.out(div);
+        slot(d) = o;
+        img[laneOff + (uint32_t)d * step] = o;
+    };
+    auto load8 = [&](uint32_t (&q)[8], int b) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) q[i] = img[laneOff + (uint32_t)min(b + i, len - 1) * step];
+    };
+    // positions b..b+7
+    auto chunk = [&](const uint32_t (&q)[8], int b) {
+        if (b >= em && b + 8 <= nAdd) {
+#pragma unroll
+            for (int i = 0; i < 8; i += K) {
+                uint32_t v[K];
+#pragma unroll
+                for (int j = 0; j < K; ++j) v[j] = slot(b + i + j - em);
+#pragma unroll
+                for (int j = 0; j < K; ++j) {
+                    s.add(q[i + j]);
+                    s.sub(v[j]);
+                    emit(b + i + j - e, fullDiv);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int pos = b + i;
+                if (pos < nAdd) {
+                    s.add(q[i]);
+                    if (pos >= e) {
+                        if (pos >= em) s.sub(slot(pos - em));
+                        const int d = pos - e;
+                        emit(d, d < m ? (uint32_t)divs[d] : fullDiv);
+                    }
+                }
+            }
+        }
+    };
+    uint32_t q0[8], q1[8], q2[8], q3[8];
+    load8(q0, 0);
+    load8(q1, 8);
+    load8(q2, 16);
+    load8(q3, 24);
+    for (int b = 0; b < nAdd; b += 32) {
+        chunk(q0, b);
+        if (b + 8 >= nAdd) break;
+        load8(q0, b + 32);
+        chunk(q1, b + 8);
+        if (b + 16 >= nAdd) break;
+        load8(q1, b + 40);
+        chunk(q2, b + 16);
+        if (b + 24 >= nAdd) break;
+        load8(q2, b + 48);
+        chunk(q3, b + 24);
+        load8(q3, b + 56);
+    }
+    if constexpr (kSplit) {
+        state[0] = s.acc;
+        state[1] = s.addRem;
+        state[2] = s.subRem;
+        return;
+    }
+    if constexpr (kEven) s.acc = __builtin_elementwise_add_sat((uint16_t)s.acc, (uint16_t)s.addRem);
+    // the end phase: e steps of subtract + emit, descending weights
+    int d = len - e, i = 0;
+    for (; i + K <= e; i += K, d += K) {
+        uint32_t v[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) v[j] = slot(d + j - m);
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            s.sub(v[j]);
+            emit(d + j, divs[e - 1 - (i + j)]);
+        }
+    }
+    for (; i < e; ++i, ++d) {
+        s.sub(slot(d - m));
+        emit(d, divs[e - 1 - i]);
+    }
+}
+
+template <int K, bool kSplit>
+CKD_DEV void b07_ring_block(const B07Job &J, int nLines, int len, int columns, int ringLen,
+                            uint32_t *states, uint8_t *ringBuf, uint16_t *divs)
+{
+    divs[threadIdx.x] = J.p.edgeDivs[threadIdx.x];
+    __syncthreads();
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int line = t >> 2, k = t & 3;
+    if (line >= nLines) return;
+    const uint32_t lineStride = columns ? 1u : (uint32_t)len;
+    const uint32_t step = (columns ? (uint32_t)nLines : 1u) * 4u;
+    global_u8 *img = (global_u8 *)J.img;
+    const uint32_t laneOff = (uint32_t)line * lineStride * 4u + (uint32_t)k;
+    uint8_t *lane = ringBuf + threadIdx.x * 4;
+    uint32_t *st = kSplit ? states + (((size_t)blockIdx.y * nLines + line) * 4 + k) * 3 : nullptr;
+    if (J.p.subEdges)
+        b07_ring_line<K, true, kSplit>(img, laneOff, step, len, J.p, divs, lane, ringLen - 1, st);
+    else
+        b07_ring_line<K, false, kSplit>(img, laneOff, step, len, J.p, divs, lane, ringLen - 1, st);
+}
+
+// blockIdx.y picks the job
+template <int K, bool kSplit>
+__global__ __launch_bounds__(256) void k_blur07_ring_jobs(const B07Job *__restrict__ jobs,
+                                                          int nLines, int len, int columns,
+                                                          int ringLen, uint32_t *__restrict__ states)
+{
+    extern __shared__ uint8_t ringBuf[];
+    __shared__ uint16_t divs[256];
+    b07_ring_block<K, kSplit>(jobs[blockIdx.y], nLines, len, columns, ringLen, states, ringBuf, divs);
+}
+
+// one image, its job passed by value
+template <int K>
+__global__ __launch_bounds__(256) void k_blur07_ring_one(B07Job job, int nLines, int len,
+                                                         int columns, int ringLen)
+{
+    extern __shared__ uint8_t ringBuf[];
+    __shared__ uint16_t divs[256];
+    b07_ring_block<K, false>(job, nLines, len, columns, ringLen, nullptr, ringBuf, divs);
+}
+
 // the chain of a batch of render targets whose last column carries from one
 // image to the next (the ball's frames with beams): one lane per line and
 // channel walks the images in order.  mode 0: the image's last column is
@@ -630,13 +658,6 @@ ckd_status validate(const char *fn, const uint32_t *dst, const uint32_t *src, un
     return CKD_OK;
 }
 
-// power of two > median, at least 4 (the ring holds whole dwords per lane)
-int ring_for(unsigned median)
-{
-    int r = 4;
-    while (r <= (int)median) r <<= 1;
-    return r;
-}
 
 } // namespace
 
@@ -651,16 +672,37 @@ int lines_per_block(int len)
     return 0;
 }
 
+// the ring kernel's ring length (a power of two >= median + 8) and its
+// read-ahead (eight ring entries per chunk while the median allows)
+int ring_len(unsigned mHi)
+{
+    int r = 8;
+    while (r < (int)mHi + 8) r <<= 1;
+    return r;
+}
+int ring_k(unsigned mLo) { return mLo >= 8 ? 8 : mLo >= 4 ? 4 : mLo >= 2 ? 2 : 1; }
+
 ckd_status launch_inplace(ckd_ctx *ctx, uint32_t *img, int nLines, int len, size_t lineStride,
                           size_t pixStride, const B07Params &p, const char *what)
 {
     const int nl = lines_per_block(len);
-    if (nl == 0) {
-        // lines longer than 16383 pixels: the recurrence reads the image
-        const int ring = ring_for(p.median);
-        const size_t lds = (size_t)ring * 256;
-        k_blur07_inplace<<<ckd_div_up((size_t)nLines * 4, 256), 256, lds, ctx->stream>>>(
-            img, nLines, len, lineStride, pixStride, ring, p);
+    if (pixStride != 1 || nl == 0) {
+        // columns (the ring kernel reads them coalesced across the wave,
+        // where staging strided columns in LDS is slow), or lines longer
+        // than LDS holds
+        B07Job job;
+        job.img = img;
+        job.p = p;
+        const int ringLen = ring_len(p.median);
+        const size_t lds = (size_t)ringLen * 256;
+        const int columns = pixStride != 1 ? 1 : 0;
+        const unsigned grid = ckd_div_up((size_t)nLines * 4, 256);
+        switch (ring_k(p.median)) {
+        case 8: k_blur07_ring_one<8><<<grid, 256, lds, ctx->stream>>>(job, nLines, len, columns, ringLen); break;
+        case 4: k_blur07_ring_one<4><<<grid, 256, lds, ctx->stream>>>(job, nLines, len, columns, ringLen); break;
+        case 2: k_blur07_ring_one<2><<<grid, 256, lds, ctx->stream>>>(job, nLines, len, columns, ringLen); break;
+        default: k_blur07_ring_one<1><<<grid, 256, lds, ctx->stream>>>(job, nLines, len, columns, ringLen); break;
+        }
         return ckd::check_launch(what);
     }
     const size_t lds = (size_t)nl * (size_t)(len + 1) * 4;
@@ -725,14 +767,58 @@ void launch_lines_jobs(int nl, dim3 grid, size_t lds, hipStream_t st, const B07J
     }
 }
 
-ckd_status blur07_inplace_jobs(ckd_ctx *ctx, const B07Job *devJobs, unsigned n, unsigned W,
-                               unsigned H, bool columns, bool vec4, uint32_t *splitStates)
+template <bool kSplit>
+void launch_ring_jobs(int K, dim3 grid, size_t lds, hipStream_t st, const B07Job *devJobs,
+                      int nLines, int len, int columns, int ringLen, uint32_t *states)
+{
+    switch (K) {
+    case 8: k_blur07_ring_jobs<8, kSplit><<<grid, 256, lds, st>>>(devJobs, nLines, len, columns, ringLen, states); break;
+    case 4: k_blur07_ring_jobs<4, kSplit><<<grid, 256, lds, st>>>(devJobs, nLines, len, columns, ringLen, states); break;
+    case 2: k_blur07_ring_jobs<2, kSplit><<<grid, 256, lds, st>>>(devJobs, nLines, len, columns, ringLen, states); break;
+    default: k_blur07_ring_jobs<1, kSplit><<<grid, 256, lds, st>>>(devJobs, nLines, len, columns, ringLen, states); break;
+    }
+}
+
+ckd_status blur07_inplace_jobs(ckd_ctx *ctx, const B07Job *hostJobs, const B07Job *devJobs,
+                               unsigned n, unsigned W, unsigned H, bool columns, bool vec4,
+                               uint32_t *splitStates)
 {
     if (n == 0) return CKD_OK;
     const unsigned len = columns ? H : W, nLines = columns ? W : H;
-    const int nl = lines_per_block((int)len);
-    if (nl == 0) return fail(CKD_EINVAL, "blur07 (in place, batch): lines of %u pixels, above 16383", len);
     CKD_REQUIRE(!(splitStates && columns), "blur07 (in place, batch): split blurs run along rows");
+    unsigned mLo = 256, mHi = 0;
+    for (unsigned j = 0; j < n; ++j) {
+        mLo = hostJobs[j].p.median < mLo ? hostJobs[j].p.median : mLo;
+        mHi = hostJobs[j].p.median > mHi ? hostJobs[j].p.median : mHi;
+    }
+    const int nl = lines_per_block((int)len);
+    // the staged-lines kernel holds whole lines in LDS; the ring kernel only
+    // the last median + 8 outputs per lane, so its lines all run at once,
+    // but a step costs it more (each output is stored as it is made).
+    // Automatic choice: the ring kernel for columns (staging strided
+    // columns is slow) and for rows once the staged kernel would need more
+    // than three rounds of blocks on the device.
+    bool ring = ctx->tuning.blurKernel == 2 || nl == 0;
+    if (ctx->tuning.blurKernel == 0 && nl > 0) {
+        const size_t ldsLines = (size_t)nl * (size_t)(len + 1) * 4;
+        const size_t perCU = (160u * 1024u) / ldsLines;
+        const size_t blocks = ckd_div_up((size_t)nLines, (size_t)nl) * n;
+        ring = columns || blocks > 3 * perCU * (size_t)ctx->cuCount;
+    }
+    const char *what = columns ? "blur07_v (in place, batch)" : "blur07_h (in place, batch)";
+    if (ring) {
+        const int ringLen = ring_len(mHi);
+        const int K = ring_k(mLo);
+        const size_t lds = (size_t)ringLen * 256;
+        const dim3 grid(ckd_div_up((size_t)nLines * 4, 256), n);
+        if (splitStates)
+            launch_ring_jobs<true>(K, grid, lds, ctx->stream, devJobs, (int)nLines, (int)len, 0, ringLen,
+                                   splitStates);
+        else
+            launch_ring_jobs<false>(K, grid, lds, ctx->stream, devJobs, (int)nLines, (int)len, columns ? 1 : 0,
+                                    ringLen, nullptr);
+        return check_launch(what);
+    }
     const size_t lds = (size_t)nl * (size_t)(len + 1) * 4;
     const int v4 = vec4 && !columns && (W % 4) == 0;
     const dim3 grid(ckd_div_up((size_t)nLines, (size_t)nl), n);
@@ -741,13 +827,13 @@ ckd_status blur07_inplace_jobs(ckd_ctx *ctx, const B07Job *devJobs, unsigned n, 
     else
         launch_lines_jobs<false>(nl, grid, lds, ctx->stream, devJobs, (int)nLines, (int)len, columns ? 1 : 0, v4,
                                  nullptr);
-    return check_launch(columns ? "blur07_v (in place, batch)" : "blur07_h (in place, batch)");
+    return check_launch(what);
 }
 
-ckd_status blur07_h_inplace_jobs(ckd_ctx *ctx, const B07Job *devJobs, unsigned n, unsigned W,
-                                 unsigned H, bool vec4, uint32_t *splitStates)
+ckd_status blur07_h_inplace_jobs(ckd_ctx *ctx, const B07Job *hostJobs, const B07Job *devJobs,
+                                 unsigned n, unsigned W, unsigned H, bool vec4, uint32_t *splitStates)
 {
-    return blur07_inplace_jobs(ctx, devJobs, n, W, H, false, vec4, splitStates);
+    return blur07_inplace_jobs(ctx, hostJobs, devJobs, n, W, H, false, vec4, splitStates);
 }
 
 ckd_status blur07_h_chain(ckd_ctx *ctx, const B07Chain *devChain, unsigned n, uint32_t *carry,
@@ -807,6 +893,33 @@ ckd_status ckd_blur07(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, unsigned
     s = launch_h(ctx, dst, src, xRes, yRes, p);
     if (s != CKD_OK) return s;
     return launch_v(ctx, dst, dst, xRes, yRes, p);
+}
+
+ckd_status ckd_blur07_inplace_batch(ckd_ctx *ctx, uint32_t *const *imgs, const float *strengths,
+                                    unsigned n, unsigned xRes, unsigned yRes, int passes)
+{
+    CKD_REQUIRE(n == 0 || (imgs && strengths), "ckd_blur07_inplace_batch: null array");
+    CKD_REQUIRE(passes >= 1 && passes <= 3, "ckd_blur07_inplace_batch: passes %d not in 1..3", passes);
+    const unsigned passLen = passes == 1 ? xRes : passes == 2 ? yRes : (xRes < yRes ? xRes : yRes);
+    std::vector<B07Job> jobs(n);
+    bool vec4 = true;
+    for (unsigned j = 0; j < n; ++j) {
+        jobs[j].img = imgs[j];
+        ckd_status s = validate("ckd_blur07_inplace_batch", imgs[j], imgs[j], xRes, yRes,
+                                strengths[j], passLen, jobs[j].p);
+        if (s != CKD_OK) return s;
+        vec4 = vec4 && (reinterpret_cast<uintptr_t>(imgs[j]) & 15) == 0;
+    }
+    CKD_REQUIRE_CTX(ctx);
+    if (n == 0) return CKD_OK;
+    void *dev = nullptr;
+    ckd_status s = ckd::stage(ctx, jobs.data(), n * sizeof(B07Job), &dev);
+    const B07Job *d = static_cast<const B07Job *>(dev);
+    if (s == CKD_OK && (passes & 1))
+        s = ckd::blur07_inplace_jobs(ctx, jobs.data(), d, n, xRes, yRes, false, vec4);
+    if (s == CKD_OK && (passes & 2))
+        s = ckd::blur07_inplace_jobs(ctx, jobs.data(), d, n, xRes, yRes, true, false);
+    return s;
 }
 
 } // extern "C"

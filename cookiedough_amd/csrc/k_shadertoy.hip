@@ -1028,8 +1028,8 @@ ckd_status st_draw(ckd_ctx *ctx, int fx, const P *params, unsigned nFrames, uint
         ckd_status r = ckd::stage(ctx, blurs.data(), blurs.size() * sizeof(ckd::B07Job), &dev);
         const ckd::B07Job *jobs = static_cast<const ckd::B07Job *>(dev);
         const unsigned n = (unsigned)blurs.size();
-        if (r == CKD_OK) r = ckd::blur07_inplace_jobs(ctx, jobs, n, W, H, false, vec4);
-        if (r == CKD_OK) r = ckd::blur07_inplace_jobs(ctx, jobs, n, W, H, true, false);
+        if (r == CKD_OK) r = ckd::blur07_inplace_jobs(ctx, blurs.data(), jobs, n, W, H, false, vec4);
+        if (r == CKD_OK) r = ckd::blur07_inplace_jobs(ctx, blurs.data(), jobs, n, W, H, true, false);
         return r;
     };
     if constexpr (std::is_same<P, ckd_fdtunnel_params>::value) {

@@ -134,6 +134,14 @@ class Context:
     def blur07(self, dst, src, W, H, strength):
         check(self.lib.ckd_blur07(self.h, ptr(dst), ptr(src), W, H, strength))
 
+    def blur07_inplace_batch(self, imgs, strengths, W, H, passes):
+        """in-place blur07 of every image of `imgs` (device tensors of W*H
+        pixels), passes 1 = rows, 2 = columns, 3 = both"""
+        n = len(imgs)
+        arr = (C.c_void_p * max(n, 1))(*[ptr(t) for t in imgs])
+        st = (C.c_float * max(n, 1))(*[float(x) for x in strengths])
+        check(self.lib.ckd_blur07_inplace_batch(self.h, arr, st, n, W, H, passes))
+
     def blur26_h(self, dst, src, W, H, strength, gain, passes):
         check(self.lib.ckd_blur26_h(self.h, ptr(dst), ptr(src), W, H, strength, gain, passes))
 

@@ -123,6 +123,14 @@ CKD_API ckd_status ckd_blur07_v(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src
 /* BoxBlur32 (H, then V in place on dst), code/deprecated/boxblur.cpp:222-231 */
 CKD_API ckd_status ckd_blur07(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src,
                               unsigned xRes, unsigned yRes, float strength);
+/* The in-place forms (pDest == pSrc) of HorizontalBoxBlur32 (passes 1),
+ * VerticalBoxBlur32 (2) or BoxBlur32 (3) on n device images of one size,
+ * each with its own strength: imgs and strengths are host arrays of n.  One
+ * launch per pass for all images; the kernel is the tuning's blurKernel.
+ * Each span must fit every pass it runs. */
+CKD_API ckd_status ckd_blur07_inplace_batch(ckd_ctx *ctx, uint32_t *const *imgs,
+                                            const float *strengths, unsigned n, unsigned xRes,
+                                            unsigned yRes, int passes);
 
 /* ---- 2026 box blur, code/boxblur.cpp -------------------------------------
  * xRes, yRes <= 2048; strength in [0,100]; gain in [0,1]; passes >= 1;
@@ -249,12 +257,15 @@ typedef struct ckd_request {
 } ckd_request;
 
 /* Launch shapes of the batched renderer, chosen by ckd_autotune_batch (or
- * set directly).  Defaults: {2, 32, 8, 32}. */
+ * set directly), and the kernel of the batched in-place 2007 blurs of the
+ * Draw calls.  Defaults: {2, 32, 8, 32, 0}. */
 typedef struct ckd_tuning {
     int sdfStepsPerRefill; /* march steps between lane refills: 1, 2 or 4   */
     int sdfChunk;          /* side of the square pixel chunks: 16 or 32     */
     int sdfBlocksPerCU;    /* persistent 4-wave blocks per CU: 1..32         */
     int tunnelVSeg;        /* rows per lane in the vertical blur: 8..512     */
+    int blurKernel;        /* in-place blur batches: 0 automatic, 1 lines
+                              staged in LDS, 2 an LDS ring per lane          */
 } ckd_tuning;
 CKD_API ckd_status ckd_get_tuning(ckd_ctx *ctx, ckd_tuning *out);
 CKD_API ckd_status ckd_set_tuning(ckd_ctx *ctx, const ckd_tuning *t);

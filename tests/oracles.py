@@ -226,9 +226,11 @@ def ref() -> C.CDLL:
     return _ref
 
 
-def P(a: np.ndarray) -> int:
+def P(a: np.ndarray) -> C.c_void_p:
+    """pointer argument for a C call; it holds a reference to the array, so a
+    temporary (P(np.zeros(n))) stays alive until the call returns"""
     assert a.flags["C_CONTIGUOUS"]
-    return a.ctypes.data
+    return a.ctypes.data_as(C.c_void_p)
 
 
 def aligned_empty(n: int, dtype=np.uint32, align: int = 64, guard: int = 0) -> np.ndarray:

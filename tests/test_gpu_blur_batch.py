@@ -1,0 +1,160 @@
+"""GPU parity of the batched in-place 2007 blur (code/deprecated/boxblur.cpp
+with pDest == pSrc: the recurrence whose subtract reads already-blurred
+pixels) through ckd_blur07_inplace_batch, and of the Draw calls that use it
+(the ball's blurs and its carried last column, Nautilus, the tunnel's glow),
+with each of its two kernels forced through the tuning: lines staged in LDS
+(blurKernel 1) and an LDS ring per lane (blurKernel 2), and the automatic
+choice (0).  Bit-exact against the oracle."""
+from __future__ import annotations
+
+import contextlib
+
+import numpy as np
+import pytest
+
+from helpers import rand_img
+from oracles import P, aligned_empty
+from cookiedough_amd import _lib
+from cookiedough_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+KERNELS = [0, 1, 2]
+
+
+@contextlib.contextmanager
+def blur_kernel(ctx, k):
+    saved = ctx.get_tuning()
+    t = _lib.Tuning.from_buffer_copy(saved)
+    t.blurKernel = k
+    ctx.set_tuning(t)
+    try:
+        yield
+    finally:
+        ctx.set_tuning(saved)
+
+
+def strength_of_span(span):
+    return (span + 0.5) / 255.0
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("passes", [1, 2, 3])
+def test_inplace_batch_matches_oracle(ctx, orc, kernel, passes):
+    rng = np.random.default_rng(100 + 10 * kernel + passes)
+    f_orc = {1: orc.orc_blur07_h, 2: orc.orc_blur07_v, 3: orc.orc_blur07}[passes]
+    for (W, H, n) in [(1280, 720, 5), (644, 364, 7), (300, 260, 9), (37, 23, 6), (8, 300, 3)]:
+        passLen = {1: W, 2: H, 3: min(W, H)}[passes]
+        spans = [int(rng.integers(1, min(255, passLen) + 1)) for _ in range(n)]
+        spans[0] = min(255, passLen)            # the longest span the pass allows
+        spans[-1] = 1
+        print("blur07 batch", kernel, passes, W, H, spans, flush=True)
+        srcs = [rand_img(int(rng.integers(1 << 30)), W * H) for _ in range(n)]
+        dev = [ctx.upload(s) for s in srcs]
+        strengths = [strength_of_span(s) for s in spans]
+        with blur_kernel(ctx, kernel):
+            ctx.blur07_inplace_batch(dev, strengths, W, H, passes)
+            ctx.sync()
+        for j in range(n):
+            want = srcs[j].copy()
+            f_orc(P(want), P(want), W, H, strengths[j], 8)
+            assert np.array_equal(dev[j].numpy(), want), (kernel, passes, W, H, j, spans[j])
+
+
+@pytest.mark.parametrize("kind", ["h", "v"])
+def test_lines_longer_than_lds_holds(ctx, orc, kind):
+    """16384-pixel lines (more than the staged-lines kernel holds): the ring
+    kernel, through the single-image and the batched entry points"""
+    W, H = (16384, 3) if kind == "h" else (3, 16384)
+    for span in [1, 2, 77, 254, 255]:
+        print("long lines", kind, span, flush=True)
+        s = strength_of_span(span)
+        src = rand_img(500 + span, W * H)
+        want = src.copy()
+        (orc.orc_blur07_h if kind == "h" else orc.orc_blur07_v)(P(want), P(want), W, H, s, 8)
+        d = ctx.upload(src)
+        (ctx.blur07_h if kind == "h" else ctx.blur07_v)(d, d, W, H, s)
+        assert np.array_equal(d.numpy(), want), (kind, span)
+        d2 = ctx.upload(src)
+        ctx.blur07_inplace_batch([d2], [s], W, H, 1 if kind == "h" else 2)
+        assert np.array_equal(d2.numpy(), want), (kind, span, "batch")
+
+
+def test_inplace_batch_rejects_bad_input(ctx):
+    d = ctx.empty(64 * 32)
+    with pytest.raises(_lib.CkdError, match="passes"):
+        ctx.blur07_inplace_batch([d], [0.1], 64, 32, 4)
+    with pytest.raises(_lib.CkdError, match="span"):
+        ctx.blur07_inplace_batch([d], [strength_of_span(40)], 64, 32, 3)   # 40 > 32 rows
+    with pytest.raises(_lib.CkdError, match="strength"):
+        ctx.blur07_inplace_batch([d], [float("nan")], 64, 32, 1)
+    ctx.blur07_inplace_batch([], [], 64, 32, 3)                            # nothing to do
+
+
+W, H = 1280, 720
+
+
+@pytest.fixture(scope="module")
+def ball(ctx, orc):
+    bm = wl.ball_maps(W, H)
+    ctx.ball_create(bm, W, H)
+    m0, m1 = np.zeros(2 * W * H, np.int32), np.zeros(2 * W * H, np.int32)
+    orc.orc_polar_maps(P(m0), P(m1), W, H, W, H)
+    return bm, m0
+
+
+def orc_ball(orc, bm, m0, prm, target):
+    d = aligned_empty(W * H)
+    assert orc.orc_ball_draw(P(d), P(target), W, H, P(bm["heightMaps"]), P(bm["colorMaps"]),
+                             P(bm["beamMaps"]), P(bm["envMap"]), P(bm["backgrounds"]),
+                             P(bm["halo"]), P(m0), P(prm.as_array()), 16) == 0
+    return d
+
+
+@pytest.mark.parametrize("kernel", [1, 2])
+def test_ball_batch_with_each_blur_kernel(ctx, orc, ball, kernel):
+    """40 frames (two groups: the carried last column crosses the group
+    boundary), beams and environment map mixed, every blur strength class"""
+    bm, m0 = ball
+    rng = np.random.default_rng(40 + kernel)
+    frames = []
+    for k in range(40):
+        p = wl.ball_frames(1, int(rng.integers(0, 2)), t0=float(rng.uniform(0, 90)))[0]
+        p.blur = float(rng.choice([0.0, 0.5, 3.0, 10.0, 30.0, 99.0, 150.0]))
+        p.hasBeams = float(rng.integers(0, 2))
+        frames.append(p)
+    tgt = ctx.empty(W * H)
+    ctx.memset32(tgt, 0x01020304, W * H)
+    host_t = np.full(W * H, 0x01020304, np.uint32)
+    d = ctx.empty(40 * W * H)
+    with blur_kernel(ctx, kernel):
+        ctx.ball_draw(d, tgt, W, H, frames)
+        ctx.sync()
+    got = d.numpy().reshape(40, -1)
+    for k, prm in enumerate(frames):
+        want = orc_ball(orc, bm, m0, prm, host_t)
+        assert np.array_equal(got[k], want), (kernel, k, prm.blur, prm.hasBeams)
+    assert np.array_equal(tgt.numpy(), host_t)
+
+
+@pytest.mark.parametrize("effect", ["nautilus", "fdtunnel"])
+def test_effect_batches_equal_across_blur_kernels(ctx, effect):
+    """the same 24 frames with either kernel: bit-identical (the blurs are
+    integer; the staged-lines kernel is pinned to the oracle elsewhere)"""
+    if effect == "fdtunnel":
+        ctx.fdtunnel_create(np.ascontiguousarray(wl.noise_texture(11, 10).reshape(-1)),
+                            np.ascontiguousarray(wl.noise_texture(12, 10).reshape(-1)))
+    frames = wl.st_frames(effect, 24, t0=12.0)
+    for k, p in enumerate(frames):
+        if effect == "nautilus":
+            p.blur = [0.0, 5.0, 32.2, 100.0][k % 4]
+        else:
+            p.litBlur = [0.0, 7.0, 40.0, 100.0][k % 4]
+    outs = []
+    for kernel in (1, 2):
+        d = ctx.empty(24 * W * H)
+        with blur_kernel(ctx, kernel):
+            ctx.st_draw(effect, d, W, H, frames)
+            ctx.sync()
+        outs.append(d.numpy())
+    assert np.array_equal(outs[0], outs[1]), effect
