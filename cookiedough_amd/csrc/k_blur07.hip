@@ -412,56 +412,105 @@ __global__ __launch_bounds__(64) void k_blur07_lines_jobs(const B07Job *__restri
 
 // The same in-place blur over a job array with an LDS ring per lane instead
 // of whole lines in LDS, so a batch's lines are all resident at once: one
-// lane per line and channel (64 lines per 256-lane block, lines of
+// lane per line and channel (32 lines per 128-lane block, lines of
 // blockIdx.y's image).  The recurrence adds position p at step p and emits
-// dest = p - edge; the subtract reads the output written `median` steps
-// earlier, which the ring keeps (ringLen >= median + 8 entries per lane,
-// a power of two, laid out [entry/4][lane][entry%4]).  Every input pixel is
-// read before its own output is stored (dest <= p), so inputs come straight
-// from the image, loaded 24..31 steps ahead in four rotating register
-// chunks of eight; outputs are stored to the image as they are made.
-// kSplit: stop before the line's last pixel is added and save the lane's
-// state, as k_blur07_lines_jobs does.
-// the image in the global address space, addressed by 32-bit byte offsets
-// (images stay below 4 GiB), so loads and stores are global_* instructions
-// whose completion the LDS ring's waits do not include
-typedef __attribute__((address_space(1))) uint8_t global_u8;
+// dest = p - edge.  Outputs go to the ring only (ringLen >= max(median + 8,
+// 64) entries per lane, a power of two; each lane's entries are contiguous,
+// lanes ringLen + 4 bytes apart, an odd number of dwords, so the lanes of a
+// wave hit distinct banks): the subtract reads the output written `median`
+// steps earlier from it, and
+// every completed aligned group of four outputs is flushed to the image, the
+// quad of a line exchanging bytes so that each of its lanes stores one whole
+// pixel.  Inputs come straight from the image: every input pixel is read
+// before its own output is stored (dest <= p), so each lane loads one pixel
+// of every group of four, 24..31 steps ahead, and the quad exchange hands
+// each lane its channel of the four.  Per eight steps a lane thus issues two
+// loads and two stores instead of sixteen byte accesses.
+// kSplit: stop before the line's last pixel is added, store every output
+// made and save the lane's state, as k_blur07_lines_jobs does.
+
+// the image in the global address space, addressed by 32-bit pixel offsets
+// (images stay below 4 GiB)
+typedef __attribute__((address_space(1))) uint32_t global_u32;
+
+// 4 x 4 byte transpose across the four lanes of a quad: lane k holds bytes
+// x[k][0..3] and receives x[0..3][k].  Two DPP exchanges (xor 2, then xor
+// 1), each followed by a rotate of the partner's value and a bit-field
+// insert under a per-lane mask, so no lane branches: after the first lanes
+// 0,1 hold [x0 x1 | partner's x0 x1] and lanes 2,3 [partner's x2 x3 | x2 x3];
+// the second does the same with single bytes.
+struct QuadT {
+    uint32_t m1, m2, r2;
+    CKD_DEV explicit QuadT(int k)
+        : m1((k & 2) ? 0xffff0000u : 0x0000ffffu), m2((k & 1) ? 0xff00ff00u : 0x00ff00ffu),
+          r2((k & 1) ? 8u : 24u)
+    {
+    }
+    CKD_DEV static uint32_t rotr(uint32_t x, uint32_t r) { return __builtin_amdgcn_alignbit(x, x, r); }
+    CKD_DEV uint32_t operator()(uint32_t v) const
+    {
+        uint32_t o = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);
+        v = (v & m1) | (rotr(o, 16) & ~m1);
+        o = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);
+        return (v & m2) | (rotr(o, r2) & ~m2);
+    }
+};
 
 template <int K, bool kEven, bool kSplit>
-CKD_DEV void b07_ring_line(global_u8 *img, uint32_t laneOff, uint32_t step, int len,
+CKD_DEV void b07_ring_line(global_u32 *img, uint32_t lineOff, uint32_t pstep, int len, int k,
                            const B07Params &p, const uint16_t *divs, uint8_t *lane, int rmask,
                            uint32_t *state)
 {
-    auto slot = [&](int entry) -> uint8_t & {
-        return lane[(size_t)((entry & rmask) >> 2) * 1024 + (entry & 3)];
-    };
+    auto slot = [&](int entry) -> uint8_t & { return lane[entry & rmask]; };
     const int e = (int)p.edgeSpan, m = (int)p.median, em = e + m;
     const int nAdd = kSplit ? len - 1 : len;
     const uint32_t fullDiv = p.fullDiv;
     B07Lane<kEven> s;
     s.acc = s.addRem = s.subRem = 0;
-    auto emit = [&](int d, uint32_t div) {
-        const uint8_t o = s.out(div);
-        slot(d) = o;
-        img[laneOff + (uint32_t)d * step] = o;
+    const QuadT quad_transpose(k);
+    // outputs 0..flushed-1 are in the image
+    int flushed = 0;
+    auto flush_group = [&](int made) {
+        const uint32_t x = *reinterpret_cast<const uint32_t *>(lane + (flushed & rmask));
+        const uint32_t px = quad_transpose(x);
+        if (flushed + k < made) img[lineOff + (uint32_t)(flushed + k) * pstep] = px;
+        flushed += 4;
     };
-    auto load8 = [&](uint32_t (&q)[8], int b) {
-#pragma unroll
-        for (int i = 0; i < 8; ++i) q[i] = img[laneOff + (uint32_t)min(b + i, len - 1) * step];
+    // after at most eight new outputs: fewer than 4 were pending before, so
+    // at most two groups are complete (no loop, so the waits on the
+    // prefetched loads stay exact)
+    auto flush_to = [&](int made) {
+        if (flushed + 4 <= made) flush_group(made);
+        if (flushed + 4 <= made) flush_group(made);
+    };
+    // every output made, the last group partly
+    auto flush_all = [&](int made) {
+        while (flushed < made) flush_group(made);
+    };
+    // this lane's pixel of the groups at b and b + 4
+    auto load2 = [&](uint32_t (&q)[2], int b) {
+        q[0] = img[lineOff + (uint32_t)min(b + k, len - 1) * pstep];
+        q[1] = img[lineOff + (uint32_t)min(b + 4 + k, len - 1) * pstep];
     };
     // positions b..b+7
-    auto chunk = [&](const uint32_t (&q)[8], int b) {
-        if (b >= em && b + 8 <= nAdd) {
+    auto chunk = [&](const uint32_t (&q)[2], int b) {
+        const uint32_t t0 = quad_transpose(q[0]), t1 = quad_transpose(q[1]);
+        auto in = [&](int i) -> uint32_t { return ((i < 4 ? t0 : t1) >> (8 * (i & 3))) & 0xffu; };
+        const int rd0 = (b - em) & rmask, wr0 = (b - e) & rmask;
+        if (b >= em && b + 8 <= nAdd && rd0 <= rmask - 7 && wr0 <= rmask - 7) {
+            // the middle run, neither ring run wrapping: fixed offsets
+            const uint8_t *rd = lane + rd0;
+            uint8_t *wr = lane + wr0;
 #pragma unroll
             for (int i = 0; i < 8; i += K) {
                 uint32_t v[K];
 #pragma unroll
-                for (int j = 0; j < K; ++j) v[j] = slot(b + i + j - em);
+                for (int j = 0; j < K; ++j) v[j] = rd[i + j];
 #pragma unroll
                 for (int j = 0; j < K; ++j) {
-                    s.add(q[i + j]);
+                    s.add(in(i + j));
                     s.sub(v[j]);
-                    emit(b + i + j - e, fullDiv);
+                    wr[i + j] = s.out(fullDiv);
                 }
             }
         } else {
@@ -469,35 +518,43 @@ CKD_DEV void b07_ring_line(global_u8 *img, uint32_t laneOff, uint32_t step, int 
             for (int i = 0; i < 8; ++i) {
                 const int pos = b + i;
                 if (pos < nAdd) {
-                    s.add(q[i]);
+                    s.add(in(i));
                     if (pos >= e) {
                         if (pos >= em) s.sub(slot(pos - em));
                         const int d = pos - e;
-                        emit(d, d < m ? (uint32_t)divs[d] : fullDiv);
+                        slot(d) = s.out(d < m ? (uint32_t)divs[d] : fullDiv);
                     }
                 }
             }
         }
+        flush_to(max(0, min(b + 8, nAdd) - e));
     };
-    uint32_t q0[8], q1[8], q2[8], q3[8];
-    load8(q0, 0);
-    load8(q1, 8);
-    load8(q2, 16);
-    load8(q3, 24);
+    // the first four chunks' loads in chunk order (the barriers keep the
+    // scheduler from reordering them, which would make the first chunk of
+    // every pass wait for the youngest loads)
+    uint32_t q0[2], q1[2], q2[2], q3[2];
+    load2(q0, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    load2(q1, 8);
+    __builtin_amdgcn_sched_barrier(0);
+    load2(q2, 16);
+    __builtin_amdgcn_sched_barrier(0);
+    load2(q3, 24);
+    __builtin_amdgcn_sched_barrier(0);
+    // four chunks per pass, always (chunks past the end do nothing): one
+    // exit, so the waits on the loads stay exact across the back edge
     for (int b = 0; b < nAdd; b += 32) {
         chunk(q0, b);
-        if (b + 8 >= nAdd) break;
-        load8(q0, b + 32);
+        load2(q0, b + 32);
         chunk(q1, b + 8);
-        if (b + 16 >= nAdd) break;
-        load8(q1, b + 40);
+        load2(q1, b + 40);
         chunk(q2, b + 16);
-        if (b + 24 >= nAdd) break;
-        load8(q2, b + 48);
+        load2(q2, b + 48);
         chunk(q3, b + 24);
-        load8(q3, b + 56);
+        load2(q3, b + 56);
     }
     if constexpr (kSplit) {
+        flush_all(max(0, nAdd - e));
         state[0] = s.acc;
         state[1] = s.addRem;
         state[2] = s.subRem;
@@ -513,39 +570,42 @@ CKD_DEV void b07_ring_line(global_u8 *img, uint32_t laneOff, uint32_t step, int 
 #pragma unroll
         for (int j = 0; j < K; ++j) {
             s.sub(v[j]);
-            emit(d + j, divs[e - 1 - (i + j)]);
+            slot(d + j) = s.out(divs[e - 1 - (i + j)]);
         }
+        flush_to(d + K);
     }
     for (; i < e; ++i, ++d) {
         s.sub(slot(d - m));
-        emit(d, divs[e - 1 - i]);
+        slot(d) = s.out(divs[e - 1 - i]);
     }
+    flush_all(len);
 }
+
+constexpr int kRingBlock = 128; // lanes per block of the ring kernel (32 lines)
 
 template <int K, bool kSplit>
 CKD_DEV void b07_ring_block(const B07Job &J, int nLines, int len, int columns, int ringLen,
                             uint32_t *states, uint8_t *ringBuf, uint16_t *divs)
 {
-    divs[threadIdx.x] = J.p.edgeDivs[threadIdx.x];
+    for (int i = threadIdx.x; i < 256; i += kRingBlock) divs[i] = J.p.edgeDivs[i];
     __syncthreads();
-    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int t = blockIdx.x * kRingBlock + threadIdx.x;
     const int line = t >> 2, k = t & 3;
     if (line >= nLines) return;
-    const uint32_t lineStride = columns ? 1u : (uint32_t)len;
-    const uint32_t step = (columns ? (uint32_t)nLines : 1u) * 4u;
-    global_u8 *img = (global_u8 *)J.img;
-    const uint32_t laneOff = (uint32_t)line * lineStride * 4u + (uint32_t)k;
-    uint8_t *lane = ringBuf + threadIdx.x * 4;
+    const uint32_t lineOff = (uint32_t)line * (columns ? 1u : (uint32_t)len);
+    const uint32_t pstep = columns ? (uint32_t)nLines : 1u;
+    global_u32 *img = (global_u32 *)J.img;
+    uint8_t *lane = ringBuf + threadIdx.x * (ringLen + 4);
     uint32_t *st = kSplit ? states + (((size_t)blockIdx.y * nLines + line) * 4 + k) * 3 : nullptr;
     if (J.p.subEdges)
-        b07_ring_line<K, true, kSplit>(img, laneOff, step, len, J.p, divs, lane, ringLen - 1, st);
+        b07_ring_line<K, true, kSplit>(img, lineOff, pstep, len, k, J.p, divs, lane, ringLen - 1, st);
     else
-        b07_ring_line<K, false, kSplit>(img, laneOff, step, len, J.p, divs, lane, ringLen - 1, st);
+        b07_ring_line<K, false, kSplit>(img, lineOff, pstep, len, k, J.p, divs, lane, ringLen - 1, st);
 }
 
 // blockIdx.y picks the job
 template <int K, bool kSplit>
-__global__ __launch_bounds__(256) void k_blur07_ring_jobs(const B07Job *__restrict__ jobs,
+__global__ __launch_bounds__(kRingBlock) void k_blur07_ring_jobs(const B07Job *__restrict__ jobs,
                                                           int nLines, int len, int columns,
                                                           int ringLen, uint32_t *__restrict__ states)
 {
@@ -556,7 +616,7 @@ __global__ __launch_bounds__(256) void k_blur07_ring_jobs(const B07Job *__restri
 
 // one image, its job passed by value
 template <int K>
-__global__ __launch_bounds__(256) void k_blur07_ring_one(B07Job job, int nLines, int len,
+__global__ __launch_bounds__(kRingBlock) void k_blur07_ring_one(B07Job job, int nLines, int len,
                                                          int columns, int ringLen)
 {
     extern __shared__ uint8_t ringBuf[];
@@ -672,14 +732,15 @@ int lines_per_block(int len)
     return 0;
 }
 
-// the ring kernel's ring length (a power of two >= median + 8) and its
+// the ring kernel's ring length (a power of two >= max(median + 8, 64)), its LDS and its
 // read-ahead (eight ring entries per chunk while the median allows)
 int ring_len(unsigned mHi)
 {
-    int r = 8;
+    int r = 64;
     while (r < (int)mHi + 8) r <<= 1;
     return r;
 }
+size_t ring_lds(int ringLen) { return (size_t)kRingBlock * (size_t)(ringLen + 4); }
 int ring_k(unsigned mLo) { return mLo >= 8 ? 8 : mLo >= 4 ? 4 : mLo >= 2 ? 2 : 1; }
 
 ckd_status launch_inplace(ckd_ctx *ctx, uint32_t *img, int nLines, int len, size_t lineStride,
@@ -694,14 +755,14 @@ ckd_status launch_inplace(ckd_ctx *ctx, uint32_t *img, int nLines, int len, size
         job.img = img;
         job.p = p;
         const int ringLen = ring_len(p.median);
-        const size_t lds = (size_t)ringLen * 256;
+        const size_t lds = ring_lds(ringLen);
         const int columns = pixStride != 1 ? 1 : 0;
-        const unsigned grid = ckd_div_up((size_t)nLines * 4, 256);
+        const unsigned grid = ckd_div_up((size_t)nLines * 4, kRingBlock);
         switch (ring_k(p.median)) {
-        case 8: k_blur07_ring_one<8><<<grid, 256, lds, ctx->stream>>>(job, nLines, len, columns, ringLen); break;
-        case 4: k_blur07_ring_one<4><<<grid, 256, lds, ctx->stream>>>(job, nLines, len, columns, ringLen); break;
-        case 2: k_blur07_ring_one<2><<<grid, 256, lds, ctx->stream>>>(job, nLines, len, columns, ringLen); break;
-        default: k_blur07_ring_one<1><<<grid, 256, lds, ctx->stream>>>(job, nLines, len, columns, ringLen); break;
+        case 8: k_blur07_ring_one<8><<<grid, kRingBlock, lds, ctx->stream>>>(job, nLines, len, columns, ringLen); break;
+        case 4: k_blur07_ring_one<4><<<grid, kRingBlock, lds, ctx->stream>>>(job, nLines, len, columns, ringLen); break;
+        case 2: k_blur07_ring_one<2><<<grid, kRingBlock, lds, ctx->stream>>>(job, nLines, len, columns, ringLen); break;
+        default: k_blur07_ring_one<1><<<grid, kRingBlock, lds, ctx->stream>>>(job, nLines, len, columns, ringLen); break;
         }
         return ckd::check_launch(what);
     }
@@ -772,10 +833,10 @@ void launch_ring_jobs(int K, dim3 grid, size_t lds, hipStream_t st, const B07Job
                       int nLines, int len, int columns, int ringLen, uint32_t *states)
 {
     switch (K) {
-    case 8: k_blur07_ring_jobs<8, kSplit><<<grid, 256, lds, st>>>(devJobs, nLines, len, columns, ringLen, states); break;
-    case 4: k_blur07_ring_jobs<4, kSplit><<<grid, 256, lds, st>>>(devJobs, nLines, len, columns, ringLen, states); break;
-    case 2: k_blur07_ring_jobs<2, kSplit><<<grid, 256, lds, st>>>(devJobs, nLines, len, columns, ringLen, states); break;
-    default: k_blur07_ring_jobs<1, kSplit><<<grid, 256, lds, st>>>(devJobs, nLines, len, columns, ringLen, states); break;
+    case 8: k_blur07_ring_jobs<8, kSplit><<<grid, kRingBlock, lds, st>>>(devJobs, nLines, len, columns, ringLen, states); break;
+    case 4: k_blur07_ring_jobs<4, kSplit><<<grid, kRingBlock, lds, st>>>(devJobs, nLines, len, columns, ringLen, states); break;
+    case 2: k_blur07_ring_jobs<2, kSplit><<<grid, kRingBlock, lds, st>>>(devJobs, nLines, len, columns, ringLen, states); break;
+    default: k_blur07_ring_jobs<1, kSplit><<<grid, kRingBlock, lds, st>>>(devJobs, nLines, len, columns, ringLen, states); break;
     }
 }
 
@@ -793,24 +854,24 @@ ckd_status blur07_inplace_jobs(ckd_ctx *ctx, const B07Job *hostJobs, const B07Jo
     }
     const int nl = lines_per_block((int)len);
     // the staged-lines kernel holds whole lines in LDS; the ring kernel only
-    // the last median + 8 outputs per lane, so its lines all run at once,
-    // but a step costs it more (each output is stored as it is made).
+    // the last outputs per lane, so its lines all run at once, but one line
+    // takes it longer (about 85 against 50 ns a step at 1280 px, measured).
     // Automatic choice: the ring kernel for columns (staging strided
     // columns is slow) and for rows once the staged kernel would need more
-    // than three rounds of blocks on the device.
+    // than one round of blocks on the device.
     bool ring = ctx->tuning.blurKernel == 2 || nl == 0;
     if (ctx->tuning.blurKernel == 0 && nl > 0) {
         const size_t ldsLines = (size_t)nl * (size_t)(len + 1) * 4;
         const size_t perCU = (160u * 1024u) / ldsLines;
         const size_t blocks = ckd_div_up((size_t)nLines, (size_t)nl) * n;
-        ring = columns || blocks > 3 * perCU * (size_t)ctx->cuCount;
+        ring = columns || blocks > perCU * (size_t)ctx->cuCount;
     }
     const char *what = columns ? "blur07_v (in place, batch)" : "blur07_h (in place, batch)";
     if (ring) {
         const int ringLen = ring_len(mHi);
         const int K = ring_k(mLo);
-        const size_t lds = (size_t)ringLen * 256;
-        const dim3 grid(ckd_div_up((size_t)nLines * 4, 256), n);
+        const size_t lds = ring_lds(ringLen);
+        const dim3 grid(ckd_div_up((size_t)nLines * 4, kRingBlock), n);
         if (splitStates)
             launch_ring_jobs<true>(K, grid, lds, ctx->stream, devJobs, (int)nLines, (int)len, 0, ringLen,
                                    splitStates);
