@@ -101,6 +101,15 @@ struct ckd_ctx {
 };
 
 namespace ckd {
+// Fx_Blit_2x2 of nFrames FX maps ((W/2+4) x (H/2+4) each, back to back)
+// into nFrames W x H frames (back to back), one launch; sizes as
+// ckd_fx_blit_2x2 checks them
+ckd_status fx_blit_frames(ckd_ctx *ctx, uint32_t *dst, const uint32_t *maps, unsigned dstW,
+                          unsigned dstH, unsigned nFrames);
+// dst = sat(dst + src) per byte over `rows` rows of `width` pixels (Add32 /
+// BlitAdd32), rows destStride / srcStride pixels apart
+ckd_status launch_blit_add(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, unsigned destStride,
+                           unsigned srcStride, unsigned width, unsigned rows);
 
 // Records a message for ckd_last_error() on this thread and returns s.
 ckd_status fail(ckd_status s, const char *fmt, ...)

@@ -26,11 +26,15 @@ __global__ __launch_bounds__(256) void k_memset32(uint4 *__restrict__ dst4, uint
 __global__ __launch_bounds__(256) void k_fx_blit_2x2(uint32_t *__restrict__ dst,
                                                      const uint32_t *__restrict__ src,
                                                      unsigned dstW, unsigned mapW,
-                                                     unsigned halfW)
+                                                     unsigned halfW, size_t dstFrame,
+                                                     size_t mapFrame)
 {
     const unsigned x = blockIdx.x * blockDim.x + threadIdx.x;
     const unsigned y = blockIdx.y;
     if (x >= halfW) return;
+    // blockIdx.z: the frame of a batch, frames dstFrame / mapFrame pixels apart
+    dst += blockIdx.z * dstFrame;
+    src += blockIdx.z * mapFrame;
     const uint32_t *r0 = src + (size_t)y * mapW + x;
     const uint32_t *r1 = r0 + mapW;
     const uint32_t s00 = r0[0], s10 = r0[1], s01 = r1[0], s11 = r1[1];
@@ -173,6 +177,25 @@ ckd_status launch_blit_add(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, uns
 
 } // namespace ckd
 
+namespace ckd {
+
+ckd_status fx_blit_frames(ckd_ctx *ctx, uint32_t *dst, const uint32_t *maps, unsigned dstW,
+                          unsigned dstH, unsigned nFrames)
+{
+    if (nFrames == 0) return CKD_OK;
+    const unsigned halfW = dstW / 2, mapW = halfW + 4;
+    const size_t mapFrame = (size_t)mapW * (dstH / 2 + 4), dstFrame = (size_t)dstW * dstH;
+    for (unsigned f0 = 0; f0 < nFrames; f0 += 65535) {
+        const unsigned n = nFrames - f0 < 65535 ? nFrames - f0 : 65535;
+        dim3 grid(ckd_div_up(halfW, 256), dstH / 2, n);
+        k_fx_blit_2x2<<<grid, 256, 0, ctx->stream>>>(dst + f0 * dstFrame, maps + f0 * mapFrame, dstW,
+                                                     mapW, halfW, dstFrame, mapFrame);
+    }
+    return check_launch("fx_blit_2x2");
+}
+
+} // namespace ckd
+
 extern "C" {
 
 ckd_status ckd_tape_warp32(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, unsigned xRes,
@@ -216,10 +239,7 @@ ckd_status ckd_fx_blit_2x2(ckd_ctx *ctx, uint32_t *dst, const uint32_t *fxMap, u
                 "ckd_fx_blit_2x2: %ux%u: need even sizes with (W/2) %% 4 == 0", dstW, dstH);
     CKD_REQUIRE(((uintptr_t)dst & 7) == 0, "ckd_fx_blit_2x2: dst must be 8-byte aligned");
     CKD_REQUIRE_CTX(ctx);
-    const unsigned halfW = dstW / 2, mapW = halfW + 4;
-    dim3 grid(ckd_div_up(halfW, 256), dstH / 2);
-    k_fx_blit_2x2<<<grid, 256, 0, ctx->stream>>>(dst, fxMap, dstW, mapW, halfW);
-    return ckd::check_launch("fx_blit_2x2");
+    return ckd::fx_blit_frames(ctx, dst, fxMap, dstW, dstH, 1);
 }
 
 ckd_status ckd_add32(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, unsigned numPixels)

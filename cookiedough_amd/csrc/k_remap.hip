@@ -101,8 +101,6 @@ ckd_status launch_tunnel_jobs(ckd_ctx *ctx, const TunJob *jobs, int nJobs, const
                               const uint32_t *tex, unsigned texLog2);
 ckd_status blur07_hv_oop(ckd_ctx *ctx, uint32_t *tmp, uint32_t *dst, const uint32_t *src,
                          unsigned W, unsigned H, float strength);
-ckd_status launch_blit_add(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, unsigned destStride,
-                           unsigned srcStride, unsigned width, unsigned rows);
 
 ckd_status launch_grid_remap(ckd_ctx *ctx, uint32_t *dst, unsigned W, unsigned H,
                              const int32_t *grid, const uint32_t *tex, unsigned texLog2)

@@ -1017,10 +1017,12 @@ ckd_status st_draw(ckd_ctx *ctx, int fx, const P *params, unsigned nFrames, uint
     uint32_t *maps = scratch, *glows = glow ? scratch + mapPx * nFrames : nullptr;
     uint32_t *work = scratch + mapPx * nFrames * (glow ? 2 : 1);
     if ((s = st_maps(ctx, fx, params, nFrames, maps, glows, resX, resY, close)) != CKD_OK) return s;
-    // the frames' in-place blurs (independent per frame) run as one batch:
-    // every row pass in one launch, then every column pass
+    CKD_REQUIRE(((uintptr_t)dst & 7) == 0, "st_draw: dst must be 8-byte aligned");
+    // The frames are independent, so each stage runs for all of them at once:
+    // the map's own post-processing, one Fx_Blit_2x2 launch, the output's
+    // blur.  In-place blurs of many frames are one launch per pass.
     std::vector<ckd::B07Job> blurs;
-    auto run_blurs = [&](unsigned W, unsigned H) -> ckd_status {
+    auto run_blurs = [&](unsigned W, unsigned H, bool columns) -> ckd_status {
         if (blurs.empty()) return CKD_OK;
         bool vec4 = true;
         for (const ckd::B07Job &j : blurs) vec4 = vec4 && (reinterpret_cast<uintptr_t>(j.img) & 15) == 0;
@@ -1029,54 +1031,71 @@ ckd_status st_draw(ckd_ctx *ctx, int fx, const P *params, unsigned nFrames, uint
         const ckd::B07Job *jobs = static_cast<const ckd::B07Job *>(dev);
         const unsigned n = (unsigned)blurs.size();
         if (r == CKD_OK) r = ckd::blur07_inplace_jobs(ctx, blurs.data(), jobs, n, W, H, false, vec4);
-        if (r == CKD_OK) r = ckd::blur07_inplace_jobs(ctx, blurs.data(), jobs, n, W, H, true, false);
+        if (r == CKD_OK && columns) r = ckd::blur07_inplace_jobs(ctx, blurs.data(), jobs, n, W, H, true, false);
+        blurs.clear();
         return r;
     };
+    if constexpr (std::is_same<P, ckd_spikey_params>::value) {
+        // Spikey_Draw, code/shadertoy.cpp:663-725: the close-up's mix-blur
+        // layer (its two work images are shared, so frame after frame), or
+        // the distant spike's HorizontalBoxBlur32 of the map in place
+        for (unsigned k = 0; k < nFrames; ++k) {
+            uint32_t *map = maps + mapPx * k;
+            if (close) {
+                if ((s = spikey_mix_blur(ctx, params[k], map, mapW, mapH, work, work + mapPx)) != CKD_OK)
+                    return s;
+            } else if (params[k].distWarmup != 0.f) {
+                blurs.emplace_back();
+                if ((s = ckd::b07_h_job(map, mapW, mapH, box_blur_scale(1.f + params[k].distWarmup),
+                                        blurs.back())) != CKD_OK)
+                    return s;
+            }
+        }
+        if ((s = run_blurs(mapW, mapH, false)) != CKD_OK) return s;
+    }
     if constexpr (std::is_same<P, ckd_fdtunnel_params>::value) {
-        // Tunnel_Draw, code/shadertoy.cpp:842-853: the glow map's BoxBlur32
+        // Tunnel_Draw, code/shadertoy.cpp:842-853: the glow map's BoxBlur32,
+        // then Add32 of the glow onto the map
+        bool allLit = true;
         for (unsigned k = 0; k < nFrames; ++k) {
             const float lb = params[k].litBlur;
             const float litBlur = (0.f < ((lb < 100.f) ? lb : 100.f)) ? ((lb < 100.f) ? lb : 100.f) : 0.f;
-            if (0 != (int)roundf(params[k].litTiles) && litBlur >= 1.f) {
+            const bool lit = 0 != (int)roundf(params[k].litTiles);
+            allLit = allLit && lit;
+            if (lit && litBlur >= 1.f) {
                 blurs.emplace_back();
                 if ((s = ckd::b07_job(glows + mapPx * k, mapW, mapH, box_blur_scale(litBlur),
                                       blurs.back())) != CKD_OK)
                     return s;
             }
         }
-        if ((s = run_blurs(mapW, mapH)) != CKD_OK) return s;
+        if ((s = run_blurs(mapW, mapH, true)) != CKD_OK) return s;
+        const size_t all = mapPx * nFrames;
+        if (allLit && all <= 0xffffffffu) {
+            // maps and glow maps are both back to back: one Add32
+            if ((s = ckd::launch_blit_add(ctx, maps, glows, (unsigned)all, (unsigned)all, (unsigned)all,
+                                          1)) != CKD_OK)
+                return s;
+        } else {
+            for (unsigned k = 0; k < nFrames; ++k)
+                if (0 != (int)roundf(params[k].litTiles) &&
+                    (s = ckd_add32(ctx, maps + mapPx * k, glows + mapPx * k, (unsigned)mapPx)) != CKD_OK)
+                    return s;
+        }
     }
-    for (unsigned k = 0; k < nFrames; ++k) {
-        uint32_t *map = maps + mapPx * k, *out = dst + outPx * k;
-        if constexpr (std::is_same<P, ckd_spikey_params>::value) {
-            // Spikey_Draw, code/shadertoy.cpp:663-725
-            if (close && (s = spikey_mix_blur(ctx, params[k], map, mapW, mapH, work,
-                                              work + mapPx)) != CKD_OK)
-                return s;
-            if (!close && params[k].distWarmup != 0.f &&
-                (s = ckd_blur07_h(ctx, map, map, mapW, mapH,
-                                  box_blur_scale(1.f + params[k].distWarmup))) != CKD_OK)
-                return s;
-        }
-        if constexpr (std::is_same<P, ckd_fdtunnel_params>::value) {
-            // Tunnel_Draw, code/shadertoy.cpp:842-853
-            // (the glow map was blurred above)
-            if (0 != (int)roundf(params[k].litTiles) &&
-                (s = ckd_add32(ctx, map, glows + mapPx * k, (unsigned)mapPx)) != CKD_OK)
-                return s;
-        }
-        if ((s = ckd_fx_blit_2x2(ctx, out, map, resX, resY)) != CKD_OK) return s;
-        if constexpr (std::is_same<P, ckd_nautilus_params>::value) {
-            // Nautilus_Draw, code/shadertoy.cpp:385-396: BoxBlur32 of the
-            // output in place, run for every frame below
+    if ((s = ckd::fx_blit_frames(ctx, dst, maps, resX, resY, nFrames)) != CKD_OK) return s;
+    if constexpr (std::is_same<P, ckd_nautilus_params>::value) {
+        // Nautilus_Draw, code/shadertoy.cpp:385-396: BoxBlur32 of the output
+        // in place
+        for (unsigned k = 0; k < nFrames; ++k) {
             const float blur = box_blur_scale(params[k].blur);
             if (0.f != blur) {
                 blurs.emplace_back();
-                if ((s = ckd::b07_job(out, resX, resY, blur, blurs.back())) != CKD_OK) return s;
+                if ((s = ckd::b07_job(dst + outPx * k, resX, resY, blur, blurs.back())) != CKD_OK) return s;
             }
         }
+        return run_blurs(resX, resY, true);
     }
-    if constexpr (std::is_same<P, ckd_nautilus_params>::value) return run_blurs(resX, resY);
     return CKD_OK;
 }
 
