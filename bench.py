@@ -34,6 +34,7 @@ from cookiedough_amd._lib import (CKD_FX_SDF, CKD_FX_TUNNEL, CKD_FX_VOXEL,  # no
 
 PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 vector (packed) peak
 PEAK_HBM_GBS = 8000.0
+PEAK_FP32_UNPACKED_TFLOPS = 78.6  # without packed FP32 math
 
 
 def log(*a):
@@ -274,11 +275,33 @@ def main() -> int:
                 plain()
                 batched()
                 mp = T.run(plain)
+                st0 = int(pcount.numpy()[0])
                 mb = T.run(batched)
+                steps_b = int(pcount.numpy()[0]) - st0
                 px = frames * W * H
                 per[name] = {"frames": frames, "plain_fp32_mpx_s": round(px / mp / 1e3, 1),
                              "batched_fp16_mpx_s": round(px / mb / 1e3, 1),
                              "speedup": round(mp / mb, 3)}
+                # the batched call against its bound, whole-call time
+                if name == "sdf_c0":
+                    ach = sdf_flops(steps_b, px, 8) / (mb * 1e-3) / 1e12
+                    rl = {"bound": "valu", "achieved": round(ach, 2), "peak": PEAK_FP32_TFLOPS,
+                          "unit": "TFLOP/s", "algorithmic": "sum(steps)*97 + px*580 FLOP"}
+                elif name == "voxel_c1":
+                    # 80 integer ops per column step, 10 per span pixel,
+                    # counted as 2 FLOP each against the unpacked FP32 rate
+                    ops = frames * W * 512 * 80 + px * 10
+                    ach = 2 * ops / (mb * 1e-3) / 1e12
+                    rl = {"bound": "valu", "achieved": round(ach, 2), "peak": PEAK_FP32_UNPACKED_TFLOPS,
+                          "unit": "TFLOP/s (2 x int ops)",
+                          "algorithmic": "512*80 ops per column + 10 per pixel"}
+                else:
+                    # remap 4 B/px written, blur H+V 8 B/px, BlitAdd32 12 B/px
+                    ach = 24 * px / (mb * 1e-3) / 1e9
+                    rl = {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                          "algorithmic": "24 B per pixel"}
+                rl["frac"] = round(ach / rl["peak"], 4)
+                per[name]["roofline"] = rl
                 log(name, per[name])
 
         if "c3" in sections:
@@ -388,6 +411,43 @@ def main() -> int:
             from helpers import error_stats  # noqa: E402
             orc = oracles.oracle()
             nthr = min(16, os.cpu_count() or 1)
+            if "classic" in sections:
+                # c0: the batched FP16 frames against the FP64 render on a row
+                # sample; c1 / c2 (integer paths): first and last frame against
+                # the oracle, which must be bit-exact
+                sdf_batched()
+                errs = []
+                for k in [0, 63]:
+                    img = fr_sdf.view(k * W * H, (W * H,)).numpy().reshape(H, W)
+                    rows = np.arange(5, H, 24)
+                    ref = np.zeros(len(rows) * W, np.uint32)
+                    orc.orc_sdf_rows(oracles.P(ref), W, H, C.byref(scene8), k / 60, 64, 5, 24, None, nthr)
+                    errs.append((img[rows].reshape(-1), ref))
+                per["sdf_c0"]["error_vs_fp64"] = error_stats(np.concatenate([e[0] for e in errs]),
+                                                             np.concatenate([e[1] for e in errs]))
+                vox_batched()
+                got, want = [], []
+                for k in [0, 119]:
+                    buf = np.zeros(W * H, np.uint32)
+                    oc = oracles.orc_cam(lcams[k])
+                    orc.orc_landscape_frame(oracles.P(buf), W, H, oracles.P(maps.height),
+                                            oracles.P(maps.color), oracles.P(maps.fog), C.byref(oc), nthr)
+                    got.append(fr_vox.view(k * W * H, (W * H,)).numpy())
+                    want.append(buf)
+                per["voxel_c1"]["error_vs_oracle"] = error_stats(np.concatenate(got), np.concatenate(want))
+                tun_batched()
+                got, want = [], []
+                for k in [0, 239]:
+                    buf = np.zeros(W * H, np.uint32)
+                    b0, b1 = np.zeros(W * H, np.uint32), np.zeros(W * H, np.uint32)
+                    g = wl.tunnel_grid(W, H, k / 60)
+                    orc.orc_tunnel_frame(oracles.P(buf), oracles.P(b0), oracles.P(b1), W, H, oracles.P(g),
+                                         oracles.P(tex), 9, int(radii[k]), nthr)
+                    got.append(fr_tun.view(k * W * H, (W * H,)).numpy())
+                    want.append(buf)
+                per["tunnel_c2"]["error_vs_oracle"] = error_stats(np.concatenate(got), np.concatenate(want))
+                log("classic errors", {k: per[k].get("error_vs_fp64", per[k].get("error_vs_oracle"))
+                                       for k in ("sdf_c0", "voxel_c1", "tunnel_c2")})
             if "c3" in sections:
                 # precision form: FP16 shading vs the FP64 render on a row sample
                 errs = []
