@@ -260,7 +260,7 @@ CKD_DEV void b07_run_sub(B07Lane<kEven> &s, int n, int median, Div div)
 // the whole line (code/deprecated/boxblur.cpp:81-125).  kSplit: stop just
 // before the step that adds the line's last pixel (the middle run's last
 // step; the caller guarantees full >= 1) and save the lane's state to
-// state[0..2]; b07_tail finishes the line later.
+// state[0..2]; k_blur07_chain (b07_chain_tail) finishes the line later.
 template <bool kEven, bool kSplit = false>
 CKD_DEV void b07_lds_line(uint8_t *px, int len, const B07Params &p, const uint16_t *edgeDivs,
                           uint32_t *state = nullptr)
@@ -418,10 +418,9 @@ __global__ __launch_bounds__(64) void k_blur07_lines_jobs(const B07Job *__restri
 // 64) entries per lane, a power of two; each lane's entries are contiguous,
 // lanes ringLen + 4 bytes apart, an odd number of dwords, so the lanes of a
 // wave hit distinct banks): the subtract reads the output written `median`
-// steps earlier from it, and
-// every completed aligned group of four outputs is flushed to the image, the
-// quad of a line exchanging bytes so that each of its lanes stores one whole
-// pixel.  Inputs come straight from the image: every input pixel is read
+// steps earlier from it, and every completed aligned group of four outputs
+// is flushed to the image, the quad of a line exchanging bytes so that each
+// of its lanes stores one whole pixel.  Inputs come straight from the image: every input pixel is read
 // before its own output is stored (dest <= p), so each lane loads one pixel
 // of every group of four, 24..31 steps ahead, and the quad exchange hands
 // each lane its channel of the four.  Per eight steps a lane thus issues two
