@@ -158,3 +158,43 @@ def test_effect_batches_equal_across_blur_kernels(ctx, effect):
             ctx.sync()
         outs.append(d.numpy())
     assert np.array_equal(outs[0], outs[1]), effect
+
+
+def test_ball_blur_spanning_the_whole_row():
+    """a ball 200 pixels wide whose blur span is exactly the row (blur 78.6:
+    span 200): the frames with beams cannot split their blurs before the
+    carried last column, so they go one after another; against the oracle
+    with the target carried over, beams and environment map mixed"""
+    import oracles
+    from cookiedough_amd.device import Context
+    W2, H2 = 200, 100
+    orc = oracles.oracle()
+    bm = wl.ball_maps(W2, H2)
+    m0, m1 = np.zeros(2 * W2 * H2, np.int32), np.zeros(2 * W2 * H2, np.int32)
+    orc.orc_polar_maps(P(m0), P(m1), W2, H2, W2, H2)
+    frames = []
+    for k in range(6):
+        p = wl.ball_frames(1, k % 2, t0=5.0 + k)[0]
+        p.radius = 150.0                  # heights <= 160: projections stay below 94 px
+        p.blur = 78.6 if k != 3 else 12.0
+        p.hasBeams = 1.0 if k in (0, 1, 3, 4) else 0.0
+        frames.append(p)
+    c = Context(0)
+    try:
+        c.ball_create(bm, W2, H2)
+        tgt = c.empty(W2 * H2)
+        c.memset32(tgt, 0x05060708, W2 * H2)
+        host_t = np.full(W2 * H2, 0x05060708, np.uint32)
+        d = c.empty(len(frames) * W2 * H2)
+        c.ball_draw(d, tgt, W2, H2, frames)
+        c.sync()
+        got = d.numpy().reshape(len(frames), -1)
+        for k, prm in enumerate(frames):
+            want = aligned_empty(W2 * H2)
+            assert orc.orc_ball_draw(P(want), P(host_t), W2, H2, P(bm["heightMaps"]), P(bm["colorMaps"]),
+                                     P(bm["beamMaps"]), P(bm["envMap"]), P(bm["backgrounds"]),
+                                     P(bm["halo"]), P(m0), P(prm.as_array()), 8) == 0
+            assert np.array_equal(got[k], want), (k, prm.blur, prm.hasBeams)
+        assert np.array_equal(tgt.numpy(), host_t)
+    finally:
+        c.close()
