@@ -23,6 +23,10 @@ struct ckd_ctx {
     // beside the persistent SDF kernel), joined through two events
     hipStream_t aux = nullptr;
     hipEvent_t forkEv = nullptr, joinEv = nullptr;
+    // third stream: the batch's tunnel work beside its voxel work, joined
+    // into the second through tunEv
+    hipStream_t aux2 = nullptr;
+    hipEvent_t tunEv = nullptr;
 
     // interpolated cosine LUT, 2049 floats (CalculateCosLUT), on the
     // device and the host copy the per-frame set-up reads

@@ -160,6 +160,19 @@ ckd_status launch_plan(ckd_ctx *ctx, const Plan &P, unsigned char *dev, const ck
                                        *scene, shading, stepCount, false);
         if (s != CKD_OK) return s;
     }
+    // with both voxel and tunnel work beside the march, the tunnel work runs
+    // on a third stream, beside the voxel work
+    const bool split = fork && P.nVox > 0 && P.nTun > 0;
+    if (split) {
+        CKD_HIP(hipStreamWaitEvent(ctx->aux2, ctx->forkEv, 0));
+        ckd::StreamSwap swap(ctx, ctx->aux2);
+        TunJob none;
+        memset(&none, 0, sizeof(none));
+        s = ckd::launch_tunnel_jobs(ctx, reinterpret_cast<const TunJob *>(dev + P.oTun), P.nTun, none,
+                                    P.tunRows, P.tunV, P.tunMaxW, tex, texLog2);
+        if (s != CKD_OK) return s;
+        CKD_HIP(hipEventRecord(ctx->tunEv, ctx->aux2));
+    }
     {
         ckd::StreamSwap swap(ctx, fork ? ctx->aux : ctx->stream);
         if (P.nVox > 0) {
@@ -167,13 +180,14 @@ ckd_status launch_plan(ckd_ctx *ctx, const Plan &P, unsigned char *dev, const ck
                                            P.nVox, P.voxBlocks);
             if (s != CKD_OK) return s;
         }
-        if (P.nTun > 0) {
+        if (P.nTun > 0 && !split) {
             TunJob none;
             memset(&none, 0, sizeof(none));
             s = ckd::launch_tunnel_jobs(ctx, reinterpret_cast<const TunJob *>(dev + P.oTun), P.nTun,
                                         none, P.tunRows, P.tunV, P.tunMaxW, tex, texLog2);
             if (s != CKD_OK) return s;
         }
+        if (split) CKD_HIP(hipStreamWaitEvent(ctx->aux, ctx->tunEv, 0));
         if (fork) {
             // the voxel / tunnel work is done: its CUs join the SDF march
             s = ckd::launch_sdf_persistent(ctx, reinterpret_cast<uint32_t *>(dev), P.chunks,

@@ -182,8 +182,10 @@ ckd_status ckd_create(int device, ckd_ctx **out)
         }
     }
     if (hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->aux2, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->forkEv, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->joinEv, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&ctx->joinEv, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->tunEv, hipEventDisableTiming) != hipSuccess) {
         ckd_destroy(ctx);
         return ckd::fail(CKD_EDEVICE, "ckd_create: stream/event creation failed");
     }
@@ -207,7 +209,9 @@ void ckd_destroy(ckd_ctx *ctx)
     for (hipEvent_t e : ctx->eventPool) (void)hipEventDestroy(e);
     if (ctx->forkEv) (void)hipEventDestroy(ctx->forkEv);
     if (ctx->joinEv) (void)hipEventDestroy(ctx->joinEv);
+    if (ctx->tunEv) (void)hipEventDestroy(ctx->tunEv);
     if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
+    if (ctx->aux2) (void)hipStreamDestroy(ctx->aux2);
     for (auto &pm : ctx->polar) {
         if (pm.map) (void)hipFree(pm.map);
         if (pm.inv) (void)hipFree(pm.inv);
