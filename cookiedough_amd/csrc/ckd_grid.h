@@ -10,16 +10,20 @@ namespace ckd {
 //   L = C00*(8-j) + C01*j, R = C10*(8-j) + C11*j, UV = (L*(8-i) + R*i) >> 6
 // then bsamp_prepUVs (code/bilinear.h:10-31) with wrap mask 2^texLog2-1 and
 // bsamp32_32 (code/bilinear.h:86-125).
-CKD_DEV uint32_t grid_pixel(const int2 *__restrict__ grid, unsigned GW,
-                            const uint32_t *__restrict__ tex, unsigned texLog2, unsigned x,
-                            unsigned y)
+// The grid is read as (U, V) int pairs through IntPtr: `const int32_t *`, or
+// a pointer into the global address space when the caller's grid comes from
+// memory (global loads rather than flat ones).
+template <typename IntPtr>
+CKD_DEV uint32_t grid_pixel(IntPtr grid, unsigned GW, const uint32_t *__restrict__ tex,
+                            unsigned texLog2, unsigned x, unsigned y)
 {
     const unsigned bx = x >> 3, by = y >> 3;
     const int i = (int)(x & 7), j = (int)(y & 7);
-    const int2 *g = grid + (size_t)by * GW + bx;
-    const int2 c00 = g[0], c10 = g[1], c01 = g[GW], c11 = g[GW + 1];
-    const int Lu = c00.x * (8 - j) + c01.x * j, Ru = c10.x * (8 - j) + c11.x * j;
-    const int Lv = c00.y * (8 - j) + c01.y * j, Rv = c10.y * (8 - j) + c11.y * j;
+    const auto g0 = grid + 2 * ((size_t)by * GW + bx), g1 = g0 + 2 * (size_t)GW;
+    const int c00x = g0[0], c00y = g0[1], c10x = g0[2], c10y = g0[3];
+    const int c01x = g1[0], c01y = g1[1], c11x = g1[2], c11y = g1[3];
+    const int Lu = c00x * (8 - j) + c01x * j, Ru = c10x * (8 - j) + c11x * j;
+    const int Lv = c00y * (8 - j) + c01y * j, Rv = c10y * (8 - j) + c11y * j;
     const int U = (Lu * (8 - i) + Ru * i) >> 6;
     const int V = (Lv * (8 - i) + Rv * i) >> 6;
     const unsigned mask = (1u << texLog2) - 1;

@@ -80,7 +80,8 @@ __global__ __launch_bounds__(256) void k_grid_remap(uint32_t *__restrict__ dst, 
     const unsigned y = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (x >= W || y >= H) return;
     const unsigned GW = (W + 7) / 8 + 1;
-    dst[(size_t)y * W + x] = ckd::grid_pixel(grid, GW, tex, texLog2, x, y);
+    dst[(size_t)y * W + x] =
+        ckd::grid_pixel(reinterpret_cast<const int32_t *>(grid), GW, tex, texLog2, x, y);
 }
 
 } // namespace

@@ -20,6 +20,12 @@ namespace {
 
 constexpr int kT1Block = 256;
 
+// the job's images and grid are global memory: pointers in that address
+// space make their accesses global_* instructions rather than flat ones,
+// which also wait on LDS traffic
+typedef __attribute__((address_space(1))) const int32_t global_i32;
+typedef __attribute__((address_space(1))) uint32_t global_u32;
+
 CKD_DEV const TunJob &pick(const TunJob *jobs, int nJobs, const TunJob &single, uint32_t unit,
                            uint32_t TunJob::*base)
 {
@@ -43,12 +49,27 @@ __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict_
 
     D[threadIdx.x] = ckd_b07_count_div(threadIdx.x);
     const unsigned GW = (J.W + 7) / 8 + 1;
-    const int2 *grid = reinterpret_cast<const int2 *>(J.grid);
-    uint32_t *drow = J.dst + (size_t)y * W;
-    for (int x = threadIdx.x; x < W; x += kT1Block) {
-        const uint32_t v = ckd::grid_pixel(grid, GW, tex, texLog2, (unsigned)x, y);
-        px[x] = v;
-        drow[x] = v;
+    const global_i32 *grid = (const global_i32 *)(J.grid);
+    global_u32 *drow = (global_u32 *)(J.dst + (size_t)y * W);
+    // four pixels per thread at a time, all their loads before any store
+    // (the stores could alias the grid as far as the compiler knows)
+    for (int x0 = threadIdx.x; x0 < W; x0 += 4 * kT1Block) {
+        // positions past the row are computed at its last pixel and dropped,
+        // so the four fetches are one straight run the scheduler interleaves
+        uint32_t v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int x = min(x0 + i * kT1Block, W - 1);
+            v[i] = ckd::grid_pixel(grid, GW, tex, texLog2, (unsigned)x, y);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int x = x0 + i * kT1Block;
+            if (x < W) {
+                px[x] = v[i];
+                drow[x] = v[i];
+            }
+        }
     }
     __syncthreads();
 
@@ -75,7 +96,7 @@ __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict_
     __syncthreads();
 
     const int e = (int)J.edge;
-    uint32_t *trow = J.tmp + (size_t)y * W;
+    global_u32 *trow = (global_u32 *)(J.tmp + (size_t)y * W);
     for (int x = threadIdx.x; x < W; x += kT1Block) {
         const int lo = max(x - e, 0), hi = min(x + e, W - 1);
         trow[x] = ckd_b07_div_pack(P[hi + 1] - P[lo], D[hi - lo + 1]);
@@ -94,17 +115,34 @@ __global__ __launch_bounds__(64) void k_tunnel_v(const TunJob *__restrict__ jobs
     const int y0 = (int)((local / J.vBlocksX) * kT2Seg);
     const int W = (int)J.W, H = (int)J.H, e = (int)J.edge;
     if (x >= W) return;
-    const uint32_t *col = J.tmp + x;
+    // tmp and dst are distinct images: restrict lets the loads of four rows
+    // go out before any of their stores
+    const global_u32 *__restrict__ col = (const global_u32 *)(J.tmp + x);
+    global_u32 *__restrict__ dcol = (global_u32 *)(J.dst + x);
     const int y1 = min(y0 + kT2Seg, H);
     uint64_t F = 0;
     for (int r = max(0, y0 - e); r <= min(H - 1, y0 + e); ++r) F += ckd_spread(col[(size_t)r * W]);
-    for (int y = y0; y < y1; ++y) {
-        const int lo = max(y - e, 0), hi = min(y + e, H - 1);
-        const uint32_t b = ckd_b07_div_pack(F, D[hi - lo + 1]);
-        uint32_t *d = J.dst + (size_t)y * W + x;
-        *d = ckd_adds_u8x4(*d, b);
-        if (y + e + 1 <= H - 1) F += ckd_spread(col[(size_t)(y + e + 1) * W]);
-        if (y - e >= 0) F -= ckd_spread(col[(size_t)(y - e) * W]);
+    for (int y = y0; y < y1; y += 4) {
+        // rows past the segment are read at its last row and dropped
+        uint32_t dv[4], av[4], sv[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int yy = min(y + j, y1 - 1);
+            dv[j] = dcol[(size_t)yy * W];
+            av[j] = col[(size_t)min(yy + e + 1, H - 1) * W];
+            sv[j] = col[(size_t)max(yy - e, 0) * W];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int yy = y + j;
+            if (yy < y1) {
+                const int lo = max(yy - e, 0), hi = min(yy + e, H - 1);
+                const uint32_t b = ckd_b07_div_pack(F, D[hi - lo + 1]);
+                dcol[(size_t)yy * W] = ckd_adds_u8x4(dv[j], b);
+                if (yy + e + 1 <= H - 1) F += ckd_spread(av[j]);
+                if (yy - e >= 0) F -= ckd_spread(sv[j]);
+            }
+        }
     }
 }
 
