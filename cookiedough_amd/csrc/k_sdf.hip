@@ -21,6 +21,9 @@
 
 namespace {
 
+// output pixels in the global address space
+typedef __attribute__((address_space(1))) uint32_t global_px;
+
 typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
@@ -434,7 +437,9 @@ __global__ __launch_bounds__(256) void k_sdf_persistent(uint32_t *__restrict__ q
                 const uint32_t x = e.pix % J.W, y = e.pix / J.W;
                 float ex, ey, ez;
                 ray_dir(J.cam, J.W, J.H, x, y, ex, ey, ez);
-                J.dst[e.pix] = shade_hit<kHalf, kMaxSph>(sc, J.cam, ex, ey, ez, e.tHit, e.total);
+                // a global store (the job's pointer would make a flat one,
+                // whose completion the next LDS reads would wait for)
+                ((global_px *)J.dst)[e.pix] = shade_hit<kHalf, kMaxSph>(sc, J.cam, ex, ey, ez, e.tHit, e.total);
                 qn -= 64;
                 __builtin_amdgcn_wave_barrier();
             }
@@ -448,7 +453,7 @@ __global__ __launch_bounds__(256) void k_sdf_persistent(uint32_t *__restrict__ q
         const uint32_t x = e.pix % J.W, y = e.pix / J.W;
         float ex, ey, ez;
         ray_dir(J.cam, J.W, J.H, x, y, ex, ey, ez);
-        J.dst[e.pix] = shade_hit<kHalf, kMaxSph>(sc, J.cam, ex, ey, ez, e.tHit, e.total);
+        ((global_px *)J.dst)[e.pix] = shade_hit<kHalf, kMaxSph>(sc, J.cam, ex, ey, ez, e.tHit, e.total);
     }
     if (stepCount) {
         unsigned long long s = mySteps;
