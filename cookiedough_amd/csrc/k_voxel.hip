@@ -71,9 +71,14 @@ CKD_DEV void ls_setup(float rayY, float colScale, float halfW, float X1, float Y
 }
 #pragma clang fp contract(on)
 
+// frame pixels in the global address space (a job table's pointer would
+// make flat stores, whose completion the fog table's scalar loads would
+// wait for)
+typedef __attribute__((address_space(1))) uint32_t global_u32;
+
 // vscape_ray (code/landscape.cpp:56-107) for column iRay of one frame, then
 // the fused memset32(fog[0]) for the rows no span reached
-CKD_DEV void ls_column(uint32_t *__restrict__ col, unsigned W, unsigned H, unsigned iRay,
+CKD_DEV void ls_column(global_u32 *__restrict__ col, unsigned W, unsigned H, unsigned iRay,
                        float rayY, float colScale, float halfW, int mapScale, float X1, float Y1,
                        float viewCos, float viewSin, int mapTilt,
                        const uint8_t *__restrict__ hmap, const uint32_t *__restrict__ cmap,
@@ -109,7 +114,7 @@ CKD_DEV void ls_column(uint32_t *__restrict__ col, unsigned W, unsigned H, unsig
             const uint32_t drawLength = (uint32_t)(lastDrawn - hgt);
             CkdSpan sp;
             ckd_span_setup(sp, color, lastColor, (uint32_t)(lastHeight - hgt), drawLength);
-            uint32_t *p = col + (size_t)hgt * W;
+            global_u32 *p = col + (size_t)hgt * W;
             for (uint32_t k = 0; k < drawLength; ++k) {
                 *p = ckd_span_pixel(sp.acc);
                 p += W;
@@ -133,7 +138,7 @@ __global__ __launch_bounds__(64) void k_landscape(uint32_t *__restrict__ dst, un
     const unsigned iRay = blockIdx.x * 64 + threadIdx.x;
     if (iRay >= W) return;
     const LsFrame &F = P.f[blockIdx.y];
-    ls_column(dst + (size_t)blockIdx.y * W * H + iRay, W, H, iRay, P.rayY, P.colScale, P.halfW,
+    ls_column((global_u32 *)(dst + (size_t)blockIdx.y * W * H + iRay), W, H, iRay, P.rayY, P.colScale, P.halfW,
               P.mapScale, F.X1, F.Y1, F.viewCos, F.viewSin, F.mapTilt, hmap, cmap, fog);
 }
 
@@ -147,7 +152,7 @@ __global__ __launch_bounds__(64) void k_landscape_jobs(const VoxJob *__restrict_
     const VoxJob &J = jobs[j];
     const unsigned iRay = (blockIdx.x - J.blockBase) * 64 + threadIdx.x;
     if (iRay >= J.W) return;
-    ls_column(J.dst + iRay, J.W, J.H, iRay, J.rayY, J.colScale, J.halfW, J.mapScale, J.X1, J.Y1,
+    ls_column((global_u32 *)(J.dst + iRay), J.W, J.H, iRay, J.rayY, J.colScale, J.halfW, J.mapScale, J.X1, J.Y1,
               J.viewCos, J.viewSin, J.mapTilt, hmap, cmap, fog);
 }
 
