@@ -5,6 +5,18 @@
 
 namespace ckd {
 
+// The texture half of grid_pixel: the bilinear sample at the interpolated
+// (U, V), both 24:8 fixed point.
+CKD_DEV uint32_t grid_sample(const uint32_t *__restrict__ tex, unsigned texLog2, int U, int V)
+{
+    const unsigned mask = (1u << texLog2) - 1;
+    const unsigned u0 = ((unsigned)(U >> 8)) & mask, u1 = ((unsigned)(U >> 8) + 1) & mask;
+    const unsigned v0 = (((unsigned)(V >> 8)) & mask) << texLog2;
+    const unsigned v1 = (((unsigned)(V >> 8) + 1) & mask) << texLog2;
+    return ckd_bilerp32(tex[u0 + v0], tex[u1 + v0], tex[u0 + v1], tex[u1 + v1],
+                        (uint32_t)U & 0xff, (uint32_t)V & 0xff);
+}
+
 // Grid remap of one pixel (build-defined law):
 //   corners C00,C10,C01,C11 of block (x>>3, y>>3), local (i,j) = (x&7, y&7)
 //   L = C00*(8-j) + C01*j, R = C10*(8-j) + C11*j, UV = (L*(8-i) + R*i) >> 6
@@ -24,14 +36,7 @@ CKD_DEV uint32_t grid_pixel(IntPtr grid, unsigned GW, const uint32_t *__restrict
     const int c01x = g1[0], c01y = g1[1], c11x = g1[2], c11y = g1[3];
     const int Lu = c00x * (8 - j) + c01x * j, Ru = c10x * (8 - j) + c11x * j;
     const int Lv = c00y * (8 - j) + c01y * j, Rv = c10y * (8 - j) + c11y * j;
-    const int U = (Lu * (8 - i) + Ru * i) >> 6;
-    const int V = (Lv * (8 - i) + Rv * i) >> 6;
-    const unsigned mask = (1u << texLog2) - 1;
-    const unsigned u0 = ((unsigned)(U >> 8)) & mask, u1 = ((unsigned)(U >> 8) + 1) & mask;
-    const unsigned v0 = (((unsigned)(V >> 8)) & mask) << texLog2;
-    const unsigned v1 = (((unsigned)(V >> 8) + 1) & mask) << texLog2;
-    return ckd_bilerp32(tex[u0 + v0], tex[u1 + v0], tex[u0 + v1], tex[u1 + v1],
-                        (uint32_t)U & 0xff, (uint32_t)V & 0xff);
+    return grid_sample(tex, texLog2, (Lu * (8 - i) + Ru * i) >> 6, (Lv * (8 - i) + Rv * i) >> 6);
 }
 
 } // namespace ckd

@@ -3,8 +3,9 @@
 // and BlitAdd32 of the blur onto the remap (code/deprecated/boxblur.cpp,
 // code/util.cpp:754-773), in two fused kernels:
 //
-//   T1  one block per row: remap the row into LDS and into dst, prefix-sum
-//       it, write the horizontal blur to tmp
+//   T1  one block per row: interpolate the row's grid corners once, remap
+//       the row into LDS and into dst, prefix-sum it, write the horizontal
+//       blur to tmp
 //   T2  one wave per 64 columns x 32 rows: vertical blur of tmp and the
 //       saturating add into dst
 //
@@ -48,28 +49,39 @@ __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict_
     uint32_t *px = reinterpret_cast<uint32_t *>(lds + W + 1);
 
     D[threadIdx.x] = ckd_b07_count_div(threadIdx.x);
+    // the row's grid corners interpolated vertically once, at each block
+    // edge: E[2b], E[2b+1] = (U, V) of edge b, shared by the 8 pixels either
+    // side of it
     const unsigned GW = (J.W + 7) / 8 + 1;
-    const global_i32 *grid = (const global_i32 *)(J.grid);
+    const int j = (int)(y & 7);
+    const global_i32 *g0 = (const global_i32 *)(J.grid) + 2 * (size_t)(y >> 3) * GW;
+    const global_i32 *g1 = g0 + 2 * (size_t)GW;
+    int *E = reinterpret_cast<int *>(px + W);
+    for (int b = threadIdx.x; b < 2 * (int)GW; b += kT1Block) E[b] = g0[b] * (8 - j) + g1[b] * j;
+    __syncthreads();
+    auto remap = [&](int x) {
+        const int b = 2 * (x >> 3), i = x & 7;
+        return ckd::grid_sample(tex, texLog2, (E[b] * (8 - i) + E[b + 2] * i) >> 6,
+                                (E[b + 1] * (8 - i) + E[b + 3] * i) >> 6);
+    };
     global_u32 *drow = (global_u32 *)(J.dst + (size_t)y * W);
-    // four pixels per thread at a time, all their loads before any store
-    // (the stores could alias the grid as far as the compiler knows)
-    for (int x0 = threadIdx.x; x0 < W; x0 += 4 * kT1Block) {
-        // positions past the row are computed at its last pixel and dropped,
-        // so the four fetches are one straight run the scheduler interleaves
+    // four pixels per thread at a time while all four are in the row, all
+    // their loads before any store; then the rest one at a time
+    int x0 = threadIdx.x;
+    for (; x0 + 3 * kT1Block < W; x0 += 4 * kT1Block) {
         uint32_t v[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int x = min(x0 + i * kT1Block, W - 1);
-            v[i] = ckd::grid_pixel(grid, GW, tex, texLog2, (unsigned)x, y);
-        }
+        for (int i = 0; i < 4; ++i) v[i] = remap(x0 + i * kT1Block);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int x = x0 + i * kT1Block;
-            if (x < W) {
-                px[x] = v[i];
-                drow[x] = v[i];
-            }
+            px[x0 + i * kT1Block] = v[i];
+            drow[x0 + i * kT1Block] = v[i];
         }
+    }
+    for (; x0 < W; x0 += kT1Block) {
+        const uint32_t v = remap(x0);
+        px[x0] = v;
+        drow[x0] = v;
     }
     __syncthreads();
 
@@ -169,7 +181,9 @@ ckd_status launch_tunnel_jobs(ckd_ctx *ctx, const TunJob *jobs, int nJobs, const
                               const uint32_t *tex, unsigned texLog2)
 {
     if (totalRows == 0) return CKD_OK;
-    const size_t lds = (size_t)(maxW + 1) * sizeof(uint64_t) + (size_t)maxW * sizeof(uint32_t);
+    // prefix sums, the row's pixels, the row's interpolated block edges
+    const size_t lds = (size_t)(maxW + 1) * sizeof(uint64_t) + (size_t)maxW * sizeof(uint32_t) +
+                       2 * (size_t)((maxW + 7) / 8 + 1) * sizeof(int32_t);
     {
         KernelTimer kt(ctx, CKD_KT_TUNNEL_H);
         k_tunnel_h<<<totalRows, kT1Block, lds, ctx->stream>>>(jobs, nJobs, single, tex, texLog2);
