@@ -145,7 +145,7 @@ CKD_DEV uint32_t ckd_b07_div_pack(uint64_t acc, uint32_t div)
 
 // WeightToDiv(16*count) (code/deprecated/boxblur.cpp:11-14) stored through
 // _mm_set1_epi16: the divisor of an odd-span window of `count` pixels
-CKD_DEV uint32_t ckd_b07_count_div(uint32_t count)
+__host__ __device__ constexpr inline uint32_t ckd_b07_count_div(uint32_t count)
 {
     return count ? ((((1u << 24) / (16u * count)) >> 4) & 0xffffu) : 0u;
 }

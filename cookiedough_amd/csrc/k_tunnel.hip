@@ -21,6 +21,17 @@ namespace {
 
 constexpr int kT1Block = 256;
 
+// ckd_b07_count_div of every window count 0..255 (radius <= 127), evaluated
+// by the compiler: blocks copy it instead of dividing
+struct B07DivTable {
+    uint16_t d[256];
+    constexpr B07DivTable() : d{}
+    {
+        for (uint32_t c = 0; c < 256; ++c) d[c] = (uint16_t)ckd_b07_count_div(c);
+    }
+};
+__constant__ const B07DivTable kB07Div = B07DivTable();
+
 // the job's images and grid are global memory: pointers in that address
 // space make their accesses global_* instructions rather than flat ones,
 // which also wait on LDS traffic
@@ -48,7 +59,7 @@ __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict_
     uint64_t *P = lds;
     uint32_t *px = reinterpret_cast<uint32_t *>(lds + W + 1);
 
-    D[threadIdx.x] = ckd_b07_count_div(threadIdx.x);
+    D[threadIdx.x] = kB07Div.d[threadIdx.x];
     // the row's grid corners interpolated vertically once, at each block
     // edge: E[2b], E[2b+1] = (U, V) of edge b, shared by the 8 pixels either
     // side of it
@@ -120,7 +131,7 @@ __global__ __launch_bounds__(64) void k_tunnel_v(const TunJob *__restrict__ jobs
 {
     __shared__ uint32_t D[256];
     const TunJob &J = pick(jobs, nJobs, single, blockIdx.x, &TunJob::vBase);
-    for (int i = threadIdx.x; i < 256; i += 64) D[i] = ckd_b07_count_div((uint32_t)i);
+    for (int i = threadIdx.x; i < 256; i += 64) D[i] = kB07Div.d[i];
     __syncthreads();
     const uint32_t local = blockIdx.x - J.vBase;
     const int x = (int)((local % J.vBlocksX) * 64 + threadIdx.x);
