@@ -13,8 +13,11 @@ CKD_DEV uint32_t grid_sample(const uint32_t *__restrict__ tex, unsigned texLog2,
     const unsigned u0 = ((unsigned)(U >> 8)) & mask, u1 = ((unsigned)(U >> 8) + 1) & mask;
     const unsigned v0 = (((unsigned)(V >> 8)) & mask) << texLog2;
     const unsigned v1 = (((unsigned)(V >> 8) + 1) & mask) << texLog2;
-    return ckd_bilerp32(tex[u0 + v0], tex[u1 + v0], tex[u0 + v1], tex[u1 + v1],
-                        (uint32_t)U & 0xff, (uint32_t)V & 0xff);
+    // 32-bit byte offsets from the texture's base (texLog2 <= 14: below
+    // 2^30), so each fetch adds one offset register to a uniform base
+    auto at = [tex](uint32_t k) { return *(const uint32_t *)((const char *)tex + (k << 2)); };
+    return ckd_bilerp32(at(u0 + v0), at(u1 + v0), at(u0 + v1), at(u1 + v1), (uint32_t)U & 0xff,
+                        (uint32_t)V & 0xff);
 }
 
 // Grid remap of one pixel (build-defined law):

@@ -3,9 +3,9 @@
 // and BlitAdd32 of the blur onto the remap (code/deprecated/boxblur.cpp,
 // code/util.cpp:754-773), in two fused kernels:
 //
-//   T1  one block per row: interpolate the row's grid corners once, remap
-//       the row into LDS and into dst, prefix-sum it, write the horizontal
-//       blur to tmp
+//   T1  one block per row: interpolate the row's grid corners once per
+//       8-pixel block, remap the row into LDS and into dst, prefix-sum it,
+//       write the horizontal blur to tmp
 //   T2  one wave per 64 columns x 32 rows: vertical blur of tmp and the
 //       saturating add into dst
 //
@@ -16,6 +16,8 @@
 #include "ckd_pixel.h"
 #include "ckd_grid.h"
 #include "ckd_jobs.h"
+
+#include <type_traits>
 
 namespace {
 
@@ -35,7 +37,7 @@ __constant__ const B07DivTable kB07Div = B07DivTable();
 // the job's images and grid are global memory: pointers in that address
 // space make their accesses global_* instructions rather than flat ones,
 // which also wait on LDS traffic
-typedef __attribute__((address_space(1))) const int32_t global_i32;
+typedef __attribute__((address_space(1))) const uint32_t global_u32c;
 typedef __attribute__((address_space(1))) uint32_t global_u32;
 
 CKD_DEV const TunJob &pick(const TunJob *jobs, int nJobs, const TunJob &single, uint32_t unit,
@@ -50,50 +52,82 @@ __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict_
                                                        const uint32_t *__restrict__ tex,
                                                        unsigned texLog2)
 {
-    extern __shared__ uint64_t lds[];
+    extern __shared__ uint4 lds[];
     __shared__ uint32_t D[256];
     __shared__ uint64_t waveTot[kT1Block / 64];
     const TunJob &J = pick(jobs, nJobs, single, blockIdx.x, &TunJob::rowBase);
     const int W = (int)J.W;
     const unsigned y = blockIdx.x - J.rowBase;
-    uint64_t *P = lds;
-    uint32_t *px = reinterpret_cast<uint32_t *>(lds + W + 1);
+    const int NB = (W + 7) / 8;
+    // per 8-pixel block b of the row: E[b] = (8*U_b, 8*V_b, U_b+1 - U_b,
+    // V_b+1 - V_b), where (U_e, V_e) is grid corner column e interpolated
+    // vertically to this row.  Pixel 8b+i then has U = (8*U_b + i*(U_b+1 -
+    // U_b)) >> 6, equal mod 2^32 to (U_b*(8-i) + U_b+1*i) >> 6.
+    uint4 *E = lds;
+    uint64_t *P = reinterpret_cast<uint64_t *>(lds + NB);
+    uint32_t *px = reinterpret_cast<uint32_t *>(P + W + 1);
 
     D[threadIdx.x] = kB07Div.d[threadIdx.x];
-    // the row's grid corners interpolated vertically once, at each block
-    // edge: E[2b], E[2b+1] = (U, V) of edge b, shared by the 8 pixels either
-    // side of it
-    const unsigned GW = (J.W + 7) / 8 + 1;
-    const int j = (int)(y & 7);
-    const global_i32 *g0 = (const global_i32 *)(J.grid) + 2 * (size_t)(y >> 3) * GW;
-    const global_i32 *g1 = g0 + 2 * (size_t)GW;
-    int *E = reinterpret_cast<int *>(px + W);
-    for (int b = threadIdx.x; b < 2 * (int)GW; b += kT1Block) E[b] = g0[b] * (8 - j) + g1[b] * j;
+    const unsigned GW = (unsigned)NB + 1;
+    const uint32_t j = y & 7;
+    const global_u32c *g0 = (const global_u32c *)(J.grid) + 2 * (size_t)(y >> 3) * GW;
+    const global_u32c *g1 = g0 + 2 * (size_t)GW;
+    // whether every step fits a signed 24-bit multiply (exact then)
+    bool wide = false;
+    for (int b = threadIdx.x; b < NB; b += kT1Block) {
+        const uint32_t u0 = g0[2 * b] * (8 - j) + g1[2 * b] * j;
+        const uint32_t v0 = g0[2 * b + 1] * (8 - j) + g1[2 * b + 1] * j;
+        const uint32_t u1 = g0[2 * b + 2] * (8 - j) + g1[2 * b + 2] * j;
+        const uint32_t v1 = g0[2 * b + 3] * (8 - j) + g1[2 * b + 3] * j;
+        const uint32_t du = u1 - u0, dv = v1 - v0;
+        wide |= (du + (1u << 23)) >= (1u << 24) || (dv + (1u << 23)) >= (1u << 24);
+        E[b] = make_uint4(8 * u0, 8 * v0, du, dv);
+    }
+    __shared__ uint32_t waveWide[kT1Block / 64];
+    const bool anyWide = __ballot(wide) != 0;
+    if ((threadIdx.x & 63) == 0) waveWide[threadIdx.x >> 6] = anyWide;
     __syncthreads();
-    auto remap = [&](int x) {
-        const int b = 2 * (x >> 3), i = x & 7;
-        return ckd::grid_sample(tex, texLog2, (E[b] * (8 - i) + E[b + 2] * i) >> 6,
-                                (E[b + 1] * (8 - i) + E[b + 3] * i) >> 6);
-    };
+    bool narrow = true;
+#pragma unroll
+    for (int w = 0; w < kT1Block / 64; ++w) narrow &= waveWide[w] == 0;
     global_u32 *drow = (global_u32 *)(J.dst + (size_t)y * W);
-    // four pixels per thread at a time while all four are in the row, all
-    // their loads before any store; then the rest one at a time
-    int x0 = threadIdx.x;
-    for (; x0 + 3 * kT1Block < W; x0 += 4 * kT1Block) {
-        uint32_t v[4];
+    // the remap of the row into LDS and dst: four pixels per thread at a time
+    // while all four are in the row, all their loads before any store; then
+    // the rest one at a time.  A thread's pixels share i = x & 7 (the stride
+    // is a multiple of 8).
+    auto remap_row = [&](auto narrowTag) {
+        constexpr bool kNarrow = decltype(narrowTag)::value;
+        uint32_t i = threadIdx.x & 7;
+        // opaque to the compiler, so that __mul24 stays v_mad_i32_i24 rather
+        // than becoming a 64-bit multiply-add on i's known small range
+        asm("" : "+v"(i));
+        auto remap = [&](int x) {
+            const uint4 e = E[x >> 3];
+            const uint32_t iu = kNarrow ? (uint32_t)__mul24((int)i, (int)e.z) : i * e.z;
+            const uint32_t iv = kNarrow ? (uint32_t)__mul24((int)i, (int)e.w) : i * e.w;
+            return ckd::grid_sample(tex, texLog2, (int)(e.x + iu) >> 6, (int)(e.y + iv) >> 6);
+        };
+        int x0 = threadIdx.x;
+        for (; x0 + 3 * kT1Block < W; x0 += 4 * kT1Block) {
+            uint32_t v[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) v[i] = remap(x0 + i * kT1Block);
+            for (int k = 0; k < 4; ++k) v[k] = remap(x0 + k * kT1Block);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            px[x0 + i * kT1Block] = v[i];
-            drow[x0 + i * kT1Block] = v[i];
+            for (int k = 0; k < 4; ++k) {
+                px[x0 + k * kT1Block] = v[k];
+                drow[x0 + k * kT1Block] = v[k];
+            }
         }
-    }
-    for (; x0 < W; x0 += kT1Block) {
-        const uint32_t v = remap(x0);
-        px[x0] = v;
-        drow[x0] = v;
-    }
+        for (; x0 < W; x0 += kT1Block) {
+            const uint32_t v = remap(x0);
+            px[x0] = v;
+            drow[x0] = v;
+        }
+    };
+    if (narrow)
+        remap_row(std::true_type{});
+    else
+        remap_row(std::false_type{});
     __syncthreads();
 
     const int chunk = (W + kT1Block - 1) / kT1Block;
@@ -192,9 +226,9 @@ ckd_status launch_tunnel_jobs(ckd_ctx *ctx, const TunJob *jobs, int nJobs, const
                               const uint32_t *tex, unsigned texLog2)
 {
     if (totalRows == 0) return CKD_OK;
-    // prefix sums, the row's pixels, the row's interpolated block edges
-    const size_t lds = (size_t)(maxW + 1) * sizeof(uint64_t) + (size_t)maxW * sizeof(uint32_t) +
-                       2 * (size_t)((maxW + 7) / 8 + 1) * sizeof(int32_t);
+    // the row's per-block interpolation terms, prefix sums, pixels
+    const size_t lds = (size_t)((maxW + 7) / 8) * sizeof(uint4) + (size_t)(maxW + 1) * sizeof(uint64_t) +
+                       (size_t)maxW * sizeof(uint32_t);
     {
         KernelTimer kt(ctx, CKD_KT_TUNNEL_H);
         k_tunnel_h<<<totalRows, kT1Block, lds, ctx->stream>>>(jobs, nJobs, single, tex, texLog2);

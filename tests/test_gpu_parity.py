@@ -210,6 +210,65 @@ def test_tunnel_frame(ctx, orc, W, H, r):
     assert np.array_equal(d_dst.numpy(), ref)
 
 
+# The remap of a tunnel row interpolates each 8-pixel block as 8*U_b +
+# i*(U_b+1 - U_b) with a 24-bit multiply while every step of the row fits 24
+# bits signed, and with a 32-bit one otherwise.  The grids: random corners up
+# to 2^24 (steps up to 2^27), and sawtooth U with steps of 2^23 - 8 (the
+# 24-bit form at its largest) and of 2^23 (just past it).  Their values keep
+# the oracle's int arithmetic from overflowing.
+@pytest.mark.parametrize("kind", ["random", "saw_narrow", "saw_wide"])
+def test_tunnel_frame_grid_steps(ctx, orc, kind):
+    _say("tunnel frame grid", kind)
+    W, H, r = 640, 360, 9
+    gh, gw = wl.grid_shape(W, H)
+    rng = np.random.default_rng(7)
+    if kind == "random":
+        grid = rng.integers(-(1 << 24), 1 << 24, size=(gh, gw, 2), dtype=np.int32)
+    else:
+        grid = np.zeros((gh, gw, 2), np.int32)
+        grid[:, 1::2, 0] = (1 << 20) - 1 if kind == "saw_narrow" else 1 << 20
+        grid[:, :, 1] = rng.integers(0, 1 << 16, size=(gh, gw), dtype=np.int32)
+    tex = wl.noise_texture(0, 9)
+    d_tex, d_grid = ctx.upload(tex), ctx.upload(grid)
+    d_dst, t0 = ctx.empty(W * H), ctx.empty(W * H)
+    ctx.tunnel_frame(d_dst, t0, W, H, d_grid, d_tex, 9, r)
+    ref, o0, o1 = (np.zeros(W * H, np.uint32) for _ in range(3))
+    orc.orc_tunnel_frame(P(ref), P(o0), P(o1), W, H, P(grid), P(tex), 9, r, 8)
+    assert np.array_equal(d_dst.numpy(), ref)
+    d_rm = ctx.empty(W * H)
+    ctx.grid_remap(d_rm, W, H, d_grid, d_tex, 9)
+    rm = np.zeros(W * H, np.uint32)
+    orc.orc_grid_remap(P(rm), W, H, P(grid), P(tex), 9, 8)
+    assert np.array_equal(d_rm.numpy(), rm)
+
+
+# The vertical pass splits each column into segments of tunnelVSeg rows, each
+# starting from a window summed over its halo: radii up to the limit, segments
+# from the smallest to the largest, one that is not a multiple of four (the
+# pass's row group) and widths that are not a multiple of 64 (its column
+# group).
+@pytest.mark.parametrize("W,H,r,seg", [(1000, 600, 80, 32), (1000, 600, 127, 32), (328, 200, 5, 50),
+                                       (328, 200, 99, 8), (1280, 720, 32, 512), (1280, 720, 16, 128)])
+def test_tunnel_frame_vertical_segments(ctx, orc, W, H, r, seg):
+    _say("tunnel frame", W, H, r, "seg", seg)
+    saved = ctx.get_tuning()
+    t = ctx.get_tuning()
+    t.tunnelVSeg = seg
+    ctx.set_tuning(t)
+    try:
+        tex = wl.noise_texture(0, 9)
+        grid = wl.tunnel_grid(W, H, 5.0 + r)
+        d_tex, d_grid = ctx.upload(tex), ctx.upload(grid)
+        d_dst, t0 = ctx.empty(W * H), ctx.empty(W * H)
+        ctx.tunnel_frame(d_dst, t0, W, H, d_grid, d_tex, 9, r)
+        got = d_dst.numpy()
+    finally:
+        ctx.set_tuning(saved)
+    ref, o0, o1 = (np.zeros(W * H, np.uint32) for _ in range(3))
+    orc.orc_tunnel_frame(P(ref), P(o0), P(o1), W, H, P(grid), P(tex), 9, r, 8)
+    assert np.array_equal(got, ref)
+
+
 # -------------------------------------------------------------------- voxel
 
 @pytest.fixture(scope="module")
