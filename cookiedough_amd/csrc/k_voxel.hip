@@ -16,19 +16,25 @@
 namespace {
 
 constexpr unsigned kMapAnd = 1023, kMapShift = 10, kRayLength = 512;
+// march steps whose texels are fetched ahead of the step being drawn
+constexpr int kLsAhead = 4;
 
 struct LsFrame {
     float X1, Y1, viewCos, viewSin;
     int mapTilt; // already scaled to H rows
 };
 
-constexpr int kLsFramesPerLaunch = 64;
+// frames per launch, their parameters passed by value: 160 keeps the
+// kernel's arguments inside the 4 KiB limit, and one launch of that many
+// 1280-column frames is 3200 waves, enough to fill every SIMD
+constexpr int kLsFramesPerLaunch = 160;
 
 struct LsLaunch {
     float rayY, colScale, halfW;
     int mapScale;
     LsFrame f[kLsFramesPerLaunch];
 };
+static_assert(sizeof(LsLaunch) + 64 <= 4096, "k_landscape's arguments fit 4 KiB");
 
 // exact floor(num/den) for num < 2^24, 0 < den < 2^24: a reciprocal
 // estimate is within one of the quotient; one correction each way fixes it
@@ -92,18 +98,34 @@ CKD_DEV void ls_column(global_u32 *__restrict__ col, unsigned W, unsigned H, uns
     uint32_t lastColor =
         cmap[(((unsigned)(curX >> 8)) & kMapAnd) | ((((unsigned)(curY >> 8)) & kMapAnd) << kMapShift)];
 
-    for (unsigned iStep = 0; iStep < kRayLength; ++iStep) {
+    // The march is software-pipelined: the texels of the next kLsAhead steps
+    // are in flight while a step draws its span (only curX/curY carry from
+    // step to step, and every fetch is masked into the maps, so fetches past
+    // the last step are harmless).
+    struct Texels {
+        uint32_t h[4], c[4], fu, fv;
+    };
+    auto fetch = [&](Texels &t) {
         curX += dX;
         curY += dY;
         const unsigned u0 = ((unsigned)(curX >> 8)) & kMapAnd;
         const unsigned u1 = ((unsigned)(curX >> 8) + 1) & kMapAnd;
         const unsigned v0 = (((unsigned)(curY >> 8)) & kMapAnd) << kMapShift;
         const unsigned v1 = (((unsigned)(curY >> 8) + 1) & kMapAnd) << kMapShift;
-        const uint32_t fu = (uint32_t)curX & 0xff, fv = (uint32_t)curY & 0xff;
-        const uint32_t h = ckd_bilerp8(hmap[u0 + v0], hmap[u1 + v0], hmap[u0 + v1],
-                                       hmap[u1 + v1], fu, fv);
-        uint32_t color = ckd_bilerp32(cmap[u0 + v0], cmap[u1 + v0], cmap[u0 + v1],
-                                      cmap[u1 + v1], fu, fv);
+        t.fu = (uint32_t)curX & 0xff;
+        t.fv = (uint32_t)curY & 0xff;
+        t.h[0] = hmap[u0 + v0];
+        t.h[1] = hmap[u1 + v0];
+        t.h[2] = hmap[u0 + v1];
+        t.h[3] = hmap[u1 + v1];
+        t.c[0] = cmap[u0 + v0];
+        t.c[1] = cmap[u1 + v0];
+        t.c[2] = cmap[u0 + v1];
+        t.c[3] = cmap[u1 + v1];
+    };
+    auto step = [&](unsigned iStep, const Texels &t) {
+        const uint32_t h = ckd_bilerp8(t.h[0], t.h[1], t.h[2], t.h[3], t.fu, t.fv);
+        uint32_t color = ckd_bilerp32(t.c[0], t.c[1], t.c[2], t.c[3], t.fu, t.fv);
         color = ckd_subs_u8x4(color, fog[iStep >> 1]);
 
         // ((255-h)<<16) / (fpFishMul*(iStep+1)) is an unsigned division
@@ -125,6 +147,25 @@ CKD_DEV void ls_column(global_u32 *__restrict__ col, unsigned W, unsigned H, uns
         }
         lastHeight = hgt;
         lastColor = color;
+    };
+    // two register sets, A and B, take turns: a set is refilled only after
+    // it has been consumed, so each fetched set lands in the registers its
+    // consumer reads and the loop carries no copies of values in flight
+    static_assert(kRayLength % (2 * kLsAhead) == 0, "the march runs in whole groups");
+    Texels A[kLsAhead], B[kLsAhead];
+#pragma unroll
+    for (int k = 0; k < kLsAhead; ++k) fetch(A[k]);
+    for (unsigned iStep = 0; iStep < kRayLength; iStep += 2 * kLsAhead) {
+#pragma unroll
+        for (int k = 0; k < kLsAhead; ++k) {
+            fetch(B[k]);
+            step(iStep + k, A[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < kLsAhead; ++k) {
+            fetch(A[k]);
+            step(iStep + kLsAhead + k, B[k]);
+        }
     }
     const uint32_t sky = fog[0];
     for (int y = 0; y < lastDrawn; ++y) col[(size_t)y * W] = sky;

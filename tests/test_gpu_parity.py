@@ -295,6 +295,29 @@ def test_landscape(ctx, orc, lmaps, W, H):
         assert np.array_equal(out[k], ref), (W, H, times[k])
 
 
+def test_landscape_many_frames_across_launches(ctx, orc, lmaps):
+    """a draw of 330 frames runs as launches of at most 160: every frame
+    equals its own one-frame draw, and the frames either side of each launch
+    boundary equal the oracle"""
+    W, H, n = 96, 64, 330
+    path = wl.landscape_path(0)
+    cams = [wl.landscape_cam(path, 0.05 * k) for k in range(n)]
+    _say("landscape", n, "frames")
+    d = ctx.empty(n * W * H)
+    ctx.landscape_draw(d, W, H, cams)
+    out = d.numpy().reshape(n, W * H)
+    one = ctx.empty(W * H)
+    for k in range(n):
+        ctx.landscape_draw(one, W, H, [cams[k]])
+        assert np.array_equal(out[k], one.numpy()), k
+    for k in (0, 159, 160, 161, 319, 320, 329):
+        ref = np.zeros(W * H, np.uint32)
+        oc = orc_cam(cams[k])
+        orc.orc_landscape_frame(P(ref), W, H, P(lmaps.height), P(lmaps.color), P(lmaps.fog),
+                                C.byref(oc), 8)
+        assert np.array_equal(out[k], ref), k
+
+
 def test_tunnelscape(ctx, orc):
     _say("tunnelscape")
     m = wl.tunnelscape_maps(0)
