@@ -4,8 +4,9 @@
 // code/util.cpp:754-773), in two fused kernels:
 //
 //   T1  one block per row: interpolate the row's grid corners once per
-//       8-pixel block, remap the row into LDS and into dst, prefix-sum it,
-//       write the horizontal blur to tmp
+//       8-pixel block, remap the row into dst and (spread into 16-bit
+//       fields) into LDS, prefix-sum it there in place, write the
+//       horizontal blur to tmp
 //   T2  one wave per 64 columns x 32 rows: vertical blur of tmp and the
 //       saturating add into dst
 //
@@ -64,8 +65,9 @@ __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict_
     // vertically to this row.  Pixel 8b+i then has U = (8*U_b + i*(U_b+1 -
     // U_b)) >> 6, equal mod 2^32 to (U_b*(8-i) + U_b+1*i) >> 6.
     uint4 *E = lds;
+    // P[x + 1] first holds pixel x spread into 16-bit fields, then (scanned
+    // in place) the prefix sum of pixels 0..x; P[0] = 0
     uint64_t *P = reinterpret_cast<uint64_t *>(lds + NB);
-    uint32_t *px = reinterpret_cast<uint32_t *>(P + W + 1);
 
     D[threadIdx.x] = kB07Div.d[threadIdx.x];
     const unsigned GW = (unsigned)NB + 1;
@@ -114,13 +116,13 @@ __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict_
             for (int k = 0; k < 4; ++k) v[k] = remap(x0 + k * kT1Block);
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                px[x0 + k * kT1Block] = v[k];
+                P[x0 + k * kT1Block + 1] = ckd_spread(v[k]);
                 drow[x0 + k * kT1Block] = v[k];
             }
         }
         for (; x0 < W; x0 += kT1Block) {
             const uint32_t v = remap(x0);
-            px[x0] = v;
+            P[x0 + 1] = ckd_spread(v);
             drow[x0] = v;
         }
     };
@@ -133,7 +135,7 @@ __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict_
     const int chunk = (W + kT1Block - 1) / kT1Block;
     const int c0 = threadIdx.x * chunk, c1 = min(c0 + chunk, W);
     uint64_t local = 0;
-    for (int i = c0; i < c1; ++i) local += ckd_spread(px[i]);
+    for (int i = c0; i < c1; ++i) local += P[i + 1];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint64_t incl = local;
 #pragma unroll
@@ -147,7 +149,7 @@ __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict_
     for (int w = 0; w < wave; ++w) run += waveTot[w];
     if (threadIdx.x == 0) P[0] = 0;
     for (int i = c0; i < c1; ++i) {
-        run += ckd_spread(px[i]);
+        run += P[i + 1];
         P[i + 1] = run;
     }
     __syncthreads();
@@ -226,9 +228,8 @@ ckd_status launch_tunnel_jobs(ckd_ctx *ctx, const TunJob *jobs, int nJobs, const
                               const uint32_t *tex, unsigned texLog2)
 {
     if (totalRows == 0) return CKD_OK;
-    // the row's per-block interpolation terms, prefix sums, pixels
-    const size_t lds = (size_t)((maxW + 7) / 8) * sizeof(uint4) + (size_t)(maxW + 1) * sizeof(uint64_t) +
-                       (size_t)maxW * sizeof(uint32_t);
+    // the row's per-block interpolation terms, its spread pixels / prefix sums
+    const size_t lds = (size_t)((maxW + 7) / 8) * sizeof(uint4) + (size_t)(maxW + 1) * sizeof(uint64_t);
     {
         KernelTimer kt(ctx, CKD_KT_TUNNEL_H);
         k_tunnel_h<<<totalRows, kT1Block, lds, ctx->stream>>>(jobs, nJobs, single, tex, texLog2);
