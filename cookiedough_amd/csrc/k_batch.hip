@@ -18,7 +18,8 @@
 namespace ckd {
 ckd_status launch_sdf_persistent(ckd_ctx *ctx, uint32_t *queueCounter, uint32_t totalChunks,
                                  const SdfJob *jobs, int nJobs, const ckd_sdf_scene &sc,
-                                 int shading, unsigned long long *stepCount, bool helper);
+                                 int shading, unsigned long long *stepCount, bool helper,
+                                 bool alone);
 void voxel_job_geometry(VoxJob &J, unsigned W, unsigned H, const ckd_landscape_cam &c);
 ckd_status launch_landscape_jobs(ckd_ctx *ctx, const VoxJob *jobs, int nJobs, uint32_t totalBlocks);
 void tunnel_job_geometry(TunJob &J, uint32_t *dst, uint32_t *tmp, const int32_t *grid, unsigned W,
@@ -157,7 +158,7 @@ ckd_status launch_plan(ckd_ctx *ctx, const Plan &P, unsigned char *dev, const ck
     if (P.nSdf > 0) {
         s = ckd::launch_sdf_persistent(ctx, reinterpret_cast<uint32_t *>(dev), P.chunks,
                                        reinterpret_cast<const SdfJob *>(dev + P.oSdf), P.nSdf,
-                                       *scene, shading, stepCount, false);
+                                       *scene, shading, stepCount, false, !fork);
         if (s != CKD_OK) return s;
     }
     // with both voxel and tunnel work beside the march, the tunnel work runs
@@ -192,7 +193,7 @@ ckd_status launch_plan(ckd_ctx *ctx, const Plan &P, unsigned char *dev, const ck
             // the voxel / tunnel work is done: its CUs join the SDF march
             s = ckd::launch_sdf_persistent(ctx, reinterpret_cast<uint32_t *>(dev), P.chunks,
                                            reinterpret_cast<const SdfJob *>(dev + P.oSdf), P.nSdf,
-                                           *scene, shading, stepCount, true);
+                                           *scene, shading, stepCount, true, false);
             if (s != CKD_OK) return s;
         }
     }

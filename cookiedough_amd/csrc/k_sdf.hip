@@ -527,7 +527,7 @@ void launch_persistent_steps(int steps, unsigned blocks, hipStream_t st, uint32_
 
 ckd_status launch_sdf_persistent(ckd_ctx *ctx, uint32_t *queueCounter, uint32_t totalChunks,
                                  const SdfJob *jobs, int nJobs, const ckd_sdf_scene &sc,
-                                 int shading, unsigned long long *stepCount, bool helper)
+                                 int shading, unsigned long long *stepCount, bool helper, bool alone)
 {
     if (totalChunks == 0) return CKD_OK;
     const bool big = sc.nSpheres > 8;
@@ -536,14 +536,18 @@ ckd_status launch_sdf_persistent(ckd_ctx *ctx, uint32_t *queueCounter, uint32_t 
     // persistent: sdfBlocksPerCU 4-wave blocks per CU; surplus blocks find
     // the queue empty and exit.  A helper launch (the batch's second stream,
     // once its voxel and tunnel work is done) adds the blocks that fill each
-    // CU up to the kernel's occupancy and drains the same queue.
+    // CU up to the kernel's occupancy and drains the same queue.  A batch
+    // with nothing beside the march (alone) gets at least that occupancy:
+    // the tuned count leaves room for work that is not there.
     int perCU = ctx->tuning.sdfBlocksPerCU;
+    const int fit = shading == CKD_SHADE_FP16
+                        ? (big ? persistent_blocks_per_cu<true, 32>(steps) : persistent_blocks_per_cu<true, 8>(steps))
+                        : (big ? persistent_blocks_per_cu<false, 32>(steps) : persistent_blocks_per_cu<false, 8>(steps));
     if (helper) {
-        const int fit = shading == CKD_SHADE_FP16
-                            ? (big ? persistent_blocks_per_cu<true, 32>(steps) : persistent_blocks_per_cu<true, 8>(steps))
-                            : (big ? persistent_blocks_per_cu<false, 32>(steps) : persistent_blocks_per_cu<false, 8>(steps));
         perCU = fit - perCU;
         if (perCU <= 0) return CKD_OK;
+    } else if (alone && perCU < fit) {
+        perCU = fit;
     }
     unsigned blocks = (unsigned)ctx->cuCount * (unsigned)perCU;
     const unsigned useful = (totalChunks + kWavesPerBlock - 1) / kWavesPerBlock;

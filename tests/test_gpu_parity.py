@@ -197,7 +197,8 @@ def test_grid_remap(ctx, orc, W, H):
     assert np.array_equal(d_dst.numpy(), ref)
 
 
-@pytest.mark.parametrize("W,H,r", [(1280, 720, 1), (1280, 720, 17), (1280, 720, 32), (320, 180, 32)])
+@pytest.mark.parametrize("W,H,r", [(1280, 720, 1), (1280, 720, 17), (1280, 720, 32), (320, 180, 32),
+                                   (1001, 603, 7), (8, 8, 1), (4096, 64, 31)])
 def test_tunnel_frame(ctx, orc, W, H, r):
     _say("tunnel frame", W, H, r)
     tex = wl.noise_texture(0, 9)
