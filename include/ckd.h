@@ -262,7 +262,10 @@ typedef struct ckd_request {
 typedef struct ckd_tuning {
     int sdfStepsPerRefill; /* march steps between lane refills: 1, 2 or 4   */
     int sdfChunk;          /* side of the square pixel chunks: 16 or 32     */
-    int sdfBlocksPerCU;    /* persistent 4-wave blocks per CU: 1..32         */
+    int sdfBlocksPerCU;    /* persistent 4-wave blocks per CU: 1..32 while
+                              voxel or tunnel work runs beside the march; a
+                              batch of SDF requests only gets at least the
+                              kernel's occupancy                            */
     int tunnelVSeg;        /* rows per lane in the vertical blur: 8..512     */
     int blurKernel;        /* in-place blur batches: 0 automatic, 1 lines
                               staged in LDS, 2 an LDS ring per lane          */
