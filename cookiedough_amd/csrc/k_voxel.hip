@@ -224,18 +224,32 @@ __global__ __launch_bounds__(64) void k_tunnelscape(uint32_t *__restrict__ dst, 
     uint32_t lastColor =
         cmap[(((unsigned)(curX >> 8)) & kTsAnd) | ((((unsigned)(curY >> 8)) & kTsAnd) << kTsShift)];
     unsigned written = 0;
-    for (unsigned iStep = 0; iStep < kRayLength; ++iStep) {
+    // pipelined as in ls_column: the texels of the next kLsAhead steps are in
+    // flight while a step draws its span (fetches are masked into the maps)
+    struct Texels {
+        uint32_t h[4], c[4], fu, fv;
+    };
+    auto fetch = [&](Texels &t) {
         curX += dX;
         curY += dY;
         const unsigned u0 = ((unsigned)(curX >> 8)) & kTsAnd;
         const unsigned u1 = ((unsigned)(curX >> 8) + 1) & kTsAnd;
         const unsigned v0 = (((unsigned)(curY >> 8)) & kTsAnd) << kTsShift;
         const unsigned v1 = (((unsigned)(curY >> 8) + 1) & kTsAnd) << kTsShift;
-        const uint32_t fu = (uint32_t)curX & 0xff, fv = (uint32_t)curY & 0xff;
-        const uint32_t h = ckd_bilerp8(hmap[u0 + v0], hmap[u1 + v0], hmap[u0 + v1],
-                                       hmap[u1 + v1], fu, fv);
-        uint32_t color = ckd_bilerp32(cmap[u0 + v0], cmap[u1 + v0], cmap[u0 + v1],
-                                      cmap[u1 + v1], fu, fv);
+        t.fu = (uint32_t)curX & 0xff;
+        t.fv = (uint32_t)curY & 0xff;
+        t.h[0] = hmap[u0 + v0];
+        t.h[1] = hmap[u1 + v0];
+        t.h[2] = hmap[u0 + v1];
+        t.h[3] = hmap[u1 + v1];
+        t.c[0] = cmap[u0 + v0];
+        t.c[1] = cmap[u1 + v0];
+        t.c[2] = cmap[u0 + v1];
+        t.c[3] = cmap[u1 + v1];
+    };
+    auto step = [&](unsigned iStep, const Texels &t) {
+        const uint32_t h = ckd_bilerp8(t.h[0], t.h[1], t.h[2], t.h[3], t.fu, t.fv);
+        uint32_t color = ckd_bilerp32(t.c[0], t.c[1], t.c[2], t.c[3], t.fu, t.fv);
         color = ckd_subs_u8x4(color, fog[iStep >> 1]);
 
         int height = 255 - (int)h - 96;
@@ -260,6 +274,21 @@ __global__ __launch_bounds__(64) void k_tunnelscape(uint32_t *__restrict__ dst, 
         }
         lastHeight = height;
         lastColor = color;
+    };
+    Texels A[kLsAhead], B[kLsAhead];
+#pragma unroll
+    for (int k = 0; k < kLsAhead; ++k) fetch(A[k]);
+    for (unsigned iStep = 0; iStep < kRayLength; iStep += 2 * kLsAhead) {
+#pragma unroll
+        for (int k = 0; k < kLsAhead; ++k) {
+            fetch(B[k]);
+            step(iStep + k, A[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < kLsAhead; ++k) {
+            fetch(A[k]);
+            step(iStep + kLsAhead + k, B[k]);
+        }
     }
     const uint32_t sky = fog[0];
     for (unsigned x = written; x < W; ++x) row[x] = sky;
