@@ -289,24 +289,33 @@ CKD_DEV void b07_lds_line(uint8_t *px, int len, const B07Params &p, const uint16
 // lines of `len` pixels: line l, pixel x at img[l*lineStride + x*pixStride]
 constexpr int kLdsBatch = 16;
 
+// images are global memory: pointers in that address space make the
+// staging global_* accesses rather than flat ones (which LDS waits would
+// also wait for); the 16-byte form is a clang vector, which can be copied
+// through such a pointer
+typedef unsigned int b07_u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) b07_u32x4 global_u32x4;
+typedef __attribute__((address_space(1))) uint32_t global_px32;
+
 template <bool kStore>
-CKD_DEV void b07_stage(uint32_t *img, uint32_t *lds, int line0, int nl, int len, int stride,
+CKD_DEV void b07_stage(uint32_t *image, uint32_t *lds, int line0, int nl, int len, int stride,
                        size_t lineStride, size_t pixStride, bool vec4)
 {
     const int tid = threadIdx.x;
+    global_px32 *img = (global_px32 *)image;
     if (pixStride == 1 && vec4) {
         // rows, 16-byte accesses along each row
         const int perLine = len >> 2, total = nl * perLine;
         for (int base = 0; base < total; base += 64 * kLdsBatch) {
-            uint4 r[kLdsBatch];
+            b07_u32x4 r[kLdsBatch];
 #pragma unroll
             for (int b = 0; b < kLdsBatch; ++b) {
                 const int idx = base + b * 64 + tid;
                 if (idx < total) {
                     const int l = idx / perLine, q = idx - l * perLine;
-                    uint4 *g = reinterpret_cast<uint4 *>(img + (size_t)(line0 + l) * lineStride) + q;
+                    global_u32x4 *g = (global_u32x4 *)(img + (size_t)(line0 + l) * lineStride) + q;
                     uint32_t *d = lds + l * stride + 4 * q;
-                    if constexpr (kStore) *g = make_uint4(d[0], d[1], d[2], d[3]);
+                    if constexpr (kStore) *g = b07_u32x4{d[0], d[1], d[2], d[3]};
                     else r[b] = *g;
                 }
             }
@@ -336,7 +345,7 @@ CKD_DEV void b07_stage(uint32_t *img, uint32_t *lds, int line0, int nl, int len,
             const int idx = base + b * 64 + tid;
             if (idx < total) {
                 const int x = idx / nl, l = idx - x * nl;
-                uint32_t *g = img + (size_t)(line0 + l) * lineStride + (size_t)x * pixStride;
+                global_px32 *g = img + (size_t)(line0 + l) * lineStride + (size_t)x * pixStride;
                 if constexpr (kStore) *g = lds[l * stride + x];
                 else r[b] = *g;
             }
