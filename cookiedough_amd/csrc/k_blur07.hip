@@ -647,8 +647,11 @@ __global__ __launch_bounds__(kRingBlock) void k_blur07_ring_one(B07Job job, int 
 // LDS slots first and only the recurrence itself waits on the carry.
 constexpr int kChainLanes = 64;
 
+typedef __attribute__((address_space(1))) uint8_t global_b8;
+typedef __attribute__((address_space(1))) const uint32_t global_cu32;
+
 template <bool kEven>
-CKD_DEV uint32_t b07_chain_tail(uint8_t *px, int len, const B07Params &p, const uint32_t *state,
+CKD_DEV uint32_t b07_chain_tail(global_b8 *px, int len, const B07Params &p, global_cu32 *state,
                                 uint32_t c, uint8_t *slots)
 {
     const int e = (int)p.edgeSpan, m = (int)p.median;
@@ -665,7 +668,7 @@ CKD_DEV uint32_t b07_chain_tail(uint8_t *px, int len, const B07Params &p, const 
     px[(m + full - 1) * 4] = o0;
     uint32_t last = o0;
     if constexpr (kEven) s.acc = __builtin_elementwise_add_sat((uint16_t)s.acc, (uint16_t)s.addRem);
-    uint8_t *out = px + (m + full) * 4;
+    global_b8 *out = px + (m + full) * 4;
     int i = 0;
     for (; i + 8 <= e; i += 8) {
         uint32_t v[8];
@@ -698,13 +701,16 @@ __global__ __launch_bounds__(kChainLanes) void k_blur07_chain(const ckd::B07Chai
     uint32_t c = *cp;
     for (int f = 0; f < n; ++f) {
         const ckd::B07Chain &J = chain[f];
-        uint8_t *px = reinterpret_cast<uint8_t *>(J.img + (size_t)line * len) + k;
+        // image bytes and saved states through global-address-space
+        // pointers: global accesses, which the LDS slots' waits do not
+        // wait for
+        global_b8 *px = (global_b8 *)(reinterpret_cast<uint8_t *>(J.img + (size_t)line * len) + k);
         if (J.mode == 0) {
             c = px[(size_t)(len - 1) * 4];
         } else if (J.mode == 1) {
             px[(size_t)(len - 1) * 4] = (uint8_t)c;
         } else {
-            const uint32_t *st = J.state + ((size_t)line * 4 + k) * 3;
+            global_cu32 *st = (global_cu32 *)(J.state + ((size_t)line * 4 + k) * 3);
             c = J.p.subEdges ? b07_chain_tail<true>(px, len, J.p, st, c, slots + threadIdx.x)
                              : b07_chain_tail<false>(px, len, J.p, st, c, slots + threadIdx.x);
         }
