@@ -209,9 +209,18 @@ __global__ __launch_bounds__(256) void k_sdf(uint32_t *__restrict__ dst, unsigne
 // refilled with the chunk's next pixels (ballot + mbcnt ranks), so a
 // long-marching ray no longer holds 63 idle lanes.  Finished rays are
 // pushed to a per-wave LDS queue and shaded 64 at a time, so shading also
-// runs with full waves.  The march evaluates |o + t*d - c|^2 as
-// t^2 + t*B + C with B = 2 d.(o-c) per ray and C = |o-c|^2 per chunk:
-// two VALU ops, a square root, a subtract and a min per sphere and step.
+// runs with full waves.
+//
+// The march evaluates |o + t*d - c|^2 about the ray's closest approach to
+// each centre: with m = d.(o-c) and w = (o-c) - m*d (the centre's offset
+// perpendicular to the ray, |w| about the radius near a silhouette),
+// |o + t*d - c|^2 = (t + m)^2 + |w|^2 for a unit d.  Per sphere and step
+// that is an add, an FMA, a square root, a subtract and a min.  Both
+// per-ray constants carry absolute errors of order 1e-6, as the direct
+// form |p - c|^2 does; the expanded t^2 + t*B + |o-c|^2 cancels terms of
+// several hundred and is off by up to 1e-4 near a surface, which decides
+// hit or miss for rays that pass within that much of the 1e-3 threshold
+// and moves rays that run out of steps.
 
 struct ShadeEntry {
     uint32_t job, pix;
@@ -300,7 +309,7 @@ CKD_DEV uint32_t shade_hit(const ckd_sdf_scene &sc, const float *cam, float dx, 
 }
 
 template <bool kHalf, int kMaxSph, int kStepsPerRefill>
-__global__ __launch_bounds__(256) void k_sdf_persistent(uint32_t *__restrict__ queueCounter,
+__global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uint32_t *__restrict__ queueCounter,
                                                         uint32_t totalChunks,
                                                         const SdfJob *__restrict__ jobs, int nJobs,
                                                         const ckd_sdf_scene sc,
@@ -308,9 +317,9 @@ __global__ __launch_bounds__(256) void k_sdf_persistent(uint32_t *__restrict__ q
                                                         uint32_t kChunkSide)
 {
     __shared__ ShadeEntry queue[kWavesPerBlock][kQueueCap];
-    // per-chunk march constants (C_i, r_i) of each wave: read every step with
-    // broadcast LDS loads instead of occupying 2*kMaxSph scalar registers
-    __shared__ f4 sphCR[kWavesPerBlock][kMaxSph / 2]; // {C_2k, r_2k, C_2k+1, r_2k+1}
+    // the radii, read every step with broadcast LDS loads instead of
+    // occupying kMaxSph scalar registers
+    __shared__ f4 sphR[kWavesPerBlock][kMaxSph / 4]; // {r_4k .. r_4k+3}
     // o - c_i of the chunk's camera, read when a lane starts a ray
     __shared__ f4 sphOC[kWavesPerBlock][kMaxSph];
     const uint32_t lane = lane_id();
@@ -329,7 +338,7 @@ __global__ __launch_bounds__(256) void k_sdf_persistent(uint32_t *__restrict__ q
     uint32_t pix = 0;
     float t = 0.f, d = 1.f, tHit = 0.f, dx = 0.f, dy = 0.f, dz = 0.f;
     int steps = 0;
-    float B[kMaxSph];
+    float M[kMaxSph], E2[kMaxSph]; // d.(o-c_i) and |w_i|^2 of the lane's ray
     unsigned long long mySteps = 0;
 
     while (true) {
@@ -347,7 +356,12 @@ __global__ __launch_bounds__(256) void k_sdf_persistent(uint32_t *__restrict__ q
 #pragma unroll
                     for (int i = 0; i < kMaxSph; ++i) {
                         const f4 oc = sphOC[wave][i + z];
-                        B[i] = 2.f * (dx * oc.x + dy * oc.y + dz * oc.z);
+                        const float m = dx * oc.x + dy * oc.y + dz * oc.z;
+                        const float wx = fmaf(-m, dx, oc.x);
+                        const float wy = fmaf(-m, dy, oc.y);
+                        const float wz = fmaf(-m, dz, oc.z);
+                        M[i] = m;
+                        E2[i] = wx * wx + wy * wy + wz * wz;
                     }
                     t = 0.f;
                     d = 1.f;
@@ -378,19 +392,17 @@ __global__ __launch_bounds__(256) void k_sdf_persistent(uint32_t *__restrict__ q
             next = 0;
             end = cw * chh;
             // lane i stages sphere i; spheres past nSpheres become a neutral
-            // sphere whose distance is never the minimum
+            // sphere whose distance is never the minimum (a centre 1e15 away
+            // from the camera, radius 0)
             if (lane < (uint32_t)kMaxSph) {
-                float c = 1e30f, r = 0.f, ox = 0.f, oy = 0.f, oz = 0.f;
+                float r = 0.f, ox = 1e15f, oy = 0.f, oz = 0.f;
                 if ((int)lane < sc.nSpheres) {
                     ox = cam[0] - sc.sphere[lane][0];
                     oy = cam[1] - sc.sphere[lane][1];
                     oz = cam[2] - sc.sphere[lane][2];
-                    c = ox * ox + oy * oy + oz * oz;
                     r = sc.sphere[lane][3];
                 }
-                float *cr = reinterpret_cast<float *>(&sphCR[wave][lane >> 1]) + 2 * (lane & 1);
-                cr[0] = c;
-                cr[1] = r;
+                reinterpret_cast<float *>(&sphR[wave][lane >> 2])[lane & 3] = r;
                 sphOC[wave][lane] = f4{ox, oy, oz, 0.f};
             }
             __builtin_amdgcn_wave_barrier();
@@ -401,16 +413,16 @@ __global__ __launch_bounds__(256) void k_sdf_persistent(uint32_t *__restrict__ q
 #pragma unroll
         for (int k = 0; k < kStepsPerRefill; ++k) {
             if (active && d > sc.eps && steps < sc.maxSteps && t < sc.tMax) {
-                const float tt = t * t;
                 float dist = fmaf(t, dy, oyg);
                 const uint32_t z = opaque_zero();
 #pragma unroll
-                for (int i = 0; i < kMaxSph / 2; ++i) {
-                    const f4 cr = sphCR[wave][i + z];
-                    const float q0 = fmaf(t, B[2 * i], cr.x + tt);
-                    const float q1 = fmaf(t, B[2 * i + 1], cr.z + tt);
-                    dist = fminf(dist, fsqrt(q0) - cr.y);
-                    dist = fminf(dist, fsqrt(q1) - cr.w);
+                for (int i = 0; i < kMaxSph / 4; ++i) {
+                    const f4 r = sphR[wave][i + z];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const float u = t + M[4 * i + j];
+                        dist = fminf(dist, fsqrt(fmaf(u, u, E2[4 * i + j])) - r[j]);
+                    }
                 }
                 tHit = t;
                 d = dist;

@@ -47,6 +47,54 @@ void release()
     }
 }
 
+// The reference's free functions (the blits and blends of util.cpp, the
+// blurs, Fx_Blit_2x2) need no Create call.  The first of them made while no
+// part holds the context takes one reference for them, held until the
+// process ends; later calls reuse it.
+bool g_lazyHeld = false;
+
+bool ensure_ctx()
+{
+    if (g_ctx) return true;
+    if (!acquire()) return false;
+    g_lazyHeld = true;
+    return true;
+}
+
+// The context reference of one part (landscape, polar, ...), taken by its
+// Create call.  A Create that fails after taking it gives it back when the
+// guard goes out of scope; a Create that succeeds keeps it (done()) until
+// the part's Destroy.  Creating a part that exists destroys it first.
+struct PartRef {
+    bool &ready;
+    bool took = false;
+    explicit PartRef(bool &r) : ready(r) {}
+    bool take()
+    {
+        if (ready) {
+            ready = false;
+            release();
+        }
+        return took = acquire();
+    }
+    bool done()
+    {
+        ready = true;
+        took = false;
+        return true;
+    }
+    ~PartRef()
+    {
+        if (took) release();
+    }
+};
+
+void drop(bool &ready)
+{
+    if (ready) release();
+    ready = false;
+}
+
 // device image that grows on demand
 struct DevImage {
     uint32_t *p = nullptr;
@@ -83,7 +131,7 @@ template <typename F>
 void filter(const char *name, uint32_t *pDest, const uint32_t *pSrc, size_t pixels, bool inPlace,
             F f)
 {
-    if (!g_ctx && !acquire()) return;
+    if (!ensure_ctx()) return;
     uint32_t *a = g_a.get(pixels);
     uint32_t *b = inPlace ? a : g_b.get(pixels);
     if (!a || !b || !up(a, pSrc, pixels) || f(b, a) != CKD_OK || !down(pDest, b, pixels))
@@ -95,7 +143,7 @@ template <typename F>
 void blit2(const char *name, uint32_t *pDest, size_t dstPixels, const uint32_t *pSrc,
            size_t srcPixels, F f)
 {
-    if (!g_ctx && !acquire()) return;
+    if (!ensure_ctx()) return;
     uint32_t *a = g_a.get(dstPixels), *b = g_b.get(srcPixels);
     if (!a || !b || !up(a, pDest, dstPixels) || !up(b, pSrc, srcPixels) || f(a, b) != CKD_OK ||
         !down(pDest, a, dstPixels))
@@ -114,6 +162,9 @@ float clampf(float lo, float hi, float v) { return std::fmax(lo, std::fmin(hi, v
 
 // ---- tunnelscape state (code/tunnelscape.cpp:19-20) ----------------------
 bool s_tsReady = false;
+
+// ---- filters: whether each part's Create holds a context reference -------
+bool s_polarReady = false, s_fxReady = false, s_blurReady = false;
 SyncTrack trackStarsStepU, trackStarsStepV, trackStarsSpeed, trackStarsBlur;
 
 // ---- raymarched effects, ball, twister ------------------------------------
@@ -184,11 +235,14 @@ template <typename F> void st_draw(const char *name, uint32_t *pDest, F render)
 
 // ===================================================================== parts
 
+int ckd_shim_context_refs() { return g_users; }
+
 uint32_t *g_pFxMap[kShimNumFxMaps] = {nullptr};
 
 bool Landscape_Create()
 {
-    if (!acquire()) return false;
+    PartRef part(s_lsReady);
+    if (!part.take()) return false;
     const uint8_t *hm = Image_Load8("assets/scape/D17.png");
     const uint32_t *cm = Image_Load32("assets/scape/C17W-edit.png");
     if (!hm || !cm) return false;
@@ -202,15 +256,10 @@ bool Landscape_Create()
     trackVoxelScapeTilt = Rocket::AddTrack("voxelScape:Tilt");
     trackWarpSpeed = Rocket::AddTrack("voxelScape:WarpSpeed");
     trackWarpStrength = Rocket::AddTrack("voxelScape:WarpStrength");
-    s_lsReady = true;
-    return true;
+    return part.done();
 }
 
-void Landscape_Destroy()
-{
-    if (s_lsReady) release();
-    s_lsReady = false;
-}
+void Landscape_Destroy() { drop(s_lsReady); }
 
 void Landscape_Draw(uint32_t *pDest, float time, float delta)
 {
@@ -268,7 +317,8 @@ void Landscape_Draw(uint32_t *pDest, float time, float delta)
 
 bool Tunnelscape_Create()
 {
-    if (!acquire()) return false;
+    PartRef part(s_tsReady);
+    if (!part.take()) return false;
     const uint8_t *hm = Image_Load8("assets/scape/tscape-D7-edit.png");
     const uint32_t *cm = Image_Load32("assets/scape/tscape-C7W-edit.png");
     if (!hm || !cm) return false;
@@ -283,15 +333,10 @@ bool Tunnelscape_Create()
     trackStarsStepV = Rocket::AddTrack("starsTunnel:stepV");
     trackStarsSpeed = Rocket::AddTrack("starsTunnel:Speed");
     trackStarsBlur = Rocket::AddTrack("starsTunnel:Blur");
-    s_tsReady = true;
-    return true;
+    return part.done();
 }
 
-void Tunnelscape_Destroy()
-{
-    if (s_tsReady) release();
-    s_tsReady = false;
-}
+void Tunnelscape_Destroy() { drop(s_tsReady); }
 
 void Tunnelscape_Draw(uint32_t *pDest, float time, float delta)
 {
@@ -322,12 +367,12 @@ void Tunnelscape_Draw(uint32_t *pDest, float time, float delta)
 
 bool Shadertoy_Create()
 {
-    if (!acquire()) return false;
+    PartRef part(s_stReady);
+    if (!part.take()) return false;
     if (!s_plasmaTracks.add("plasma") || !s_nautilusTracks.add("nautilus") ||
         !s_spikeyTracks.add("spikey") || !s_sinusesTracks.add("sinuses") ||
         !s_lauraTracks.add("laura") || !s_tunnelTracks.add("fdtunnel")) {
         report("Shadertoy_Create (tracks)");
-        release();
         return false;
     }
     // the free-directional tunnel's 1024x1024 textures and the close-up's
@@ -343,15 +388,10 @@ bool Shadertoy_Create()
         report("Shadertoy_Create");
         return false;
     }
-    s_stReady = true;
-    return true;
+    return part.done();
 }
 
-void Shadertoy_Destroy()
-{
-    if (s_stReady) release();
-    s_stReady = false;
-}
+void Shadertoy_Destroy() { drop(s_stReady); }
 
 void Plasma_Draw(uint32_t *pDest, float time, float delta)
 {
@@ -404,17 +444,18 @@ void Tunnel_Draw(uint32_t *pDest, float time, float delta)
 
 bool Polar_Create()
 {
-    if (!acquire()) return false;
+    PartRef part(s_polarReady);
+    if (!part.take()) return false;
     // the output's maps and the FX map's (code/polar.cpp:61-70)
     if (ckd_polar_create(g_ctx, kResX, kResY) != CKD_OK ||
         ckd_polar_create(g_ctx, kFxMapResX, kFxMapResY) != CKD_OK) {
         report("Polar_Create");
         return false;
     }
-    return true;
+    return part.done();
 }
 
-void Polar_Destroy() { release(); }
+void Polar_Destroy() { drop(s_polarReady); }
 
 void Polar_Blit(uint32_t *pDest, const uint32_t *pSrc, bool inverse)
 {
@@ -425,27 +466,33 @@ void Polar_Blit(uint32_t *pDest, const uint32_t *pSrc, bool inverse)
 
 bool FxBlitter_Create()
 {
-    if (!acquire()) return false;
+    PartRef part(s_fxReady);
+    if (!part.take()) return false;
     // one allocation for the four maps, 16-byte aligned (code/fx-blitter.cpp:10-20)
+    free(g_pFxMap[0]);
+    for (auto &m : g_pFxMap) m = nullptr;
     const size_t mapPixels = (size_t)kFxMapResX * kFxMapResY;
     void *base = nullptr;
-    if (posix_memalign(&base, 16, mapPixels * 4 * kShimNumFxMaps) != 0) return false;
+    if (posix_memalign(&base, 16, mapPixels * 4 * kShimNumFxMaps) != 0) {
+        SetLastError("FxBlitter_Create: out of host memory");
+        return false;
+    }
     memset(base, 0, mapPixels * 4 * kShimNumFxMaps);
     for (size_t i = 0; i < kShimNumFxMaps; ++i)
         g_pFxMap[i] = static_cast<uint32_t *>(base) + i * mapPixels;
-    return true;
+    return part.done();
 }
 
 void FxBlitter_Destroy()
 {
     free(g_pFxMap[0]);
     for (auto &m : g_pFxMap) m = nullptr;
-    release();
+    drop(s_fxReady);
 }
 
 void Fx_Blit_2x2(uint32_t *pDest, const uint32_t *pSrc)
 {
-    if (!g_ctx && !acquire()) return;
+    if (!ensure_ctx()) return;
     const size_t mapPixels = (size_t)kFxMapResX * kFxMapResY, n = (size_t)kResX * kResY;
     uint32_t *a = g_a.get(mapPixels), *b = g_b.get(n);
     if (!a || !b || !up(a, pSrc, mapPixels) || ckd_fx_blit_2x2(g_ctx, b, a, kResX, kResY) != CKD_OK ||
@@ -453,8 +500,30 @@ void Fx_Blit_2x2(uint32_t *pDest, const uint32_t *pSrc)
         report("Fx_Blit_2x2");
 }
 
-bool BoxBlur_Create() { return acquire(); }
-void BoxBlur_Destroy() { release(); }
+// code/fx-blitter.cpp:77-100: the test pattern (upper half rows, lower half
+// columns of alternating black and white) written into g_pFxMap[0], then
+// Fx_Blit_2x2 of it
+void FxBlitter_DrawTestPattern(uint32_t *pDest)
+{
+    if (!g_pFxMap[0]) {
+        SetLastError("FxBlitter_DrawTestPattern: FxBlitter_Create has not been called");
+        return;
+    }
+    for (unsigned y = 0; y < kFxMapResY; ++y)
+        for (unsigned x = 0; x < kFxMapResX; ++x) {
+            const unsigned bit = y < kFxMapResY / 2 ? y : x;
+            g_pFxMap[0][y * kFxMapResX + x] = (bit & 1) ? 0xffffffffu : 0u;
+        }
+    Fx_Blit_2x2(pDest, g_pFxMap[0]);
+}
+
+bool BoxBlur_Create()
+{
+    PartRef part(s_blurReady);
+    return part.take() && part.done();
+}
+
+void BoxBlur_Destroy() { drop(s_blurReady); }
 
 void BoxBlur_Horz32(uint32_t *pDest, const uint32_t *pSrc, unsigned xRes, unsigned yRes,
                     float strength, float gain, unsigned numPasses)
@@ -628,7 +697,8 @@ void Polar_BlitA(uint32_t *pDest, const uint32_t *pSrc, bool inverse)
 
 bool Ball_Create()
 {
-    if (!acquire()) return false;
+    PartRef part(s_ballReady);
+    if (!part.take()) return false;
     static const char *const kHeightMaps[5] = {"assets/ball/hmap_1_1k.jpg", "assets/ball/hmap_4_1k.jpg",
                                                "assets/ball/hmap_2_1k.jpg", "assets/ball/hmap_3_1k.jpg",
                                                "assets/ball/hmap_5_1k.jpg"};
@@ -673,25 +743,16 @@ bool Ball_Create()
     }
     delete[] heights;
     delete[] images;
-    if (!ok) {
-        release();
-        return false;
-    }
+    if (!ok) return false;
     if (!s_ballTracks.add("ball")) {
         report("Ball_Create (tracks)");
-        release();
         return false;
     }
     s_ballBackground = bg[0];
-    s_ballReady = true;
-    return true;
+    return part.done();
 }
 
-void Ball_Destroy()
-{
-    if (s_ballReady) release();
-    s_ballReady = false;
-}
+void Ball_Destroy() { drop(s_ballReady); }
 
 void Ball_Draw(uint32_t *pDest, float time, float delta)
 {
@@ -716,30 +777,24 @@ bool Ball_HasBeams()
 
 bool Twister_Create()
 {
-    if (!acquire()) return false;
+    PartRef part(s_twReady);
+    if (!part.take()) return false;
     const uint8_t *hm = Image_Load8("assets/twister/hmap_2_1k.jpg");
     const uint32_t *cm = Image_Load32("assets/twister/colormap_1k.jpg");
     const uint32_t *bg = (hm && cm) ? Image_Load32("assets/twister/nytrik-background_1280x720.png")
                                     : nullptr;
     if (!bg || ckd_twister_create(g_ctx, hm, cm, bg, kResX, kResY) != CKD_OK) {
         if (bg) report("Twister_Create");
-        release();
         return false;
     }
     if (!s_twisterTracks.add("twister")) {
         report("Twister_Create (tracks)");
-        release();
         return false;
     }
-    s_twReady = true;
-    return true;
+    return part.done();
 }
 
-void Twister_Destroy()
-{
-    if (s_twReady) release();
-    s_twReady = false;
-}
+void Twister_Destroy() { drop(s_twReady); }
 
 void Twister_Draw(uint32_t *pDest, float time, float delta)
 {

@@ -47,6 +47,7 @@ extern uint32_t *g_pFxMap[kShimNumFxMaps];
 bool FxBlitter_Create();
 void FxBlitter_Destroy();
 void Fx_Blit_2x2(uint32_t *pDest, const uint32_t *pSrc);
+void FxBlitter_DrawTestPattern(uint32_t *pDest);
 
 // code/boxblur.h:7-20
 bool BoxBlur_Create();
@@ -107,3 +108,8 @@ void BlitSrc32(uint32_t *pDest, const uint32_t *pSrc, unsigned destResX, unsigne
 void BlitSrc32A(uint32_t *pDest, const uint32_t *pSrc, unsigned destResX, unsigned srcResX,
                 unsigned yRes, float alpha);
 void Fade32(uint32_t *pDest, unsigned int numPixels, uint32_t RGB, uint8_t alpha);
+
+// No reference counterpart: the number of references to the shim's one GPU
+// context that Create calls (and the first call of a free function) hold;
+// 0 once every part is destroyed.  For tests of the parts' lifetimes.
+int ckd_shim_context_refs();

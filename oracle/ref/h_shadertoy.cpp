@@ -130,6 +130,16 @@ REF_API void ref_st_gamma(const float *in4, float gamma, float *out4)
     _mm_storeu_ps(out4, Shadertoy::GammaAdj(v, gamma));
 }
 
+REF_API float ref_st_expfog(float distance, float scale)
+{
+    return Shadertoy::ExpFog(distance, scale);
+}
+
+REF_API void ref_st_vlerp4(const float *a4, const float *b4, float factor, float *out4)
+{
+    _mm_storeu_ps(out4, Shadertoy::vLerp4(_mm_loadu_ps(a4), _mm_loadu_ps(b4), factor));
+}
+
 REF_API void ref_st_topixel4(const float *in16, uint32_t *out4)
 {
     __m128 c[4];

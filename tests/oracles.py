@@ -100,6 +100,10 @@ _REF = {
     "ref_vscape_ray": (None, [_p, _p, _p, _p, _i, _i, _i, _i, _i, _f]),
     "ref_memset32": (None, [_p, C.c_uint32, C.c_size_t]),
     "ref_tscape_ray": (None, [_p, _p, _p, _p, _i, _i, _i, _i]),
+    "ref_tunnelscape_setup": (_i, [C.c_char_p, _p, _p, _p]),
+    "ref_tscape": (None, [_p, _f]),
+    "ref_tunnelscape_draw": (None, [_p, _f, _f]),
+    "ref_fx_test_pattern": (_i, [_p]),
     "ref_polar_maps": (None, [_p, _p, _u, _u, _u, _u]),
     "ref_polar_create": (_i, []),
     "ref_polar_blit_2x2": (None, [_p, _p, _i]),
@@ -131,6 +135,8 @@ _REF = {
     "ref_st_draw": (None, [_i, _p, _f, _f]),
     "ref_st_gamma": (None, [_p, _f, _p]),
     "ref_st_topixel4": (None, [_p, _p]),
+    "ref_st_expfog": (_f, [_f, _f]),
+    "ref_st_vlerp4": (None, [_p, _p, _f, _p]),
     "ref_mix32": (None, [_p, _p, _u, C.c_uint8]),
     "ref_mixover32": (None, [_p, _p, _u]),
     "ref_sub32": (None, [_p, _p, _u]),

@@ -19,6 +19,11 @@
 //   shim_test twister <time> <out>
 //   shim_test polar_blit_a <dst> <src> <inverse 0|1> <out>
 //   shim_test polar2x2 <in 644x364> <inverse 0|1> <out>
+//   shim_test tunnelscape <time> <frames> <out>     (frames successive 1280x720
+//                                                     frames at the same time)
+//   shim_test testpattern <out>                     (FxBlitter_DrawTestPattern)
+//   shim_test lifecycle                             (Create failures and
+//             Create/Destroy pairs leave no context reference behind; prints ok)
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -217,6 +222,53 @@ int main(int argc, char **argv)
         Polar_BlitA(d.data(), s.data(), atoi(argv[4]) != 0);
         Polar_Destroy();
         write_pixels(argv[5], d.data(), n);
+    } else if (cmd == "tunnelscape" && argc == 5) {
+        const int frames = atoi(argv[3]);
+        if (frames < 1 || !Tunnelscape_Create()) return 1;
+        std::vector<uint32_t> all((size_t)frames * n);
+        for (int k = 0; k < frames; ++k)
+            Tunnelscape_Draw(all.data() + (size_t)k * n, (float)atof(argv[2]), 1.f);
+        Tunnelscape_Destroy();
+        write_pixels(argv[4], all.data(), all.size());
+    } else if (cmd == "testpattern" && argc == 3) {
+        if (!FxBlitter_Create()) return 1;
+        FxBlitter_DrawTestPattern(out.data());
+        FxBlitter_Destroy();
+        write_pixels(argv[2], out.data(), n);
+    } else if (cmd == "lifecycle" && argc == 2) {
+        // run with CKD_SHIM_ASSETS naming an empty directory: every Create
+        // that loads an asset fails, and must give its reference back
+        auto expect = [](int refs, const char *where) {
+            if (ckd_shim_context_refs() != refs) {
+                fprintf(stderr, "%s: %d context references, expected %d\n", where,
+                        ckd_shim_context_refs(), refs);
+                exit(1);
+            }
+        };
+        expect(0, "start");
+        if (Landscape_Create() || Tunnelscape_Create() || Shadertoy_Create() || Ball_Create() ||
+            Twister_Create())
+            return 1;
+        expect(0, "after failed Creates");
+        g_failed = false; // the failures above reported through SetLastError
+        if (!Polar_Create() || !Polar_Create()) return 1;
+        expect(1, "Polar_Create twice");
+        Polar_Destroy();
+        Polar_Destroy();
+        expect(0, "Polar_Destroy twice");
+        if (!FxBlitter_Create() || !BoxBlur_Create()) return 1;
+        expect(2, "FxBlitter_Create + BoxBlur_Create");
+        FxBlitter_DrawTestPattern(out.data());
+        BoxBlur_Destroy();
+        FxBlitter_Destroy();
+        expect(0, "after their Destroys");
+        std::vector<uint32_t> a(64, 0x01020304u), b(64, 0x10203040u);
+        Add32(a.data(), b.data(), 64);
+        expect(1, "a free function holds one reference");
+        Add32(a.data(), b.data(), 64);
+        expect(1, "and only one");
+        if (a[0] != 0x11223344u) return 1;
+        printf("ok\n");
     } else {
         return 2;
     }
