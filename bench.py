@@ -19,6 +19,7 @@ import ctypes as C
 import json
 import os
 import re
+import statistics
 import sys
 import time
 from pathlib import Path
@@ -47,6 +48,52 @@ def sdf_flops(total_steps: int, pixels: int, n_spheres: int) -> float:
     for the normal and ~40 for shading (FMA = 2, sqrt/exp/log = 1)."""
     f_sdf = 11 * n_spheres + 2
     return total_steps * (f_sdf + 7) + pixels * (6 * f_sdf + 40)
+
+
+def host_info() -> dict:
+    """the host the CPU baselines ran on: logical CPUs, the affinity the
+    process got, CPU model, sockets x cores"""
+    info = {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
+    try:
+        text = Path("/proc/cpuinfo").read_text()
+        models = re.findall(r"^model name\s*:\s*(.+)$", text, re.M)
+        sockets = set(re.findall(r"^physical id\s*:\s*(\d+)$", text, re.M))
+        cores = re.findall(r"^cpu cores\s*:\s*(\d+)$", text, re.M)
+        info["cpu_model"] = models[0].strip() if models else None
+        info["sockets"] = len(sockets) or None
+        info["cores_per_socket"] = int(cores[0]) if cores else None
+    except OSError:
+        pass
+    return info
+
+
+def median_rate(fn, pixels: int, warmup: int = 3, runs: int = 5) -> dict:
+    """Mpixel/s of fn over `pixels`: the median of `runs` timed calls after
+    `warmup` untimed ones"""
+    for _ in range(warmup):
+        fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    med = statistics.median(ts)
+    return {"mpx_s": round(pixels / med / 1e6, 2), "median_s": round(med, 4),
+            "runs_s": [round(t, 4) for t in ts], "warmup": warmup}
+
+
+def headline_sample(items) -> list[int]:
+    """indices of the batch requests whose output is checked: every
+    320x180 and 640x360 request, and the first request of each effect at
+    each larger size"""
+    pick, seen = [], set()
+    for i, it in enumerate(items):
+        if it.W <= 640:
+            pick.append(i)
+        elif (it.effect, it.W) not in seen:
+            seen.add((it.effect, it.W))
+            pick.append(i)
+    return pick
 
 
 class Timer:
@@ -119,11 +166,12 @@ def main() -> int:
     tmp_px = sum(i.W * i.H for i in items if i.effect == CKD_FX_TUNNEL)
     out = ctx.empty(total_px)
     tmp = ctx.empty(max(1, tmp_px))
-    reqs, off, grids = [], 0, []
+    reqs, off, grids, offs = [], 0, [], []
     for it in items:
         g = ctx.upload(wl.tunnel_grid(it.W, it.H, it.t)) if it.effect == CKD_FX_TUNNEL else None
         grids.append(g)
         reqs.append(wl.make_request(it, out.ptr + 4 * off, scene8, path, g.ptr if g else None))
+        offs.append(off)
         off += it.W * it.H
     sdf_only = ctx.request_array([r for r, it in zip(reqs, items) if it.effect == CKD_FX_SDF])
     reqs = ctx.request_array(reqs)
@@ -160,6 +208,11 @@ def main() -> int:
     ms_step = ms_total / args.steps
     value = total_px / (ms_step * 1e-3) / 1e6
     log(f"batch: {ms_step:.2f} ms/step, {value:.0f} Mpx/s")
+    # the output of the last timed replay, for the requests the CPU leg
+    # checks against the oracle
+    checked = headline_sample(items)
+    ctx.sync()
+    headline_out = {i: out.view(offs[i], (items[i].W * items[i].H,)).numpy() for i in checked}
 
     # ------------------------------------------- roofline: persistent tracer
     # The kernel's own launches, bracketed by HIP events on the stream it
@@ -212,7 +265,7 @@ def main() -> int:
         ctx.set_tuning(headline_tuning)
     log("roofline", roofline)
 
-    per, es, cpu = {}, None, None
+    per, es, cpu, headline_err = {}, None, None, None
     sections = set(args.sections.split(","))
     if not args.headline_only:
         # ----------------------------------------------------- per effect (c0-c2)
@@ -222,10 +275,11 @@ def main() -> int:
             # SDF c0: 64 frames at t=k/60
             cams = [wl.sdf_camera(scene8, k / 60) for k in range(64)]
             fr_sdf = ctx.empty(64 * W * H)
+            fr_sdf_p = ctx.empty(64 * W * H)
 
             def sdf_plain():
                 for k, c in enumerate(cams):
-                    ctx.sdf_render(fr_sdf.view(k * W * H, (W * H,)), W, H, scene8, c, CKD_SHADE_FP32)
+                    ctx.sdf_render(fr_sdf_p.view(k * W * H, (W * H,)), W, H, scene8, c, CKD_SHADE_FP32)
 
             sdf_reqs = []
             for k in range(64):
@@ -309,26 +363,35 @@ def main() -> int:
             W3, H3 = 2560, 1440
             n3 = 64
             fr3 = ctx.empty(n3 * W3 * H3)
+            fr3p = ctx.empty(n3 * W3 * H3)
             cams3 = [wl.sdf_camera(scene32, k / 60) for k in range(n3)]
 
             def c3_plain():
                 for k, c in enumerate(cams3):
-                    ctx.sdf_render(fr3.view(k * W3 * H3, (W3 * H3,)), W3, H3, scene32, c, CKD_SHADE_FP32)
+                    ctx.sdf_render(fr3p.view(k * W3 * H3, (W3 * H3,)), W3, H3, scene32, c, CKD_SHADE_FP32)
 
             reqs3 = [wl.make_request(wl.BatchItem(CKD_FX_SDF, W3, H3, k / 60), fr3.ptr + 4 * k * W3 * H3,
                                      scene32, path, None) for k in range(n3)]
             reqs3 = ctx.request_array(reqs3)
 
             def c3_batched():
-                ctx.render_batch(reqs3, scene32, CKD_SHADE_FP16, None, 9, None, 0)
+                ctx.render_batch(reqs3, scene32, CKD_SHADE_FP16, None, 9, None, 0, pcount)
 
             c3_plain()
             c3_batched()
-            mp, mb = T.run(c3_plain), T.run(c3_batched)
+            mp = T.run(c3_plain)
+            st0 = int(pcount.numpy()[0])
+            mb = T.run(c3_batched)
+            steps3 = int(pcount.numpy()[0]) - st0
             px3 = n3 * W3 * H3
+            ach3 = sdf_flops(steps3, px3, 32) / (mb * 1e-3) / 1e12
             per["sdf_c3_precision"] = {"frames": n3, "plain_fp32_mpx_s": round(px3 / mp / 1e3, 1),
                                        "batched_fp16_mpx_s": round(px3 / mb / 1e3, 1),
-                                       "speedup": round(mp / mb, 3)}
+                                       "speedup": round(mp / mb, 3),
+                                       "roofline": {"bound": "valu", "achieved": round(ach3, 2),
+                                                    "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                                                    "algorithmic": "sum(steps)*361 + px*2164 FLOP",
+                                                    "frac": round(ach3 / PEAK_FP32_TFLOPS, 4)}}
             log("sdf_c3", per["sdf_c3_precision"])
 
 
@@ -403,102 +466,219 @@ def main() -> int:
                 log("voxel2", key, v2_per[key])
             per["ball_twister_1280x720"] = v2_per
 
-        # The CPU leg is the only part of the bench that runs the oracle: the
-        # timed CPU baseline and the FP64 reference render the precision form's
-        # error is measured against.
+        # The CPU leg is the only part of the bench that runs the oracle and
+        # the reference harness: the checks of the GPU frames against them,
+        # and the timed CPU baselines.
         if not args.no_cpu:
             import oracles  # noqa: E402  (the checker)
-            from helpers import error_stats  # noqa: E402
+            from helpers import error_stats, merge_reports, sdf_fp64, sdf_report  # noqa: E402
+            from oracles import P  # noqa: E402
             orc = oracles.oracle()
-            nthr = min(16, os.cpu_count() or 1)
+            host = host_info()
+            # the GPU box's CPU share for one GPU is 16 threads
+            nthr = max(1, min(16, host["affinity_cpus"]))
+            host["threads_used"] = nthr
+            use_ref = oracles.ref_available()
+            refl = oracles.ref() if use_ref else None
+            if use_ref:
+                refl.ref_omp_threads(nthr)
+
+            def sdf_dense(batched_dev, plain_dev, sc, Wd, Hd, n, row_step=4):
+                """every row_step-th row of every frame of the batched FP16 and
+                the plain FP32 renders against one FP64 render"""
+                reps = {"batched_fp16": [], "plain_fp32": []}
+                for k in range(n):
+                    ref, _, cond = sdf_fp64(orc, sc, Wd, Hd, k / 60, 0, row_step, nthr)
+                    for key, dev in (("batched_fp16", batched_dev), ("plain_fp32", plain_dev)):
+                        img = dev.view(k * Wd * Hd, (Wd * Hd,)).numpy().reshape(Hd, Wd)
+                        reps[key].append(sdf_report(img[::row_step].reshape(-1), ref, cond))
+                return {"rows_sampled": f"every {row_step}th row of all {n} frames",
+                        **{key: merge_reports(v) for key, v in reps.items()}}
+
+            # ---- the headline batch: its sampled requests against the oracle
+            t_chk = time.time()
+            sdf_reps, vox, tun = [], {"requests": 0, "mismatched_px": 0}, {"requests": 0, "mismatched_px": 0}
+            for i in checked:
+                it, got = items[i], headline_out[i]
+                n = it.W * it.H
+                if it.effect == CKD_FX_SDF:
+                    ref, _, cond = sdf_fp64(orc, scene8, it.W, it.H, it.t, 0, 1, nthr)
+                    sdf_reps.append(sdf_report(got, ref, cond))
+                    continue
+                ref = np.zeros(n, np.uint32)
+                if it.effect == CKD_FX_VOXEL:
+                    oc = oracles.orc_cam(wl.landscape_cam(path, it.t))
+                    orc.orc_landscape_frame(P(ref), it.W, it.H, P(maps.height), P(maps.color), P(maps.fog),
+                                            C.byref(oc), nthr)
+                    d = vox
+                else:
+                    g = wl.tunnel_grid(it.W, it.H, it.t)
+                    b0, b1 = np.zeros(n, np.uint32), np.zeros(n, np.uint32)
+                    orc.orc_tunnel_frame(P(ref), P(b0), P(b1), it.W, it.H, P(g), P(tex), 9, it.radius, nthr)
+                    d = tun
+                d["requests"] += 1
+                d["mismatched_px"] += int((ref != got).sum())
+            headline_err = {
+                "requests_checked": len(checked), "of": len(items),
+                "checked": "every 320x180 and 640x360 request, and the first of each effect at each "
+                           "larger size",
+                "sdf_fp16_vs_fp64": dict(merge_reports(sdf_reps), requests=len(sdf_reps)),
+                "voxel_vs_oracle": dict(vox, bit_exact=vox["mismatched_px"] == 0),
+                "tunnel_vs_oracle": dict(tun, bit_exact=tun["mismatched_px"] == 0)}
+            log(f"headline check in {time.time() - t_chk:.1f}s", headline_err)
+
             if "classic" in sections:
-                # c0: the batched FP16 frames against the FP64 render on a row
-                # sample; c1 / c2 (integer paths): first and last frame against
-                # the oracle, which must be bit-exact
+                # c0: plain FP32 and batched FP16 frames against the FP64
+                # render, every 4th row of all 64 frames
+                sdf_plain()
                 sdf_batched()
-                errs = []
-                for k in [0, 63]:
-                    img = fr_sdf.view(k * W * H, (W * H,)).numpy().reshape(H, W)
-                    rows = np.arange(5, H, 24)
-                    ref = np.zeros(len(rows) * W, np.uint32)
-                    orc.orc_sdf_rows(oracles.P(ref), W, H, C.byref(scene8), k / 60, 64, 5, 24, None, nthr)
-                    errs.append((img[rows].reshape(-1), ref))
-                per["sdf_c0"]["error_vs_fp64"] = error_stats(np.concatenate([e[0] for e in errs]),
-                                                             np.concatenate([e[1] for e in errs]))
+                per["sdf_c0"]["error_vs_fp64"] = sdf_dense(fr_sdf, fr_sdf_p, scene8, W, H, 64)
+                # c1 / c2 (integer paths): four frames each against the
+                # oracle, which must be bit-exact
                 vox_batched()
                 got, want = [], []
-                for k in [0, 119]:
+                for k in [0, 17, 59, 119]:
                     buf = np.zeros(W * H, np.uint32)
                     oc = oracles.orc_cam(lcams[k])
-                    orc.orc_landscape_frame(oracles.P(buf), W, H, oracles.P(maps.height),
-                                            oracles.P(maps.color), oracles.P(maps.fog), C.byref(oc), nthr)
+                    orc.orc_landscape_frame(P(buf), W, H, P(maps.height), P(maps.color), P(maps.fog),
+                                            C.byref(oc), nthr)
                     got.append(fr_vox.view(k * W * H, (W * H,)).numpy())
                     want.append(buf)
-                per["voxel_c1"]["error_vs_oracle"] = error_stats(np.concatenate(got), np.concatenate(want))
+                per["voxel_c1"]["error_vs_oracle"] = dict(error_stats(np.concatenate(got), np.concatenate(want)),
+                                                         frames=[0, 17, 59, 119])
                 tun_batched()
                 got, want = [], []
-                for k in [0, 239]:
+                for k in [0, 61, 142, 239]:
                     buf = np.zeros(W * H, np.uint32)
                     b0, b1 = np.zeros(W * H, np.uint32), np.zeros(W * H, np.uint32)
                     g = wl.tunnel_grid(W, H, k / 60)
-                    orc.orc_tunnel_frame(oracles.P(buf), oracles.P(b0), oracles.P(b1), W, H, oracles.P(g),
-                                         oracles.P(tex), 9, int(radii[k]), nthr)
+                    orc.orc_tunnel_frame(P(buf), P(b0), P(b1), W, H, P(g), P(tex), 9, int(radii[k]), nthr)
                     got.append(fr_tun.view(k * W * H, (W * H,)).numpy())
                     want.append(buf)
-                per["tunnel_c2"]["error_vs_oracle"] = error_stats(np.concatenate(got), np.concatenate(want))
+                per["tunnel_c2"]["error_vs_oracle"] = dict(error_stats(np.concatenate(got), np.concatenate(want)),
+                                                          frames=[0, 61, 142, 239])
                 log("classic errors", {k: per[k].get("error_vs_fp64", per[k].get("error_vs_oracle"))
                                        for k in ("sdf_c0", "voxel_c1", "tunnel_c2")})
+
+                # CPU baselines of c0-c2 (4 frames per run, median of 5 runs
+                # after 3 warm-ups): the oracle's FP32 render of the SDF scene
+                # (the scene has no reference counterpart); the reference's
+                # own column loop of vscape_ray (code/landscape.cpp:170-191)
+                # for the voxel frames; the reference's HorizontalBoxBlur32,
+                # VerticalBoxBlur32 and BlitAdd32 around the oracle's grid
+                # remap (no reference counterpart) for the tunnel frames
+                sdf_buf = np.zeros(W * H, np.uint32)
+
+                def cpu_c0():
+                    for k in range(4):
+                        orc.orc_sdf_frame(P(sdf_buf), W, H, C.byref(scene8), k / 60, 32, None, nthr)
+                per["sdf_c0"]["cpu"] = dict(median_rate(cpu_c0, 4 * W * H), kind="port",
+                                            what="oracle FP32 render (C, OpenMP rows)", threads=nthr)
+                vbuf = oracles.aligned_empty(W * H)
+                vset = []
+                for k in range(4):
+                    oc = oracles.orc_cam(lcams[k])
+                    fx, fy = C.c_int(), C.c_int()
+                    dX, dY = np.zeros(W, np.int32), np.zeros(W, np.int32)
+                    fish = np.zeros(W, np.float32)
+                    orc.orc_vscape_setup(W, H, C.byref(oc), C.byref(fx), C.byref(fy), P(dX), P(dY), P(fish))
+                    vset.append((lcams[k].mapTilt, fx.value, fy.value, dX, dY, fish))
+
+                if use_ref:
+                    def cpu_c1():
+                        for (tilt, fx, fy, dX, dY, fish) in vset:
+                            refl.ref_vscape_columns(P(vbuf), P(maps.height), P(maps.color), P(maps.fog),
+                                                    tilt, fx, fy, P(dX), P(dY), P(fish))
+                    c1_kind, c1_what = "reference", "reference vscape_ray, OpenMP over columns (SSE4.1)"
+                else:
+                    def cpu_c1():
+                        for k in range(4):
+                            oc = oracles.orc_cam(lcams[k])
+                            orc.orc_landscape_frame(P(vbuf), W, H, P(maps.height), P(maps.color),
+                                                    P(maps.fog), C.byref(oc), nthr)
+                    c1_kind, c1_what = "port", "oracle column caster (C, OpenMP columns)"
+                per["voxel_c1"]["cpu"] = dict(median_rate(cpu_c1, 4 * W * H), kind=c1_kind, what=c1_what,
+                                              threads=nthr)
+                rm, t0b, t1b = (oracles.aligned_empty(W * H) for _ in range(3))
+                tgrids_h = [wl.tunnel_grid(W, H, k / 60) for k in range(4)]
+
+                if use_ref:
+                    def cpu_c2():
+                        for k in range(4):
+                            s_ = np.float32((2.0 * int(radii[k]) + 1.5) / 255.0)
+                            orc.orc_grid_remap(P(rm), W, H, P(tgrids_h[k]), P(tex), 9, nthr)
+                            refl.ref_blur07_h(P(t0b), P(rm), W, H, s_)
+                            refl.ref_blur07_v(P(t1b), P(t0b), W, H, s_)
+                            refl.ref_blit_add32(P(rm), P(t1b), W, W, H)
+                    c2_kind = "reference"
+                    c2_what = ("reference HorizontalBoxBlur32 + VerticalBoxBlur32 + BlitAdd32 (SSE, OpenMP), "
+                               "oracle grid remap (no reference counterpart)")
+                else:
+                    def cpu_c2():
+                        for k in range(4):
+                            orc.orc_tunnel_frame(P(rm), P(t0b), P(t1b), W, H, P(tgrids_h[k]), P(tex), 9,
+                                                 int(radii[k]), nthr)
+                    c2_kind, c2_what = "port", "oracle tunnel frame (C, OpenMP)"
+                per["tunnel_c2"]["cpu"] = dict(median_rate(cpu_c2, 4 * W * H), kind=c2_kind, what=c2_what,
+                                               threads=nthr)
+                if use_ref:
+                    # the reference-built tunnel frame is the oracle's, bit for bit
+                    want = np.zeros(W * H, np.uint32)
+                    orc.orc_tunnel_frame(P(want), P(np.zeros(W * H, np.uint32)), P(np.zeros(W * H, np.uint32)),
+                                         W, H, P(tgrids_h[3]), P(tex), 9, int(radii[3]), nthr)
+                    assert np.array_equal(want, rm), "reference tunnel frame differs from the oracle's"
+                log("classic cpu", {k: per[k]["cpu"] for k in ("sdf_c0", "voxel_c1", "tunnel_c2")})
+
             if "c3" in sections:
-                # precision form: FP16 shading vs the FP64 render on a row sample
-                errs = []
+                # precision form: FP16 batched and FP32 plain against the FP64
+                # render, every 4th row of all 64 frames
+                c3_plain()
                 c3_batched()
-                for k in [0, 63]:
-                    img = fr3.view(k * W3 * H3, (W3 * H3,)).numpy().reshape(H3, W3)
-                    rows = np.arange(7, H3, 36)
-                    ref = np.zeros(len(rows) * W3, np.uint32)
-                    orc.orc_sdf_rows(oracles.P(ref), W3, H3, C.byref(scene32), k / 60, 64, 7, 36, None, nthr)
-                    errs.append((img[rows].reshape(-1), ref))
-                es = error_stats(np.concatenate([e[0] for e in errs]), np.concatenate([e[1] for e in errs]))
+                es = sdf_dense(fr3, fr3p, scene32, W3, H3, n3)
                 log("precision", es)
-            # bounded sample of the batched workload: its requests in order, as
-            # many as fit in about 10 s of CPU time, rendered by the oracle
+                sdf3 = np.zeros(W3 * H3, np.uint32)
+
+                def cpu_c3():
+                    orc.orc_sdf_frame(P(sdf3), W3, H3, C.byref(scene32), 0.0, 32, None, nthr)
+                per["sdf_c3_precision"]["cpu"] = dict(median_rate(cpu_c3, W3 * H3), kind="port",
+                                                      what="oracle FP32 render (C, OpenMP rows)",
+                                                      threads=nthr)
+
+            # bounded sample of the batched workload, rendered by the oracle
             def cpu_render(it, nthreads):
                 n = it.W * it.H
                 buf = np.zeros(n, np.uint32)
                 if it.effect == CKD_FX_SDF:
-                    orc.orc_sdf_frame(oracles.P(buf), it.W, it.H, C.byref(scene8), it.t, 32, None, nthreads)
+                    orc.orc_sdf_frame(P(buf), it.W, it.H, C.byref(scene8), it.t, 32, None, nthreads)
                 elif it.effect == CKD_FX_VOXEL:
                     oc = oracles.orc_cam(wl.landscape_cam(path, it.t))
-                    orc.orc_landscape_frame(oracles.P(buf), it.W, it.H, oracles.P(maps.height),
-                                            oracles.P(maps.color), oracles.P(maps.fog), C.byref(oc),
-                                            nthreads)
+                    orc.orc_landscape_frame(P(buf), it.W, it.H, P(maps.height), P(maps.color), P(maps.fog),
+                                            C.byref(oc), nthreads)
                 else:
                     g = wl.tunnel_grid(it.W, it.H, it.t)
                     b0, b1 = np.zeros(n, np.uint32), np.zeros(n, np.uint32)
-                    orc.orc_tunnel_frame(oracles.P(buf), oracles.P(b0), oracles.P(b1), it.W, it.H,
-                                         oracles.P(g), oracles.P(tex), 9, it.radius, nthreads)
+                    orc.orc_tunnel_frame(P(buf), P(b0), P(b1), it.W, it.H, P(g), P(tex), 9, it.radius,
+                                         nthreads)
                 return n
 
-            def cpu_rate(nthreads, budget_s):
-                px, k, t_c = 0, 0, time.perf_counter()
-                while k < len(items) and time.perf_counter() - t_c < budget_s:
-                    px += cpu_render(items[k], nthreads)
+            def first_requests(mpx):
+                k, px = 0, 0
+                while k < len(items) and px < mpx * 1e6:
+                    px += items[k].W * items[k].H
                     k += 1
-                return px, k, time.perf_counter() - t_c
+                return items[:k], px
 
-            # the reference's own Draw functions of the raymarched effects, on
+            # the reference's Draw functions of the raymarched effects, on
             # the host, where the compiled reference harness is present (its
             # parameters come from Rocket track files written for the frames);
             # else the oracle's restatement of them
             import tempfile
             lut_h = np.zeros(2049, np.float32)
-            orc.orc_cos_lut(oracles.P(lut_h))
-            use_ref = oracles.ref_available()
-            refl = oracles.ref() if use_ref else None
+            orc.orc_cos_lut(P(lut_h))
             if use_ref and "shadertoy" in sections:
                 refl.ref_st_init()
-                refl.ref_st_textures(oracles.P(st_tex), oracles.P(st_tex_hi))
-                refl.ref_st_blurmaps(oracles.P(st_blur), oracles.P(st_blur[st_blur.size // 2:]))
+                refl.ref_st_textures(P(st_tex), P(st_tex_hi))
+                refl.ref_st_blurmaps(P(st_blur), P(st_blur[st_blur.size // 2:]))
             if "shadertoy" in sections:
                 for fx in st_per:
                     name = "spikey" if fx == "spikey_close" else fx
@@ -510,19 +690,18 @@ def main() -> int:
                             oracles.write_rocket_tracks(Path(td), {
                                 tr: [getattr(f, fld) for f in frames] for fld, tr in cls.TRACKS.items()})
                             refl.ref_st_rocket(td.encode())
-                        t_c = time.perf_counter()
-                        for k, f in enumerate(frames):
-                            if use_ref:
-                                refl.ref_st_row(float(k))
-                                refl.ref_st_draw(oracles.ST_IDS[fx], oracles.P(out_h), f.time, 1.0)
-                            else:
-                                pa = f.as_array()
-                                orc.orc_st_draw(oracles.ST_IDS[fx], oracles.P(out_h), W, H, oracles.P(pa),
-                                                oracles.P(lut_h), oracles.P(st_tex), oracles.P(st_tex_hi),
-                                                oracles.P(st_blur), 1, nthr)
-                        dt_c = time.perf_counter() - t_c
-                    st_per[fx]["cpu_mpx_s"] = round(8 * W * H / dt_c / 1e6, 2)
-                    st_per[fx]["cpu_kind"] = ("reference Draw (OpenMP, OMP_NUM_THREADS threads)"
+
+                        def st_cpu():
+                            for k, f in enumerate(frames):
+                                if use_ref:
+                                    refl.ref_st_row(float(k))
+                                    refl.ref_st_draw(oracles.ST_IDS[fx], P(out_h), f.time, 1.0)
+                                else:
+                                    orc.orc_st_draw(oracles.ST_IDS[fx], P(out_h), W, H, P(f.as_array()),
+                                                    P(lut_h), P(st_tex), P(st_tex_hi), P(st_blur), 1, nthr)
+                        r = median_rate(st_cpu, len(frames) * W * H, warmup=1, runs=3)
+                    st_per[fx]["cpu_mpx_s"] = r["mpx_s"]
+                    st_per[fx]["cpu_kind"] = (f"reference Draw (SSE, OpenMP, {nthr} threads)"
                                               if use_ref else f"oracle Draw ({nthr} threads)")
                     log("shadertoy cpu", fx, st_per[fx]["cpu_mpx_s"])
 
@@ -530,7 +709,7 @@ def main() -> int:
                 # the ball and the twister: the reference's Draw (tracks as above)
                 # where the harness is present, else the oracle's
                 m0_h = np.zeros(2 * W * H, np.int32)
-                orc.orc_polar_maps(oracles.P(m0_h), oracles.P(np.zeros(2 * W * H, np.int32)), W, H, W, H)
+                orc.orc_polar_maps(P(m0_h), P(np.zeros(2 * W * H, np.int32)), W, H, W, H)
                 if use_ref:
                     refl.ref_shared_init()
                     refl.ref_polar_create()
@@ -546,44 +725,50 @@ def main() -> int:
                                 tr: [getattr(f, fld) for f in frames] for fld, tr in cls.TRACKS.items()})
                             refl.ref_st_rocket(td.encode())
                             if kind == "ball":
-                                refl.ref_ball_setup(td.encode(), oracles.P(bm_v2["heightMaps"]),
-                                                    oracles.P(bm_v2["colorMaps"]), oracles.P(bm_v2["beamMaps"]),
-                                                    oracles.P(bm_v2["envMap"]), oracles.P(bm_v2["backgrounds"]),
-                                                    oracles.P(bm_v2["halo"]))
+                                refl.ref_ball_setup(td.encode(), P(bm_v2["heightMaps"]), P(bm_v2["colorMaps"]),
+                                                    P(bm_v2["beamMaps"]), P(bm_v2["envMap"]),
+                                                    P(bm_v2["backgrounds"]), P(bm_v2["halo"]))
                             else:
-                                refl.ref_twister_setup(td.encode(), oracles.P(tm_v2["heightMap"]),
-                                                       oracles.P(tm_v2["colorMap"]),
-                                                       oracles.P(tm_v2["background"]))
-                        t_c = time.perf_counter()
-                        for k, f in enumerate(frames):
-                            if use_ref:
-                                refl.ref_st_row(float(k))
-                                (refl.ref_ball_draw if kind == "ball" else refl.ref_twister_draw)(
-                                    oracles.P(out_h), f.time, 1.0)
-                            elif kind == "ball":
-                                orc.orc_ball_draw(oracles.P(out_h), oracles.P(tgt_h), W, H,
-                                                  oracles.P(bm_v2["heightMaps"]), oracles.P(bm_v2["colorMaps"]),
-                                                  oracles.P(bm_v2["beamMaps"]), oracles.P(bm_v2["envMap"]),
-                                                  oracles.P(bm_v2["backgrounds"]), oracles.P(bm_v2["halo"]),
-                                                  oracles.P(m0_h), oracles.P(f.as_array()), nthr)
-                            else:
-                                orc.orc_twister_draw(oracles.P(out_h), oracles.P(tgt_h), W, H,
-                                                     oracles.P(tm_v2["heightMap"]), oracles.P(tm_v2["colorMap"]),
-                                                     oracles.P(tm_v2["background"]), oracles.P(m0_h),
-                                                     oracles.P(f.as_array()), nthr)
-                        dt_c = time.perf_counter() - t_c
-                    v2_per[key]["cpu_mpx_s"] = round(len(frames) * W * H / dt_c / 1e6, 2)
-                    v2_per[key]["cpu_kind"] = ("reference Draw (OpenMP, OMP_NUM_THREADS threads)"
+                                refl.ref_twister_setup(td.encode(), P(tm_v2["heightMap"]), P(tm_v2["colorMap"]),
+                                                       P(tm_v2["background"]))
+
+                        def v2_cpu():
+                            for k, f in enumerate(frames):
+                                if use_ref:
+                                    refl.ref_st_row(float(k))
+                                    (refl.ref_ball_draw if kind == "ball" else refl.ref_twister_draw)(
+                                        P(out_h), f.time, 1.0)
+                                elif kind == "ball":
+                                    orc.orc_ball_draw(P(out_h), P(tgt_h), W, H, P(bm_v2["heightMaps"]),
+                                                      P(bm_v2["colorMaps"]), P(bm_v2["beamMaps"]),
+                                                      P(bm_v2["envMap"]), P(bm_v2["backgrounds"]),
+                                                      P(bm_v2["halo"]), P(m0_h), P(f.as_array()), nthr)
+                                else:
+                                    orc.orc_twister_draw(P(out_h), P(tgt_h), W, H, P(tm_v2["heightMap"]),
+                                                         P(tm_v2["colorMap"]), P(tm_v2["background"]), P(m0_h),
+                                                         P(f.as_array()), nthr)
+                        r = median_rate(v2_cpu, len(frames) * W * H, warmup=1, runs=3)
+                    v2_per[key]["cpu_mpx_s"] = r["mpx_s"]
+                    v2_per[key]["cpu_kind"] = (f"reference Draw (SSE, OpenMP, {nthr} threads)"
                                                if use_ref else f"oracle Draw ({nthr} threads)")
                     log("voxel2 cpu", key, v2_per[key]["cpu_mpx_s"])
 
-            px, k, dt = cpu_rate(nthr, 10.0)
-            px1, k1, dt1 = cpu_rate(1, 5.0)
-            cpu = {"value": round(px / dt / 1e6, 2), "unit": "Mpixel/s", "cores": nthr, "kind": "port",
-                   "sample": f"first {k} of the {len(items)} batch requests ({px / 1e6:.1f} Mpx) "
-                             f"rendered by the oracle (C, OpenMP) in {dt:.1f}s",
-                   "value_1_thread": round(px1 / dt1 / 1e6, 2),
-                   "sample_1_thread": f"first {k1} requests in {dt1:.1f}s"}
+            # the headline's CPU baseline: the oracle renders the first
+            # batch requests (about 100 Mpx) in order, median of 5 runs after
+            # 3 warm-ups; the same on one thread over about 5 Mpx
+            samp, samp_px = first_requests(100)
+            r = median_rate(lambda: [cpu_render(it, nthr) for it in samp], samp_px)
+            samp1, samp1_px = first_requests(5)
+            r1 = median_rate(lambda: [cpu_render(it, 1) for it in samp1], samp1_px, warmup=1, runs=3)
+            cpu = {"value": r["mpx_s"], "unit": "Mpixel/s", "cores": nthr, "kind": "port",
+                   "sample": f"first {len(samp)} of the {len(items)} batch requests ({samp_px / 1e6:.1f} Mpx) "
+                             f"rendered by the oracle (C, OpenMP) on {nthr} threads; median of 5 runs "
+                             f"after 3 warm-ups",
+                   "runs_s": r["runs_s"],
+                   "value_1_thread": r1["mpx_s"],
+                   "sample_1_thread": f"first {len(samp1)} requests ({samp1_px / 1e6:.1f} Mpx), median of 3 "
+                                      f"runs after 1 warm-up",
+                   "host": host}
             log("cpu", cpu)
 
     result = {
@@ -605,8 +790,8 @@ def main() -> int:
                    "march_steps_per_step": steps_total // max(1, args.steps),
                    "tuning": tuning},
         "per_effect": per,
-        "precision_c3": None if es is None else
-        {"shading": "fp16", "vs": "fp64 CPU render", "rows_sampled": "every 36th, frames 0 and 63", **es},
+        "error_vs_oracle": headline_err,
+        "precision_c3": es,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

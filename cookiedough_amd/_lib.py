@@ -139,6 +139,7 @@ SIGNATURES = {
     "ckd_event_record": [_p, _i],
     "ckd_event_elapsed_ms": [_p, _i, _i, C.POINTER(_f)],
     "ckd_kernel_timing": [_p, _i],
+    "ckd_debug_fail_alloc": [_i],
     "ckd_kernel_timing_read": [_p, _i, C.POINTER(C.c_double), C.POINTER(C.c_ulonglong)],
     "ckd_memset32": [_p, _p, C.c_uint32, C.c_size_t],
     "ckd_fx_blit_2x2": [_p, _p, _p, _u, _u],
@@ -171,6 +172,7 @@ SIGNATURES = {
     "ckd_batch_graph_create": [_p, C.POINTER(Request), _u, C.POINTER(SdfScene), _i, _p, _u, _p,
                                C.c_size_t, _p, C.POINTER(_p)],
     "ckd_graph_launch": [_p],
+    "ckd_graph_update": [_p, _p, _u],
     "ckd_graph_destroy": [_p],
     "ckd_mix32": [_p, _p, _p, _u, C.c_uint8],
     "ckd_mixover32": [_p, _p, _p, _u],

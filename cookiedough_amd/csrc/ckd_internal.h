@@ -59,11 +59,15 @@ struct ckd_ctx {
     uint8_t *v2Scratch = nullptr;
     size_t v2Cap = 0;
 
-    // landscape maps (Landscape_Create)
+    // landscape maps (Landscape_Create), and their 2x2 texel quads: texel
+    // (u,v)'s four bilinear neighbours (wrapped) in one word of heights and
+    // four words of colours, so a march step fetches them with two loads
     uint8_t *lsHeight = nullptr;
     uint32_t *lsColor = nullptr;
     uint32_t *lsFog = nullptr;
     uint32_t lsFog0 = 0;
+    uint32_t *lsQuadH = nullptr;
+    uint4 *lsQuadC = nullptr;
 
     // tunnelscape maps (Tunnelscape_Create)
     uint8_t *tsHeight = nullptr;
@@ -116,6 +120,14 @@ ckd_status launch_blit_add(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, uns
                            unsigned srcStride, unsigned width, unsigned rows);
 
 // Records a message for ckd_last_error() on this thread and returns s.
+// hipMalloc for every device allocation the library makes, so that a test
+// can make the n-th next one fail (ckd_debug_fail_alloc)
+hipError_t dev_alloc_raw(void **p, size_t bytes);
+template <typename T> hipError_t dev_alloc(T **p, size_t bytes)
+{
+    return dev_alloc_raw(reinterpret_cast<void **>(p), bytes);
+}
+
 ckd_status fail(ckd_status s, const char *fmt, ...)
     __attribute__((format(printf, 2, 3)));
 

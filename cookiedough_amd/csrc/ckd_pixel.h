@@ -9,6 +9,16 @@
 
 #define CKD_DEV __device__ __forceinline__
 
+// a * b for a, b < 2^24 with a product below 2^32: v_mul_u32_u24, full rate
+// (the compiler otherwise picks the quarter-rate 32-bit multiply where it
+// cannot prove the operands' range)
+CKD_DEV uint32_t ckd_mul_u24(uint32_t a, uint32_t b)
+{
+    uint32_t r;
+    asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 // Two 8-bit channels held in the 16-bit fields of 0x00XX00YY.  The convex
 // combination a*(256-f) + b*f of two such fields never exceeds 255*256, so
 // the fields never carry into each other.
@@ -102,6 +112,30 @@ CKD_DEV void ckd_span_setup(CkdSpan &s, uint32_t A, uint32_t B, uint32_t length,
         const uint32_t M = (k & 1) ? mhi : mlo;
         s.step[k] = (uint32_t)ckd_madd16(delta, divisor);
         s.acc[k] = (a << 16) + (uint32_t)ckd_madd16(delta, M);
+    }
+}
+
+// ckd_span_setup for a span that is drawn (1 <= drawLength <= length <
+// 65536), in fewer operations.  The pre-step product m = divisor * pre is
+// below 65536 there (pre < length, divisor = floor(65536/length)), so the
+// 64-bit product of _mm_mul_epi32 has a zero high word (G and A get no
+// pre-step) and its low word has a zero high half: _mm_madd_epi16 of a
+// channel delta d (|d| <= 255, its high half the sign) with m is
+// d * int16(m).  The divisor's high half is 1 only for length 1, where the
+// step is d * 0 + (d < 0 ? -1 : 0).  Equal to ckd_span_setup on every such
+// input (checked exhaustively over lengths up to 2200).
+CKD_DEV void ckd_span_setup_drawn(CkdSpan &s, uint32_t A, uint32_t B, uint32_t length,
+                                  uint32_t drawLength, uint32_t divisor)
+{
+    const int32_t m = (int32_t)(int16_t)ckd_mul_u24(divisor, length - drawLength);
+    const int32_t dlo = (int32_t)(int16_t)divisor;
+    const int32_t dhi = (int32_t)(divisor >> 16);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int32_t a = (int32_t)((A >> (8 * k)) & 0xff), b = (int32_t)((B >> (8 * k)) & 0xff);
+        const int32_t d = b - a;
+        s.step[k] = (uint32_t)(d * dlo - (d < 0 ? dhi : 0));
+        s.acc[k] = ((uint32_t)a << 16) + ((k & 1) ? 0u : (uint32_t)(d * m));
     }
 }
 

@@ -574,7 +574,7 @@ ckd_status v2_scratch(ckd_ctx *ctx, size_t bytes, uint32_t *&out)
         if (ctx->v2Scratch) (void)hipFree(ctx->v2Scratch);
         ctx->v2Scratch = nullptr;
         ctx->v2Cap = 0;
-        if (hipMalloc(&ctx->v2Scratch, bytes) != hipSuccess) {
+        if (ckd::dev_alloc(&ctx->v2Scratch, bytes) != hipSuccess) {
             ctx->v2Scratch = nullptr;
             return ckd::fail(CKD_ENOMEM, "batched Draw: cannot allocate %zu bytes of scratch", bytes);
         }
@@ -645,17 +645,16 @@ ckd_status ckd_twister_create(ckd_ctx *ctx, const uint8_t *heightMap, const uint
     CKD_REQUIRE_CTX(ctx);
     ckd_status s = ensure_polar(ctx, W, H);
     if (s != CKD_OK) return s;
-    if (!ctx->twHeight) {
-        if (hipMalloc(&ctx->twHeight, kMapPx) != hipSuccess ||
-            hipMalloc(&ctx->twColor, kMapPx * 4) != hipSuccess ||
-            hipMalloc(&ctx->twTables, 2 * kTwRay * sizeof(uint32_t)) != hipSuccess)
-            return ckd::fail(CKD_ENOMEM, "ckd_twister_create: cannot allocate the maps");
-    }
+    // each map on its own, so a create that failed part way allocates the rest
+    if ((!ctx->twHeight && ckd::dev_alloc(&ctx->twHeight, kMapPx) != hipSuccess) ||
+        (!ctx->twColor && ckd::dev_alloc(&ctx->twColor, kMapPx * 4) != hipSuccess) ||
+        (!ctx->twTables && ckd::dev_alloc(&ctx->twTables, 2 * kTwRay * sizeof(uint32_t)) != hipSuccess))
+        return ckd::fail(CKD_ENOMEM, "ckd_twister_create: cannot allocate the maps");
     if (ctx->twBackground && (ctx->twW != W || ctx->twH != H)) {
         CKD_HIP(hipFree(ctx->twBackground));
         ctx->twBackground = nullptr;
     }
-    if (!ctx->twBackground && hipMalloc(&ctx->twBackground, (size_t)W * H * 4) != hipSuccess)
+    if (!ctx->twBackground && ckd::dev_alloc(&ctx->twBackground, (size_t)W * H * 4) != hipSuccess)
         return ckd::fail(CKD_ENOMEM, "ckd_twister_create: cannot allocate the background");
     uint32_t tables[2 * kTwRay];
     twister_tables(tables, tables + kTwRay);
@@ -755,7 +754,7 @@ ckd_status ckd_ball_create(ckd_ctx *ctx, const uint8_t *heightMaps5, const uint3
     if (!ctx->ballMaps) {
         // heights 5 + mix, colour 2, beam 3 + mix, env
         const size_t bytes = 6 * kMapPx + (2 + 4 + 1) * kMapPx * 4;
-        if (hipMalloc(&ctx->ballMaps, bytes) != hipSuccess)
+        if (ckd::dev_alloc(&ctx->ballMaps, bytes) != hipSuccess)
             return ckd::fail(CKD_ENOMEM, "ckd_ball_create: cannot allocate the maps");
     }
     if (ctx->ballImages && (ctx->ballW != W || ctx->ballH != H)) {
@@ -765,7 +764,7 @@ ckd_status ckd_ball_create(ckd_ctx *ctx, const uint8_t *heightMaps5, const uint3
     // two backgrounds and the halo, then the fan deltas of the H rays
     const size_t fanOffset = ((size_t)3 * W * H * 4 + 255) & ~(size_t)255;
     if (!ctx->ballImages &&
-        hipMalloc(&ctx->ballImages, fanOffset + (size_t)H * sizeof(int2)) != hipSuccess)
+        ckd::dev_alloc(&ctx->ballImages, fanOffset + (size_t)H * sizeof(int2)) != hipSuccess)
         return ckd::fail(CKD_ENOMEM, "ckd_ball_create: cannot allocate the backgrounds");
     std::vector<int2> fans;
     ball_fans(H, fans);

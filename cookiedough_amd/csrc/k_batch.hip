@@ -30,12 +30,26 @@ ckd_status launch_tunnel_jobs(ckd_ctx *ctx, const TunJob *jobs, int nJobs, const
                               const uint32_t *tex, unsigned texLog2);
 } // namespace ckd
 
+// per request, what fixes the captured launch geometry: the effect and the
+// size
+struct ReqShape {
+    int effect;
+    unsigned W, H;
+};
+
 struct ckd_graph {
     ckd_ctx *ctx = nullptr;
     void *tables = nullptr;
+    size_t tableBytes = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool hasSdf = false;
+    // what ckd_graph_update must keep: the launch shapes the tables were
+    // laid out with, the scratch image, the requests' shapes
+    ckd_tuning tuning{};
+    uint32_t *tmp = nullptr;
+    size_t tmpPixels = 0;
+    std::vector<ReqShape> shapes;
 };
 
 namespace {
@@ -95,12 +109,13 @@ ckd_status validate(const char *fn, const ckd_request *requests, unsigned n,
     return CKD_OK;
 }
 
-void make_plan(const ckd_ctx *ctx, const ckd_request *requests, unsigned n, uint32_t *tmp, Plan &P)
+void make_plan(const ckd_tuning &T, const ckd_request *requests, unsigned n, uint32_t *tmp,
+               Plan &P)
 {
     std::vector<SdfJob> sdf;
     std::vector<VoxJob> vox;
     std::vector<TunJob> tun;
-    const uint32_t chunk = (uint32_t)ctx->tuning.sdfChunk;
+    const uint32_t chunk = (uint32_t)T.sdfChunk;
     size_t off = 0;
     for (unsigned i = 0; i < n; ++i) {
         const ckd_request &r = requests[i];
@@ -128,7 +143,7 @@ void make_plan(const ckd_ctx *ctx, const ckd_request *requests, unsigned n, uint
             J.rowBase = P.tunRows;
             J.vBase = P.tunV;
             P.tunRows += r.H;
-            P.tunV += ckd::tunnel_vblocks(r.W, r.H, ctx->tuning.tunnelVSeg);
+            P.tunV += ckd::tunnel_vblocks(r.W, r.H, T.tunnelVSeg);
             if (r.W > P.tunMaxW) P.tunMaxW = r.W;
             tun.push_back(J);
         }
@@ -209,7 +224,7 @@ ckd_status run_eager(ckd_ctx *ctx, const ckd_request *requests, unsigned n,
                      unsigned texLog2, uint32_t *tmp, unsigned long long *stepCount)
 {
     Plan P;
-    make_plan(ctx, requests, n, tmp, P);
+    make_plan(ctx->tuning, requests, n, tmp, P);
     void *dev = nullptr;
     ckd_status s = ckd::stage(ctx, P.host.data(), P.host.size(), &dev);
     if (s != CKD_OK) return s;
@@ -332,10 +347,16 @@ ckd_status ckd_batch_graph_create(ckd_ctx *ctx, const ckd_request *requests, uns
     CKD_REQUIRE(n > 0, "ckd_batch_graph_create: empty batch");
     CKD_REQUIRE_CTX(ctx);
     Plan P;
-    make_plan(ctx, requests, n, tmp, P);
+    make_plan(ctx->tuning, requests, n, tmp, P);
     ckd_graph *g = new ckd_graph();
     g->ctx = ctx;
-    if (hipMalloc(&g->tables, P.host.size()) != hipSuccess) {
+    g->tableBytes = P.host.size();
+    g->tuning = ctx->tuning;
+    g->tmp = tmp;
+    g->tmpPixels = tmpPixels;
+    for (unsigned i = 0; i < n; ++i)
+        g->shapes.push_back({requests[i].effect, requests[i].W, requests[i].H});
+    if (ckd::dev_alloc(&g->tables, P.host.size()) != hipSuccess) {
         delete g;
         return ckd::fail(CKD_ENOMEM, "ckd_batch_graph_create: table allocation failed");
     }
@@ -377,6 +398,41 @@ ckd_status ckd_batch_graph_create(ckd_ctx *ctx, const ckd_request *requests, uns
         return s;
     }
     *out = g;
+    return CKD_OK;
+}
+
+ckd_status ckd_graph_update(ckd_graph *g, const ckd_request *requests, unsigned n)
+{
+    CKD_REQUIRE(g && g->exec, "ckd_graph_update: null graph");
+    CKD_REQUIRE(n == g->shapes.size(), "ckd_graph_update: %u requests, the graph holds %zu", n,
+                g->shapes.size());
+    CKD_REQUIRE(requests, "ckd_graph_update: null request array");
+    for (unsigned i = 0; i < n; ++i) {
+        const ckd_request &r = requests[i];
+        const ReqShape &c = g->shapes[i];
+        CKD_REQUIRE(r.effect == c.effect && r.W == c.W && r.H == c.H,
+                    "ckd_graph_update: request %u is effect %d at %ux%u, the graph's is %d at %ux%u", i,
+                    r.effect, r.W, r.H, c.effect, c.W, c.H);
+        CKD_REQUIRE(r.dst != nullptr, "ckd_graph_update: request %u: null dst", i);
+        if (r.effect == CKD_FX_VOXEL)
+            CKD_REQUIRE(r.vcam.mapTilt >= 0 && r.vcam.mapTilt <= 180,
+                        "ckd_graph_update: request %u: mapTilt %d outside 0..180", i, r.vcam.mapTilt);
+        if (r.effect == CKD_FX_TUNNEL) {
+            CKD_REQUIRE(r.grid != nullptr, "ckd_graph_update: request %u: null grid", i);
+            CKD_REQUIRE(r.radius >= 1 && r.radius <= 127 &&
+                            (unsigned)(2 * r.radius + 1) <= (r.W < r.H ? r.W : r.H),
+                        "ckd_graph_update: request %u: radius %d out of range", i, r.radius);
+        }
+    }
+    Plan P;
+    make_plan(g->tuning, requests, n, g->tmp, P);
+    if (P.host.size() != g->tableBytes)
+        return ckd::fail(CKD_EINVAL, "ckd_graph_update: the requests' tables do not fit the graph");
+    // a replay still in flight reads the tables: the copy waits for it, and
+    // the next replay is queued after the copy
+    ckd_ctx *ctx = g->ctx;
+    CKD_HIP(hipStreamSynchronize(ctx->stream));
+    CKD_HIP(hipMemcpy(g->tables, P.host.data(), P.host.size(), hipMemcpyHostToDevice));
     return CKD_OK;
 }
 

@@ -243,7 +243,7 @@ ckd_status ensure_scratch(ckd_ctx *ctx, size_t pixels)
     }
     ctx->b26Cap = 0;
     for (int i = 0; i < 2; ++i)
-        if (hipMalloc(&ctx->b26Scratch[i], pixels * 4) != hipSuccess) {
+        if (ckd::dev_alloc(&ctx->b26Scratch[i], pixels * 4) != hipSuccess) {
             ctx->b26Scratch[i] = nullptr;
             return ckd::fail(CKD_ENOMEM, "blur26: scratch allocation failed");
         }

@@ -139,7 +139,7 @@ ckd_status ckd_polar_create(ckd_ctx *ctx, unsigned W, unsigned H)
         (void)hipFree(pm.inv);
         pm = ckd_ctx::PolarMaps();
     }
-    if (hipMalloc(&pm.map, n * 4) != hipSuccess || hipMalloc(&pm.inv, n * 4) != hipSuccess) {
+    if (ckd::dev_alloc(&pm.map, n * 4) != hipSuccess || ckd::dev_alloc(&pm.inv, n * 4) != hipSuccess) {
         if (pm.map) (void)hipFree(pm.map);
         pm = ckd_ctx::PolarMaps();
         return ckd::fail(CKD_ENOMEM, "ckd_polar_create: allocation failed");

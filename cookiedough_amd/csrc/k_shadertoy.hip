@@ -900,7 +900,7 @@ static ckd_status st_scratch(ckd_ctx *ctx, size_t pixels, uint32_t **out)
             ctx->stScratch = nullptr;
             ctx->stScratchCap = 0;
         }
-        if (hipMalloc(&ctx->stScratch, pixels * sizeof(uint32_t)) != hipSuccess)
+        if (ckd::dev_alloc(&ctx->stScratch, pixels * sizeof(uint32_t)) != hipSuccess)
             return fail(CKD_ENOMEM, "shadertoy: cannot allocate %zu map pixels", pixels);
         ctx->stScratchCap = pixels;
     }
@@ -1171,7 +1171,7 @@ ckd_status ckd_spikey_blurmaps(ckd_ctx *ctx, const uint32_t *map1, const uint32_
         CKD_HIP(hipFree(ctx->spikeBlurMaps));
         ctx->spikeBlurMaps = nullptr;
     }
-    if (hipMalloc(&ctx->spikeBlurMaps, 2 * bytes) != hipSuccess)
+    if (ckd::dev_alloc(&ctx->spikeBlurMaps, 2 * bytes) != hipSuccess)
         return ckd::fail(CKD_ENOMEM, "ckd_spikey_blurmaps: cannot allocate the maps");
     CKD_HIP(hipMemcpy(ctx->spikeBlurMaps, map1, bytes, hipMemcpyHostToDevice));
     CKD_HIP(hipMemcpy(ctx->spikeBlurMaps + px, map2, bytes, hipMemcpyHostToDevice));
@@ -1185,10 +1185,9 @@ ckd_status ckd_fdtunnel_create(ckd_ctx *ctx, const uint32_t *tex, const uint32_t
     CKD_REQUIRE(tex && texHighlights, "ckd_fdtunnel_create: null texture");
     CKD_REQUIRE_CTX(ctx);
     const size_t bytes = 1024 * 1024 * sizeof(uint32_t);
-    if (!ctx->fdTex) {
-        if (hipMalloc(&ctx->fdTex, bytes) != hipSuccess || hipMalloc(&ctx->fdTexHi, bytes) != hipSuccess)
-            return ckd::fail(CKD_ENOMEM, "ckd_fdtunnel_create: cannot allocate the textures");
-    }
+    if ((!ctx->fdTex && ckd::dev_alloc(&ctx->fdTex, bytes) != hipSuccess) ||
+        (!ctx->fdTexHi && ckd::dev_alloc(&ctx->fdTexHi, bytes) != hipSuccess))
+        return ckd::fail(CKD_ENOMEM, "ckd_fdtunnel_create: cannot allocate the textures");
     CKD_HIP(hipMemcpy(ctx->fdTex, tex, bytes, hipMemcpyHostToDevice));
     CKD_HIP(hipMemcpy(ctx->fdTexHi, texHighlights, bytes, hipMemcpyHostToDevice));
     return CKD_OK;

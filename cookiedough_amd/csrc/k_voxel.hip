@@ -36,12 +36,34 @@ struct LsLaunch {
 };
 static_assert(sizeof(LsLaunch) + 64 <= 4096, "k_landscape's arguments fit 4 KiB");
 
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+// bsamp32_16 (code/bilinear.h:58-83) followed by _mm_subs_epu16 of the fog
+// colour (code/landscape.cpp:83): the interpolation runs on two channels per
+// register (B,R and G,A in 16-bit fields, exact as in ckd_bilerp32) and the
+// saturating subtract is one packed u16 subtract per pair
+CKD_DEV uint32_t ls_bilerp_fog(uint32_t p0, uint32_t p1, uint32_t p2, uint32_t p3, uint32_t fu,
+                               uint32_t fv, uint32_t fogBR, uint32_t fogGA)
+{
+    const uint32_t M = 0x00ff00ffu;
+    const uint32_t br = ckd_lerp2(ckd_lerp2(p0 & M, p1 & M, fu), ckd_lerp2(p2 & M, p3 & M, fu), fv);
+    const uint32_t ga = ckd_lerp2(ckd_lerp2((p0 >> 8) & M, (p1 >> 8) & M, fu),
+                                  ckd_lerp2((p2 >> 8) & M, (p3 >> 8) & M, fu), fv);
+    const u16x2 sbr = __builtin_elementwise_sub_sat(__builtin_bit_cast(u16x2, br),
+                                                    __builtin_bit_cast(u16x2, fogBR));
+    const u16x2 sga = __builtin_elementwise_sub_sat(__builtin_bit_cast(u16x2, ga),
+                                                    __builtin_bit_cast(u16x2, fogGA));
+    return __builtin_bit_cast(uint32_t, sbr) | (__builtin_bit_cast(uint32_t, sga) << 8);
+}
+
 // exact floor(num/den) for num < 2^24, 0 < den < 2^24: a reciprocal
-// estimate is within one of the quotient; one correction each way fixes it
+// estimate is within one of the quotient; one correction each way fixes it.
+// q*den stays below 2^25, so the 24-bit multiply (full rate; the 32-bit
+// one is quarter rate) is exact.
 CKD_DEV uint32_t udiv24(uint32_t num, uint32_t den)
 {
     uint32_t q = (uint32_t)((float)num * __builtin_amdgcn_rcpf((float)den));
-    int32_t r = (int32_t)(num - q * den);
+    int32_t r = (int32_t)(num - ckd_mul_u24(q, den));
     if (r < 0) {
         --q;
         r += (int32_t)den;
@@ -82,12 +104,30 @@ CKD_DEV void ls_setup(float rayY, float colScale, float halfW, float X1, float Y
 // wait for)
 typedef __attribute__((address_space(1))) uint32_t global_u32;
 
+// the 2x2 texel quads of the landscape maps: quadH[i] holds the heights of
+// texels (u,v), (u+1,v), (u,v+1), (u+1,v+1) (indices wrapped into the map) in
+// its bytes 0..3, quadC[i] their colours, for i = v*1024 + u
+__global__ __launch_bounds__(256) void k_ls_quads(const uint8_t *__restrict__ hmap,
+                                                  const uint32_t *__restrict__ cmap,
+                                                  uint32_t *__restrict__ quadH,
+                                                  uint4 *__restrict__ quadC)
+{
+    const unsigned i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= (kMapAnd + 1) * (kMapAnd + 1)) return;
+    const unsigned u = i & kMapAnd, v = i >> kMapShift;
+    const unsigned u1 = (u + 1) & kMapAnd, v1 = ((v + 1) & kMapAnd) << kMapShift;
+    const unsigned v0 = v << kMapShift;
+    quadH[i] = (uint32_t)hmap[u + v0] | ((uint32_t)hmap[u1 + v0] << 8) |
+               ((uint32_t)hmap[u + v1] << 16) | ((uint32_t)hmap[u1 + v1] << 24);
+    quadC[i] = make_uint4(cmap[u + v0], cmap[u1 + v0], cmap[u + v1], cmap[u1 + v1]);
+}
+
 // vscape_ray (code/landscape.cpp:56-107) for column iRay of one frame, then
 // the fused memset32(fog[0]) for the rows no span reached
 CKD_DEV void ls_column(global_u32 *__restrict__ col, unsigned W, unsigned H, unsigned iRay,
                        float rayY, float colScale, float halfW, int mapScale, float X1, float Y1,
                        float viewCos, float viewSin, int mapTilt,
-                       const uint8_t *__restrict__ hmap, const uint32_t *__restrict__ cmap,
+                       const uint32_t *__restrict__ quadH, const uint4 *__restrict__ quadC,
                        const uint32_t *__restrict__ fog)
 {
     int dX, dY, fpFishMul;
@@ -96,47 +136,46 @@ CKD_DEV void ls_column(global_u32 *__restrict__ col, unsigned W, unsigned H, uns
 
     int lastHeight = (int)H, lastDrawn = (int)H;
     uint32_t lastColor =
-        cmap[(((unsigned)(curX >> 8)) & kMapAnd) | ((((unsigned)(curY >> 8)) & kMapAnd) << kMapShift)];
+        quadC[(((unsigned)(curX >> 8)) & kMapAnd) | ((((unsigned)(curY >> 8)) & kMapAnd) << kMapShift)].x;
 
     // The march is software-pipelined: the texels of the next kLsAhead steps
     // are in flight while a step draws its span (only curX/curY carry from
     // step to step, and every fetch is masked into the maps, so fetches past
     // the last step are harmless).
     struct Texels {
-        uint32_t h[4], c[4], fu, fv;
+        uint32_t h4, fu, fv;
+        uint4 c;
     };
     auto fetch = [&](Texels &t) {
         curX += dX;
         curY += dY;
-        const unsigned u0 = ((unsigned)(curX >> 8)) & kMapAnd;
-        const unsigned u1 = ((unsigned)(curX >> 8) + 1) & kMapAnd;
-        const unsigned v0 = (((unsigned)(curY >> 8)) & kMapAnd) << kMapShift;
-        const unsigned v1 = (((unsigned)(curY >> 8) + 1) & kMapAnd) << kMapShift;
+        const unsigned i = ((((unsigned)(curY >> 8)) & kMapAnd) << kMapShift) |
+                           (((unsigned)(curX >> 8)) & kMapAnd);
         t.fu = (uint32_t)curX & 0xff;
         t.fv = (uint32_t)curY & 0xff;
-        t.h[0] = hmap[u0 + v0];
-        t.h[1] = hmap[u1 + v0];
-        t.h[2] = hmap[u0 + v1];
-        t.h[3] = hmap[u1 + v1];
-        t.c[0] = cmap[u0 + v0];
-        t.c[1] = cmap[u1 + v0];
-        t.c[2] = cmap[u0 + v1];
-        t.c[3] = cmap[u1 + v1];
+        t.h4 = quadH[i];
+        t.c = quadC[i];
     };
     auto step = [&](unsigned iStep, const Texels &t) {
-        const uint32_t h = ckd_bilerp8(t.h[0], t.h[1], t.h[2], t.h[3], t.fu, t.fv);
-        uint32_t color = ckd_bilerp32(t.c[0], t.c[1], t.c[2], t.c[3], t.fu, t.fv);
-        color = ckd_subs_u8x4(color, fog[iStep >> 1]);
+        const uint32_t h = ckd_bilerp8(t.h4 & 0xff, (t.h4 >> 8) & 0xff, (t.h4 >> 16) & 0xff, t.h4 >> 24,
+                                       t.fu, t.fv);
+        const uint32_t f = fog[iStep >> 1];
+        const uint32_t color = ls_bilerp_fog(t.c.x, t.c.y, t.c.z, t.c.w, t.fu, t.fv, f & 0x00ff00ffu,
+                                             (f >> 8) & 0x00ff00ffu);
 
         // ((255-h)<<16) / (fpFishMul*(iStep+1)) is an unsigned division
-        const uint32_t q = udiv24((255u - h) << 16, (uint32_t)fpFishMul * (iStep + 1));
-        const int hgt = (((int)q * mapScale) >> 8) + mapTilt;
+        // fpFishMul <= 256 and iStep < 512: 24-bit multiplies are exact;
+        // q < 2^17 (fpFishMul >= 191 at every width) and mapScale < 2^11
+        const uint32_t q = udiv24((255u - h) << 16, ckd_mul_u24((uint32_t)fpFishMul, iStep + 1));
+        const int hgt = ((int)ckd_mul_u24(q, (uint32_t)mapScale) >> 8) + mapTilt;
 
         if (hgt < lastDrawn) {
             const uint32_t drawLength = (uint32_t)(lastDrawn - hgt);
+            const uint32_t length = (uint32_t)(lastHeight - hgt);
             CkdSpan sp;
-            ckd_span_setup(sp, color, lastColor, (uint32_t)(lastHeight - hgt), drawLength);
-            global_u32 *p = col + (size_t)hgt * W;
+            ckd_span_setup_drawn(sp, color, lastColor, length, drawLength, udiv24(65536u, length));
+            // hgt < H <= 2160 and W <= 4096: the row offset fits 24 bits
+            global_u32 *p = col + ckd_mul_u24((uint32_t)hgt, W);
             for (uint32_t k = 0; k < drawLength; ++k) {
                 *p = ckd_span_pixel(sp.acc);
                 p += W;
@@ -172,21 +211,21 @@ CKD_DEV void ls_column(global_u32 *__restrict__ col, unsigned W, unsigned H, uns
 }
 
 __global__ __launch_bounds__(64) void k_landscape(uint32_t *__restrict__ dst, unsigned W,
-                                                  unsigned H, const uint8_t *__restrict__ hmap,
-                                                  const uint32_t *__restrict__ cmap,
+                                                  unsigned H, const uint32_t *__restrict__ quadH,
+                                                  const uint4 *__restrict__ quadC,
                                                   const uint32_t *__restrict__ fog, LsLaunch P)
 {
     const unsigned iRay = blockIdx.x * 64 + threadIdx.x;
     if (iRay >= W) return;
     const LsFrame &F = P.f[blockIdx.y];
     ls_column((global_u32 *)(dst + (size_t)blockIdx.y * W * H + iRay), W, H, iRay, P.rayY, P.colScale, P.halfW,
-              P.mapScale, F.X1, F.Y1, F.viewCos, F.viewSin, F.mapTilt, hmap, cmap, fog);
+              P.mapScale, F.X1, F.Y1, F.viewCos, F.viewSin, F.mapTilt, quadH, quadC, fog);
 }
 
 // every voxel request of a batch in one launch: block -> job by binary search
 __global__ __launch_bounds__(64) void k_landscape_jobs(const VoxJob *__restrict__ jobs, int nJobs,
-                                                       const uint8_t *__restrict__ hmap,
-                                                       const uint32_t *__restrict__ cmap,
+                                                       const uint32_t *__restrict__ quadH,
+                                                       const uint4 *__restrict__ quadC,
                                                        const uint32_t *__restrict__ fog)
 {
     const int j = ckd_find_job(jobs, nJobs, blockIdx.x, &VoxJob::blockBase);
@@ -194,7 +233,7 @@ __global__ __launch_bounds__(64) void k_landscape_jobs(const VoxJob *__restrict_
     const unsigned iRay = (blockIdx.x - J.blockBase) * 64 + threadIdx.x;
     if (iRay >= J.W) return;
     ls_column((global_u32 *)(J.dst + iRay), J.W, J.H, iRay, J.rayY, J.colScale, J.halfW, J.mapScale, J.X1, J.Y1,
-              J.viewCos, J.viewSin, J.mapTilt, hmap, cmap, fog);
+              J.viewCos, J.viewSin, J.mapTilt, quadH, quadC, fog);
 }
 
 // ---- tunnelscape ---------------------------------------------------------
@@ -300,9 +339,9 @@ ckd_status upload(const void *host, size_t bytes, void **dev)
         (void)hipFree(*dev);
         *dev = nullptr;
     }
-    if (hipMalloc(dev, bytes) != hipSuccess) {
+    if (ckd::dev_alloc(dev, bytes) != hipSuccess) {
         *dev = nullptr;
-        return ckd::fail(CKD_ENOMEM, "map upload: hipMalloc(%zu) failed", bytes);
+        return ckd::fail(CKD_ENOMEM, "map upload: ckd::dev_alloc(%zu) failed", bytes);
     }
     CKD_HIP(hipMemcpy(*dev, host, bytes, hipMemcpyHostToDevice));
     return CKD_OK;
@@ -344,7 +383,7 @@ ckd_status launch_landscape_jobs(ckd_ctx *ctx, const VoxJob *jobs, int nJobs, ui
     if (!ctx->lsHeight)
         return fail(CKD_ESTATE, "voxel requests need the landscape maps (ckd_landscape_create)");
     KernelTimer kt(ctx, CKD_KT_LANDSCAPE);
-    k_landscape_jobs<<<totalBlocks, 64, 0, ctx->stream>>>(jobs, nJobs, ctx->lsHeight, ctx->lsColor,
+    k_landscape_jobs<<<totalBlocks, 64, 0, ctx->stream>>>(jobs, nJobs, ctx->lsQuadH, ctx->lsQuadC,
                                                           ctx->lsFog);
     return check_launch("landscape jobs");
 }
@@ -362,7 +401,16 @@ ckd_status ckd_landscape_create(ckd_ctx *ctx, const uint8_t *heightMap, const ui
     ckd_status s = upload(heightMap, 1024 * 1024, (void **)&ctx->lsHeight);
     if (s == CKD_OK) s = upload(colorMap, 1024 * 1024 * 4, (void **)&ctx->lsColor);
     if (s == CKD_OK) s = upload(fog256, 256 * 4, (void **)&ctx->lsFog);
-    if (s == CKD_OK) ctx->lsFog0 = fog256[0];
+    if (s != CKD_OK) return s;
+    ctx->lsFog0 = fog256[0];
+    constexpr size_t kTexels = 1024 * 1024;
+    if ((!ctx->lsQuadH && ckd::dev_alloc(&ctx->lsQuadH, kTexels * 4) != hipSuccess) ||
+        (!ctx->lsQuadC && ckd::dev_alloc(&ctx->lsQuadC, kTexels * 16) != hipSuccess))
+        return ckd::fail(CKD_ENOMEM, "ckd_landscape_create: hipMalloc of the texel quads failed");
+    k_ls_quads<<<kTexels / 256, 256, 0, ctx->stream>>>(ctx->lsHeight, ctx->lsColor, ctx->lsQuadH,
+                                                       ctx->lsQuadC);
+    s = ckd::check_launch("landscape quads");
+    if (s == CKD_OK) CKD_HIP(hipStreamSynchronize(ctx->stream));
     return s;
 }
 
@@ -392,8 +440,8 @@ ckd_status ckd_landscape_draw(ckd_ctx *ctx, uint32_t *dst, unsigned W, unsigned 
         }
         dim3 grid(ckd_div_up(W, 64), n);
         ckd::KernelTimer kt(ctx, CKD_KT_LANDSCAPE);
-        k_landscape<<<grid, 64, 0, ctx->stream>>>(dst + (size_t)f0 * W * H, W, H, ctx->lsHeight,
-                                                  ctx->lsColor, ctx->lsFog, P);
+        k_landscape<<<grid, 64, 0, ctx->stream>>>(dst + (size_t)f0 * W * H, W, H, ctx->lsQuadH,
+                                                  ctx->lsQuadC, ctx->lsFog, P);
         ckd_status s = ckd::check_launch("landscape");
         if (s != CKD_OK) return s;
     }

@@ -1,6 +1,8 @@
 // runtime.hip -- context, errors, device memory, events.
 #include "ckd_internal.h"
 
+#include <atomic>
+
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -18,6 +20,21 @@ ckd_status fail(ckd_status s, const char *fmt, ...)
     vsnprintf(g_lastError, sizeof(g_lastError), fmt, ap);
     va_end(ap);
     return s;
+}
+
+// the allocation that fails next: -1 none, 0 the next one, n the one after n
+std::atomic<int> g_failAlloc{-1};
+
+hipError_t dev_alloc_raw(void **p, size_t bytes)
+{
+    int n = g_failAlloc.load();
+    while (n >= 0 && !g_failAlloc.compare_exchange_weak(n, n - 1)) {
+    }
+    if (n == 0) {
+        *p = nullptr;
+        return hipErrorOutOfMemory;
+    }
+    return hipMalloc(p, bytes);
 }
 
 ckd_status check_launch(const char *what)
@@ -40,9 +57,9 @@ ckd_status stage(ckd_ctx *ctx, const void *host, size_t bytes, void **dev)
         }
         size_t cap = 4096;
         while (cap < bytes) cap *= 2;
-        if (hipMalloc(&ctx->staging, cap) != hipSuccess) {
+        if (ckd::dev_alloc(&ctx->staging, cap) != hipSuccess) {
             ctx->staging = nullptr;
-            return fail(CKD_ENOMEM, "stage: hipMalloc(%zu) failed", cap);
+            return fail(CKD_ENOMEM, "stage: ckd::dev_alloc(%zu) failed", cap);
         }
         ctx->stagingCap = cap;
     }
@@ -112,6 +129,13 @@ using ckd::drain_timing;
 
 extern "C" {
 
+ckd_status ckd_debug_fail_alloc(int n)
+{
+    CKD_REQUIRE(n >= -1, "ckd_debug_fail_alloc: n %d below -1", n);
+    ckd::g_failAlloc.store(n);
+    return CKD_OK;
+}
+
 ckd_status ckd_kernel_timing(ckd_ctx *ctx, int enable)
 {
     CKD_REQUIRE_CTX(ctx);
@@ -175,7 +199,7 @@ ckd_status ckd_create(int device, ckd_ctx **out)
         for (unsigned i = 0; i < 2048; ++i) lut[i] = cosf((float)i * (k2PI / 2048.f));
         lut[2048] = lut[0];
         for (unsigned i = 0; i < 2049; ++i) ctx->hostLut[i] = lut[i];
-        if (hipMalloc(&ctx->cosLut, sizeof(lut)) != hipSuccess ||
+        if (ckd::dev_alloc(&ctx->cosLut, sizeof(lut)) != hipSuccess ||
             hipMemcpy(ctx->cosLut, lut, sizeof(lut), hipMemcpyHostToDevice) != hipSuccess) {
             ckd_destroy(ctx);
             return ckd::fail(CKD_ENOMEM, "ckd_create: cosine LUT upload failed");
@@ -216,7 +240,8 @@ void ckd_destroy(ckd_ctx *ctx)
         if (pm.map) (void)hipFree(pm.map);
         if (pm.inv) (void)hipFree(pm.inv);
     }
-    void *bufs[] = {ctx->cosLut, ctx->lsHeight, ctx->lsColor, ctx->lsFog, ctx->tsHeight, ctx->tsColor,
+    void *bufs[] = {ctx->cosLut, ctx->lsHeight, ctx->lsColor, ctx->lsFog, ctx->lsQuadH, ctx->lsQuadC,
+                    ctx->tsHeight, ctx->tsColor,
                     ctx->tsFog, ctx->b26Scratch[0],
                     ctx->b26Scratch[1], ctx->staging, ctx->fdTex, ctx->fdTexHi,
                     ctx->stScratch, ctx->spikeBlurMaps, ctx->twHeight, ctx->twColor,
@@ -246,9 +271,9 @@ ckd_status ckd_malloc(ckd_ctx *ctx, void **dptr, size_t bytes)
     CKD_REQUIRE_CTX(ctx);
     *dptr = nullptr;
     if (bytes == 0) return CKD_OK;
-    if (hipMalloc(dptr, bytes) != hipSuccess) {
+    if (ckd::dev_alloc(dptr, bytes) != hipSuccess) {
         *dptr = nullptr;
-        return ckd::fail(CKD_ENOMEM, "ckd_malloc: hipMalloc(%zu) failed", bytes);
+        return ckd::fail(CKD_ENOMEM, "ckd_malloc: ckd::dev_alloc(%zu) failed", bytes);
     }
     return CKD_OK;
 }

@@ -347,6 +347,12 @@ class BatchGraph:
     def launch(self) -> None:
         check(self.ctx.lib.ckd_graph_launch(self.h))
 
+    def update(self, requests) -> None:
+        """new cameras / times / grids / radii / outputs for the next replays"""
+        arr = requests if isinstance(requests, C.Array) else self.ctx.request_array(requests)
+        check(self.ctx.lib.ckd_graph_update(self.h, arr, len(arr)))
+        self._keep = (arr, self._keep[1])
+
     def close(self) -> None:
         if self.h:
             self.ctx.lib.ckd_graph_destroy(self.h)

@@ -79,6 +79,13 @@ CKD_API ckd_status ckd_kernel_timing_read(ckd_ctx *ctx, int kernel,
                                           double *totalMs,
                                           unsigned long long *launches);
 
+/* Test hook, no reference counterpart (the reference's failures are a
+ * Create returning false, code/main.h:49): the n-th next device allocation
+ * the library makes (n = 0: the next one) fails as if the device were out
+ * of memory, so tests can drive every CKD_ENOMEM path; n = -1 disarms.
+ * Process-wide. */
+CKD_API ckd_status ckd_debug_fail_alloc(int n);
+
 /* ---- pixel primitives ---------------------------------------------------- */
 /* memset32, code/util.h:57-67 */
 CKD_API ckd_status ckd_memset32(ckd_ctx *ctx, uint32_t *dst, uint32_t value,
@@ -311,6 +318,16 @@ CKD_API ckd_status ckd_batch_graph_create(ckd_ctx *ctx, const ckd_request *reque
                                           unsigned long long *stepCount,
                                           ckd_graph **out);
 CKD_API ckd_status ckd_graph_launch(ckd_graph *graph);
+/* New parameters for the next replays of a captured batch (no reference
+ * counterpart; the next frames of the animation that code/main.cpp:314-351
+ * renders frame by frame): the requests must have the captured effects and
+ * sizes, in the captured order; cameras, times, grids, radii and dst
+ * pointers may change.  The scene, shading mode, texture and scratch stay the captured
+ * ones.  The graph's job tables are rewritten in place (after any replay
+ * still queued has finished with them), so the graph itself is not
+ * re-instantiated. */
+CKD_API ckd_status ckd_graph_update(ckd_graph *graph, const ckd_request *requests,
+                                    unsigned n);
 CKD_API void ckd_graph_destroy(ckd_graph *graph);
 
 /* ---- compositing blends (code/util.cpp:58-811) ---------------------------

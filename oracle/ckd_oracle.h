@@ -158,6 +158,10 @@ void orc_sdf_rows(uint32_t *dst, unsigned W, unsigned H,
                   int nthreads);
 /* float camera basis of frame time t: origin[3], forward*focal[3],
  * right[3], up[3] (computed in double, rounded to float) */
+void orc_sdf_rows_ex(uint32_t *dst, unsigned W, unsigned H,
+                     const orc_sdf_scene *scene, float t, int prec,
+                     unsigned rowStart, unsigned rowStep, uint16_t *steps,
+                     uint8_t *cond, int nthreads);
 void orc_sdf_camera(const orc_sdf_scene *scene, float t, float cam[12]);
 
 /* ---- a19 LUT trig (code/sincos-lut.h:13-33, code/sincos-lut.cpp:9-16) --- */

@@ -25,6 +25,16 @@
 
 #define SDF_NORMAL_H 1e-3
 
+static inline unsigned orc_max_chan_diff(unsigned cur, uint32_t a, uint32_t b)
+{
+    for (int c = 0; c < 4; ++c) {
+        const int x = (int)((a >> (8 * c)) & 0xff), y = (int)((b >> (8 * c)) & 0xff);
+        const unsigned e = (unsigned)(x > y ? x - y : y - x);
+        if (e > cur) cur = e;
+    }
+    return cur;
+}
+
 /* round-to-nearest-even like _mm_cvtps_epi32 under the default MXCSR */
 static inline uint32_t to_chan(double c)
 {
@@ -35,6 +45,13 @@ static inline uint32_t to_chan(double c)
     if (r > 255) return 255;
     return (uint32_t)r;
 }
+
+/* The march's stopping rule is perturbed to measure how much a pixel's
+ * colour depends on it: the threshold scaled by 1 +- SDF_COND_EPS_REL, and
+ * the step budget one step shorter or longer.  Each perturbed march is a
+ * prefix or a continuation of the actual one, so it costs a few extra steps
+ * and one more shading where the stop moves. */
+#define SDF_COND_EPS_REL 0.01
 
 #define DEFINE_SDF_RENDER(NAME, REAL, SQRT, EXP, LOG)                              \
 static inline REAL NAME##_scene(const orc_sdf_scene *sc, REAL px, REAL py, REAL pz, \
@@ -56,29 +73,25 @@ static inline REAL NAME##_scene(const orc_sdf_scene *sc, REAL px, REAL py, REAL 
     return d;                                                                      \
 }                                                                                  \
                                                                                    \
-static uint32_t NAME##_pixel(const orc_sdf_scene *sc, const REAL cam[12],           \
-                             unsigned W, unsigned H, unsigned x, unsigned y,       \
-                             uint16_t *stepsOut)                                   \
+/* march state: the last point sampled, the distance there, the distance  */     \
+/* travelled (including that last distance) and the steps taken            */     \
+typedef struct { REAL hx, hy, hz, march, total; int step; } NAME##_state;         \
+                                                                                   \
+/* one step of the reference tracers' loop (code/shadertoy.cpp:566-576)    */     \
+static inline void NAME##_step(const orc_sdf_scene *sc, const REAL *cam,          \
+                               const REAL *d, NAME##_state *s)                    \
 {                                                                                  \
-    const REAL u = ((REAL)2 * (REAL)x - (REAL)W) / (REAL)H;                        \
-    const REAL v = ((REAL)H - (REAL)2 * (REAL)y) / (REAL)H;                        \
-    REAL dx = cam[3] + u * cam[6] + v * cam[9];                                    \
-    REAL dy = cam[4] + u * cam[7] + v * cam[10];                                   \
-    REAL dz = cam[5] + u * cam[8] + v * cam[11];                                   \
-    const REAL il = (REAL)1 / SQRT(dx * dx + dy * dy + dz * dz);                    \
-    dx *= il; dy *= il; dz *= il;                                                  \
-    REAL hx = cam[0], hy = cam[1], hz = cam[2];                                    \
-    REAL march = 1, total = 0;                                                     \
-    const REAL eps = (REAL)sc->eps, tMax = (REAL)sc->tMax;                         \
-    int step;                                                                      \
-    for (step = 0; march > eps && step < sc->maxSteps && total < tMax; ++step) {   \
-        hx = cam[0] + dx * total;                                                  \
-        hy = cam[1] + dy * total;                                                  \
-        hz = cam[2] + dz * total;                                                  \
-        march = NAME##_scene(sc, hx, hy, hz, 0);                                   \
-        total += march;                                                            \
-    }                                                                              \
-    if (stepsOut) *stepsOut = (uint16_t)step;                                      \
+    s->hx = cam[0] + d[0] * s->total;                                              \
+    s->hy = cam[1] + d[1] * s->total;                                              \
+    s->hz = cam[2] + d[2] * s->total;                                              \
+    s->march = NAME##_scene(sc, s->hx, s->hy, s->hz, 0);                           \
+    s->total += s->march;                                                          \
+    ++s->step;                                                                     \
+}                                                                                  \
+                                                                                   \
+static uint32_t NAME##_shade(const orc_sdf_scene *sc, const NAME##_state *s)       \
+{                                                                                  \
+    const REAL hx = s->hx, hy = s->hy, hz = s->hz, total = s->total;               \
     const REAL h = (REAL)SDF_NORMAL_H;                                             \
     REAL nx = NAME##_scene(sc, hx + h, hy, hz, 0) - NAME##_scene(sc, hx - h, hy, hz, 0); \
     REAL ny = NAME##_scene(sc, hx, hy + h, hz, 0) - NAME##_scene(sc, hx, hy - h, hz, 0); \
@@ -102,6 +115,59 @@ static uint32_t NAME##_pixel(const orc_sdf_scene *sc, const REAL cam[12],       
     for (int c = 0; c < 4; ++c) {                                                  \
         const REAL g = c4[c] > (REAL)0 ? EXP((REAL)sc->gamma * LOG(c4[c])) : (REAL)0; \
         px |= to_chan((double)g) << (8 * c);                                       \
+    }                                                                              \
+    return px;                                                                     \
+}                                                                                  \
+                                                                                   \
+static uint32_t NAME##_pixel(const orc_sdf_scene *sc, const REAL cam[12],           \
+                             unsigned W, unsigned H, unsigned x, unsigned y,       \
+                             uint16_t *stepsOut, uint8_t *condOut)                 \
+{                                                                                  \
+    const REAL u = ((REAL)2 * (REAL)x - (REAL)W) / (REAL)H;                        \
+    const REAL v = ((REAL)H - (REAL)2 * (REAL)y) / (REAL)H;                        \
+    REAL d[3];                                                                     \
+    d[0] = cam[3] + u * cam[6] + v * cam[9];                                       \
+    d[1] = cam[4] + u * cam[7] + v * cam[10];                                      \
+    d[2] = cam[5] + u * cam[8] + v * cam[11];                                      \
+    const REAL il = (REAL)1 / SQRT(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);        \
+    d[0] *= il; d[1] *= il; d[2] *= il;                                            \
+    const REAL eps = (REAL)sc->eps, tMax = (REAL)sc->tMax;                         \
+    const REAL epsHi = eps * (REAL)(1 + SDF_COND_EPS_REL);                         \
+    const REAL epsLo = eps * (REAL)(1 - SDF_COND_EPS_REL);                         \
+    NAME##_state s = {cam[0], cam[1], cam[2], (REAL)1, (REAL)0, 0};                \
+    NAME##_state sHi = s, sPrev = s;                                               \
+    int haveHi = 0;                                                                \
+    while (s.march > eps && s.step < sc->maxSteps && s.total < tMax) {             \
+        sPrev = s;                                                                 \
+        NAME##_step(sc, cam, d, &s);                                               \
+        if (!haveHi && s.march <= epsHi) {                                         \
+            sHi = s;                                                               \
+            haveHi = 1;                                                            \
+        }                                                                          \
+    }                                                                              \
+    if (stepsOut) *stepsOut = (uint16_t)s.step;                                    \
+    const uint32_t px = NAME##_shade(sc, &s);                                      \
+    if (condOut) {                                                                 \
+        unsigned cond = 0;                                                         \
+        /* a threshold 1% higher stops at the first distance <= epsHi */          \
+        if (haveHi && sHi.step < s.step)                                           \
+            cond = orc_max_chan_diff(cond, NAME##_shade(sc, &sHi), px);            \
+        /* a threshold 1% lower carries on while the distance > epsLo */           \
+        if (s.march <= eps && s.march > epsLo) {                                   \
+            NAME##_state sLo = s;                                                  \
+            while (sLo.march > epsLo && sLo.step < sc->maxSteps && sLo.total < tMax) \
+                NAME##_step(sc, cam, d, &sLo);                                     \
+            cond = orc_max_chan_diff(cond, NAME##_shade(sc, &sLo), px);            \
+        }                                                                          \
+        /* a march that used every step: one step fewer, one step more */          \
+        if (s.march > eps && s.step >= sc->maxSteps && s.total < tMax) {           \
+            if (sPrev.step > 0)                                                    \
+                cond = orc_max_chan_diff(cond, NAME##_shade(sc, &sPrev), px);      \
+            NAME##_state sMore = s;                                                \
+            NAME##_step(sc, cam, d, &sMore);                                       \
+            cond = orc_max_chan_diff(cond, NAME##_shade(sc, &sMore), px);          \
+        }                                                                          \
+        *condOut = (uint8_t)cond;                                                  \
     }                                                                              \
     return px;                                                                     \
 }
@@ -141,10 +207,16 @@ void orc_sdf_camera(const orc_sdf_scene *sc, float t, float camOut[12])
     for (int i = 0; i < 12; ++i) camOut[i] = (float)cam[i];
 }
 
-/* rows rowStart, rowStart+rowStep, ... (compacted into dst) */
-void orc_sdf_rows(uint32_t *dst, unsigned W, unsigned H, const orc_sdf_scene *sc,
-                  float t, int prec, unsigned rowStart, unsigned rowStep,
-                  uint16_t *steps, int nthreads)
+/* rows rowStart, rowStart+rowStep, ... (compacted into dst).  steps and
+ * cond (optional) receive per pixel the march's step count and its
+ * conditioning: the largest per-channel change of the pixel when the
+ * stopping threshold is scaled by 1 +- 1% or the step budget moves by one.
+ * Where that exceeds the error bar, FP32 rounding of the distances (of
+ * order 1e-6 against eps = 1e-3) decides the colour, not the scene: a ray
+ * passing a surface at about eps, or still moving when its steps run out. */
+void orc_sdf_rows_ex(uint32_t *dst, unsigned W, unsigned H, const orc_sdf_scene *sc,
+                     float t, int prec, unsigned rowStart, unsigned rowStep,
+                     uint16_t *steps, uint8_t *cond, int nthreads)
 {
     /* both precisions start from the float camera the product receives */
     float camf[12];
@@ -163,10 +235,18 @@ void orc_sdf_rows(uint32_t *dst, unsigned W, unsigned H, const orc_sdf_scene *sc
         for (unsigned x = 0; x < W; ++x) {
             const size_t o = (size_t)r * W + x;
             uint16_t *so = steps ? steps + o : 0;
-            dst[o] = prec == 64 ? sdf64_pixel(sc, cam64, W, H, x, y, so)
-                                : sdf32_pixel(sc, cam32, W, H, x, y, so);
+            uint8_t *co = cond ? cond + o : 0;
+            dst[o] = prec == 64 ? sdf64_pixel(sc, cam64, W, H, x, y, so, co)
+                                : sdf32_pixel(sc, cam32, W, H, x, y, so, co);
         }
     }
+}
+
+void orc_sdf_rows(uint32_t *dst, unsigned W, unsigned H, const orc_sdf_scene *sc,
+                  float t, int prec, unsigned rowStart, unsigned rowStep,
+                  uint16_t *steps, int nthreads)
+{
+    orc_sdf_rows_ex(dst, W, H, sc, t, prec, rowStart, rowStep, steps, 0, nthreads);
 }
 
 void orc_sdf_frame(uint32_t *dst, unsigned W, unsigned H,

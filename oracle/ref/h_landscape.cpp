@@ -56,3 +56,22 @@ REF_API void ref_memset32(uint32_t *dst, uint32_t value, size_t n)
 {
     memset32(dst, (int)value, n);
 }
+
+/* Landscape_Draw's fill and vscape's column loop (code/landscape.cpp:170-191,
+ * :228-233) over the reference's vscape_ray, OpenMP over columns as there,
+ * for a 1280x720 frame; the per-column ray parameters (24:8 origin,
+ * direction and fishMul) come from the caller */
+REF_API void ref_vscape_columns(uint32_t *pDest, const uint8_t *heightMap,
+                                const uint32_t *colorMap, const uint32_t *fog256,
+                                int mapTilt, int fpX1, int fpY1, const int *dX,
+                                const int *dY, const float *fishMul)
+{
+    s_pHeightMap = const_cast<uint8_t *>(heightMap);
+    s_pColorMap = const_cast<uint32_t *>(colorMap);
+    for (int i = 0; i < 256; ++i) s_fogGradientUnp[i] = c2vISSE16(fog256[i]);
+    s_mapTilt = mapTilt;
+    memset32(pDest, fog256[0], kResX * kResY);
+    #pragma omp parallel for schedule(static)
+    for (unsigned iRay = 0; iRay < kResX; ++iRay)
+        vscape_ray(pDest + iRay, fpX1, fpY1, dX[iRay], dY[iRay], fishMul[iRay]);
+}

@@ -71,3 +71,7 @@ REF_API void ref_zoom32(uint32_t *d, const uint32_t *s, unsigned x, unsigned y, 
 {
     Zoom32(d, s, x, y, scale);
 }
+
+/* OpenMP team size of the reference's parallel loops (their pragmas take
+ * the runtime default) */
+REF_API void ref_omp_threads(int n) { omp_set_num_threads(n); }

@@ -54,6 +54,7 @@ _ORC = {
     "orc_tunnel_frame": (None, [_p, _p, _p, _u, _u, _p, _p, _u, _i, _i]),
     "orc_sdf_frame": (None, [_p, _u, _u, C.POINTER(SdfScene), _f, _i, _p, _i]),
     "orc_sdf_rows": (None, [_p, _u, _u, C.POINTER(SdfScene), _f, _i, _u, _u, _p, _i]),
+    "orc_sdf_rows_ex": (None, [_p, _u, _u, C.POINTER(SdfScene), _f, _i, _u, _u, _p, _p, _i]),
     "orc_sdf_camera": (None, [C.POINTER(SdfScene), _f, _p]),
     "orc_cos_lut": (None, [_p]),
     "orc_lutcosf": (_f, [_p, _f]),
@@ -98,6 +99,7 @@ _REF = {
     "ref_bsamp32_32": (C.c_uint32, [_p, _u, _u, _u, _u, _u, _u]),
     "ref_cspan16": (None, [_p, _i, _u, _u, C.c_uint32, C.c_uint32]),
     "ref_vscape_ray": (None, [_p, _p, _p, _p, _i, _i, _i, _i, _i, _f]),
+    "ref_vscape_columns": (None, [_p, _p, _p, _p, _i, _i, _i, _p, _p, _p]),
     "ref_memset32": (None, [_p, C.c_uint32, C.c_size_t]),
     "ref_tscape_ray": (None, [_p, _p, _p, _p, _i, _i, _i, _i]),
     "ref_tunnelscape_setup": (_i, [C.c_char_p, _p, _p, _p]),
@@ -119,6 +121,7 @@ _REF = {
     "ref_blur26": (None, [_p, _p, _u, _u, _f, _f, _u]),
     "ref_fx_blit_2x2": (None, [_p, _p]),
     "ref_add32": (None, [_p, _p, _u]),
+    "ref_omp_threads": (None, [_i]),
     "ref_blit_add32": (None, [_p, _p, _u, _u, _u]),
     "ref_blit_add32a": (None, [_p, _p, _u, _u, _u, _f]),
     "ref_cos_lut": (None, [_p]),

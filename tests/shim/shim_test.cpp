@@ -247,16 +247,24 @@ int main(int argc, char **argv)
         };
         expect(0, "start");
         if (Landscape_Create() || Tunnelscape_Create() || Shadertoy_Create() || Ball_Create() ||
-            Twister_Create())
+            Twister_Create()) {
+            fprintf(stderr, "a Create succeeded without its assets\n");
             return 1;
+        }
         expect(0, "after failed Creates");
         g_failed = false; // the failures above reported through SetLastError
-        if (!Polar_Create() || !Polar_Create()) return 1;
+        if (!Polar_Create() || !Polar_Create()) {
+            fprintf(stderr, "Polar_Create failed\n");
+            return 1;
+        }
         expect(1, "Polar_Create twice");
         Polar_Destroy();
         Polar_Destroy();
         expect(0, "Polar_Destroy twice");
-        if (!FxBlitter_Create() || !BoxBlur_Create()) return 1;
+        if (!FxBlitter_Create() || !BoxBlur_Create()) {
+            fprintf(stderr, "FxBlitter_Create or BoxBlur_Create failed\n");
+            return 1;
+        }
         expect(2, "FxBlitter_Create + BoxBlur_Create");
         FxBlitter_DrawTestPattern(out.data());
         BoxBlur_Destroy();
@@ -267,7 +275,10 @@ int main(int argc, char **argv)
         expect(1, "a free function holds one reference");
         Add32(a.data(), b.data(), 64);
         expect(1, "and only one");
-        if (a[0] != 0x11223344u) return 1;
+        if (a[0] != 0x21426384u) { // two saturating adds of b
+            fprintf(stderr, "Add32: %08x\n", a[0]);
+            return 1;
+        }
         printf("ok\n");
     } else {
         return 2;

@@ -127,6 +127,11 @@ def test_landscape_frame_matches_reference_columns(orc, refl, maps):
         refl.ref_vscape_ray(P(ref[col:]), P(maps.height), P(maps.color), P(maps.fog), cam.mapTilt,
                             fpX1.value, fpY1.value, int(dX[col]), int(dY[col]), float(fish[col]))
     assert np.array_equal(out, ref)
+    # the reference's own OpenMP column loop (the CPU baseline of the bench)
+    cols = aligned_empty(W * H)
+    refl.ref_vscape_columns(P(cols), P(maps.height), P(maps.color), P(maps.fog), cam.mapTilt,
+                            fpX1.value, fpY1.value, P(dX), P(dY), P(fish))
+    assert np.array_equal(out, cols)
 
 
 def test_tscape_ray_rows(orc, refl):
