@@ -140,6 +140,7 @@ SIGNATURES = {
     "ckd_event_elapsed_ms": [_p, _i, _i, C.POINTER(_f)],
     "ckd_kernel_timing": [_p, _i],
     "ckd_debug_fail_alloc": [_i],
+    "ckd_tunnel_frame_noalloc": [_p, _p, _u, _u, _p, _p, _u, _i],
     "ckd_kernel_timing_read": [_p, _i, C.POINTER(C.c_double), C.POINTER(C.c_ulonglong)],
     "ckd_memset32": [_p, _p, C.c_uint32, C.c_size_t],
     "ckd_fx_blit_2x2": [_p, _p, _p, _u, _u],

@@ -120,6 +120,10 @@ ckd_status launch_blit_add(ckd_ctx *ctx, uint32_t *dst, const uint32_t *src, uns
                            unsigned srcStride, unsigned width, unsigned rows);
 
 // Records a message for ckd_last_error() on this thread and returns s.
+// the device address of ckd_tunnel_frame_noalloc's scratch array on the
+// current device, looked up once per device (k_remap.hip)
+ckd_status noalloc_scratch_setup(int device);
+
 // hipMalloc for every device allocation the library makes, so that a test
 // can make the n-th next one fail (ckd_debug_fail_alloc)
 hipError_t dev_alloc_raw(void **p, size_t bytes);

@@ -27,7 +27,7 @@ void tunnel_job_geometry(TunJob &J, uint32_t *dst, uint32_t *tmp, const int32_t 
 uint32_t tunnel_vblocks(unsigned W, unsigned H, int seg);
 ckd_status launch_tunnel_jobs(ckd_ctx *ctx, const TunJob *jobs, int nJobs, const TunJob &single,
                               uint32_t totalRows, uint32_t totalVBlocks, unsigned maxW,
-                              const uint32_t *tex, unsigned texLog2);
+                              const uint32_t *tex, unsigned texLog2, bool timed);
 } // namespace ckd
 
 // per request, what fixes the captured launch geometry: the effect and the
@@ -185,7 +185,7 @@ ckd_status launch_plan(ckd_ctx *ctx, const Plan &P, unsigned char *dev, const ck
         TunJob none;
         memset(&none, 0, sizeof(none));
         s = ckd::launch_tunnel_jobs(ctx, reinterpret_cast<const TunJob *>(dev + P.oTun), P.nTun, none,
-                                    P.tunRows, P.tunV, P.tunMaxW, tex, texLog2);
+                                    P.tunRows, P.tunV, P.tunMaxW, tex, texLog2, true);
         if (s != CKD_OK) return s;
         CKD_HIP(hipEventRecord(ctx->tunEv, ctx->aux2));
     }
@@ -200,7 +200,7 @@ ckd_status launch_plan(ckd_ctx *ctx, const Plan &P, unsigned char *dev, const ck
             TunJob none;
             memset(&none, 0, sizeof(none));
             s = ckd::launch_tunnel_jobs(ctx, reinterpret_cast<const TunJob *>(dev + P.oTun), P.nTun,
-                                        none, P.tunRows, P.tunV, P.tunMaxW, tex, texLog2);
+                                        none, P.tunRows, P.tunV, P.tunMaxW, tex, texLog2, true);
             if (s != CKD_OK) return s;
         }
         if (split) CKD_HIP(hipStreamWaitEvent(ctx->aux, ctx->tunEv, 0));

@@ -10,6 +10,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 namespace {
@@ -99,7 +100,7 @@ void tunnel_job_geometry(TunJob &J, uint32_t *dst, uint32_t *tmp, const int32_t 
 uint32_t tunnel_vblocks(unsigned W, unsigned H, int seg);
 ckd_status launch_tunnel_jobs(ckd_ctx *ctx, const TunJob *jobs, int nJobs, const TunJob &single,
                               uint32_t totalRows, uint32_t totalVBlocks, unsigned maxW,
-                              const uint32_t *tex, unsigned texLog2);
+                              const uint32_t *tex, unsigned texLog2, bool timed);
 ckd_status blur07_hv_oop(ckd_ctx *ctx, uint32_t *tmp, uint32_t *dst, const uint32_t *src,
                          unsigned W, unsigned H, float strength);
 
@@ -190,7 +191,83 @@ ckd_status ckd_tunnel_frame(ckd_ctx *ctx, uint32_t *dst, uint32_t *tmp, unsigned
     ckd::tunnel_job_geometry(J, dst, tmp, grid, W, H, radius);
     J.rowBase = 0;
     J.vBase = 0;
-    return ckd::launch_tunnel_jobs(ctx, nullptr, 1, J, H, ckd::tunnel_vblocks(W, H, ctx->tuning.tunnelVSeg), W, tex, texLog2);
+    return ckd::launch_tunnel_jobs(ctx, nullptr, 1, J, H, ckd::tunnel_vblocks(W, H, ctx->tuning.tunnelVSeg), W, tex, texLog2,
+                                   true);
+}
+
+// ---- the tunnel frame without allocation ---------------------------------
+//
+// ckd_tunnel_frame_noalloc keeps the frame's horizontal-blur image in
+// `scratch`, a fixed array of device memory: every GPU the library runs on
+// holds one, 14.7 MB, until the process ends.  On the host the array's name
+// is only a stand-in for the symbol; the device address of each GPU's copy
+// comes from hipGetSymbolAddress, called once per device with that device
+// current (ckd_create), and is the only thing host code passes to kernels.
+
+} // extern "C"
+
+namespace {
+
+constexpr size_t kNoAllocPixels = 2560u * 1440u;
+static __device__ __align__(256) uint32_t scratch[kNoAllocPixels];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMutex;
+uint32_t *g_scratchPtr[kMaxDevices] = {};
+
+uint32_t *scratch_of(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    return device >= 0 && device < kMaxDevices ? g_scratchPtr[device] : nullptr;
+}
+
+} // namespace
+
+namespace ckd {
+
+ckd_status noalloc_scratch_setup(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if (device < 0 || device >= kMaxDevices)
+        return fail(CKD_EDEVICE, "device %d beyond the %d the library keeps scratch for", device,
+                    kMaxDevices);
+    if (g_scratchPtr[device]) return CKD_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(CKD_EDEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", device,
+                    hipGetErrorString(e));
+    g_scratchPtr[device] = static_cast<uint32_t *>(p);
+    return CKD_OK;
+}
+
+} // namespace ckd
+
+extern "C" {
+
+ckd_status ckd_tunnel_frame_noalloc(ckd_ctx *ctx, uint32_t *dst, unsigned W, unsigned H,
+                                    const int32_t *grid, const uint32_t *tex, unsigned texLog2,
+                                    int radius)
+{
+    CKD_REQUIRE(dst && grid && tex, "ckd_tunnel_frame_noalloc: null buffer");
+    CKD_REQUIRE(W > 0 && H > 0 && W <= 4096 && H <= 4096, "ckd_tunnel_frame_noalloc: bad size %ux%u",
+                W, H);
+    CKD_REQUIRE((size_t)W * H <= kNoAllocPixels,
+                "ckd_tunnel_frame_noalloc: %ux%u exceeds its scratch of %zu pixels (2560x1440)", W, H,
+                kNoAllocPixels);
+    CKD_REQUIRE(texLog2 >= 1 && texLog2 <= 14, "ckd_tunnel_frame_noalloc: texLog2 %u out of 1..14",
+                texLog2);
+    CKD_REQUIRE(radius >= 1 && radius <= 127 && (unsigned)(2 * radius + 1) <= (W < H ? W : H),
+                "ckd_tunnel_frame_noalloc: radius %d out of range", radius);
+    CKD_REQUIRE_CTX(ctx);
+    uint32_t *tmp = scratch_of(ctx->device);
+    if (!tmp) return ckd::fail(CKD_ESTATE, "ckd_tunnel_frame_noalloc: no scratch on device %d", ctx->device);
+    TunJob J;
+    ckd::tunnel_job_geometry(J, dst, tmp, grid, W, H, radius);
+    J.rowBase = 0;
+    J.vBase = 0;
+    return ckd::launch_tunnel_jobs(ctx, nullptr, 1, J, H, ckd::tunnel_vblocks(W, H, ctx->tuning.tunnelVSeg),
+                                   W, tex, texLog2, false);
 }
 
 } // extern "C"

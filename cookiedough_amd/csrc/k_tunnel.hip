@@ -225,18 +225,18 @@ uint32_t tunnel_vblocks(unsigned W, unsigned H, int seg) { return ((W + 63) / 64
 
 ckd_status launch_tunnel_jobs(ckd_ctx *ctx, const TunJob *jobs, int nJobs, const TunJob &single,
                               uint32_t totalRows, uint32_t totalVBlocks, unsigned maxW,
-                              const uint32_t *tex, unsigned texLog2)
+                              const uint32_t *tex, unsigned texLog2, bool timed)
 {
     if (totalRows == 0) return CKD_OK;
     // the row's per-block interpolation terms, its spread pixels / prefix sums
     const size_t lds = (size_t)((maxW + 7) / 8) * sizeof(uint4) + (size_t)(maxW + 1) * sizeof(uint64_t);
     {
-        KernelTimer kt(ctx, CKD_KT_TUNNEL_H);
+        KernelTimer kt(ctx, timed ? CKD_KT_TUNNEL_H : -1);
         k_tunnel_h<<<totalRows, kT1Block, lds, ctx->stream>>>(jobs, nJobs, single, tex, texLog2);
     }
     ckd_status s = check_launch("tunnel remap+H");
     if (s != CKD_OK) return s;
-    KernelTimer kt(ctx, CKD_KT_TUNNEL_V);
+    KernelTimer kt(ctx, timed ? CKD_KT_TUNNEL_V : -1);
     k_tunnel_v<<<totalVBlocks, 64, 0, ctx->stream>>>(jobs, nJobs, single, ctx->tuning.tunnelVSeg);
     return check_launch("tunnel V+add");
 }

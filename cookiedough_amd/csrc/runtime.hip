@@ -182,6 +182,11 @@ ckd_status ckd_create(int device, ckd_ctx **out)
     if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return ckd::fail(CKD_EDEVICE, "ckd_create: device %d is %s, this library is built for gfx950",
                          device, prop.gcnArchName);
+    {
+        // the scratch array of ckd_tunnel_frame_noalloc, once per device
+        const ckd_status s = ckd::noalloc_scratch_setup(device);
+        if (s != CKD_OK) return s;
+    }
     ckd_ctx *ctx = new ckd_ctx();
     ctx->device = device;
     ctx->cuCount = prop.multiProcessorCount;

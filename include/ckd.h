@@ -249,6 +249,19 @@ CKD_API ckd_status ckd_tunnel_frame(ckd_ctx *ctx, uint32_t *dst, uint32_t *tmp,
                                     const uint32_t *tex, unsigned texLog2,
                                     int radius);
 
+/* No reference counterpart: ckd_tunnel_frame for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The W*H scratch image is a fixed array of device
+ * memory the library keeps on every GPU it runs on (14.7 MB, until the
+ * process ends), so W*H is limited to 2560*1440; larger frames get
+ * CKD_EINVAL.  The call allocates nothing (device or host) and is untimed by
+ * ckd_kernel_timing.  Calls on one GPU share that array: calls that overlap
+ * on one GPU (other streams, other contexts, other threads) race. */
+CKD_API ckd_status ckd_tunnel_frame_noalloc(ckd_ctx *ctx, uint32_t *dst, unsigned W,
+                                            unsigned H, const int32_t *grid,
+                                            const uint32_t *tex, unsigned texLog2,
+                                            int radius);
+
 /* ---- batched renderer (no reference counterpart: many frames, effects and
  * resolutions in one persistent tile-queue launch) ------------------------ */
 enum { CKD_FX_SDF = 0, CKD_FX_VOXEL = 1, CKD_FX_TUNNEL = 2 };
