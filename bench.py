@@ -293,6 +293,7 @@ def main() -> int:
     
             # voxel c1: 120 frames on the Lissajous path
             lcams = [wl.landscape_cam(path, k / 60) for k in range(120)]
+            lcams_arr = ctx.landscape_cams(lcams)
             fr_vox = ctx.empty(120 * W * H)
 
             def vox_plain():
@@ -300,7 +301,7 @@ def main() -> int:
                     ctx.landscape_draw(fr_vox.view(k * W * H, (W * H,)), W, H, [c])
 
             def vox_batched():
-                ctx.landscape_draw(fr_vox, W, H, lcams)
+                ctx.landscape_draw(fr_vox, W, H, lcams_arr)
 
             # tunnel c2: 240 frames, radius U{1..32}
             radii = wl.tunnel_radii(0, 240)

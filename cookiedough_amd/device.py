@@ -278,8 +278,12 @@ class Context:
         arr = (_lib.BallParams * len(frames))(*frames)
         check(self.lib.ckd_ball_draw(self.h, ptr(dst), ptr(target), W, H, arr, len(frames)))
 
+    def landscape_cams(self, cams):
+        """a ctypes array of cameras, built once and reusable across calls"""
+        return (_lib.LandscapeCam * len(cams))(*cams)
+
     def landscape_draw(self, dst, W, H, cams):
-        arr = (_lib.LandscapeCam * len(cams))(*cams)
+        arr = cams if isinstance(cams, C.Array) else self.landscape_cams(cams)
         check(self.lib.ckd_landscape_draw(self.h, ptr(dst), W, H, arr, len(cams)))
 
     def tunnelscape_create(self, hmap, cmap, fog):
