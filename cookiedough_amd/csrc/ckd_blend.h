@@ -37,7 +37,10 @@ CKD_DEV uint32_t chan(uint32_t px, int k) { return (px >> (8 * k)) & 0xffu; }
 CKD_DEV uint32_t mix_uniform(uint32_t d, uint32_t s, uint32_t a)
 {
     const uint32_t M = 0x00ff00ffu;
-    return ckd_lerp2(d & M, s & M, a) | (ckd_lerp2((d >> 8) & M, (s >> 8) & M, a) << 8);
+    const ckd_u16x2 f = ckd_splat16(a), nf = (unsigned short)256 - f;
+    return ckd_u32_of(ckd_lerp2x(ckd_u16x2_of(d & M), ckd_u16x2_of(s & M), nf, f)) |
+           (ckd_u32_of(ckd_lerp2x(ckd_u16x2_of(ckd_odd_bytes(d)), ckd_u16x2_of(ckd_odd_bytes(s)), nf, f))
+            << 8);
 }
 
 // SoftLightBlend (code/util.cpp:227-240)

@@ -5,19 +5,30 @@
 
 namespace ckd {
 
-// The texture half of grid_pixel: the bilinear sample at the interpolated
-// (U, V), both 24:8 fixed point.
+// The texture half of grid_pixel, from (U, V) << 6: 18:14 fixed point, as
+// the per-block interpolation (8 * corner + i * step) produces them before
+// its >> 6.  Texel column (U >> 8) & mask is bits [14, 14 + texLog2) of
+// X, the next column ((U >> 8) + 1) & mask those bits of X + 2^14, the
+// fraction U & 0xff bits [6, 14) (texLog2 <= 14).
+CKD_DEV uint32_t grid_sample_q14(const uint32_t *__restrict__ tex, unsigned texLog2, uint32_t X,
+                                 uint32_t Y)
+{
+    const uint32_t u0 = __builtin_amdgcn_ubfe(X, 14, texLog2);
+    const uint32_t u1 = __builtin_amdgcn_ubfe(X + (1u << 14), 14, texLog2);
+    const uint32_t v0 = __builtin_amdgcn_ubfe(Y, 14, texLog2) << texLog2;
+    const uint32_t v1 = __builtin_amdgcn_ubfe(Y + (1u << 14), 14, texLog2) << texLog2;
+    // 32-bit byte offsets from the texture's base (below 2^30), so each
+    // fetch adds one offset register to a uniform base
+    auto at = [tex](uint32_t k) { return *(const uint32_t *)((const char *)tex + (k << 2)); };
+    return ckd_bilerp32(at(u0 + v0), at(u1 + v0), at(u0 + v1), at(u1 + v1),
+                        __builtin_amdgcn_ubfe(X, 6, 8), __builtin_amdgcn_ubfe(Y, 6, 8));
+}
+
+// The bilinear sample at the interpolated (U, V), both 24:8 fixed point
+// (the bits grid_sample_q14 reads are U's bits [0, 8 + texLog2)).
 CKD_DEV uint32_t grid_sample(const uint32_t *__restrict__ tex, unsigned texLog2, int U, int V)
 {
-    const unsigned mask = (1u << texLog2) - 1;
-    const unsigned u0 = ((unsigned)(U >> 8)) & mask, u1 = ((unsigned)(U >> 8) + 1) & mask;
-    const unsigned v0 = (((unsigned)(V >> 8)) & mask) << texLog2;
-    const unsigned v1 = (((unsigned)(V >> 8) + 1) & mask) << texLog2;
-    // 32-bit byte offsets from the texture's base (texLog2 <= 14: below
-    // 2^30), so each fetch adds one offset register to a uniform base
-    auto at = [tex](uint32_t k) { return *(const uint32_t *)((const char *)tex + (k << 2)); };
-    return ckd_bilerp32(at(u0 + v0), at(u1 + v0), at(u0 + v1), at(u1 + v1), (uint32_t)U & 0xff,
-                        (uint32_t)V & 0xff);
+    return grid_sample_q14(tex, texLog2, (uint32_t)U << 6, (uint32_t)V << 6);
 }
 
 // Grid remap of one pixel (build-defined law):

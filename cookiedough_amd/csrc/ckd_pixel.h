@@ -27,19 +27,54 @@ CKD_DEV uint32_t ckd_lerp2(uint32_t a, uint32_t b, uint32_t f)
     return ((a * (256u - f) + b * f) >> 8) & 0x00ff00ffu;
 }
 
+// Two 16-bit lanes, for the packed 16-bit integer VALU operations
+// (v_pk_mul_lo_u16, v_pk_mad_u16, v_pk_lshrrev_b16, v_pk_add_u16, ...): one
+// instruction works on both channels of a 0x00XX00YY register, and the
+// lanes never carry into each other.
+typedef unsigned short ckd_u16x2 __attribute__((ext_vector_type(2)));
+
+CKD_DEV ckd_u16x2 ckd_u16x2_of(uint32_t v) { return __builtin_bit_cast(ckd_u16x2, v); }
+CKD_DEV uint32_t ckd_u32_of(ckd_u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+
+// v_perm_b32: byte k of the result is byte sel.k of the 8-byte value
+// hi:lo (0-3 select lo's bytes, 4-7 hi's, 0x0c gives 0)
+CKD_DEV uint32_t ckd_perm(uint32_t hi, uint32_t lo, uint32_t sel)
+{
+    return __builtin_amdgcn_perm(hi, lo, sel);
+}
+
+// (v >> 8) & 0x00ff00ff: bytes 1 and 3 into the low bytes of the 16-bit fields
+CKD_DEV uint32_t ckd_odd_bytes(uint32_t v) { return ckd_perm(0u, v, 0x0c030c01u); }
+
+// a*(256-f) + b*f >> 8 in each 16-bit lane: f and nf = 256 - f are splat
+// into both lanes; every lane stays below 2^16 (ckd_lerp2), so the packed
+// mod-2^16 arithmetic is exact
+CKD_DEV ckd_u16x2 ckd_lerp2x(ckd_u16x2 a, ckd_u16x2 b, ckd_u16x2 nf, ckd_u16x2 f)
+{
+    return (a * nf + b * f) >> (unsigned short)8;
+}
+
+CKD_DEV ckd_u16x2 ckd_splat16(uint32_t f) { return ckd_u16x2{(unsigned short)f, (unsigned short)f}; }
+
 // Bilinear blend of four ARGB texels with 8-bit fractions.  Equals
 // bsamp32_16 and bsamp32_32 (code/bilinear.h:58-125) channel for channel:
 // their 16-bit wrap-around intermediates equal S0*(256-f)+S1*f, which lies
 // in [0, 65280], so the mod-2^16 and 32-bit forms agree with this one.
+// B,R and G,A go through the packed 16-bit lerps in pairs.
 CKD_DEV uint32_t ckd_bilerp32(uint32_t p0, uint32_t p1, uint32_t p2, uint32_t p3,
                               uint32_t fu, uint32_t fv)
 {
     const uint32_t M = 0x00ff00ffu;
-    const uint32_t br = ckd_lerp2(ckd_lerp2(p0 & M, p1 & M, fu),
-                                  ckd_lerp2(p2 & M, p3 & M, fu), fv);
-    const uint32_t ga = ckd_lerp2(ckd_lerp2((p0 >> 8) & M, (p1 >> 8) & M, fu),
-                                  ckd_lerp2((p2 >> 8) & M, (p3 >> 8) & M, fu), fv);
-    return br | (ga << 8);
+    const ckd_u16x2 f = ckd_splat16(fu), g = ckd_splat16(fv);
+    const ckd_u16x2 nf = (unsigned short)256 - f, ng = (unsigned short)256 - g;
+    const ckd_u16x2 br = ckd_lerp2x(ckd_lerp2x(ckd_u16x2_of(p0 & M), ckd_u16x2_of(p1 & M), nf, f),
+                                    ckd_lerp2x(ckd_u16x2_of(p2 & M), ckd_u16x2_of(p3 & M), nf, f),
+                                    ng, g);
+    const ckd_u16x2 ga =
+        ckd_lerp2x(ckd_lerp2x(ckd_u16x2_of(ckd_odd_bytes(p0)), ckd_u16x2_of(ckd_odd_bytes(p1)), nf, f),
+                   ckd_lerp2x(ckd_u16x2_of(ckd_odd_bytes(p2)), ckd_u16x2_of(ckd_odd_bytes(p3)), nf, f),
+                   ng, g);
+    return ckd_u32_of(br) | (ckd_u32_of(ga) << 8);
 }
 
 // bsamp8 (code/bilinear.h:34-53), one channel.
@@ -54,25 +89,23 @@ CKD_DEV uint32_t ckd_bilerp8(uint32_t s0, uint32_t s1, uint32_t s2, uint32_t s3,
 // _mm_subs_epu16 of two packed ARGB colours widened to 16-bit lanes.
 CKD_DEV uint32_t ckd_subs_u8x4(uint32_t c, uint32_t f)
 {
-    uint32_t r = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int v = (int)((c >> (8 * k)) & 0xff) - (int)((f >> (8 * k)) & 0xff);
-        r |= (uint32_t)(v > 0 ? v : 0) << (8 * k);
-    }
-    return r;
+    const uint32_t M = 0x00ff00ffu;
+    const ckd_u16x2 br = __builtin_elementwise_sub_sat(ckd_u16x2_of(c & M), ckd_u16x2_of(f & M));
+    const ckd_u16x2 ga =
+        __builtin_elementwise_sub_sat(ckd_u16x2_of(ckd_odd_bytes(c)), ckd_u16x2_of(ckd_odd_bytes(f)));
+    return ckd_u32_of(br) | (ckd_u32_of(ga) << 8);
 }
 
-// Saturating per-byte add (_mm_add_epi16 of widened bytes + packus_epi16).
+// Saturating per-byte add (_mm_add_epi16 of widened bytes + packus_epi16),
+// two channels per packed 16-bit add and min.
 CKD_DEV uint32_t ckd_adds_u8x4(uint32_t a, uint32_t b)
 {
-    uint32_t r = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint32_t v = ((a >> (8 * k)) & 0xff) + ((b >> (8 * k)) & 0xff);
-        r |= (v > 255u ? 255u : v) << (8 * k);
-    }
-    return r;
+    const uint32_t M = 0x00ff00ffu;
+    const ckd_u16x2 top = ckd_splat16(255u);
+    const ckd_u16x2 br = __builtin_elementwise_min(ckd_u16x2_of(a & M) + ckd_u16x2_of(b & M), top);
+    const ckd_u16x2 ga =
+        __builtin_elementwise_min(ckd_u16x2_of(ckd_odd_bytes(a)) + ckd_u16x2_of(ckd_odd_bytes(b)), top);
+    return ckd_u32_of(br) | (ckd_u32_of(ga) << 8);
 }
 
 // _mm_avg_epu8 of four bytes
@@ -159,22 +192,23 @@ CKD_DEV uint32_t ckd_span_pixel(const uint32_t acc[4])
 // spread values (prefix sums included) are exact field by field.
 CKD_DEV uint64_t ckd_spread(uint32_t px)
 {
-    return (uint64_t)(px & 0xffu) | ((uint64_t)(px & 0xff00u) << 8) |
-           ((uint64_t)(px & 0xff0000u) << 16) | ((uint64_t)(px & 0xff000000u) << 24);
+    const uint32_t lo = ckd_perm(0u, px, 0x0c010c00u);  // 0x00b1_00b0
+    const uint32_t hi = ckd_perm(0u, px, 0x0c030c02u);  // 0x00b3_00b2
+    return ((uint64_t)hi << 32) | lo;
 }
 
 // Div (code/deprecated/boxblur.cpp:39-42): mulhi_epu16(acc, div) per field,
-// then packus_epi16
+// then packus_epi16, for a window sum of `count` pixels and its divisor
+// div = ckd_b07_count_div(count).  Each field of acc is at most 255*count
+// and div at most 2^16/count, so every quotient is at most 255: packus
+// never saturates, and the quotient is byte 2 of the 32-bit product (both
+// factors below 2^16, so v_mul_u32_u24 is exact).
 CKD_DEV uint32_t ckd_b07_div_pack(uint64_t acc, uint32_t div)
 {
-    uint32_t px = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint32_t a = (uint32_t)(acc >> (16 * k)) & 0xffffu;
-        const uint32_t q = (a * div) >> 16;
-        px |= (q > 32767u ? 0u : (q > 255u ? 255u : q)) << (8 * k);
-    }
-    return px;
+    const uint32_t lo = (uint32_t)acc, hi = (uint32_t)(acc >> 32);
+    const uint32_t q0 = __umul24(lo & 0xffffu, div), q1 = __umul24(lo >> 16, div);
+    const uint32_t q2 = __umul24(hi & 0xffffu, div), q3 = __umul24(hi >> 16, div);
+    return ckd_perm(q1, q0, 0x0c0c0602u) | ckd_perm(q3, q2, 0x06020c0cu);
 }
 
 // WeightToDiv(16*count) (code/deprecated/boxblur.cpp:11-14) stored through

@@ -25,9 +25,10 @@ ckd_status launch_landscape_jobs(ckd_ctx *ctx, const VoxJob *jobs, int nJobs, ui
 void tunnel_job_geometry(TunJob &J, uint32_t *dst, uint32_t *tmp, const int32_t *grid, unsigned W,
                          unsigned H, int radius);
 uint32_t tunnel_vblocks(unsigned W, unsigned H, int seg);
+unsigned tunnel_ring_rows(unsigned maxEdge);
 ckd_status launch_tunnel_jobs(ckd_ctx *ctx, const TunJob *jobs, int nJobs, const TunJob &single,
                               uint32_t totalRows, uint32_t totalVBlocks, unsigned maxW,
-                              const uint32_t *tex, unsigned texLog2, bool timed);
+                              unsigned maxEdge, const uint32_t *tex, unsigned texLog2, bool timed);
 } // namespace ckd
 
 // per request, what fixes the captured launch geometry: the effect and the
@@ -50,6 +51,9 @@ struct ckd_graph {
     uint32_t *tmp = nullptr;
     size_t tmpPixels = 0;
     std::vector<ReqShape> shapes;
+    // window rows of the captured vertical-blur ring (0: no ring): the
+    // largest radius a replay can take
+    unsigned tunRingRows = 0;
 };
 
 namespace {
@@ -63,6 +67,7 @@ struct Plan {
     int nSdf = 0, nVox = 0, nTun = 0;
     uint32_t chunks = 0, voxBlocks = 0, tunRows = 0, tunV = 0;
     unsigned tunMaxW = 0;
+    unsigned tunMaxE = 0;
 };
 
 __global__ void k_reset_counter(uint32_t *c) { *c = 0u; }
@@ -145,6 +150,7 @@ void make_plan(const ckd_tuning &T, const ckd_request *requests, unsigned n, uin
             P.tunRows += r.H;
             P.tunV += ckd::tunnel_vblocks(r.W, r.H, T.tunnelVSeg);
             if (r.W > P.tunMaxW) P.tunMaxW = r.W;
+            if ((unsigned)r.radius > P.tunMaxE) P.tunMaxE = (unsigned)r.radius;
             tun.push_back(J);
         }
     }
@@ -185,7 +191,7 @@ ckd_status launch_plan(ckd_ctx *ctx, const Plan &P, unsigned char *dev, const ck
         TunJob none;
         memset(&none, 0, sizeof(none));
         s = ckd::launch_tunnel_jobs(ctx, reinterpret_cast<const TunJob *>(dev + P.oTun), P.nTun, none,
-                                    P.tunRows, P.tunV, P.tunMaxW, tex, texLog2, true);
+                                    P.tunRows, P.tunV, P.tunMaxW, P.tunMaxE, tex, texLog2, true);
         if (s != CKD_OK) return s;
         CKD_HIP(hipEventRecord(ctx->tunEv, ctx->aux2));
     }
@@ -200,7 +206,7 @@ ckd_status launch_plan(ckd_ctx *ctx, const Plan &P, unsigned char *dev, const ck
             TunJob none;
             memset(&none, 0, sizeof(none));
             s = ckd::launch_tunnel_jobs(ctx, reinterpret_cast<const TunJob *>(dev + P.oTun), P.nTun,
-                                        none, P.tunRows, P.tunV, P.tunMaxW, tex, texLog2, true);
+                                        none, P.tunRows, P.tunV, P.tunMaxW, P.tunMaxE, tex, texLog2, true);
             if (s != CKD_OK) return s;
         }
         if (split) CKD_HIP(hipStreamWaitEvent(ctx->aux, ctx->tunEv, 0));
@@ -354,6 +360,7 @@ ckd_status ckd_batch_graph_create(ckd_ctx *ctx, const ckd_request *requests, uns
     g->tuning = ctx->tuning;
     g->tmp = tmp;
     g->tmpPixels = tmpPixels;
+    g->tunRingRows = P.nTun ? ckd::tunnel_ring_rows(P.tunMaxE) : 0;
     for (unsigned i = 0; i < n; ++i)
         g->shapes.push_back({requests[i].effect, requests[i].W, requests[i].H});
     if (ckd::dev_alloc(&g->tables, P.host.size()) != hipSuccess) {
@@ -428,6 +435,10 @@ ckd_status ckd_graph_update(ckd_graph *g, const ckd_request *requests, unsigned 
     make_plan(g->tuning, requests, n, g->tmp, P);
     if (P.host.size() != g->tableBytes)
         return ckd::fail(CKD_EINVAL, "ckd_graph_update: the requests' tables do not fit the graph");
+    if (g->tunRingRows && 2 * P.tunMaxE + 1 > g->tunRingRows)
+        return ckd::fail(CKD_EINVAL,
+                         "ckd_graph_update: radius %u exceeds the captured batch's largest (%u)",
+                         P.tunMaxE, (g->tunRingRows - 1) / 2);
     // a replay still in flight reads the tables: the copy waits for it, and
     // the next replay is queued after the copy
     ckd_ctx *ctx = g->ctx;

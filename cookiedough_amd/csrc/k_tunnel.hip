@@ -3,12 +3,14 @@
 // and BlitAdd32 of the blur onto the remap (code/deprecated/boxblur.cpp,
 // code/util.cpp:754-773), in two fused kernels:
 //
-//   T1  one block per row: interpolate the row's grid corners once per
-//       8-pixel block, remap the row into dst and (spread into 16-bit
-//       fields) into LDS, prefix-sum it there in place, write the
+//   T1  one 128-thread block per row: interpolate the row's grid corners
+//       once per 8-pixel block, remap the row into dst and (spread into
+//       16-bit fields) into LDS, prefix-sum it there in place, write the
 //       horizontal blur to tmp
-//   T2  one wave per 64 columns x 32 rows: vertical blur of tmp and the
-//       saturating add into dst
+//   T2  one wave per 64 columns x one segment of rows: vertical blur of tmp
+//       as a running window sum and the saturating add into dst; the
+//       window's rows stay in an LDS ring (radius <= 32), so each row of
+//       tmp is read once per segment
 //
 // For an odd span the window of x is [x-e, x+e] clipped to the line and the
 // reference's divisor is WeightToDiv(16*count) of that window's pixel count
@@ -22,7 +24,9 @@
 
 namespace {
 
-constexpr int kT1Block = 256;
+// threads per row of T1: 128 gives a 1280-pixel row 10 pixels per thread,
+// which spreads the per-thread fixed work (grid set-up, scan) thinner
+constexpr int kT1Block = 128;
 
 // ckd_b07_count_div of every window count 0..255 (radius <= 127), evaluated
 // by the compiler: blocks copy it instead of dividing
@@ -41,11 +45,31 @@ __constant__ const B07DivTable kB07Div = B07DivTable();
 typedef __attribute__((address_space(1))) const uint32_t global_u32c;
 typedef __attribute__((address_space(1))) uint32_t global_u32;
 
-CKD_DEV const TunJob &pick(const TunJob *jobs, int nJobs, const TunJob &single, uint32_t unit,
-                           uint32_t TunJob::*base)
+// A block's job is the same for all its lanes.  readfirstlane says so to the
+// compiler, which then keeps the job's fields, and the row arithmetic built
+// on them, in scalar registers: a row's address is a scalar base plus the
+// lane's 32-bit offset.
+CKD_DEV uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+template <typename T> CKD_DEV T *uniform(T *p)
 {
-    if (!jobs) return single;
-    return jobs[ckd_find_job(jobs, nJobs, unit, base)];
+    const uint64_t a = (uint64_t)(uintptr_t)p;
+    return (T *)(uintptr_t)(((uint64_t)uniform((uint32_t)(a >> 32)) << 32) | uniform((uint32_t)a));
+}
+
+CKD_DEV TunJob pick(const TunJob *jobs, int nJobs, const TunJob &single, uint32_t unit,
+                    uint32_t TunJob::*base)
+{
+    TunJob J = jobs ? jobs[ckd_find_job(jobs, nJobs, unit, base)] : single;
+    J.dst = uniform(J.dst);
+    J.tmp = uniform(J.tmp);
+    J.grid = uniform(J.grid);
+    J.W = uniform(J.W);
+    J.H = uniform(J.H);
+    J.rowBase = uniform(J.rowBase);
+    J.vBase = uniform(J.vBase);
+    J.edge = uniform(J.edge);
+    J.vBlocksX = uniform(J.vBlocksX);
+    return J;
 }
 
 __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict__ jobs, int nJobs,
@@ -53,12 +77,13 @@ __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict_
                                                        const uint32_t *__restrict__ tex,
                                                        unsigned texLog2)
 {
+    const uint32_t row = blockIdx.x;
     extern __shared__ uint4 lds[];
-    __shared__ uint32_t D[256];
+    __shared__ uint16_t D[256];
     __shared__ uint64_t waveTot[kT1Block / 64];
-    const TunJob &J = pick(jobs, nJobs, single, blockIdx.x, &TunJob::rowBase);
+    const TunJob J = pick(jobs, nJobs, single, row, &TunJob::rowBase);
     const int W = (int)J.W;
-    const unsigned y = blockIdx.x - J.rowBase;
+    const unsigned y = row - J.rowBase;
     const int NB = (W + 7) / 8;
     // per 8-pixel block b of the row: E[b] = (8*U_b, 8*V_b, U_b+1 - U_b,
     // V_b+1 - V_b), where (U_e, V_e) is grid corner column e interpolated
@@ -69,7 +94,7 @@ __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict_
     // in place) the prefix sum of pixels 0..x; P[0] = 0
     uint64_t *P = reinterpret_cast<uint64_t *>(lds + NB);
 
-    D[threadIdx.x] = kB07Div.d[threadIdx.x];
+    for (int i = threadIdx.x; i < 256; i += kT1Block) D[i] = kB07Div.d[i];
     const unsigned GW = (unsigned)NB + 1;
     const uint32_t j = y & 7;
     const global_u32c *g0 = (const global_u32c *)(J.grid) + 2 * (size_t)(y >> 3) * GW;
@@ -107,7 +132,7 @@ __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict_
             const uint4 e = E[x >> 3];
             const uint32_t iu = kNarrow ? (uint32_t)__mul24((int)i, (int)e.z) : i * e.z;
             const uint32_t iv = kNarrow ? (uint32_t)__mul24((int)i, (int)e.w) : i * e.w;
-            return ckd::grid_sample(tex, texLog2, (int)(e.x + iu) >> 6, (int)(e.y + iv) >> 6);
+            return ckd::grid_sample_q14(tex, texLog2, e.x + iu, e.y + iv);
         };
         int x0 = threadIdx.x;
         for (; x0 + 3 * kT1Block < W; x0 += 4 * kT1Block) {
@@ -165,44 +190,191 @@ __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict_
 __global__ __launch_bounds__(64) void k_tunnel_v(const TunJob *__restrict__ jobs, int nJobs,
                                                  const TunJob single, int kT2Seg)
 {
-    __shared__ uint32_t D[256];
-    const TunJob &J = pick(jobs, nJobs, single, blockIdx.x, &TunJob::vBase);
+    const uint32_t unit = blockIdx.x;
+    __shared__ uint16_t D[256];
+    const TunJob J = pick(jobs, nJobs, single, unit, &TunJob::vBase);
     for (int i = threadIdx.x; i < 256; i += 64) D[i] = kB07Div.d[i];
     __syncthreads();
-    const uint32_t local = blockIdx.x - J.vBase;
+    const uint32_t local = unit - J.vBase;
     const int x = (int)((local % J.vBlocksX) * 64 + threadIdx.x);
     const int y0 = (int)((local / J.vBlocksX) * kT2Seg);
     const int W = (int)J.W, H = (int)J.H, e = (int)J.edge;
-    if (x >= W) return;
-    // tmp and dst are distinct images: restrict lets the loads of four rows
-    // go out before any of their stores
-    const global_u32 *__restrict__ col = (const global_u32 *)(J.tmp + x);
-    global_u32 *__restrict__ dcol = (global_u32 *)(J.dst + x);
+    // lanes past the right edge take the last column: they compute and store
+    // exactly what its own lane does, so every lane runs the same
+    // (wave-uniform, scalar) row loop without a branch around the stores
+    const int xc = min(x, W - 1);
     const int y1 = min(y0 + kT2Seg, H);
+    // rows are a wave-uniform base plus the lane's column: the row
+    // arithmetic stays scalar, each access one 32-bit lane offset
+    const global_u32c *__restrict__ tmp = (const global_u32c *)J.tmp;
+    global_u32 *__restrict__ dst = (global_u32 *)J.dst;
+    auto T = [&](int r) { return (tmp + (size_t)r * W)[xc]; };
+    auto Drow = [&](int r) { return dst + (size_t)r * W; };
+
     uint64_t F = 0;
-    for (int r = max(0, y0 - e); r <= min(H - 1, y0 + e); ++r) F += ckd_spread(col[(size_t)r * W]);
-    for (int y = y0; y < y1; y += 4) {
-        // rows past the segment are read at its last row and dropped
+    for (int r = max(0, y0 - e); r <= min(H - 1, y0 + e); ++r) F += ckd_spread(T(r));
+    // a row whose window is clipped by the image: its own count, and the
+    // rows entering and leaving only where they exist
+    auto edge_row = [&](int y) {
+        const int lo = max(y - e, 0), hi = min(y + e, H - 1);
+        global_u32 *d = Drow(y);
+        d[xc] = ckd_adds_u8x4(d[xc], ckd_b07_div_pack(F, D[hi - lo + 1]));
+        if (y + e + 1 <= H - 1) F += ckd_spread(T(y + e + 1));
+        if (y - e >= 0) F -= ckd_spread(T(y - e));
+    };
+    // rows y in [e, H - 2 - e]: full window of 2e+1 rows, row y+e+1 enters
+    // and row y-e leaves
+    const int iBeg = max(y0, e), iEnd = min(y1, H - 1 - e);
+    int y = y0;
+    for (; y < min(iBeg, y1); ++y) edge_row(y);
+    const uint32_t div = D[2 * e + 1];
+    for (; y + 4 <= iEnd; y += 4) {
         uint32_t dv[4], av[4], sv[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const int yy = min(y + j, y1 - 1);
-            dv[j] = dcol[(size_t)yy * W];
-            av[j] = col[(size_t)min(yy + e + 1, H - 1) * W];
-            sv[j] = col[(size_t)max(yy - e, 0) * W];
+            dv[j] = Drow(y + j)[xc];
+            av[j] = T(y + j + e + 1);
+            sv[j] = T(y + j - e);
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const int yy = y + j;
-            if (yy < y1) {
-                const int lo = max(yy - e, 0), hi = min(yy + e, H - 1);
-                const uint32_t b = ckd_b07_div_pack(F, D[hi - lo + 1]);
-                dcol[(size_t)yy * W] = ckd_adds_u8x4(dv[j], b);
-                if (yy + e + 1 <= H - 1) F += ckd_spread(av[j]);
-                if (yy - e >= 0) F -= ckd_spread(sv[j]);
-            }
+            Drow(y + j)[xc] = ckd_adds_u8x4(dv[j], ckd_b07_div_pack(F, div));
+            F += ckd_spread(av[j]) - ckd_spread(sv[j]);
         }
     }
+    for (; y < iEnd; ++y) {
+        global_u32 *d = Drow(y);
+        d[xc] = ckd_adds_u8x4(d[xc], ckd_b07_div_pack(F, div));
+        F += ckd_spread(T(y + e + 1)) - ckd_spread(T(y - e));
+    }
+    for (; y < y1; ++y) edge_row(y);
+}
+
+// T2 with the window in LDS: each wave keeps the 2e+1 rows of its 64
+// columns that the window spans, row r in slot r mod (2e+1), so the row
+// leaving the window comes from LDS instead of a second read of tmp (at
+// 2e+1 rows' distance it has usually left L2).  The row entering the
+// window takes the slot of the row leaving it.  U rows per group: their
+// loads of dst and tmp are in flight together.
+constexpr int kRingMaxEdge = 32;
+template <int U>
+__global__ __launch_bounds__(64) void k_tunnel_v_ring(const TunJob *__restrict__ jobs, int nJobs,
+                                                      const TunJob single, int kT2Seg)
+{
+    const uint32_t unit = blockIdx.x;
+    extern __shared__ uint32_t ring[]; // (2 * maxEdge + 1) slots of 64 lanes
+    __shared__ uint16_t D[256];
+    const TunJob J = pick(jobs, nJobs, single, unit, &TunJob::vBase);
+    for (int i = threadIdx.x; i < 256; i += 64) D[i] = kB07Div.d[i];
+    __syncthreads();
+    const uint32_t local = unit - J.vBase;
+    const int x = (int)((local % J.vBlocksX) * 64 + threadIdx.x);
+    const int y0 = (int)((local / J.vBlocksX) * kT2Seg);
+    const int W = (int)J.W, H = (int)J.H, e = (int)J.edge;
+    const int xc = min(x, W - 1);
+    const int y1 = min(y0 + kT2Seg, H);
+    const int R = 2 * e + 1;
+    const global_u32c *__restrict__ tmp = (const global_u32c *)J.tmp;
+    global_u32 *__restrict__ dst = (global_u32 *)J.dst;
+    auto T = [&](int r) { return (tmp + (size_t)r * W)[xc]; };
+    auto Drow = [&](int r) { return dst + (size_t)r * W; };
+    uint32_t *mine = ring + threadIdx.x;
+
+    // the window of row y0, rows clipped to the image, into F and the ring
+    uint64_t F = 0;
+    {
+        const int r0 = max(0, y0 - e), r1 = min(H - 1, y0 + e);
+        int s = r0 % R;
+        for (int r = r0; r <= r1; ++r) {
+            const uint32_t v = T(r);
+            F += ckd_spread(v);
+            mine[s * 64] = v;
+            s = s + 1 == R ? 0 : s + 1;
+        }
+    }
+    // slot of row y - e (taken mod R also where y - e < 0): the leaving row
+    // at row y, and the slot the entering row y + e + 1 takes
+    int sl = ((y0 - e) % R + R) % R;
+    auto advance = [&]() { sl = sl + 1 == R ? 0 : sl + 1; };
+    auto edge_row = [&](int y) {
+        const int lo = max(y - e, 0), hi = min(y + e, H - 1);
+        global_u32 *d = Drow(y);
+        d[xc] = ckd_adds_u8x4(d[xc], ckd_b07_div_pack(F, D[hi - lo + 1]));
+        if (y - e >= 0) F -= ckd_spread(mine[sl * 64]);
+        if (y + e + 1 <= H - 1) {
+            const uint32_t a = T(y + e + 1);
+            F += ckd_spread(a);
+            mine[sl * 64] = a;
+        }
+        advance();
+    };
+    const int iBeg = max(y0, e), iEnd = min(y1, H - 1 - e);
+    int y = y0;
+    for (; y < min(iBeg, y1); ++y) edge_row(y);
+    const uint32_t div = D[2 * e + 1];
+    // groups of U rows, the next group's loads in flight while this one is
+    // blended; with R >= U the group's U leaving rows are distinct slots and
+    // are read before any entering row is written
+    if (y + U <= iEnd) {
+        uint32_t dv[U], av[U];
+        auto load = [&](int g, uint32_t *d, uint32_t *a) {
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                d[j] = Drow(g + j)[xc];
+                a[j] = T(g + j + e + 1);
+            }
+        };
+        auto blend = [&](int g, const uint32_t *d, const uint32_t *a) {
+            uint32_t leaving[U];
+            if (R >= U) {
+                int t = sl;
+#pragma unroll
+                for (int j = 0; j < U; ++j) {
+                    leaving[j] = mine[t * 64];
+                    t = t + 1 == R ? 0 : t + 1;
+                }
+#pragma unroll
+                for (int j = 0; j < U; ++j) {
+                    mine[sl * 64] = a[j];
+                    advance();
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < U; ++j) {
+                    leaving[j] = mine[sl * 64];
+                    mine[sl * 64] = a[j];
+                    advance();
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                Drow(g + j)[xc] = ckd_adds_u8x4(d[j], ckd_b07_div_pack(F, div));
+                F += ckd_spread(a[j]) - ckd_spread(leaving[j]);
+            }
+        };
+        load(y, dv, av);
+        for (; y + 2 * U <= iEnd; y += U) {
+            uint32_t dn[U], an[U];
+            load(y + U, dn, an);
+            blend(y, dv, av);
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                dv[j] = dn[j];
+                av[j] = an[j];
+            }
+        }
+        blend(y, dv, av);
+        y += U;
+    }
+    for (; y < iEnd; ++y) {
+        global_u32 *d = Drow(y);
+        d[xc] = ckd_adds_u8x4(d[xc], ckd_b07_div_pack(F, div));
+        const uint32_t a = T(y + e + 1), leaving = mine[sl * 64];
+        mine[sl * 64] = a;
+        F += ckd_spread(a) - ckd_spread(leaving);
+        advance();
+    }
+    for (; y < y1; ++y) edge_row(y);
 }
 
 } // namespace
@@ -223,9 +395,14 @@ void tunnel_job_geometry(TunJob &J, uint32_t *dst, uint32_t *tmp, const int32_t 
 
 uint32_t tunnel_vblocks(unsigned W, unsigned H, int seg) { return ((W + 63) / 64) * ((H + seg - 1) / seg); }
 
+unsigned tunnel_ring_rows(unsigned maxEdge)
+{
+    return maxEdge <= (unsigned)kRingMaxEdge ? 2 * maxEdge + 1 : 0;
+}
+
 ckd_status launch_tunnel_jobs(ckd_ctx *ctx, const TunJob *jobs, int nJobs, const TunJob &single,
                               uint32_t totalRows, uint32_t totalVBlocks, unsigned maxW,
-                              const uint32_t *tex, unsigned texLog2, bool timed)
+                              unsigned maxEdge, const uint32_t *tex, unsigned texLog2, bool timed)
 {
     if (totalRows == 0) return CKD_OK;
     // the row's per-block interpolation terms, its spread pixels / prefix sums
@@ -237,7 +414,14 @@ ckd_status launch_tunnel_jobs(ckd_ctx *ctx, const TunJob *jobs, int nJobs, const
     ckd_status s = check_launch("tunnel remap+H");
     if (s != CKD_OK) return s;
     KernelTimer kt(ctx, timed ? CKD_KT_TUNNEL_V : -1);
-    k_tunnel_v<<<totalVBlocks, 64, 0, ctx->stream>>>(jobs, nJobs, single, ctx->tuning.tunnelVSeg);
+    // the window ring holds up to 2 * kRingMaxEdge + 1 rows (16.6 KB per
+    // wave); wider windows read their leaving rows again from memory
+    const unsigned ringRows = tunnel_ring_rows(maxEdge);
+    if (ringRows)
+        k_tunnel_v_ring<8><<<totalVBlocks, 64, (size_t)ringRows * 64 * sizeof(uint32_t), ctx->stream>>>(
+            jobs, nJobs, single, ctx->tuning.tunnelVSeg);
+    else
+        k_tunnel_v<<<totalVBlocks, 64, 0, ctx->stream>>>(jobs, nJobs, single, ctx->tuning.tunnelVSeg);
     return check_launch("tunnel V+add");
 }
 

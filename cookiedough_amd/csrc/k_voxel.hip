@@ -36,24 +36,26 @@ struct LsLaunch {
 };
 static_assert(sizeof(LsLaunch) + 64 <= 4096, "k_landscape's arguments fit 4 KiB");
 
-typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
-
 // bsamp32_16 (code/bilinear.h:58-83) followed by _mm_subs_epu16 of the fog
 // colour (code/landscape.cpp:83): the interpolation runs on two channels per
-// register (B,R and G,A in 16-bit fields, exact as in ckd_bilerp32) and the
+// register (B,R and G,A in 16-bit lanes, exact as in ckd_bilerp32) and the
 // saturating subtract is one packed u16 subtract per pair
 CKD_DEV uint32_t ls_bilerp_fog(uint32_t p0, uint32_t p1, uint32_t p2, uint32_t p3, uint32_t fu,
                                uint32_t fv, uint32_t fogBR, uint32_t fogGA)
 {
     const uint32_t M = 0x00ff00ffu;
-    const uint32_t br = ckd_lerp2(ckd_lerp2(p0 & M, p1 & M, fu), ckd_lerp2(p2 & M, p3 & M, fu), fv);
-    const uint32_t ga = ckd_lerp2(ckd_lerp2((p0 >> 8) & M, (p1 >> 8) & M, fu),
-                                  ckd_lerp2((p2 >> 8) & M, (p3 >> 8) & M, fu), fv);
-    const u16x2 sbr = __builtin_elementwise_sub_sat(__builtin_bit_cast(u16x2, br),
-                                                    __builtin_bit_cast(u16x2, fogBR));
-    const u16x2 sga = __builtin_elementwise_sub_sat(__builtin_bit_cast(u16x2, ga),
-                                                    __builtin_bit_cast(u16x2, fogGA));
-    return __builtin_bit_cast(uint32_t, sbr) | (__builtin_bit_cast(uint32_t, sga) << 8);
+    const ckd_u16x2 f = ckd_splat16(fu), g = ckd_splat16(fv);
+    const ckd_u16x2 nf = (unsigned short)256 - f, ng = (unsigned short)256 - g;
+    const ckd_u16x2 br = ckd_lerp2x(ckd_lerp2x(ckd_u16x2_of(p0 & M), ckd_u16x2_of(p1 & M), nf, f),
+                                    ckd_lerp2x(ckd_u16x2_of(p2 & M), ckd_u16x2_of(p3 & M), nf, f),
+                                    ng, g);
+    const ckd_u16x2 ga =
+        ckd_lerp2x(ckd_lerp2x(ckd_u16x2_of(ckd_odd_bytes(p0)), ckd_u16x2_of(ckd_odd_bytes(p1)), nf, f),
+                   ckd_lerp2x(ckd_u16x2_of(ckd_odd_bytes(p2)), ckd_u16x2_of(ckd_odd_bytes(p3)), nf, f),
+                   ng, g);
+    const ckd_u16x2 sbr = __builtin_elementwise_sub_sat(br, ckd_u16x2_of(fogBR));
+    const ckd_u16x2 sga = __builtin_elementwise_sub_sat(ga, ckd_u16x2_of(fogGA));
+    return ckd_u32_of(sbr) | (ckd_u32_of(sga) << 8);
 }
 
 // exact floor(num/den) for num < 2^24, 0 < den < 2^24: a reciprocal

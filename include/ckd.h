@@ -335,8 +335,10 @@ CKD_API ckd_status ckd_graph_launch(ckd_graph *graph);
  * counterpart; the next frames of the animation that code/main.cpp:314-351
  * renders frame by frame): the requests must have the captured effects and
  * sizes, in the captured order; cameras, times, grids, radii and dst
- * pointers may change.  The scene, shading mode, texture and scratch stay the captured
- * ones.  The graph's job tables are rewritten in place (after any replay
+ * pointers may change.  A tunnel radius may not exceed the largest of the
+ * captured batch where that largest is at most 32 (the vertical blur's
+ * window rows are sized for it); CKD_EINVAL otherwise.  The scene, shading
+ * mode, texture and scratch stay the captured ones.  The graph's job tables are rewritten in place (after any replay
  * still queued has finished with them), so the graph itself is not
  * re-instantiated. */
 CKD_API ckd_status ckd_graph_update(ckd_graph *graph, const ckd_request *requests,

@@ -587,6 +587,14 @@ def test_batch_graph_update_matches_eager(ctx, lmaps):
         bad[0] = wl.BatchItem(bad[0].effect, bad[0].W + 8, bad[0].H, bad[0].t, bad[0].radius)
         arr = requests(bad, out_e)
         assert ctx.lib.ckd_graph_update(graph.h, arr, len(arr)) == CKD_EINVAL
+        # the vertical blur's window rows were sized for the captured radii
+        # (all <= 32): a wider window is refused
+        rmax = max(i.radius for i in base if i.effect == CKD_FX_TUNNEL)
+        assert rmax <= 32
+        wide = [wl.BatchItem(i.effect, i.W, i.H, i.t, rmax + 1 if i.effect == CKD_FX_TUNNEL else i.radius)
+                for i in base]
+        arr = requests(wide, out_e)
+        assert ctx.lib.ckd_graph_update(graph.h, arr, len(arr)) == CKD_EINVAL
     finally:
         graph.close()
 
