@@ -24,6 +24,21 @@ CKD_DEV uint32_t grid_sample_q14(const uint32_t *__restrict__ tex, unsigned texL
                         __builtin_amdgcn_ubfe(X, 6, 8), __builtin_amdgcn_ubfe(Y, 6, 8));
 }
 
+// grid_sample_q14 from the texture's texel quads: quads[v * 2^texLog2 + u]
+// holds texels (u,v), (u+1,v), (u,v+1), (u+1,v+1), indices wrapped, so the
+// four texels of a sample are one 16-byte load.
+CKD_DEV uint32_t grid_sample_quad(const uint4 *__restrict__ quads, unsigned texLog2, uint32_t X,
+                                  uint32_t Y)
+{
+    const uint32_t u0 = __builtin_amdgcn_ubfe(X, 14, texLog2);
+    const uint32_t v0 = __builtin_amdgcn_ubfe(Y, 14, texLog2);
+    // byte offset below 2^32 (texLog2 <= 13)
+    const uint32_t k = ((v0 << texLog2) + u0) << 4;
+    const uint4 q = *(const uint4 *)((const char *)quads + k);
+    return ckd_bilerp32(q.x, q.y, q.z, q.w, __builtin_amdgcn_ubfe(X, 6, 8),
+                        __builtin_amdgcn_ubfe(Y, 6, 8));
+}
+
 // The bilinear sample at the interpolated (U, V), both 24:8 fixed point
 // (the bits grid_sample_q14 reads are U's bits [0, 8 + texLog2)).
 CKD_DEV uint32_t grid_sample(const uint32_t *__restrict__ tex, unsigned texLog2, int U, int V)

@@ -27,6 +27,10 @@ struct ckd_ctx {
     // into the second through tunEv
     hipStream_t aux2 = nullptr;
     hipEvent_t tunEv = nullptr;
+    // texel quads of the batched tunnel's texture (16 bytes per texel),
+    // grown to the largest texture a batch has used
+    uint4 *texQuads = nullptr;
+    size_t texQuadBytes = 0;
 
     // interpolated cosine LUT, 2049 floats (CalculateCosLUT), on the
     // device and the host copy the per-frame set-up reads

@@ -28,7 +28,9 @@ uint32_t tunnel_vblocks(unsigned W, unsigned H, int seg);
 unsigned tunnel_ring_rows(unsigned maxEdge);
 ckd_status launch_tunnel_jobs(ckd_ctx *ctx, const TunJob *jobs, int nJobs, const TunJob &single,
                               uint32_t totalRows, uint32_t totalVBlocks, unsigned maxW,
-                              unsigned maxEdge, const uint32_t *tex, unsigned texLog2, bool timed);
+                              unsigned maxEdge, const uint32_t *tex, uint4 *quads, unsigned texLog2,
+                              bool timed);
+size_t tex_quad_bytes(unsigned texLog2);
 } // namespace ckd
 
 // per request, what fixes the captured launch geometry: the effect and the
@@ -54,6 +56,7 @@ struct ckd_graph {
     // window rows of the captured vertical-blur ring (0: no ring): the
     // largest radius a replay can take
     unsigned tunRingRows = 0;
+    uint4 *quads = nullptr;
 };
 
 namespace {
@@ -167,7 +170,7 @@ void make_plan(const ckd_tuning &T, const ckd_request *requests, unsigned n, uin
 
 // the launch sequence over tables staged at `dev`
 ckd_status launch_plan(ckd_ctx *ctx, const Plan &P, unsigned char *dev, const ckd_sdf_scene *scene,
-                       int shading, const uint32_t *tex, unsigned texLog2,
+                       int shading, const uint32_t *tex, uint4 *quads, unsigned texLog2,
                        unsigned long long *stepCount)
 {
     ckd_status s = CKD_OK;
@@ -191,7 +194,7 @@ ckd_status launch_plan(ckd_ctx *ctx, const Plan &P, unsigned char *dev, const ck
         TunJob none;
         memset(&none, 0, sizeof(none));
         s = ckd::launch_tunnel_jobs(ctx, reinterpret_cast<const TunJob *>(dev + P.oTun), P.nTun, none,
-                                    P.tunRows, P.tunV, P.tunMaxW, P.tunMaxE, tex, texLog2, true);
+                                    P.tunRows, P.tunV, P.tunMaxW, P.tunMaxE, tex, quads, texLog2, true);
         if (s != CKD_OK) return s;
         CKD_HIP(hipEventRecord(ctx->tunEv, ctx->aux2));
     }
@@ -206,7 +209,7 @@ ckd_status launch_plan(ckd_ctx *ctx, const Plan &P, unsigned char *dev, const ck
             TunJob none;
             memset(&none, 0, sizeof(none));
             s = ckd::launch_tunnel_jobs(ctx, reinterpret_cast<const TunJob *>(dev + P.oTun), P.nTun,
-                                        none, P.tunRows, P.tunV, P.tunMaxW, P.tunMaxE, tex, texLog2, true);
+                                        none, P.tunRows, P.tunV, P.tunMaxW, P.tunMaxE, tex, quads, texLog2, true);
             if (s != CKD_OK) return s;
         }
         if (split) CKD_HIP(hipStreamWaitEvent(ctx->aux, ctx->tunEv, 0));
@@ -234,7 +237,25 @@ ckd_status run_eager(ckd_ctx *ctx, const ckd_request *requests, unsigned n,
     void *dev = nullptr;
     ckd_status s = ckd::stage(ctx, P.host.data(), P.host.size(), &dev);
     if (s != CKD_OK) return s;
-    return launch_plan(ctx, P, static_cast<unsigned char *>(dev), scene, shading, tex, texLog2,
+    // the texture's texel quads, in a buffer of the context's that grows
+    // to the largest texture seen (freed only after the work queued on it)
+    uint4 *quads = nullptr;
+    const size_t qb = P.nTun ? ckd::tex_quad_bytes(texLog2) : 0;
+    if (qb) {
+        if (ctx->texQuadBytes < qb) {
+            if (ctx->texQuads) {
+                CKD_HIP(hipStreamSynchronize(ctx->stream));
+                (void)hipFree(ctx->texQuads);
+                ctx->texQuads = nullptr;
+                ctx->texQuadBytes = 0;
+            }
+            if (ckd::dev_alloc(&ctx->texQuads, qb) != hipSuccess)
+                return ckd::fail(CKD_ENOMEM, "ckd_render_batch: texel quad allocation failed");
+            ctx->texQuadBytes = qb;
+        }
+        quads = ctx->texQuads;
+    }
+    return launch_plan(ctx, P, static_cast<unsigned char *>(dev), scene, shading, tex, quads, texLog2,
                        stepCount);
 }
 
@@ -371,6 +392,12 @@ ckd_status ckd_batch_graph_create(ckd_ctx *ctx, const ckd_request *requests, uns
         ckd_graph_destroy(g);
         return ckd::fail(CKD_EDEVICE, "ckd_batch_graph_create: table upload failed");
     }
+    // the graph's own texel quads: rebuilt from the texture by every replay
+    const size_t qb = P.nTun ? ckd::tex_quad_bytes(texLog2) : 0;
+    if (qb && ckd::dev_alloc(&g->quads, qb) != hipSuccess) {
+        ckd_graph_destroy(g);
+        return ckd::fail(CKD_ENOMEM, "ckd_batch_graph_create: texel quad allocation failed");
+    }
     // capture on a private stream so nothing else on the context stream is
     // recorded; the kernel-timing events are not part of the graph
     hipStream_t cap = nullptr;
@@ -387,7 +414,7 @@ ckd_status ckd_batch_graph_create(ckd_ctx *ctx, const ckd_request *requests, uns
         unsigned char *dev = static_cast<unsigned char *>(g->tables);
         // the queue counter is reset by a kernel node on every replay
         k_reset_counter<<<1, 1, 0, cap>>>(reinterpret_cast<uint32_t *>(dev));
-        s = launch_plan(ctx, P, dev, scene, shading, tex, texLog2, stepCount);
+        s = launch_plan(ctx, P, dev, scene, shading, tex, g->quads, texLog2, stepCount);
         const hipError_t e2 = hipStreamEndCapture(cap, &g->graph);
         if (s == CKD_OK && e2 != hipSuccess)
             s = ckd::fail(CKD_EDEVICE, "ckd_batch_graph_create: capture failed: %s",
@@ -467,6 +494,7 @@ void ckd_graph_destroy(ckd_graph *g)
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
     if (g->tables) (void)hipFree(g->tables);
+    if (g->quads) (void)hipFree(g->quads);
     delete g;
 }
 

@@ -100,7 +100,9 @@ void tunnel_job_geometry(TunJob &J, uint32_t *dst, uint32_t *tmp, const int32_t 
 uint32_t tunnel_vblocks(unsigned W, unsigned H, int seg);
 ckd_status launch_tunnel_jobs(ckd_ctx *ctx, const TunJob *jobs, int nJobs, const TunJob &single,
                               uint32_t totalRows, uint32_t totalVBlocks, unsigned maxW,
-                              unsigned maxEdge, const uint32_t *tex, unsigned texLog2, bool timed);
+                              unsigned maxEdge, const uint32_t *tex, uint4 *quads, unsigned texLog2,
+                              bool timed);
+size_t tex_quad_bytes(unsigned texLog2);
 ckd_status blur07_hv_oop(ckd_ctx *ctx, uint32_t *tmp, uint32_t *dst, const uint32_t *src,
                          unsigned W, unsigned H, float strength);
 
@@ -192,7 +194,7 @@ ckd_status ckd_tunnel_frame(ckd_ctx *ctx, uint32_t *dst, uint32_t *tmp, unsigned
     J.rowBase = 0;
     J.vBase = 0;
     return ckd::launch_tunnel_jobs(ctx, nullptr, 1, J, H, ckd::tunnel_vblocks(W, H, ctx->tuning.tunnelVSeg),
-                                   W, (unsigned)radius, tex, texLog2, true);
+                                   W, (unsigned)radius, tex, nullptr, texLog2, true);
 }
 
 // ---- the tunnel frame without allocation ---------------------------------
@@ -267,7 +269,7 @@ ckd_status ckd_tunnel_frame_noalloc(ckd_ctx *ctx, uint32_t *dst, unsigned W, uns
     J.rowBase = 0;
     J.vBase = 0;
     return ckd::launch_tunnel_jobs(ctx, nullptr, 1, J, H, ckd::tunnel_vblocks(W, H, ctx->tuning.tunnelVSeg),
-                                   W, (unsigned)radius, tex, texLog2, false);
+                                   W, (unsigned)radius, tex, nullptr, texLog2, false);
 }
 
 } // extern "C"

@@ -27,6 +27,8 @@ namespace {
 // threads per row of T1: 128 gives a 1280-pixel row 10 pixels per thread,
 // which spreads the per-thread fixed work (grid set-up, scan) thinner
 constexpr int kT1Block = 128;
+// texel quads (16 bytes per texel) for textures up to 2^11 x 2^11 (64 MB)
+constexpr int kTexQuadMaxLog2 = 11;
 
 // ckd_b07_count_div of every window count 0..255 (radius <= 127), evaluated
 // by the compiler: blocks copy it instead of dividing
@@ -72,9 +74,23 @@ CKD_DEV TunJob pick(const TunJob *jobs, int nJobs, const TunJob &single, uint32_
     return J;
 }
 
+// the texel quads of a 2^L x 2^L texture (grid_sample_quad)
+__global__ __launch_bounds__(256) void k_tex_quads(const uint32_t *__restrict__ tex, unsigned L,
+                                                   uint4 *__restrict__ quads)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= (1u << (2 * L))) return;
+    const uint32_t mask = (1u << L) - 1, u = i & mask, v = i >> L;
+    const uint32_t u1 = (u + 1) & mask, v0 = v << L, v1 = ((v + 1) & mask) << L;
+    quads[i] = make_uint4(tex[v0 + u], tex[v0 + u1], tex[v1 + u], tex[v1 + u1]);
+}
+
+// kQuads: the texture's texels come from its quads (one load per pixel)
+template <bool kQuads>
 __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict__ jobs, int nJobs,
                                                        const TunJob single,
                                                        const uint32_t *__restrict__ tex,
+                                                       const uint4 *__restrict__ quads,
                                                        unsigned texLog2)
 {
     const uint32_t row = blockIdx.x;
@@ -132,7 +148,10 @@ __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict_
             const uint4 e = E[x >> 3];
             const uint32_t iu = kNarrow ? (uint32_t)__mul24((int)i, (int)e.z) : i * e.z;
             const uint32_t iv = kNarrow ? (uint32_t)__mul24((int)i, (int)e.w) : i * e.w;
-            return ckd::grid_sample_q14(tex, texLog2, e.x + iu, e.y + iv);
+            if constexpr (kQuads)
+                return ckd::grid_sample_quad(quads, texLog2, e.x + iu, e.y + iv);
+            else
+                return ckd::grid_sample_q14(tex, texLog2, e.x + iu, e.y + iv);
         };
         int x0 = threadIdx.x;
         for (; x0 + 3 * kT1Block < W; x0 += 4 * kT1Block) {
@@ -400,16 +419,32 @@ unsigned tunnel_ring_rows(unsigned maxEdge)
     return maxEdge <= (unsigned)kRingMaxEdge ? 2 * maxEdge + 1 : 0;
 }
 
+size_t tex_quad_bytes(unsigned texLog2)
+{
+    return texLog2 <= (unsigned)kTexQuadMaxLog2 ? sizeof(uint4) << (2 * texLog2) : 0;
+}
+
 ckd_status launch_tunnel_jobs(ckd_ctx *ctx, const TunJob *jobs, int nJobs, const TunJob &single,
                               uint32_t totalRows, uint32_t totalVBlocks, unsigned maxW,
-                              unsigned maxEdge, const uint32_t *tex, unsigned texLog2, bool timed)
+                              unsigned maxEdge, const uint32_t *tex, uint4 *quads, unsigned texLog2,
+                              bool timed)
 {
     if (totalRows == 0) return CKD_OK;
     // the row's per-block interpolation terms, its spread pixels / prefix sums
     const size_t lds = (size_t)((maxW + 7) / 8) * sizeof(uint4) + (size_t)(maxW + 1) * sizeof(uint64_t);
     {
         KernelTimer kt(ctx, timed ? CKD_KT_TUNNEL_H : -1);
-        k_tunnel_h<<<totalRows, kT1Block, lds, ctx->stream>>>(jobs, nJobs, single, tex, texLog2);
+        if (quads) {
+            // rebuilt on every call (and graph replay): the texture's
+            // contents may have changed since the last one
+            k_tex_quads<<<ckd_div_up((size_t)1 << (2 * texLog2), 256), 256, 0, ctx->stream>>>(tex, texLog2,
+                                                                                             quads);
+            k_tunnel_h<true><<<totalRows, kT1Block, lds, ctx->stream>>>(jobs, nJobs, single, tex, quads,
+                                                                        texLog2);
+        } else {
+            k_tunnel_h<false><<<totalRows, kT1Block, lds, ctx->stream>>>(jobs, nJobs, single, tex, nullptr,
+                                                                         texLog2);
+        }
     }
     ckd_status s = check_launch("tunnel remap+H");
     if (s != CKD_OK) return s;

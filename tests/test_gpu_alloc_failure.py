@@ -96,3 +96,33 @@ def test_lazy_scratch_paths_recover(lib):
         c.sync()
         assert np.array_equal(out.numpy(), first)
         g.close()
+
+        # a tunnel request: the texture's texel quads (the context's for an
+        # eager batch, the graph's own for a captured one)
+        tex = wl.noise_texture(0, 9)
+        it = wl.BatchItem(2, W, H, 3.0, 7)
+        results = []
+        for ctx, fail in ((ok, False), (c, True)):
+            d_tex, grid = ctx.upload(tex), ctx.upload(wl.tunnel_grid(W, H, it.t))
+            o, tmp = ctx.empty(W * H), ctx.empty(W * H)
+            treq = ctx.request_array([wl.make_request(it, o.ptr, sc, path, grid.ptr)])
+            if fail:
+                lib.ckd_debug_fail_alloc(0)
+                with pytest.raises(CkdError) as ei:
+                    ctx.render_batch(treq, sc, 1, d_tex, 9, tmp, W * H)
+                assert ei.value.status == CKD_ENOMEM and "quad" in str(ei.value), str(ei.value)
+            ctx.render_batch(treq, sc, 1, d_tex, 9, tmp, W * H)
+            ctx.sync()
+            results.append(o.numpy())
+            if fail:
+                lib.ckd_debug_fail_alloc(1)  # the graph's tables, then its quads
+                with pytest.raises(CkdError) as ei:
+                    ctx.batch_graph(treq, sc, 1, d_tex, 9, tmp, W * H)
+                assert ei.value.status == CKD_ENOMEM and "quad" in str(ei.value), str(ei.value)
+                g = ctx.batch_graph(treq, sc, 1, d_tex, 9, tmp, W * H)
+                ctx.memset32(o, 0, W * H)
+                g.launch()
+                ctx.sync()
+                results.append(o.numpy())
+                g.close()
+        assert np.array_equal(results[0], results[1]) and np.array_equal(results[0], results[2])
