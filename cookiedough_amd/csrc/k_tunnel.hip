@@ -107,8 +107,9 @@ __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict_
     // U_b)) >> 6, equal mod 2^32 to (U_b*(8-i) + U_b+1*i) >> 6.
     uint4 *E = lds;
     // P[x + 1] first holds pixel x spread into 16-bit fields, then (scanned
-    // in place) the prefix sum of pixels 0..x; P[0] = 0
-    uint64_t *P = reinterpret_cast<uint64_t *>(lds + NB);
+    // in place) the prefix sum of pixels 0..x; P[0] = 0.  P starts 8 bytes
+    // into its 16-byte slot, so that P[1] is 16-byte aligned.
+    uint64_t *P = reinterpret_cast<uint64_t *>(lds + NB) + 1;
 
     for (int i = threadIdx.x; i < 256; i += kT1Block) D[i] = kB07Div.d[i];
     const unsigned GW = (unsigned)NB + 1;
@@ -176,10 +177,19 @@ __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict_
         remap_row(std::false_type{});
     __syncthreads();
 
-    const int chunk = (W + kT1Block - 1) / kT1Block;
-    const int c0 = threadIdx.x * chunk, c1 = min(c0 + chunk, W);
+    // thread t scans pixels [c0, c1), an even run: P[c0 + 1] is 16-byte
+    // aligned, so the run's entries move in pairs
+    const int chunk = ((W + kT1Block - 1) / kT1Block + 1) & ~1;
+    const int c0 = min((int)threadIdx.x * chunk, W), c1 = min(c0 + chunk, W);
+    uint4 *Q = reinterpret_cast<uint4 *>(P + 1); // Q[j] = (P[2j + 1], P[2j + 2])
+    auto pair_lo = [](const uint4 &q) { return ((uint64_t)q.y << 32) | q.x; };
+    auto pair_hi = [](const uint4 &q) { return ((uint64_t)q.w << 32) | q.z; };
     uint64_t local = 0;
-    for (int i = c0; i < c1; ++i) local += P[i + 1];
+    for (int i = c0; i + 1 < c1; i += 2) {
+        const uint4 q = Q[i >> 1];
+        local += pair_lo(q) + pair_hi(q);
+    }
+    if ((c1 - c0) & 1) local += P[c1];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint64_t incl = local;
 #pragma unroll
@@ -192,17 +202,28 @@ __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict_
     uint64_t run = incl - local;
     for (int w = 0; w < wave; ++w) run += waveTot[w];
     if (threadIdx.x == 0) P[0] = 0;
-    for (int i = c0; i < c1; ++i) {
-        run += P[i + 1];
-        P[i + 1] = run;
+    for (int i = c0; i + 1 < c1; i += 2) {
+        const uint4 q = Q[i >> 1];
+        const uint64_t a = run + pair_lo(q), b = a + pair_hi(q);
+        Q[i >> 1] = make_uint4((uint32_t)a, (uint32_t)(a >> 32), (uint32_t)b, (uint32_t)(b >> 32));
+        run = b;
     }
+    if ((c1 - c0) & 1) P[c1] += run;
     __syncthreads();
 
     const int e = (int)J.edge;
+    const uint32_t divI = D[2 * e + 1];
     global_u32 *trow = (global_u32 *)(J.tmp + (size_t)y * W);
     for (int x = threadIdx.x; x < W; x += kT1Block) {
-        const int lo = max(x - e, 0), hi = min(x + e, W - 1);
-        trow[x] = ckd_b07_div_pack(P[hi + 1] - P[lo], D[hi - lo + 1]);
+        // the wave's 64 pixels [xw, xw + 63] (lane 0 is active whenever
+        // any lane is): a full window for all of them, or clipped ones
+        const int xw = __builtin_amdgcn_readfirstlane(x);
+        if (xw >= e && xw + 63 + e <= W - 1) {
+            trow[x] = ckd_b07_div_pack(P[x + e + 1] - P[x - e], divI);
+        } else {
+            const int lo = max(x - e, 0), hi = min(x + e, W - 1);
+            trow[x] = ckd_b07_div_pack(P[hi + 1] - P[lo], D[hi - lo + 1]);
+        }
     }
 }
 
@@ -431,7 +452,7 @@ ckd_status launch_tunnel_jobs(ckd_ctx *ctx, const TunJob *jobs, int nJobs, const
 {
     if (totalRows == 0) return CKD_OK;
     // the row's per-block interpolation terms, its spread pixels / prefix sums
-    const size_t lds = (size_t)((maxW + 7) / 8) * sizeof(uint4) + (size_t)(maxW + 1) * sizeof(uint64_t);
+    const size_t lds = (size_t)((maxW + 7) / 8) * sizeof(uint4) + (size_t)(maxW + 2) * sizeof(uint64_t);
     {
         KernelTimer kt(ctx, timed ? CKD_KT_TUNNEL_H : -1);
         if (quads) {
