@@ -37,6 +37,29 @@ PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 vector (packed) peak
 PEAK_HBM_GBS = 8000.0
 PEAK_FP32_UNPACKED_TFLOPS = 78.6  # without packed FP32 math
 
+# Algorithmic FP32 FLOP of the raymarched FX-map effects (code/shadertoy.cpp),
+# (per march step, per map pixel outside the march): + - * / sqrt min max
+# and each float<->int conversion count 1; lutcosf 6 (scale, trunc, frac,
+# lerp 3), lutsinf 7; powf 40, expf and atan2f 20; the Cephes log / exp of
+# GammaAdj 33 / 28, so GammaAdj of four lanes 248.
+#   plasma:   fPlasma 42 + step bookkeeping 5; map pixel: UV + direction 14,
+#             the second fPlasma 45, colour 14, gamma 248, pack 8
+#   nautilus: fNautilus 50 + 5; pixel: UV, roll, vNorm 25, two gradients
+#             337, specular 46, colour 26, gamma 248, pack 8
+#   spikey (distant) and spikey_close: fSpikey 27 + 8; pixel: UV, roll,
+#             vNorm 26, gradient 98, shading 83, fog 26, lerp 12, gamma
+#             248, pack 8
+#   sinuses:  fSinMap 81 + 8; pixel: UV, roll, vNorm 19, normal 260,
+#             shading 58, fog 26, lerp 12, gamma 248, pack 8
+#   laura:    fLaura 21 + 8; pixel: UV and three rotations 36, normal 80,
+#             light and specular 114, fog 26, lerp 12, gamma 248, pack 8
+#   fdtunnel: no march; pixel: direction 29, flower and box 39, two
+#             atan2 lookups 50, shade 26, two samples 16, two lerps 24,
+#             pack 8
+ST_FLOP = {"plasma": (47, 329), "nautilus": (55, 691), "spikey": (35, 475),
+           "spikey_close": (35, 473), "sinuses": (89, 638), "laura": (29, 530),
+           "fdtunnel": (0, 196)}
+
 # kernel timers (CKD_KT_*) of the batched call of each classic workload
 CLASSIC_KERNELS = {"sdf_c0": [("k_sdf_persistent", 1)], "voxel_c1": [("k_landscape", 2)],
                    "tunnel_c2": [("k_tunnel_h", 3), ("k_tunnel_v", 4)]}
@@ -434,6 +457,7 @@ def main() -> int:
             st_blur = wl.st_blur_maps(W, H)
             ctx.spikey_blurmaps(st_blur, W, H)
             fr_st = ctx.empty(n_st * W * H)
+            st_cnt = ctx.upload(np.zeros(1, np.uint64))
             st_per = {}
             for fx in ["plasma", "nautilus", "spikey", "spikey_close", "sinuses", "laura", "fdtunnel"]:
                 name = "spikey" if fx == "spikey_close" else fx
@@ -456,6 +480,31 @@ def main() -> int:
                 st_per[fx] = {"frames": n_st, "plain_mpx_s": round(px / mp / 1e3, 1),
                               "batched_mpx_s": round(px / mb / 1e3, 1),
                               "ms_per_frame_batched": round(mb / n_st, 4)}
+                # the FX-map kernel against the FP32 rate: its march steps
+                # counted on the device, its time from HIP events
+                ctx.memset32(st_cnt, 0, 2)
+                ctx.shadertoy_step_counter(st_cnt)
+                st_batched()
+                ctx.sync()
+                ctx.shadertoy_step_counter(None)
+                steps = int(st_cnt.numpy()[0])
+                ctx.kernel_timing(True)
+                for _ in range(3):
+                    st_batched()
+                ctx.sync()
+                ms_k, n_k = ctx.kernel_timing_read(5)  # CKD_KT_SHADERTOY
+                ctx.kernel_timing(False)
+                map_px = n_st * (W // 2 + 4) * (H // 2 + 4)
+                f_step, f_px = ST_FLOP[fx]
+                flop = steps * f_step + map_px * f_px
+                t_k = ms_k / max(1, n_k)
+                st_per[fx]["roofline"] = {
+                    "bound": "valu", "kernel": "k_st_map", "avg_launch_ms": round(t_k, 4),
+                    "steps_per_map_pixel": round(steps / map_px, 2),
+                    "achieved": round(flop / (t_k * 1e-3) / 1e12, 2), "peak": PEAK_FP32_TFLOPS,
+                    "unit": "TFLOP/s",
+                    "algorithmic": f"steps*{f_step} + map_px*{f_px} FLOP",
+                    "frac": round(flop / (t_k * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4)}
                 log("shadertoy", fx, st_per[fx])
             per["shadertoy_1280x720"] = st_per
 

@@ -42,6 +42,8 @@ struct ckd_ctx {
     uint32_t *fdTex = nullptr, *fdTexHi = nullptr;
     uint32_t *stScratch = nullptr;
     size_t stScratchCap = 0;
+    // march-step counter of the FX-map kernels (ckd_shadertoy_step_counter)
+    unsigned long long *stSteps = nullptr;
     // the close-up spike's two blur maps (Shadertoy_Create), for the output
     // size they were given for
     uint32_t *spikeBlurMaps = nullptr;

@@ -343,7 +343,7 @@ __device__ __forceinline__ F4 nautilus_grad(const Lut &L, const NautilusF &f, fl
 
 // RenderNautilusMap_2x2 per pixel (code/shadertoy.cpp:338-378)
 __device__ uint32_t nautilus_pixel(const Lut &L, const StGeom &g, const NautilusF &f, unsigned iX,
-                                   unsigned iY)
+                                   unsigned iY, int &steps)
 {
     float u, v;
     to_uv(g, iX, iY, 2.f, u, v);
@@ -351,13 +351,15 @@ __device__ uint32_t nautilus_pixel(const Lut &L, const StGeom &g, const Nautilus
     rot(f.rollCos, f.rollSin, dx, dy);
     const F4 d = vnorm3(dx, dy, 1.f);
     float total = 0.01f, march = 1.f, hx = 0.f, hy = 0.f, hz = 0.f;
-    for (int step = 0; march > 0.01f && step < 48; ++step) {
+    int step = 0;
+    for (; march > 0.01f && step < 48; ++step) {
         hx = d.x * total;
         hy = d.y * total;
         hz = d.z * total;
         march = f_nautilus(L, f, hx, hy, hz);
         total += march * 0.628f;
     }
+    steps = step;
     const F4 n = nautilus_grad(L, f, hx, hy, hz, march);
     float diffuse = n.z * 0.1f;
     const float spec = powf(smax(0.f, dot3(n, d)), 16.f);
@@ -394,7 +396,7 @@ __device__ __forceinline__ F4 spikey_grad(const Lut &L, const SpikeyF &f, float 
 // RenderSpikeyMap_2x2_Distant (code/shadertoy.cpp:558-593),
 // _Distant_SpecularOnly (:620-653) and _Close (:468-517) per pixel
 __device__ uint32_t spikey_pixel(const Lut &L, const StGeom &g, const SpikeyF &f, unsigned iX,
-                                 unsigned iY)
+                                 unsigned iY, int &steps)
 {
     const float ox = f.origin[0], oy = f.origin[1], oz = f.origin[2];
     float u, v, march = 1.f, total = 0.f, hx = 0.f, hy = 0.f, hz = 0.f;
@@ -404,7 +406,8 @@ __device__ uint32_t spikey_pixel(const Lut &L, const StGeom &g, const SpikeyF &f
         float dx = u + f.xOffs, dy = v + f.yOffs;
         rot(f.rollCos, f.rollSin, dx, dy);
         const F4 d = vnorm3(dx, dy, 1.f);
-        for (int step = 0; march > 0.001f && step < 48; ++step) {
+        int step = 0;
+        for (; march > 0.001f && step < 48; ++step) {
             hx = ox + d.x * total;
             hy = oy + d.y * total;
             hz = oz + d.z * total;
@@ -412,6 +415,7 @@ __device__ uint32_t spikey_pixel(const Lut &L, const StGeom &g, const SpikeyF &f
             march *= 0.314f;
             total += march;
         }
+        steps = step;
         const F4 n = spikey_grad<2>(L, f, hx, hy, hz, march, kPI * 0.02f);
         const float diffuse = smax(0.f, n.z * 0.8f + n.y * 0.2f);
         const float spec = powf(dot3(n, d), f.specPow);
@@ -430,6 +434,7 @@ __device__ uint32_t spikey_pixel(const Lut &L, const StGeom &g, const SpikeyF &f
             march = f_spikey<2>(L, f, hx, hy, hz);
             total += march * 0.075f * kGoldenRatio;
         }
+        steps = 36;
         const F4 n = spikey_grad<2>(L, f, hx, hy, hz, march, 0.01f);
         const float spec = f.warmup * powf(smax(0.f, dot3(n, d)), f.specPow);
         const float fog = exp_fog(hz - oz, 0.0133f);
@@ -440,13 +445,15 @@ __device__ uint32_t spikey_pixel(const Lut &L, const StGeom &g, const SpikeyF &f
         float dx = (u + f.xOffs) * g.aspect, dy = v + f.yOffs;
         rot(f.rollCos, f.rollSin, dx, dy);
         const F4 d = vnorm3(dx, dy, f.zDir);
-        for (int step = 0; march > 0.0001f && step < 32; ++step) {
+        int step = 0;
+        for (; march > 0.0001f && step < 32; ++step) {
             hx = ox + d.x * total;
             hy = oy + d.y * total;
             hz = oz + d.z * total;
             march = f_spikey<1>(L, f, hx, hy, hz);
             total += march * (0.05f * kPI);
         }
+        steps = step;
         const F4 n = spikey_grad<1>(L, f, hx, hy, hz, march, f.nOffs);
         // the rim factor max(1, min(0, x)) is 1 for every x
         const float diffuse = n.z;
@@ -475,7 +482,7 @@ __device__ __forceinline__ float f_sinmap(const Lut &L, float x, float y, float 
 
 // RenderSinMap_2x2 per pixel (code/shadertoy.cpp:935-970)
 __device__ uint32_t sinuses_pixel(const Lut &L, const StGeom &g, const SinusesF &f, unsigned iX,
-                                  unsigned iY)
+                                  unsigned iY, int &steps)
 {
     const float ox = f.origin[0], oy = f.origin[1], oz = f.origin[2];
     float u, v;
@@ -484,13 +491,15 @@ __device__ uint32_t sinuses_pixel(const Lut &L, const StGeom &g, const SinusesF 
     rot(f.rollCos, f.rollSin, dx, dy);
     const F4 d = vnorm3(dx, dy, 0.314f);
     float march = 1.f, total = 0.f, hx = 0.f, hy = 0.f, hz = 0.f;
-    for (int step = 0; march > 0.01f && step < 32; ++step) {
+    int step = 0;
+    for (; march > 0.01f && step < 32; ++step) {
         hx = ox + d.x * total;
         hy = oy + d.y * total;
         hz = oz + d.z * total;
         march = f_sinmap(L, hx, hy, hz);
         total += march * 0.814f;
     }
+    steps = step;
     const float nx = march - f_sinmap(L, hx + 0.2f, hy, hz);
     const float ny = march - f_sinmap(L, hx, hy + 0.2f, hz);
     const float nz = march - f_sinmap(L, hx, hy, hz + 0.2f);
@@ -611,13 +620,17 @@ __device__ void fdtunnel_pixel(const Lut &L, const StGeom &g, const TunnelF &f,
     glow = to_pixel_noconv(lerp_to(h, f.fog1, shade));
 }
 
+// steps: while non-null, every pixel's march steps (loop iterations; the
+// fixed-length marches count their length, the tunnel none) are added to
+// *steps, one atomic per wave
 template <int kFx>
 __global__ __launch_bounds__(256) void k_st_map(const StFrame *__restrict__ frames,
                                                 uint32_t *__restrict__ maps,
                                                 uint32_t *__restrict__ glows, StGeom g,
                                                 const float *__restrict__ lutG,
                                                 const uint32_t *__restrict__ tex,
-                                                const uint32_t *__restrict__ texHi)
+                                                const uint32_t *__restrict__ texHi,
+                                                unsigned long long *__restrict__ steps)
 {
     __shared__ float lut[kLutN + 3];
     for (unsigned i = threadIdx.x; i < kLutN; i += 256) lut[i] = lutG[i];
@@ -627,19 +640,33 @@ __global__ __launch_bounds__(256) void k_st_map(const StFrame *__restrict__ fram
     const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const unsigned iX = blockIdx.x * kTile + (wave & 1) * 8 + (lane & 7);
     const unsigned iY = blockIdx.y * kTile + (wave >> 1) * 8 + (lane >> 3);
-    if (iX >= g.mapW || iY >= g.mapH) return;
-    const StFrame &F = frames[blockIdx.z];
-    const size_t at = (size_t)blockIdx.z * g.mapW * g.mapH + (size_t)iY * g.mapW + iX;
-    if constexpr (kFx == FX_PLASMA) maps[at] = plasma_pixel(L, g, F.p, iX, iY);
-    if constexpr (kFx == FX_NAUTILUS) maps[at] = nautilus_pixel(L, g, F.n, iX, iY);
-    if constexpr (kFx == FX_SPIKEY) maps[at] = spikey_pixel(L, g, F.s, iX, iY);
-    if constexpr (kFx == FX_SINUSES) maps[at] = sinuses_pixel(L, g, F.si, iX, iY);
-    if constexpr (kFx == FX_LAURA) maps[at] = laura_pixel(L, g, F.l, iX, iY);
-    if constexpr (kFx == FX_FDTUNNEL) {
-        uint32_t c, h;
-        fdtunnel_pixel(L, g, F.t, tex, texHi, iX, iY, c, h);
-        maps[at] = c;
-        glows[at] = h;
+    int n = 0;
+    if (iX < g.mapW && iY < g.mapH) {
+        const StFrame &F = frames[blockIdx.z];
+        const size_t at = (size_t)blockIdx.z * g.mapW * g.mapH + (size_t)iY * g.mapW + iX;
+        if constexpr (kFx == FX_PLASMA) {
+            maps[at] = plasma_pixel(L, g, F.p, iX, iY);
+            n = 24;
+        }
+        if constexpr (kFx == FX_NAUTILUS) maps[at] = nautilus_pixel(L, g, F.n, iX, iY, n);
+        if constexpr (kFx == FX_SPIKEY) maps[at] = spikey_pixel(L, g, F.s, iX, iY, n);
+        if constexpr (kFx == FX_SINUSES) maps[at] = sinuses_pixel(L, g, F.si, iX, iY, n);
+        if constexpr (kFx == FX_LAURA) {
+            maps[at] = laura_pixel(L, g, F.l, iX, iY);
+            n = 32;
+        }
+        if constexpr (kFx == FX_FDTUNNEL) {
+            uint32_t c, h;
+            fdtunnel_pixel(L, g, F.t, tex, texHi, iX, iY, c, h);
+            maps[at] = c;
+            glows[at] = h;
+        }
+    }
+    if (steps) {
+        unsigned long long v = (unsigned long long)n;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (lane == 0) atomicAdd(steps, v);
     }
 }
 
@@ -866,23 +893,23 @@ static ckd_status st_launch(ckd_ctx *ctx, int fx, const StFrame *frames, unsigne
     KernelTimer timer(ctx, CKD_KT_SHADERTOY);
     switch (fx) {
     case FX_PLASMA:
-        k_st_map<FX_PLASMA><<<grid, 256, 0, ctx->stream>>>(df, maps, glows, g, ctx->cosLut, nullptr, nullptr);
+        k_st_map<FX_PLASMA><<<grid, 256, 0, ctx->stream>>>(df, maps, glows, g, ctx->cosLut, nullptr, nullptr, ctx->stSteps);
         break;
     case FX_NAUTILUS:
-        k_st_map<FX_NAUTILUS><<<grid, 256, 0, ctx->stream>>>(df, maps, glows, g, ctx->cosLut, nullptr, nullptr);
+        k_st_map<FX_NAUTILUS><<<grid, 256, 0, ctx->stream>>>(df, maps, glows, g, ctx->cosLut, nullptr, nullptr, ctx->stSteps);
         break;
     case FX_SPIKEY:
-        k_st_map<FX_SPIKEY><<<grid, 256, 0, ctx->stream>>>(df, maps, glows, g, ctx->cosLut, nullptr, nullptr);
+        k_st_map<FX_SPIKEY><<<grid, 256, 0, ctx->stream>>>(df, maps, glows, g, ctx->cosLut, nullptr, nullptr, ctx->stSteps);
         break;
     case FX_SINUSES:
-        k_st_map<FX_SINUSES><<<grid, 256, 0, ctx->stream>>>(df, maps, glows, g, ctx->cosLut, nullptr, nullptr);
+        k_st_map<FX_SINUSES><<<grid, 256, 0, ctx->stream>>>(df, maps, glows, g, ctx->cosLut, nullptr, nullptr, ctx->stSteps);
         break;
     case FX_LAURA:
-        k_st_map<FX_LAURA><<<grid, 256, 0, ctx->stream>>>(df, maps, glows, g, ctx->cosLut, nullptr, nullptr);
+        k_st_map<FX_LAURA><<<grid, 256, 0, ctx->stream>>>(df, maps, glows, g, ctx->cosLut, nullptr, nullptr, ctx->stSteps);
         break;
     case FX_FDTUNNEL:
         k_st_map<FX_FDTUNNEL><<<grid, 256, 0, ctx->stream>>>(df, maps, glows, g, ctx->cosLut, ctx->fdTex,
-                                                              ctx->fdTexHi);
+                                                              ctx->fdTexHi, ctx->stSteps);
         break;
     default:
         return fail(CKD_EINVAL, "shadertoy: unknown effect %d", fx);
@@ -1102,6 +1129,13 @@ ckd_status st_draw(ckd_ctx *ctx, int fx, const P *params, unsigned nFrames, uint
 } // namespace
 
 extern "C" {
+
+ckd_status ckd_shadertoy_step_counter(ckd_ctx *ctx, unsigned long long *counter)
+{
+    CKD_REQUIRE_CTX(ctx);
+    ctx->stSteps = counter;
+    return CKD_OK;
+}
 
 #define CKD_ST_ENTRY(name, P, FX)                                                               \
     ckd_status ckd_##name##_map(ckd_ctx *ctx, uint32_t *maps, unsigned resX, unsigned resY,     \

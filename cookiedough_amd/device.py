@@ -190,6 +190,10 @@ class Context:
         check(self.lib.ckd_zoom32(self.h, ptr(dst), ptr(src), W, H, scale))
 
     # ---- the reference's raymarched effects (code/shadertoy.cpp)
+    def shadertoy_step_counter(self, counter):
+        """count the FX-map kernels' march steps into a device u64 (None: off)"""
+        check(self.lib.ckd_shadertoy_step_counter(self.h, ptr(counter)))
+
     def fdtunnel_create(self, tex, tex_highlights):
         tex = np.ascontiguousarray(tex, np.uint32)
         hi = np.ascontiguousarray(tex_highlights, np.uint32)

@@ -549,6 +549,12 @@ CKD_API ckd_status ckd_fdtunnel_map(ckd_ctx *ctx, uint32_t *maps, uint32_t *glow
                                     const ckd_fdtunnel_params *frames, unsigned nFrames);
 CKD_API ckd_status ckd_fdtunnel_draw(ckd_ctx *ctx, uint32_t *dst, unsigned resX, unsigned resY,
                                      const ckd_fdtunnel_params *frames, unsigned nFrames);
+/* Measurement hook, no reference counterpart: while counter (device memory)
+ * is non-null, the FX-map kernels of the effects above add every map
+ * pixel's march steps to *counter (loop iterations of its march; Plasma and
+ * Laura count their fixed 24 and 32, Spikey's specular-only form its 36,
+ * the tunnel none).  NULL switches it off. */
+CKD_API ckd_status ckd_shadertoy_step_counter(ckd_ctx *ctx, unsigned long long *counter);
 
 /* ---- voxel ball, torus twister, Polar_BlitA ------------------------------ */
 

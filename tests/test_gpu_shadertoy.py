@@ -228,3 +228,31 @@ def test_draws_from_rocket_tracks(ctx, orc, st, tmp_path):
                 assert orc.orc_st_draw(oracles.ST_IDS[fx], P(od), W, H, P(p), P(lut), P(tex),
                                        P(tex_hi), P(blur), 0, 16) == 0
                 check_close(got[k], od, f"{fx} tracks row {rows[k]}", exact=fx == "plasma")
+
+
+def test_step_counter(ctx, st, gold):
+    """ckd_shadertoy_step_counter: the maps are unchanged while it counts;
+    fixed marches count their length per map pixel (Plasma 24, Laura 32,
+    the tunnel none), early-exit marches between 1 and their cap"""
+    g, _ = gold
+    picked = {}
+    for c in g["cases"]:
+        picked.setdefault(c["effect"], c)
+    caps = {"plasma": (24, 24), "laura": (32, 32), "fdtunnel": (0, 0), "nautilus": (1, 48),
+            "spikey": (1, 48), "spikey_close": (1, 32), "sinuses": (1, 32)}
+    mw, mh = W // 2 + 4, H // 2 + 4
+    cnt = ctx.upload(np.zeros(1, np.uint64))
+    for fid, c in picked.items():
+        name, prm = params_of(c)
+        plain, _ = gpu_map(ctx, name, prm, W, H, fid == "spikey_close")
+        ctx.memset32(cnt, 0, 2)
+        ctx.shadertoy_step_counter(cnt)
+        try:
+            counted, _ = gpu_map(ctx, name, prm, W, H, fid == "spikey_close")
+        finally:
+            ctx.shadertoy_step_counter(None)
+        n = int(cnt.numpy()[0])
+        lo, hi = caps[fid]
+        print(fid, "steps per map pixel", n / (mw * mh), flush=True)
+        assert np.array_equal(plain, counted), fid
+        assert lo * mw * mh <= n <= hi * mw * mh, (fid, n)
