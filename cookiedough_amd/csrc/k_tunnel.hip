@@ -6,7 +6,9 @@
 //   T1  one 128-thread block per row: interpolate the row's grid corners
 //       once per 8-pixel block, remap the row into dst and (spread into
 //       16-bit fields) into LDS, prefix-sum it there in place, write the
-//       horizontal blur to tmp
+//       horizontal blur to tmp.  Its stores are non-temporal: the rows it
+//       writes are read once, later, and would otherwise push the texture
+//       out of L2
 //   T2  one wave per 64 columns x one segment of rows: vertical blur of tmp
 //       as a running window sum and the saturating add into dst; the
 //       window's rows stay in an LDS ring (radius <= 32), so each row of
@@ -162,13 +164,13 @@ __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict_
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 P[x0 + k * kT1Block + 1] = ckd_spread(v[k]);
-                drow[x0 + k * kT1Block] = v[k];
+                __builtin_nontemporal_store(v[k], &drow[x0 + k * kT1Block]);
             }
         }
         for (; x0 < W; x0 += kT1Block) {
             const uint32_t v = remap(x0);
             P[x0 + 1] = ckd_spread(v);
-            drow[x0] = v;
+            __builtin_nontemporal_store(v, &drow[x0]);
         }
     };
     if (narrow)
@@ -219,10 +221,10 @@ __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict_
         // any lane is): a full window for all of them, or clipped ones
         const int xw = __builtin_amdgcn_readfirstlane(x);
         if (xw >= e && xw + 63 + e <= W - 1) {
-            trow[x] = ckd_b07_div_pack(P[x + e + 1] - P[x - e], divI);
+            __builtin_nontemporal_store(ckd_b07_div_pack(P[x + e + 1] - P[x - e], divI), &trow[x]);
         } else {
             const int lo = max(x - e, 0), hi = min(x + e, W - 1);
-            trow[x] = ckd_b07_div_pack(P[hi + 1] - P[lo], D[hi - lo + 1]);
+            __builtin_nontemporal_store(ckd_b07_div_pack(P[hi + 1] - P[lo], D[hi - lo + 1]), &trow[x]);
         }
     }
 }
