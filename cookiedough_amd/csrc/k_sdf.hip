@@ -18,6 +18,9 @@
 #include "ckd_pixel.h"
 #include "ckd_jobs.h"
 
+#include <algorithm>
+#include <cmath>
+
 
 namespace {
 
@@ -314,7 +317,7 @@ __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uin
                                                         const SdfJob *__restrict__ jobs, int nJobs,
                                                         const ckd_sdf_scene sc,
                                                         unsigned long long *__restrict__ stepCount,
-                                                        uint32_t kChunkSide)
+                                                        uint32_t kChunkSide, f4 bound)
 {
     __shared__ ShadeEntry queue[kWavesPerBlock][kQueueCap];
     // the radii, read every step with broadcast LDS loads instead of
@@ -339,6 +342,8 @@ __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uin
     float t = 0.f, d = 1.f, tHit = 0.f, dx = 0.f, dy = 0.f, dz = 0.f;
     int steps = 0;
     float M[kMaxSph], E2[kMaxSph]; // d.(o-c_i) and |w_i|^2 of the lane's ray
+    float Mb = 0.f, E2b = 0.f;     // the same for the centre of the spheres' bounding sphere
+    float bx = 0.f, by = 0.f, bz = 0.f; // o - C of the chunk's camera (uniform)
     unsigned long long mySteps = 0;
 
     while (true) {
@@ -362,6 +367,14 @@ __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uin
                         const float wz = fmaf(-m, dz, oc.z);
                         M[i] = m;
                         E2[i] = wx * wx + wy * wy + wz * wz;
+                    }
+                    {
+                        const float m = dx * bx + dy * by + dz * bz;
+                        const float wx = fmaf(-m, dx, bx);
+                        const float wy = fmaf(-m, dy, by);
+                        const float wz = fmaf(-m, dz, bz);
+                        Mb = m;
+                        E2b = wx * wx + wy * wy + wz * wz;
                     }
                     t = 0.f;
                     d = 1.f;
@@ -391,6 +404,9 @@ __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uin
             const uint32_t chh = min((uint32_t)kChunkSide, H - cy0);
             next = 0;
             end = cw * chh;
+            bx = cam[0] - bound.x;
+            by = cam[1] - bound.y;
+            bz = cam[2] - bound.z;
             // lane i stages sphere i; spheres past nSpheres become a neutral
             // sphere whose distance is never the minimum (a centre 1e15 away
             // from the camera, radius 0)
@@ -412,8 +428,20 @@ __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uin
         const float oyg = cam[1] - sc.groundY;
 #pragma unroll
         for (int k = 0; k < kStepsPerRefill; ++k) {
-            if (active && d > sc.eps && steps < sc.maxSteps && t < sc.tMax) {
-                float dist = fmaf(t, dy, oyg);
+            const bool marching = active && d > sc.eps && steps < sc.maxSteps && t < sc.tMax;
+            float dist = fmaf(t, dy, oyg);
+            // Every sphere lies inside the bounding sphere (centre C, radius
+            // R), so each sphere's distance is at least |p - C| - R.  Where
+            // that bound exceeds the ground's distance by more than the
+            // rounding of either (1e-3 + 1e-3 |ground|, against errors of
+            // order 1e-5), no sphere can be the minimum: the ground's
+            // distance is the step, exactly as the full minimum gives it.
+            // The spheres are evaluated only when some marching lane needs
+            // them.
+            const float ub = t + Mb;
+            const float db = fsqrt(fmaf(ub, ub, E2b)) - bound.w;
+            const bool near = db < fmaf(fabsf(dist), 1e-3f, dist + 1e-3f);
+            if (__ballot(marching && near)) {
                 const uint32_t z = opaque_zero();
 #pragma unroll
                 for (int i = 0; i < kMaxSph / 4; ++i) {
@@ -424,6 +452,8 @@ __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uin
                         dist = fminf(dist, fsqrt(fmaf(u, u, E2[4 * i + j])) - r[j]);
                     }
                 }
+            }
+            if (marching) {
                 tHit = t;
                 d = dist;
                 t += dist;
@@ -527,14 +557,40 @@ int persistent_blocks_per_cu(int steps)
 template <bool kHalf, int kMaxSph>
 void launch_persistent_steps(int steps, unsigned blocks, hipStream_t st, uint32_t *queueCounter,
                              uint32_t totalChunks, const SdfJob *jobs, int nJobs,
-                             const ckd_sdf_scene &sc, unsigned long long *stepCount, uint32_t chunk)
+                             const ckd_sdf_scene &sc, unsigned long long *stepCount, uint32_t chunk,
+                             f4 bound)
 {
     if (steps <= 1)
-        k_sdf_persistent<kHalf, kMaxSph, 1><<<blocks, 256, 0, st>>>(queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk);
+        k_sdf_persistent<kHalf, kMaxSph, 1><<<blocks, 256, 0, st>>>(queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk, bound);
     else if (steps == 2)
-        k_sdf_persistent<kHalf, kMaxSph, 2><<<blocks, 256, 0, st>>>(queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk);
+        k_sdf_persistent<kHalf, kMaxSph, 2><<<blocks, 256, 0, st>>>(queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk, bound);
     else
-        k_sdf_persistent<kHalf, kMaxSph, 4><<<blocks, 256, 0, st>>>(queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk);
+        k_sdf_persistent<kHalf, kMaxSph, 4><<<blocks, 256, 0, st>>>(queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk, bound);
+}
+
+// A sphere containing every sphere of the scene: centre the centroid of the
+// centres, radius the largest |c_i - C| + r_i, in double and rounded up.  A
+// scene without spheres gets one far away (every step skips it).
+f4 bounding_sphere(const ckd_sdf_scene &sc)
+{
+    if (sc.nSpheres <= 0) return f4{1e15f, 0.f, 0.f, 0.f};
+    double cx = 0, cy = 0, cz = 0;
+    for (int i = 0; i < sc.nSpheres; ++i) {
+        cx += sc.sphere[i][0];
+        cy += sc.sphere[i][1];
+        cz += sc.sphere[i][2];
+    }
+    cx /= sc.nSpheres;
+    cy /= sc.nSpheres;
+    cz /= sc.nSpheres;
+    const float fx = (float)cx, fy = (float)cy, fz = (float)cz;
+    double R = 0;
+    for (int i = 0; i < sc.nSpheres; ++i) {
+        const double ex = sc.sphere[i][0] - (double)fx, ey = sc.sphere[i][1] - (double)fy,
+                     ez = sc.sphere[i][2] - (double)fz;
+        R = std::max(R, std::sqrt(ex * ex + ey * ey + ez * ez) + (double)sc.sphere[i][3]);
+    }
+    return f4{fx, fy, fz, std::nextafter((float)R, 3.4e38f)};
 }
 
 ckd_status launch_sdf_persistent(ckd_ctx *ctx, uint32_t *queueCounter, uint32_t totalChunks,
@@ -566,13 +622,14 @@ ckd_status launch_sdf_persistent(ckd_ctx *ctx, uint32_t *queueCounter, uint32_t 
     if (blocks > useful) blocks = useful;
     if (blocks == 0) blocks = 1;
     // only the main launch is timed: it spans the whole march of the batch
+    const f4 bound = bounding_sphere(sc);
     KernelTimer kt(ctx, helper ? -1 : CKD_KT_SDF_PERSISTENT);
     if (shading == CKD_SHADE_FP16) {
-        if (big) launch_persistent_steps<true, 32>(steps, blocks, ctx->stream, queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk);
-        else launch_persistent_steps<true, 8>(steps, blocks, ctx->stream, queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk);
+        if (big) launch_persistent_steps<true, 32>(steps, blocks, ctx->stream, queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk, bound);
+        else launch_persistent_steps<true, 8>(steps, blocks, ctx->stream, queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk, bound);
     } else {
-        if (big) launch_persistent_steps<false, 32>(steps, blocks, ctx->stream, queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk);
-        else launch_persistent_steps<false, 8>(steps, blocks, ctx->stream, queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk);
+        if (big) launch_persistent_steps<false, 32>(steps, blocks, ctx->stream, queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk, bound);
+        else launch_persistent_steps<false, 8>(steps, blocks, ctx->stream, queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk, bound);
     }
     return check_launch("sdf persistent");
 }
