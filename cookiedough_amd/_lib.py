@@ -207,6 +207,7 @@ SIGNATURES = {
     "ckd_ball_draw": [_p, _p, _p, _u, _u, C.POINTER(BallParams), _u],
     "ckd_fdtunnel_create": [_p, _p, _p],
     "ckd_shadertoy_step_counter": [_p, _p],
+    "ckd_sdf_work_counter": [_p, _p],
     "ckd_spikey_blurmaps": [_p, _p, _p, _u, _u],
     "ckd_fdtunnel_map": [_p, _p, _p, _u, _u, C.POINTER(FdTunnelParams), _u],
     "ckd_fdtunnel_draw": [_p, _p, _u, _u, C.POINTER(FdTunnelParams), _u],

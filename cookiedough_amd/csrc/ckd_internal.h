@@ -44,6 +44,8 @@ struct ckd_ctx {
     size_t stScratchCap = 0;
     // march-step counter of the FX-map kernels (ckd_shadertoy_step_counter)
     unsigned long long *stSteps = nullptr;
+    // the batched SDF kernel's two work counters (ckd_sdf_work_counter)
+    unsigned long long *sdfWork = nullptr;
     // the close-up spike's two blur maps (Shadertoy_Create), for the output
     // size they were given for
     uint32_t *spikeBlurMaps = nullptr;

@@ -271,9 +271,16 @@ CKD_DEV void ray_dir(const float *cam, uint32_t W, uint32_t H, uint32_t x, uint3
 // the six offsets taken from one |p-c|^2 per sphere
 // (|p +- h e_k - c|^2 = q + h^2 +- 2 h (p-c)_k), argmin id, Lambert, fog,
 // gamma, pack
+// Where the spheres' bounding sphere (centre C, radius R) is farther from
+// the hit point p than the ground by more than 2h (an offset of h moves a
+// sphere's distance by at most h) plus the rounding margin, every sphere
+// term of the six offsets and of the argmin exceeds the ground's: the
+// minima are the ground's values and the id is the ground's, exactly as the
+// loop over the spheres gives them.  The loop runs only when some lane of
+// the wave needs it.
 template <bool kHalf, int kMaxSph>
 CKD_DEV uint32_t shade_hit(const ckd_sdf_scene &sc, const float *cam, float dx, float dy, float dz,
-                           float tHit, float total)
+                           float tHit, float total, const f4 &bound, uint32_t &evaluated)
 {
     const float hx = cam[0] + dx * tHit, hy = cam[1] + dy * tHit, hz = cam[2] + dz * tHit;
     const float h = 1e-3f, hh = h * h, h2 = 2.f * h;
@@ -281,6 +288,12 @@ CKD_DEV uint32_t shade_hit(const ckd_sdf_scene &sc, const float *cam, float dx, 
     float mxp = pl, mxm = pl, myp = pl + h, mym = pl - h, mzp = pl, mzm = pl;
     float best = pl;
     int id = sc.nSpheres;
+    const float bx = hx - bound.x, by = hy - bound.y, bz = hz - bound.z;
+    const float db = fsqrt(bx * bx + by * by + bz * bz) - bound.w;
+    const bool near = db < fmaf(fabsf(pl), 1e-3f, pl + (2.f * h + 1e-3f));
+    const bool loop = __ballot(near) != 0;
+    evaluated += loop ? 1u : 0u;
+    if (loop)
 #pragma unroll 4
     for (int i = 0; i < kMaxSph; ++i) {
         if (i < sc.nSpheres) {
@@ -317,7 +330,8 @@ __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uin
                                                         const SdfJob *__restrict__ jobs, int nJobs,
                                                         const ckd_sdf_scene sc,
                                                         unsigned long long *__restrict__ stepCount,
-                                                        uint32_t kChunkSide, f4 bound)
+                                                        uint32_t kChunkSide, f4 bound,
+                                                        unsigned long long *__restrict__ work)
 {
     __shared__ ShadeEntry queue[kWavesPerBlock][kQueueCap];
     // the radii, read every step with broadcast LDS loads instead of
@@ -345,6 +359,8 @@ __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uin
     float Mb = 0.f, E2b = 0.f;     // the same for the centre of the spheres' bounding sphere
     float bx = 0.f, by = 0.f, bz = 0.f; // o - C of the chunk's camera (uniform)
     unsigned long long mySteps = 0;
+    // march steps and shaded pixels of this lane that evaluated the spheres
+    uint32_t evalSteps = 0, evalShades = 0;
 
     while (true) {
         // refill finished lanes from the current chunk
@@ -442,6 +458,7 @@ __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uin
             const float db = fsqrt(fmaf(ub, ub, E2b)) - bound.w;
             const bool near = db < fmaf(fabsf(dist), 1e-3f, dist + 1e-3f);
             if (__ballot(marching && near)) {
+                evalSteps += marching ? 1u : 0u;
                 const uint32_t z = opaque_zero();
 #pragma unroll
                 for (int i = 0; i < kMaxSph / 4; ++i) {
@@ -481,7 +498,7 @@ __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uin
                 ray_dir(J.cam, J.W, J.H, x, y, ex, ey, ez);
                 // a global store (the job's pointer would make a flat one,
                 // whose completion the next LDS reads would wait for)
-                ((global_px *)J.dst)[e.pix] = shade_hit<kHalf, kMaxSph>(sc, J.cam, ex, ey, ez, e.tHit, e.total);
+                ((global_px *)J.dst)[e.pix] = shade_hit<kHalf, kMaxSph>(sc, J.cam, ex, ey, ez, e.tHit, e.total, bound, evalShades);
                 qn -= 64;
                 __builtin_amdgcn_wave_barrier();
             }
@@ -495,13 +512,25 @@ __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uin
         const uint32_t x = e.pix % J.W, y = e.pix / J.W;
         float ex, ey, ez;
         ray_dir(J.cam, J.W, J.H, x, y, ex, ey, ez);
-        ((global_px *)J.dst)[e.pix] = shade_hit<kHalf, kMaxSph>(sc, J.cam, ex, ey, ez, e.tHit, e.total);
+        ((global_px *)J.dst)[e.pix] = shade_hit<kHalf, kMaxSph>(sc, J.cam, ex, ey, ez, e.tHit, e.total, bound, evalShades);
     }
     if (stepCount) {
         unsigned long long s = mySteps;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
         if (lane == 0) atomicAdd(stepCount, s);
+    }
+    if (work) {
+        unsigned long long a = evalSteps, b = evalShades;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            a += __shfl_xor(a, o, 64);
+            b += __shfl_xor(b, o, 64);
+        }
+        if (lane == 0) {
+            atomicAdd(work, a);
+            atomicAdd(work + 1, b);
+        }
     }
 }
 
@@ -558,14 +587,14 @@ template <bool kHalf, int kMaxSph>
 void launch_persistent_steps(int steps, unsigned blocks, hipStream_t st, uint32_t *queueCounter,
                              uint32_t totalChunks, const SdfJob *jobs, int nJobs,
                              const ckd_sdf_scene &sc, unsigned long long *stepCount, uint32_t chunk,
-                             f4 bound)
+                             f4 bound, unsigned long long *work)
 {
     if (steps <= 1)
-        k_sdf_persistent<kHalf, kMaxSph, 1><<<blocks, 256, 0, st>>>(queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk, bound);
+        k_sdf_persistent<kHalf, kMaxSph, 1><<<blocks, 256, 0, st>>>(queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk, bound, work);
     else if (steps == 2)
-        k_sdf_persistent<kHalf, kMaxSph, 2><<<blocks, 256, 0, st>>>(queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk, bound);
+        k_sdf_persistent<kHalf, kMaxSph, 2><<<blocks, 256, 0, st>>>(queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk, bound, work);
     else
-        k_sdf_persistent<kHalf, kMaxSph, 4><<<blocks, 256, 0, st>>>(queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk, bound);
+        k_sdf_persistent<kHalf, kMaxSph, 4><<<blocks, 256, 0, st>>>(queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk, bound, work);
 }
 
 // A sphere containing every sphere of the scene: centre the centroid of the
@@ -625,11 +654,11 @@ ckd_status launch_sdf_persistent(ckd_ctx *ctx, uint32_t *queueCounter, uint32_t 
     const f4 bound = bounding_sphere(sc);
     KernelTimer kt(ctx, helper ? -1 : CKD_KT_SDF_PERSISTENT);
     if (shading == CKD_SHADE_FP16) {
-        if (big) launch_persistent_steps<true, 32>(steps, blocks, ctx->stream, queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk, bound);
-        else launch_persistent_steps<true, 8>(steps, blocks, ctx->stream, queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk, bound);
+        if (big) launch_persistent_steps<true, 32>(steps, blocks, ctx->stream, queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk, bound, ctx->sdfWork);
+        else launch_persistent_steps<true, 8>(steps, blocks, ctx->stream, queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk, bound, ctx->sdfWork);
     } else {
-        if (big) launch_persistent_steps<false, 32>(steps, blocks, ctx->stream, queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk, bound);
-        else launch_persistent_steps<false, 8>(steps, blocks, ctx->stream, queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk, bound);
+        if (big) launch_persistent_steps<false, 32>(steps, blocks, ctx->stream, queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk, bound, ctx->sdfWork);
+        else launch_persistent_steps<false, 8>(steps, blocks, ctx->stream, queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk, bound, ctx->sdfWork);
     }
     return check_launch("sdf persistent");
 }
@@ -637,6 +666,13 @@ ckd_status launch_sdf_persistent(ckd_ctx *ctx, uint32_t *queueCounter, uint32_t 
 } // namespace ckd
 
 extern "C" {
+
+ckd_status ckd_sdf_work_counter(ckd_ctx *ctx, unsigned long long *counters)
+{
+    CKD_REQUIRE_CTX(ctx);
+    ctx->sdfWork = counters;
+    return CKD_OK;
+}
 
 ckd_status ckd_sdf_render(ckd_ctx *ctx, uint32_t *dst, unsigned W, unsigned H,
                           const ckd_sdf_scene *scene, const float *cam, int shading,

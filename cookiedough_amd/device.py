@@ -190,6 +190,11 @@ class Context:
         check(self.lib.ckd_zoom32(self.h, ptr(dst), ptr(src), W, H, scale))
 
     # ---- the reference's raymarched effects (code/shadertoy.cpp)
+    def sdf_work_counter(self, counters):
+        """count the batched SDF kernel's sphere-evaluating steps and shades
+        into two device u64 (None: off)"""
+        check(self.lib.ckd_sdf_work_counter(self.h, ptr(counters)))
+
     def shadertoy_step_counter(self, counter):
         """count the FX-map kernels' march steps into a device u64 (None: off)"""
         check(self.lib.ckd_shadertoy_step_counter(self.h, ptr(counter)))

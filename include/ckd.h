@@ -238,6 +238,14 @@ CKD_API ckd_status ckd_sdf_render(ckd_ctx *ctx, uint32_t *dst, unsigned W,
                                   const float *cam, int shading,
                                   unsigned long long *stepCount);
 
+/* Measurement hook, no reference counterpart: while counters (device memory,
+ * two u64) is non-null, the batched SDF kernel adds to counters[0] the march
+ * steps that evaluated the spheres and to counters[1] the shaded pixels
+ * whose normal evaluated them.  The others took the ground's distance after
+ * a bounding-sphere test showed no sphere could be nearer (the result is the
+ * same either way).  NULL switches it off. */
+CKD_API ckd_status ckd_sdf_work_counter(ckd_ctx *ctx, unsigned long long *counters);
+
 /* ---- tunnel frame (no reference counterpart as a whole: the tunnel
  * benchmark's pipeline of grid remap, 2007 blur H then V out of place with
  * span 2r+1 -- HorizontalBoxBlur32 / VerticalBoxBlur32,
