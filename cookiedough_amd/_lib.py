@@ -43,7 +43,8 @@ class SdfScene(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("sdfStepsPerRefill", C.c_int), ("sdfChunk", C.c_int),
-                ("sdfBlocksPerCU", C.c_int), ("tunnelVSeg", C.c_int), ("blurKernel", C.c_int)]
+                ("sdfBlocksPerCU", C.c_int), ("tunnelVSeg", C.c_int), ("blurKernel", C.c_int),
+                ("voxelChunks", C.c_int), ("voxelWavesPerSIMD", C.c_int)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}

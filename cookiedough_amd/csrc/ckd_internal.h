@@ -14,6 +14,9 @@
 
 #define CKD_NUM_EVENTS 16
 
+// launch shapes of a new context
+constexpr ckd_tuning kCkdDefaultTuning = {2, 32, 8, 32, 0, 2, 2};
+
 struct ckd_ctx {
     int device = 0;
     int cuCount = 0;
@@ -76,6 +79,10 @@ struct ckd_ctx {
     uint32_t lsFog0 = 0;
     uint32_t *lsQuadH = nullptr;
     uint4 *lsQuadC = nullptr;
+    // ckd_landscape_draw's work queue of march ranges: four control words
+    // and one 64-bit state word per column of a launch's frames
+    uint32_t *lsQueue = nullptr;
+    unsigned long long *lsState = nullptr;
 
     // tunnelscape maps (Tunnelscape_Create)
     uint8_t *tsHeight = nullptr;
@@ -102,7 +109,7 @@ struct ckd_ctx {
     size_t stagingCap = 0;
 
     // launch-shape tuning (ckd_autotune_batch / ckd_set_tuning)
-    ckd_tuning tuning = {2, 32, 8, 32, 0};
+    ckd_tuning tuning = kCkdDefaultTuning;
 
     // per-kernel HIP-event timing (ckd_kernel_timing)
     bool timing = false;

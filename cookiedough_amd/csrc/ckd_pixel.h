@@ -11,8 +11,15 @@
 
 // a * b for a, b < 2^24 with a product below 2^32: v_mul_u32_u24, full rate
 // (the compiler otherwise picks the quarter-rate 32-bit multiply where it
-// cannot prove the operands' range)
+// cannot prove the operands' range); a uniform operand stays in an SGPR
 CKD_DEV uint32_t ckd_mul_u24(uint32_t a, uint32_t b)
+{
+    return __umul24(a, b);
+}
+
+// the same product where only its low bits are used: the compiler turns
+// __umul24 there into v_mul_lo_u32 (quarter rate), so it is written out
+CKD_DEV uint32_t ckd_mul_u24_v(uint32_t a, uint32_t b)
 {
     uint32_t r;
     asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
@@ -155,35 +162,34 @@ CKD_DEV void ckd_span_setup(CkdSpan &s, uint32_t A, uint32_t B, uint32_t length,
 // pre-step) and its low word has a zero high half: _mm_madd_epi16 of a
 // channel delta d (|d| <= 255, its high half the sign) with m is
 // d * int16(m).  The divisor's high half is 1 only for length 1, where the
-// step is d * 0 + (d < 0 ? -1 : 0).  Equal to ckd_span_setup on every such
-// input (checked exhaustively over lengths up to 2200).
+// step of ckd_span_setup is d * 0 + (d < 0 ? -1 : 0); such a span draws one
+// pixel, from acc, and never adds its step, so the step here is d * int16
+// of the divisor for every length.  acc, and step for length >= 2, equal
+// ckd_span_setup's.
 CKD_DEV void ckd_span_setup_drawn(CkdSpan &s, uint32_t A, uint32_t B, uint32_t length,
                                   uint32_t drawLength, uint32_t divisor)
 {
-    const int32_t m = (int32_t)(int16_t)ckd_mul_u24(divisor, length - drawLength);
+    const int32_t m = (int32_t)(int16_t)ckd_mul_u24_v(divisor, length - drawLength);
     const int32_t dlo = (int32_t)(int16_t)divisor;
-    const int32_t dhi = (int32_t)(divisor >> 16);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const int32_t a = (int32_t)((A >> (8 * k)) & 0xff), b = (int32_t)((B >> (8 * k)) & 0xff);
         const int32_t d = b - a;
-        s.step[k] = (uint32_t)(d * dlo - (d < 0 ? dhi : 0));
+        s.step[k] = (uint32_t)(d * dlo);
         s.acc[k] = ((uint32_t)a << 16) + ((k & 1) ? 0u : (uint32_t)(d * m));
     }
 }
 
 // one span pixel: v = acc>>16 (logical); packus_epi32 keeps v (<= 65535);
-// packus_epi16 maps v > 32767 to 0 and clamps to 255.
+// packus_epi16 maps v > 32767 to 0 and clamps to 255.  v > 32767 is a
+// negative acc, so the channel is byte 2 of acc clamped (signed) to
+// [0, 0xffffff]: one v_med3_i32 per channel, two byte permutes per pixel.
 CKD_DEV uint32_t ckd_span_pixel(const uint32_t acc[4])
 {
-    uint32_t px = 0;
+    uint32_t m[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint32_t v = acc[k] >> 16;
-        const uint32_t b = v > 32767u ? 0u : (v > 255u ? 255u : v);
-        px |= b << (8 * k);
-    }
-    return px;
+    for (int k = 0; k < 4; ++k) m[k] = (uint32_t)min(max((int32_t)acc[k], 0), 0xffffff);
+    return ckd_perm(m[1], m[0], 0x0c0c0602u) | ckd_perm(m[3], m[2], 0x06020c0cu);
 }
 
 // ---- 2007 box blur helpers --------------------------------------------

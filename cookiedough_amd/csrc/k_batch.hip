@@ -272,6 +272,11 @@ ckd_status valid_tuning(const ckd_tuning *t)
                 t->tunnelVSeg);
     CKD_REQUIRE(t->blurKernel >= 0 && t->blurKernel <= 2, "tuning: blurKernel %d not in {0,1,2}",
                 t->blurKernel);
+    CKD_REQUIRE(t->voxelChunks == 0 || t->voxelChunks == 1 || t->voxelChunks == 2 || t->voxelChunks == 4 ||
+                    t->voxelChunks == 8,
+                "tuning: voxelChunks %d not in {0,1,2,4,8}", t->voxelChunks);
+    CKD_REQUIRE(t->voxelWavesPerSIMD >= 0 && t->voxelWavesPerSIMD <= 8,
+                "tuning: voxelWavesPerSIMD %d outside 0..8", t->voxelWavesPerSIMD);
     return CKD_OK;
 }
 
@@ -293,6 +298,8 @@ ckd_status ckd_set_tuning(ckd_ctx *ctx, const ckd_tuning *t)
     if (s != CKD_OK) return s;
     CKD_REQUIRE_CTX(ctx);
     ctx->tuning = *t;
+    if (t->voxelChunks == 0) ctx->tuning.voxelChunks = kCkdDefaultTuning.voxelChunks;
+    if (t->voxelWavesPerSIMD == 0) ctx->tuning.voxelWavesPerSIMD = kCkdDefaultTuning.voxelWavesPerSIMD;
     return CKD_OK;
 }
 
@@ -332,7 +339,7 @@ ckd_status ckd_autotune_batch(ckd_ctx *ctx, const ckd_request *requests, unsigne
         for (int ch : chunks)
             for (int bp : bpc)
                 for (int sg : segs) {
-                    ctx->tuning = ckd_tuning{st, ch, bp, sg, saved.blurKernel};
+                    ctx->tuning = ckd_tuning{st, ch, bp, sg, saved.blurKernel, saved.voxelChunks, saved.voxelWavesPerSIMD};
                     // warm once, then time the second run
                     s = run_eager(ctx, requests, n, scene, shading, tex, texLog2, tmp, nullptr);
                     if (s == CKD_OK) {

@@ -10,6 +10,7 @@
 #include "ckd_pixel.h"
 #include "ckd_jobs.h"
 
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -124,21 +125,21 @@ __global__ __launch_bounds__(256) void k_ls_quads(const uint8_t *__restrict__ hm
     quadC[i] = make_uint4(cmap[u + v0], cmap[u1 + v0], cmap[u + v1], cmap[u1 + v1]);
 }
 
-// vscape_ray (code/landscape.cpp:56-107) for column iRay of one frame, then
-// the fused memset32(fog[0]) for the rows no span reached
-CKD_DEV void ls_column(global_u32 *__restrict__ col, unsigned W, unsigned H, unsigned iRay,
-                       float rayY, float colScale, float halfW, int mapScale, float X1, float Y1,
-                       float viewCos, float viewSin, int mapTilt,
-                       const uint32_t *__restrict__ quadH, const uint4 *__restrict__ quadC,
-                       const uint32_t *__restrict__ fog)
+// vscape_ray (code/landscape.cpp:56-107) for column iRay of one frame, march
+// steps [s0, s1).  drawnIn() gives the highest row the steps before s0 drew
+// (H when s0 is 0); it is called once the first texels are in flight.  The
+// fused memset32(fog[0]) for the rows no span reached runs when s1 ends the
+// march.  Returns the highest row drawn after step s1 - 1.
+template <typename DrawnIn>
+CKD_DEV int ls_column(global_u32 *__restrict__ col, unsigned W, unsigned H, unsigned iRay,
+                      float rayY, float colScale, float halfW, int mapScale, float X1, float Y1,
+                      float viewCos, float viewSin, int mapTilt,
+                      const uint32_t *__restrict__ quadH, const uint4 *__restrict__ quadC,
+                      const uint32_t *__restrict__ fog, unsigned s0, unsigned s1, DrawnIn drawnIn)
 {
     int dX, dY, fpFishMul;
     ls_setup(rayY, colScale, halfW, X1, Y1, viewCos, viewSin, iRay, dX, dY, fpFishMul);
     int curX = (int)(X1 * 256.f), curY = (int)(Y1 * 256.f);
-
-    int lastHeight = (int)H, lastDrawn = (int)H;
-    uint32_t lastColor =
-        quadC[(((unsigned)(curX >> 8)) & kMapAnd) | ((((unsigned)(curY >> 8)) & kMapAnd) << kMapShift)].x;
 
     // The march is software-pipelined: the texels of the next kLsAhead steps
     // are in flight while a step draws its span (only curX/curY carry from
@@ -158,19 +159,41 @@ CKD_DEV void ls_column(global_u32 *__restrict__ col, unsigned W, unsigned H, uns
         t.h4 = quadH[i];
         t.c = quadC[i];
     };
-    auto step = [&](unsigned iStep, const Texels &t) {
+    // a step's projected row and fogged colour (f = fog[iStep >> 1], den =
+    // fpFishMul * (iStep + 1))
+    auto sample = [&](const Texels &t, uint32_t f, uint32_t den, int &hgt, uint32_t &color) {
         const uint32_t h = ckd_bilerp8(t.h4 & 0xff, (t.h4 >> 8) & 0xff, (t.h4 >> 16) & 0xff, t.h4 >> 24,
                                        t.fu, t.fv);
-        const uint32_t f = fog[iStep >> 1];
-        const uint32_t color = ls_bilerp_fog(t.c.x, t.c.y, t.c.z, t.c.w, t.fu, t.fv, f & 0x00ff00ffu,
-                                             (f >> 8) & 0x00ff00ffu);
+        color = ls_bilerp_fog(t.c.x, t.c.y, t.c.z, t.c.w, t.fu, t.fv, f & 0x00ff00ffu, (f >> 8) & 0x00ff00ffu);
 
         // ((255-h)<<16) / (fpFishMul*(iStep+1)) is an unsigned division
         // fpFishMul <= 256 and iStep < 512: 24-bit multiplies are exact;
         // q < 2^17 (fpFishMul >= 191 at every width) and mapScale < 2^11
-        const uint32_t q = udiv24((255u - h) << 16, ckd_mul_u24((uint32_t)fpFishMul, iStep + 1));
-        const int hgt = ((int)ckd_mul_u24(q, (uint32_t)mapScale) >> 8) + mapTilt;
+        const uint32_t q = udiv24((255u - h) << 16, den);
+        hgt = ((int)ckd_mul_u24(q, (uint32_t)mapScale) >> 8) + mapTilt;
+    };
 
+    int lastHeight = (int)H, lastDrawn;
+    uint32_t lastColor;
+    if (s0 == 0) {
+        lastColor = quadC[(((unsigned)(curX >> 8)) & kMapAnd) | ((((unsigned)(curY >> 8)) & kMapAnd) << kMapShift)].x;
+    } else {
+        // the march's state after step s0 - 1: that step's row and colour
+        // (s0 - 1 additions of dX wrap as one multiply-add does)
+        curX = (int)((uint32_t)curX + (uint32_t)dX * (s0 - 1));
+        curY = (int)((uint32_t)curY + (uint32_t)dY * (s0 - 1));
+        Texels t;
+        fetch(t);
+        sample(t, fog[(s0 - 1) >> 1], ckd_mul_u24((uint32_t)fpFishMul, s0), lastHeight, lastColor);
+    }
+
+    // den = fpFishMul * (iStep + 1), one add per step (below 2^17)
+    uint32_t den = ckd_mul_u24((uint32_t)fpFishMul, s0);
+    auto step = [&](const Texels &t, uint32_t f) {
+        den += (uint32_t)fpFishMul;
+        int hgt;
+        uint32_t color;
+        sample(t, f, den, hgt, color);
         if (hgt < lastDrawn) {
             const uint32_t drawLength = (uint32_t)(lastDrawn - hgt);
             const uint32_t length = (uint32_t)(lastHeight - hgt);
@@ -192,24 +215,32 @@ CKD_DEV void ls_column(global_u32 *__restrict__ col, unsigned W, unsigned H, uns
     // two register sets, A and B, take turns: a set is refilled only after
     // it has been consumed, so each fetched set lands in the registers its
     // consumer reads and the loop carries no copies of values in flight
-    static_assert(kRayLength % (2 * kLsAhead) == 0, "the march runs in whole groups");
+    // s0 and s1 are multiples of 2 * kLsAhead = 8, so a group's eight fog
+    // words fog[iStep/2 .. iStep/2 + 3] are one aligned 16-byte load
+    static_assert(kRayLength % (2 * kLsAhead) == 0 && 2 * kLsAhead == 8, "the march runs in whole groups");
     Texels A[kLsAhead], B[kLsAhead];
 #pragma unroll
     for (int k = 0; k < kLsAhead; ++k) fetch(A[k]);
-    for (unsigned iStep = 0; iStep < kRayLength; iStep += 2 * kLsAhead) {
+    lastDrawn = drawnIn();
+    for (unsigned iStep = s0; iStep < s1; iStep += 2 * kLsAhead) {
+        const uint4 fg = *reinterpret_cast<const uint4 *>(fog + (iStep >> 1));
+        const uint32_t f8[4] = {fg.x, fg.y, fg.z, fg.w};
 #pragma unroll
         for (int k = 0; k < kLsAhead; ++k) {
             fetch(B[k]);
-            step(iStep + k, A[k]);
+            step(A[k], f8[k >> 1]);
         }
 #pragma unroll
         for (int k = 0; k < kLsAhead; ++k) {
             fetch(A[k]);
-            step(iStep + kLsAhead + k, B[k]);
+            step(B[k], f8[(kLsAhead + k) >> 1]);
         }
     }
-    const uint32_t sky = fog[0];
-    for (int y = 0; y < lastDrawn; ++y) col[(size_t)y * W] = sky;
+    if (s1 == kRayLength) {
+        const uint32_t sky = fog[0];
+        for (int y = 0; y < lastDrawn; ++y) col[(size_t)y * W] = sky;
+    }
+    return lastDrawn;
 }
 
 __global__ __launch_bounds__(64) void k_landscape(uint32_t *__restrict__ dst, unsigned W,
@@ -221,7 +252,96 @@ __global__ __launch_bounds__(64) void k_landscape(uint32_t *__restrict__ dst, un
     if (iRay >= W) return;
     const LsFrame &F = P.f[blockIdx.y];
     ls_column((global_u32 *)(dst + (size_t)blockIdx.y * W * H + iRay), W, H, iRay, P.rayY, P.colScale, P.halfW,
-              P.mapScale, F.X1, F.Y1, F.viewCos, F.viewSin, F.mapTilt, quadH, quadC, fog);
+              P.mapScale, F.X1, F.Y1, F.viewCos, F.viewSin, F.mapTilt, quadH, quadC, fog, 0, kRayLength,
+              [H] { return (int)H; });
+}
+
+// ---- the march split into step ranges, dequeued ---------------------------
+//
+// A launch of many frames has nGroups waves of 64 columns; when that is not
+// close to a multiple of the SIMD count, the SIMDs that hold one wave more
+// than the others set the launch time (120 frames of 1280 columns: 2400
+// waves on 1024 SIMDs take as long as 3072).  Split each column's march into 2^c
+// ranges of 512 >> c steps: unit u = k * nGroups + g is range k of group g.
+// Persistent waves take units from a counter in that order.  Range k > 0
+// needs only one value from range k - 1: the highest row drawn so far.  Range
+// k - 1 publishes it per column, tagged (epoch, k), in a 64-bit state word;
+// range k reads it (after its first texels are in flight) and waits while
+// the tag is not there yet.  Range k - 1 was taken earlier, by a wave that
+// runs, and depends only on ranges taken before it, so every wait ends; the
+// wait is also bounded, and a wait that runs out counts in ctl[3] and
+// carries on from row H.
+//
+// ctl: [0] next unit, [1] epoch, [2] waves finished, [3] waits that ran out.
+// The last wave to finish resets [0] and [2] and moves the epoch on, so the
+// next launch starts from a clean queue and tags of this one never match it.
+constexpr int kLsMaxChunkLog2 = 3;
+constexpr unsigned kLsWaitLimit = 1u << 18; // polls, with a sleep between them
+constexpr size_t kLsStateWords = (size_t)kLsFramesPerLaunch * 4096;
+
+CKD_DEV int ls_wait_drawn(const unsigned long long *w, unsigned long long want, int H,
+                          uint32_t *timeouts)
+{
+    unsigned long long v = 0;
+    bool ready = false;
+    for (unsigned spin = 0; spin < kLsWaitLimit; ++spin) {
+        if (!ready) {
+            v = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ready = (v >> 16) == want;
+        }
+        if (__ballot(!ready) == 0) return (int)(v & 0xffffu);
+        __builtin_amdgcn_s_sleep(2);
+    }
+    if (!ready) atomicAdd(timeouts, 1u);
+    return ready ? (int)(v & 0xffffu) : H;
+}
+
+__global__ __launch_bounds__(64) void k_landscape_queue(uint32_t *__restrict__ dst, unsigned W,
+                                                        unsigned H, const uint32_t *__restrict__ quadH,
+                                                        const uint4 *__restrict__ quadC,
+                                                        const uint32_t *__restrict__ fog, LsLaunch P,
+                                                        unsigned groupsPerFrame, unsigned nGroups,
+                                                        unsigned chunkLog2, unsigned nWaves,
+                                                        uint32_t *__restrict__ ctl,
+                                                        unsigned long long *__restrict__ state)
+{
+    const unsigned lane = threadIdx.x;
+    const uint32_t epoch =
+        __builtin_amdgcn_readfirstlane(__hip_atomic_load(&ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    const unsigned steps = kRayLength >> chunkLog2, nChunks = 1u << chunkLog2;
+    const unsigned nUnits = nGroups << chunkLog2;
+    for (;;) {
+        uint32_t u = 0;
+        if (lane == 0) u = atomicAdd(&ctl[0], 1u);
+        u = __builtin_amdgcn_readfirstlane(u);
+        if (u >= nUnits) break;
+        const unsigned k = u / nGroups, g = u - k * nGroups;
+        const unsigned f = g / groupsPerFrame;
+        const unsigned iRay = (g - f * groupsPerFrame) * 64 + lane;
+        if (iRay < W) {
+            const LsFrame &F = P.f[f];
+            unsigned long long *w = state + (size_t)f * W + iRay;
+            const unsigned long long want = ((unsigned long long)epoch << 16) | k;
+            const int drawn = ls_column(
+                (global_u32 *)(dst + (size_t)f * W * H + iRay), W, H, iRay, P.rayY, P.colScale, P.halfW,
+                P.mapScale, F.X1, F.Y1, F.viewCos, F.viewSin, F.mapTilt, quadH, quadC, fog, k * steps,
+                (k + 1) * steps, [&] { return k == 0 ? (int)H : ls_wait_drawn(w, want, (int)H, &ctl[3]); });
+            if (k + 1 < nChunks)
+                __hip_atomic_store(w, ((unsigned long long)epoch << 32) | ((unsigned long long)(k + 1) << 16) |
+                                          (unsigned)drawn,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    // relaxed: a wave's last dequeue returned before this increment was
+    // issued, so the last wave to count sees every dequeue done (an
+    // acquire-release here would write back and invalidate the L2 in every
+    // wave, while frames are being written)
+    if (lane == 0 &&
+        __hip_atomic_fetch_add(&ctl[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == nWaves - 1) {
+        __hip_atomic_store(&ctl[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&ctl[2], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&ctl[1], epoch + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 // every voxel request of a batch in one launch: block -> job by binary search
@@ -235,7 +355,7 @@ __global__ __launch_bounds__(64) void k_landscape_jobs(const VoxJob *__restrict_
     const unsigned iRay = (blockIdx.x - J.blockBase) * 64 + threadIdx.x;
     if (iRay >= J.W) return;
     ls_column((global_u32 *)(J.dst + iRay), J.W, J.H, iRay, J.rayY, J.colScale, J.halfW, J.mapScale, J.X1, J.Y1,
-              J.viewCos, J.viewSin, J.mapTilt, quadH, quadC, fog);
+              J.viewCos, J.viewSin, J.mapTilt, quadH, quadC, fog, 0, kRayLength, [&J] { return (int)J.H; });
 }
 
 // ---- tunnelscape ---------------------------------------------------------
@@ -409,6 +529,16 @@ ckd_status ckd_landscape_create(ckd_ctx *ctx, const uint8_t *heightMap, const ui
     if ((!ctx->lsQuadH && ckd::dev_alloc(&ctx->lsQuadH, kTexels * 4) != hipSuccess) ||
         (!ctx->lsQuadC && ckd::dev_alloc(&ctx->lsQuadC, kTexels * 16) != hipSuccess))
         return ckd::fail(CKD_ENOMEM, "ckd_landscape_create: hipMalloc of the texel quads failed");
+    if (!ctx->lsQueue) {
+        if (ckd::dev_alloc(&ctx->lsQueue, 4 * sizeof(uint32_t)) != hipSuccess)
+            return ckd::fail(CKD_ENOMEM, "ckd_landscape_create: hipMalloc of the work queue failed");
+        CKD_HIP(hipMemsetAsync(ctx->lsQueue, 0, 4 * sizeof(uint32_t), ctx->stream));
+    }
+    if (!ctx->lsState) {
+        if (ckd::dev_alloc(&ctx->lsState, kLsStateWords * sizeof(unsigned long long)) != hipSuccess)
+            return ckd::fail(CKD_ENOMEM, "ckd_landscape_create: hipMalloc of the queue's state failed");
+        CKD_HIP(hipMemsetAsync(ctx->lsState, 0, kLsStateWords * sizeof(unsigned long long), ctx->stream));
+    }
     k_ls_quads<<<kTexels / 256, 256, 0, ctx->stream>>>(ctx->lsHeight, ctx->lsColor, ctx->lsQuadH,
                                                        ctx->lsQuadC);
     s = ckd::check_launch("landscape quads");
@@ -440,10 +570,27 @@ ckd_status ckd_landscape_draw(ckd_ctx *ctx, uint32_t *dst, unsigned W, unsigned 
             const ckd_landscape_cam &c = cams[f0 + k];
             P.f[k] = LsFrame{c.X1, c.Y1, c.viewCos, c.viewSin, scale_rows(c.mapTilt, H)};
         }
-        dim3 grid(ckd_div_up(W, 64), n);
+        const unsigned groupsPerFrame = ckd_div_up(W, 64), nGroups = groupsPerFrame * n;
+        const unsigned simds = 4u * (unsigned)ctx->cuCount;
+        const int chunkLog2 = __builtin_ctz((unsigned)ctx->tuning.voxelChunks);
+        // one wave per group puts ceil(nGroups / simds) waves on the busiest
+        // SIMDs; the work queue pays when that is well above the average
+        const unsigned busiest = ckd_div_up(nGroups, simds);
+        const bool uneven = nGroups > simds && (double)nGroups < 0.9 * (double)busiest * simds;
         ckd::KernelTimer kt(ctx, CKD_KT_LANDSCAPE);
-        k_landscape<<<grid, 64, 0, ctx->stream>>>(dst + (size_t)f0 * W * H, W, H, ctx->lsQuadH,
-                                                  ctx->lsQuadC, ctx->lsFog, P);
+        if (chunkLog2 > 0 && uneven) {
+            // the march ranges through the work queue
+            static_assert(kLsMaxChunkLog2 == 3, "voxelChunks is at most 8");
+            const unsigned nUnits = nGroups << chunkLog2;
+            const unsigned nWaves = std::min(nUnits, simds * (unsigned)ctx->tuning.voxelWavesPerSIMD);
+            k_landscape_queue<<<nWaves, 64, 0, ctx->stream>>>(
+                dst + (size_t)f0 * W * H, W, H, ctx->lsQuadH, ctx->lsQuadC, ctx->lsFog, P, groupsPerFrame,
+                nGroups, (unsigned)chunkLog2, nWaves, ctx->lsQueue, ctx->lsState);
+        } else {
+            dim3 grid(groupsPerFrame, n);
+            k_landscape<<<grid, 64, 0, ctx->stream>>>(dst + (size_t)f0 * W * H, W, H, ctx->lsQuadH,
+                                                      ctx->lsQuadC, ctx->lsFog, P);
+        }
         ckd_status s = ckd::check_launch("landscape");
         if (s != CKD_OK) return s;
     }

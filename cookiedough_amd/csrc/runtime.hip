@@ -245,7 +245,7 @@ void ckd_destroy(ckd_ctx *ctx)
         if (pm.map) (void)hipFree(pm.map);
         if (pm.inv) (void)hipFree(pm.inv);
     }
-    void *bufs[] = {ctx->texQuads, ctx->cosLut, ctx->lsHeight, ctx->lsColor, ctx->lsFog, ctx->lsQuadH, ctx->lsQuadC,
+    void *bufs[] = {ctx->texQuads, ctx->cosLut, ctx->lsHeight, ctx->lsColor, ctx->lsFog, ctx->lsQuadH, ctx->lsQuadC, ctx->lsQueue, ctx->lsState,
                     ctx->tsHeight, ctx->tsColor,
                     ctx->tsFog, ctx->b26Scratch[0],
                     ctx->b26Scratch[1], ctx->staging, ctx->fdTex, ctx->fdTexHi,

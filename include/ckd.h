@@ -193,7 +193,11 @@ typedef struct ckd_landscape_cam {
 /* Landscape_Draw without TapeWarp32 (code/landscape.cpp:228-243): fills dst
  * (W x H, W <= 4096, H <= 2160) for each of nFrames cameras; frame f is
  * written at dst + f*W*H.  The 720-row constants kMapScale and s_mapTilt
- * are scaled by H/720 and the column spacing by 1280/W. */
+ * are scaled by H/720 and the column spacing by 1280/W.  Up to 160 frames
+ * run in one launch; a launch whose 64-column groups do not spread evenly
+ * over the GPU's SIMDs splits the march into ranges taken from a work queue
+ * the context holds (ckd_tuning.voxelChunks), so two draws of one context
+ * must not overlap (calls on its one stream never do). */
 CKD_API ckd_status ckd_landscape_draw(ckd_ctx *ctx, uint32_t *dst, unsigned W,
                                       unsigned H, const ckd_landscape_cam *cams,
                                       unsigned nFrames);
@@ -285,8 +289,9 @@ typedef struct ckd_request {
 } ckd_request;
 
 /* Launch shapes of the batched renderer, chosen by ckd_autotune_batch (or
- * set directly), and the kernel of the batched in-place 2007 blurs of the
- * Draw calls.  Defaults: {2, 32, 8, 32, 0}. */
+ * set directly), the kernel of the batched in-place 2007 blurs of the Draw
+ * calls and the split of ckd_landscape_draw's launches (ckd_autotune_batch
+ * leaves the last three as they are).  Defaults: {2, 32, 8, 32, 0, 2, 2}. */
 typedef struct ckd_tuning {
     int sdfStepsPerRefill; /* march steps between lane refills: 1, 2 or 4   */
     int sdfChunk;          /* side of the square pixel chunks: 16 or 32     */
@@ -297,6 +302,15 @@ typedef struct ckd_tuning {
     int tunnelVSeg;        /* rows per lane in the vertical blur: 8..512     */
     int blurKernel;        /* in-place blur batches: 0 automatic, 1 lines
                               staged in LDS, 2 an LDS ring per lane          */
+    int voxelChunks;       /* ckd_landscape_draw launches whose 64-column
+                              groups (one wave each) would load the busiest
+                              SIMDs over 1/0.9 times the average: each
+                              column's 512-step march in this many ranges
+                              taken from a work queue, so the SIMDs share
+                              the work evenly; 1 (no queue), 2, 4 or 8; 0
+                              sets the default                              */
+    int voxelWavesPerSIMD; /* persistent waves of that queue per SIMD: 1..8;
+                              0 sets the default                            */
 } ckd_tuning;
 CKD_API ckd_status ckd_get_tuning(ckd_ctx *ctx, ckd_tuning *out);
 CKD_API ckd_status ckd_set_tuning(ckd_ctx *ctx, const ckd_tuning *t);

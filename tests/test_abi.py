@@ -98,7 +98,8 @@ def test_batch_validation():
 def test_tuning_validation_without_gpu():
     lib = _lib.load()
     for bad in [(3, 32, 8, 32), (2, 24, 8, 32), (2, 32, 0, 32), (2, 32, 8, 4), (2, 32, 8, 1024),
-                (2, 32, 8, 32, 3), (2, 32, 8, 32, -1)]:
+                (2, 32, 8, 32, 3), (2, 32, 8, 32, -1), (2, 32, 8, 32, 0, 3), (2, 32, 8, 32, 0, 16),
+                (2, 32, 8, 32, 0, 8, 9), (2, 32, 8, 32, 0, 8, -1)]:
         t = _lib.Tuning(*bad)
         assert lib.ckd_set_tuning(None, C.byref(t)) == CKD_EINVAL
         assert "tuning" in _lib.last_error()

@@ -63,7 +63,11 @@ def test_lazy_scratch_paths_recover(lib):
         assert np.array_equal(*outs)
 
         m = wl.landscape_maps(0)
-        for n in range(5):  # height, colour, fog, then the two quad maps
+        # the n-th allocation of the call fails: height, colour, fog, then the
+        # first of the context's own buffers not yet allocated (a buffer that
+        # was allocated stays): the two quad maps, then the draw's work queue
+        # state
+        for n in range(6):
             lib.ckd_debug_fail_alloc(n)
             _expect_enomem(lambda: c.landscape_create(m.height, m.color, m.fog))
         lib.ckd_debug_fail_alloc(-1)
