@@ -59,6 +59,22 @@ CKD_DEV uint32_t ls_bilerp_fog(uint32_t p0, uint32_t p1, uint32_t p2, uint32_t p
     return ckd_u32_of(sbr) | (ckd_u32_of(sga) << 8);
 }
 
+// (255 - bsamp8(quad)) << 16 (code/bilinear.h:34-53, code/landscape.cpp:
+// 76-86) for a height quad h4 holding (u,v), (u+1,v), (u,v+1), (u+1,v+1) in
+// bytes 0..3: both horizontal lerps in the 16-bit lanes of (h0,h2) and
+// (h1,h3), the vertical one as the sum of the lanes of (s01*(256-fv),
+// s23*fv).  Every intermediate lies in [0, 65280], as in the scalar form.
+CKD_DEV uint32_t ls_height_num(uint32_t h4, uint32_t fu, uint32_t fv)
+{
+    const ckd_u16x2 f = ckd_splat16(fu), nf = (unsigned short)256 - f;
+    const ckd_u16x2 s = ckd_lerp2x(ckd_u16x2_of(ckd_perm(0u, h4, 0x0c020c00u)),  // h0, h2
+                                   ckd_u16x2_of(ckd_perm(0u, h4, 0x0c030c01u)),  // h1, h3
+                                   nf, f);                                       // s01, s23
+    const ckd_u16x2 p = s * ckd_u16x2{(unsigned short)(256u - fv), (unsigned short)fv};
+    const uint32_t sum = (uint32_t)p[0] + (uint32_t)p[1];
+    return 0xff0000u - ckd_perm(0u, sum, 0x0c010c0cu); // (255 - (sum >> 8)) << 16
+}
+
 // exact floor(num/den) for num < 2^24, 0 < den < 2^24: a reciprocal
 // estimate is within one of the quotient; one correction each way fixes it.
 // q*den stays below 2^25, so the 24-bit multiply (full rate; the 32-bit
@@ -162,14 +178,12 @@ CKD_DEV int ls_column(global_u32 *__restrict__ col, unsigned W, unsigned H, unsi
     // a step's projected row and fogged colour (f = fog[iStep >> 1], den =
     // fpFishMul * (iStep + 1))
     auto sample = [&](const Texels &t, uint32_t f, uint32_t den, int &hgt, uint32_t &color) {
-        const uint32_t h = ckd_bilerp8(t.h4 & 0xff, (t.h4 >> 8) & 0xff, (t.h4 >> 16) & 0xff, t.h4 >> 24,
-                                       t.fu, t.fv);
         color = ls_bilerp_fog(t.c.x, t.c.y, t.c.z, t.c.w, t.fu, t.fv, f & 0x00ff00ffu, (f >> 8) & 0x00ff00ffu);
 
         // ((255-h)<<16) / (fpFishMul*(iStep+1)) is an unsigned division
         // fpFishMul <= 256 and iStep < 512: 24-bit multiplies are exact;
         // q < 2^17 (fpFishMul >= 191 at every width) and mapScale < 2^11
-        const uint32_t q = udiv24((255u - h) << 16, den);
+        const uint32_t q = udiv24(ls_height_num(t.h4, t.fu, t.fv), den);
         hgt = ((int)ckd_mul_u24(q, (uint32_t)mapScale) >> 8) + mapTilt;
     };
 
