@@ -93,6 +93,8 @@ ckd_status validate(const char *fn, const ckd_request *requests, unsigned n,
         case CKD_FX_VOXEL:
             CKD_REQUIRE(r.vcam.mapTilt >= 0 && r.vcam.mapTilt <= 180,
                         "%s: request %u: mapTilt %d outside 0..180", fn, i, r.vcam.mapTilt);
+            CKD_REQUIRE(r.W <= 64 * r.H, "%s: request %u: voxel frame %ux%u wider than 64 times its height",
+                        fn, i, r.W, r.H);
             break;
         case CKD_FX_TUNNEL:
             CKD_REQUIRE(r.grid && tex, "%s: request %u: TUNNEL needs grid and texture", fn, i);

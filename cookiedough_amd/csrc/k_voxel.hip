@@ -75,20 +75,19 @@ CKD_DEV uint32_t ls_height_num(uint32_t h4, uint32_t fu, uint32_t fv)
     return 0xff0000u - ckd_perm(0u, sum, 0x0c010c0cu); // (255 - (sum >> 8)) << 16
 }
 
-// exact floor(num/den) for num < 2^24, 0 < den < 2^24: a reciprocal
-// estimate is within one of the quotient; one correction each way fixes it.
-// q*den stays below 2^25, so the 24-bit multiply (full rate; the 32-bit
-// one is quarter rate) is exact.
+// exact floor(num/den) for num < 2^24 and 0 < den < 2^24 with a quotient
+// below 2^22: num * rcp(den) is then within 0.5 of num/den, so the estimate
+// floor(num * rcp(den) - 0.5) is the quotient or one less, and one upward
+// correction fixes it.  q*den stays below 2^25, so the 24-bit multiply (full
+// rate; the 32-bit one is quarter rate) is exact.  Checked on the host over
+// every (num, den) of the march with fpFishMul >= 4 (W <= 64 H gives >= 8)
+// and every span divisor, with the reciprocal rounded to nearest and one ulp
+// off either way.
 CKD_DEV uint32_t udiv24(uint32_t num, uint32_t den)
 {
-    uint32_t q = (uint32_t)((float)num * __builtin_amdgcn_rcpf((float)den));
-    int32_t r = (int32_t)(num - ckd_mul_u24(q, den));
-    if (r < 0) {
-        --q;
-        r += (int32_t)den;
-    }
-    if (r >= (int32_t)den) ++q;
-    return q;
+    const float qf = __builtin_fmaf((float)num, __builtin_amdgcn_rcpf((float)den), -0.5f);
+    const uint32_t q = (uint32_t)__builtin_amdgcn_fmed3f(qf, 0.f, 16777215.f);
+    return q + (num - ckd_mul_u24(q, den) >= den ? 1u : 0u);
 }
 
 // Per-column ray set-up of vscape (code/landscape.cpp:164-190) and
@@ -566,6 +565,8 @@ ckd_status ckd_landscape_draw(ckd_ctx *ctx, uint32_t *dst, unsigned W, unsigned 
     CKD_REQUIRE(dst && cams, "ckd_landscape_draw: null buffer");
     CKD_REQUIRE(W >= 8 && H >= 8 && W <= 4096 && H <= 2160, "ckd_landscape_draw: bad size %ux%u",
                 W, H);
+    // the edge columns' fpFishMul is at least 8 then (udiv24 needs 4)
+    CKD_REQUIRE(W <= 64 * H, "ckd_landscape_draw: %ux%u is wider than 64 times its height", W, H);
     for (unsigned f = 0; f < nFrames; ++f)
         CKD_REQUIRE(cams[f].mapTilt >= 0 && cams[f].mapTilt <= 180,
                     "ckd_landscape_draw: frame %u: mapTilt %d outside 0..180", f, cams[f].mapTilt);

@@ -69,6 +69,24 @@ def test_landscape_rejects_negative_tilt():
     assert "mapTilt" in _lib.last_error()
 
 
+def test_landscape_rejects_frames_too_wide_for_their_height():
+    """W > 64 H would put the edge columns' fish-eye factor below the range
+    in which the march's division is exact"""
+    lib = _lib.load()
+    cam = _lib.LandscapeCam(0, 0, 1, 0, 90)
+    assert lib.ckd_landscape_draw(None, 8, 4096, 63, C.byref(cam), 1) == CKD_EINVAL
+    assert "wider" in _lib.last_error()
+    assert lib.ckd_landscape_draw(None, 8, 4096, 64, C.byref(cam), 1) == CKD_EINVAL
+    assert "null context" in _lib.last_error()
+    r = _lib.Request()
+    r.effect = 1
+    r.W, r.H, r.dst = 1024, 15, 8
+    r.vcam = cam
+    arr = (_lib.Request * 1)(r)
+    assert lib.ckd_render_batch(None, arr, 1, None, 0, None, 9, None, 0, None) == CKD_EINVAL
+    assert "wider" in _lib.last_error()
+
+
 def test_sdf_rejects_bad_scene():
     lib = _lib.load()
     sc = _lib.SdfScene()
