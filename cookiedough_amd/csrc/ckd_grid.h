@@ -35,8 +35,11 @@ CKD_DEV uint32_t grid_sample_quad(const uint4 *__restrict__ quads, unsigned texL
     // byte offset below 2^32 (texLog2 <= 13)
     const uint32_t k = ((v0 << texLog2) + u0) << 4;
     const uint4 q = *(const uint4 *)((const char *)quads + k);
-    return ckd_bilerp32(q.x, q.y, q.z, q.w, __builtin_amdgcn_ubfe(X, 6, 8),
-                        __builtin_amdgcn_ubfe(Y, 6, 8));
+    // the fractions as one v_bfe_u32 each (the compiler otherwise splits
+    // them into a 16-bit shift and a mask)
+    uint32_t fu = __builtin_amdgcn_ubfe(X, 6, 8), fv = __builtin_amdgcn_ubfe(Y, 6, 8);
+    asm("" : "+v"(fu), "+v"(fv));
+    return ckd_bilerp32(q.x, q.y, q.z, q.w, fu, fv);
 }
 
 // The bilinear sample at the interpolated (U, V), both 24:8 fixed point

@@ -147,8 +147,9 @@ __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict_
         // opaque to the compiler, so that __mul24 stays v_mad_i32_i24 rather
         // than becoming a 64-bit multiply-add on i's known small range
         asm("" : "+v"(i));
-        auto remap = [&](int x) {
-            const uint4 e = E[x >> 3];
+        // pixel x of the row, b = x >> 3 its block
+        auto remap = [&](int b) {
+            const uint4 e = E[b];
             const uint32_t iu = kNarrow ? (uint32_t)__mul24((int)i, (int)e.z) : i * e.z;
             const uint32_t iv = kNarrow ? (uint32_t)__mul24((int)i, (int)e.w) : i * e.w;
             if constexpr (kQuads)
@@ -158,9 +159,11 @@ __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict_
         };
         int x0 = threadIdx.x;
         for (; x0 + 3 * kT1Block < W; x0 += 4 * kT1Block) {
+            // the four pixels' blocks are 16 apart: one base, constant offsets
+            const int b0 = x0 >> 3;
             uint32_t v[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) v[k] = remap(x0 + k * kT1Block);
+            for (int k = 0; k < 4; ++k) v[k] = remap(b0 + k * (kT1Block / 8));
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 P[x0 + k * kT1Block + 1] = ckd_spread(v[k]);
@@ -168,7 +171,7 @@ __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict_
             }
         }
         for (; x0 < W; x0 += kT1Block) {
-            const uint32_t v = remap(x0);
+            const uint32_t v = remap(x0 >> 3);
             P[x0 + 1] = ckd_spread(v);
             __builtin_nontemporal_store(v, &drow[x0]);
         }
