@@ -15,7 +15,7 @@
 #define CKD_NUM_EVENTS 16
 
 // launch shapes of a new context
-constexpr ckd_tuning kCkdDefaultTuning = {2, 32, 8, 32, 0, 2, 2};
+constexpr ckd_tuning kCkdDefaultTuning = {16, 32, 8, 32, 0, 2, 2};
 
 struct ckd_ctx {
     int device = 0;

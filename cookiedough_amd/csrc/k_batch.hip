@@ -264,8 +264,9 @@ ckd_status run_eager(ckd_ctx *ctx, const ckd_request *requests, unsigned n,
 ckd_status valid_tuning(const ckd_tuning *t)
 {
     CKD_REQUIRE(t != nullptr, "tuning: null pointer");
-    CKD_REQUIRE(t->sdfStepsPerRefill == 1 || t->sdfStepsPerRefill == 2 || t->sdfStepsPerRefill == 4,
-                "tuning: sdfStepsPerRefill %d not in {1,2,4}", t->sdfStepsPerRefill);
+    CKD_REQUIRE(t->sdfStepsPerRefill == 1 || t->sdfStepsPerRefill == 2 || t->sdfStepsPerRefill == 4 ||
+                    t->sdfStepsPerRefill == 8 || t->sdfStepsPerRefill == 16,
+                "tuning: sdfStepsPerRefill %d not in {1,2,4,8,16}", t->sdfStepsPerRefill);
     CKD_REQUIRE(t->sdfChunk == 16 || t->sdfChunk == 32, "tuning: sdfChunk %d not in {16,32}",
                 t->sdfChunk);
     CKD_REQUIRE(t->sdfBlocksPerCU >= 1 && t->sdfBlocksPerCU <= 32,
@@ -329,7 +330,7 @@ ckd_status ckd_autotune_batch(ckd_ctx *ctx, const ckd_request *requests, unsigne
     if (n == 0) return CKD_OK;
     // fewer persistent SDF blocks per CU leave room for the voxel / tunnel
     // branch on the second stream, which otherwise waits for them to drain
-    const int steps[] = {1, 2, 4}, chunks[] = {16, 32}, bpc[] = {2, 3, 4, 5, 6, 8},
+    const int steps[] = {4, 8, 16}, chunks[] = {16, 32}, bpc[] = {2, 3, 4, 5, 6, 8},
               segs[] = {16, 32, 64, 128, 256};
     const ckd_tuning saved = ctx->tuning;
     const bool timing = ctx->timing;

@@ -578,8 +578,12 @@ int persistent_blocks_per_cu(int steps)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_sdf_persistent<kHalf, kMaxSph, 1>, 256, 0);
     else if (steps == 2)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_sdf_persistent<kHalf, kMaxSph, 2>, 256, 0);
-    else
+    else if (steps == 4)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_sdf_persistent<kHalf, kMaxSph, 4>, 256, 0);
+    else if (steps == 8)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_sdf_persistent<kHalf, kMaxSph, 8>, 256, 0);
+    else
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_sdf_persistent<kHalf, kMaxSph, 16>, 256, 0);
     return e == hipSuccess ? n : 0;
 }
 
@@ -593,8 +597,12 @@ void launch_persistent_steps(int steps, unsigned blocks, hipStream_t st, uint32_
         k_sdf_persistent<kHalf, kMaxSph, 1><<<blocks, 256, 0, st>>>(queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk, bound, work);
     else if (steps == 2)
         k_sdf_persistent<kHalf, kMaxSph, 2><<<blocks, 256, 0, st>>>(queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk, bound, work);
-    else
+    else if (steps == 4)
         k_sdf_persistent<kHalf, kMaxSph, 4><<<blocks, 256, 0, st>>>(queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk, bound, work);
+    else if (steps == 8)
+        k_sdf_persistent<kHalf, kMaxSph, 8><<<blocks, 256, 0, st>>>(queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk, bound, work);
+    else
+        k_sdf_persistent<kHalf, kMaxSph, 16><<<blocks, 256, 0, st>>>(queueCounter, totalChunks, jobs, nJobs, sc, stepCount, chunk, bound, work);
 }
 
 // A sphere containing every sphere of the scene: centre the centroid of the

@@ -291,9 +291,10 @@ typedef struct ckd_request {
 /* Launch shapes of the batched renderer, chosen by ckd_autotune_batch (or
  * set directly), the kernel of the batched in-place 2007 blurs of the Draw
  * calls and the split of ckd_landscape_draw's launches (ckd_autotune_batch
- * leaves the last three as they are).  Defaults: {2, 32, 8, 32, 0, 2, 2}. */
+ * leaves the last three as they are).  Defaults: {16, 32, 8, 32, 0, 2, 2}. */
 typedef struct ckd_tuning {
-    int sdfStepsPerRefill; /* march steps between lane refills: 1, 2 or 4   */
+    int sdfStepsPerRefill; /* march steps between lane refills: 1, 2, 4, 8
+                              or 16                                         */
     int sdfChunk;          /* side of the square pixel chunks: 16 or 32     */
     int sdfBlocksPerCU;    /* persistent 4-wave blocks per CU: 1..32 while
                               voxel or tunnel work runs beside the march; a

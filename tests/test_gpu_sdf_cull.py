@@ -75,3 +75,27 @@ def test_work_counter(ctx):
     finally:
         ctx.sdf_work_counter(None)
     assert [int(v) for v in work.numpy()] == [0, 0]
+
+
+@pytest.mark.parametrize("n_spheres", [8, 20])
+def test_steps_per_refill_change_nothing(ctx, n_spheres):
+    """the lanes are refilled every 1, 2, 4, 8 or 16 march steps: the order of
+    work changes, the frames and the total march step count do not"""
+    sc = wl.sdf_scene(1, n_spheres, 96)
+    times = [0.7, 13.0, 41.5]
+    saved = ctx.get_tuning()
+    outs, counts = [], []
+    try:
+        for k in (1, 2, 4, 8, 16):
+            t = ctx.get_tuning()
+            t.sdfStepsPerRefill = k
+            ctx.set_tuning(t)
+            cnt = ctx.upload(np.zeros(1, np.uint64))
+            print("steps per refill", k, flush=True)
+            outs.append(_render(ctx, sc, 640, 360, times, CKD_SHADE_FP16, cnt))
+            counts.append(int(cnt.numpy()[0]))
+    finally:
+        ctx.set_tuning(saved)
+    for k, o in zip((2, 4, 8, 16), outs[1:]):
+        assert np.array_equal(o, outs[0]), k
+    assert len(set(counts)) == 1 and counts[0] > 0, counts
