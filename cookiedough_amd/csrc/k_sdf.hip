@@ -324,6 +324,10 @@ CKD_DEV uint32_t shade_hit(const ckd_sdf_scene &sc, const float *cam, float dx, 
     return shade_lit<kHalf>(sc, m, nx, ny, nz, id);
 }
 
+// lanes that must be idle before a wave refills them (checked every
+// kStepsPerRefill march steps)
+constexpr int kRefillMinIdle = 32;
+
 template <bool kHalf, int kMaxSph, int kStepsPerRefill>
 __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uint32_t *__restrict__ queueCounter,
                                                         uint32_t totalChunks,
@@ -363,9 +367,12 @@ __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uin
     uint32_t evalSteps = 0, evalShades = 0;
 
     while (true) {
-        // refill finished lanes from the current chunk
+        // refill finished lanes from the current chunk, once half the wave
+        // is idle: a refill runs the new rays' set-up for the whole wave, so
+        // it pays to refill many lanes at once (a wave whose lanes are all
+        // idle always refills)
         const uint64_t idle = __ballot(!active);
-        if (idle && next < end) {
+        if (idle && next < end && __popcll(idle) >= kRefillMinIdle) {
             const uint32_t rank = __popcll(idle & ((1ull << lane) - 1));
             if (!active) {
                 const uint32_t k = next + rank;
