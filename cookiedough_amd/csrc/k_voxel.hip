@@ -85,8 +85,9 @@ CKD_DEV uint32_t ls_height_num(uint32_t h4, uint32_t fu, uint32_t fv)
 // off either way.
 CKD_DEV uint32_t udiv24(uint32_t num, uint32_t den)
 {
+    // qf > -0.5: converting through a signed int truncates it to 0 there
     const float qf = __builtin_fmaf((float)num, __builtin_amdgcn_rcpf((float)den), -0.5f);
-    const uint32_t q = (uint32_t)__builtin_amdgcn_fmed3f(qf, 0.f, 16777215.f);
+    const uint32_t q = (uint32_t)(int32_t)qf;
     return q + (num - ckd_mul_u24(q, den) >= den ? 1u : 0u);
 }
 
