@@ -235,6 +235,27 @@ constexpr int kWavesPerBlock = 4;
 
 CKD_DEV uint32_t lane_id() { return __lane_id(); }
 
+// the lane's bit of the wave mask m ? a : b: one v_cndmask that reads m from
+// SGPRs (a select on a bool the compiler rebuilds from the mask costs it
+// three compares again)
+CKD_DEV float sel_lane(uint64_t m, float a, float b)
+{
+    float r;
+    asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(r) : "v"(b), "v"(a), "s"(m));
+    return r;
+}
+CKD_DEV int sel_lane(uint64_t m, int a, int b)
+{
+    int r;
+    asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(r) : "v"(b), "v"(a), "s"(m));
+    return r;
+}
+
+// v_cmp into a wave mask (LLVM predicates: 2 ordered >, 4 ordered <, 40
+// signed <); a __ballot of a bool made of several compares costs a
+// v_cndmask and a v_cmp more
+constexpr int kCmpGt = 2, kCmpLt = 4, kCmpSlt = 40;
+
 // a zero the compiler cannot see through: indexing LDS tables with it keeps
 // their loads inside the loop instead of hoisting every table entry into
 // registers for the whole kernel
@@ -363,8 +384,10 @@ __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uin
     float Mb = 0.f, E2b = 0.f;     // the same for the centre of the spheres' bounding sphere
     float bx = 0.f, by = 0.f, bz = 0.f; // o - C of the chunk's camera (uniform)
     unsigned long long mySteps = 0;
-    // march steps and shaded pixels of this lane that evaluated the spheres
-    uint32_t evalSteps = 0, evalShades = 0;
+    // lane march steps of the wave that evaluated the spheres (uniform), and
+    // shaded pixels of this lane that did
+    unsigned long long evalSteps = 0;
+    uint32_t evalShades = 0;
 
     while (true) {
         // refill finished lanes from the current chunk, once half the wave
@@ -447,11 +470,15 @@ __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uin
             __builtin_amdgcn_wave_barrier();
             continue;
         }
-        // march a few steps
+        // march a few steps; the lanes' conditions are wave masks in SGPRs
         const float oyg = cam[1] - sc.groundY;
+        const float Rm = bound.w + 1e-3f;
+        const uint64_t activeM = __ballot(active);
 #pragma unroll
         for (int k = 0; k < kStepsPerRefill; ++k) {
-            const bool marching = active && d > sc.eps && steps < sc.maxSteps && t < sc.tMax;
+            const uint64_t marchM = activeM & __builtin_amdgcn_fcmpf(d, sc.eps, kCmpGt) &
+                                    __builtin_amdgcn_sicmp(steps, sc.maxSteps, kCmpSlt) &
+                                    __builtin_amdgcn_fcmpf(t, sc.tMax, kCmpLt);
             float dist = fmaf(t, dy, oyg);
             // Every sphere lies inside the bounding sphere (centre C, radius
             // R), so each sphere's distance is at least |p - C| - R.  Where
@@ -459,13 +486,17 @@ __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uin
             // rounding of either (1e-3 + 1e-3 |ground|, against errors of
             // order 1e-5), no sphere can be the minimum: the ground's
             // distance is the step, exactly as the full minimum gives it.
-            // The spheres are evaluated only when some marching lane needs
-            // them.
+            // The skip test is |p - C|^2 >= y^2 with y = ground + R +
+            // margin, without a square root: for y > 0 that is |p - C| >= y;
+            // for y <= 0 the test with the root skips too, and this one can
+            // only evaluate the spheres where it need not, which changes
+            // nothing.  The spheres are evaluated only when some marching
+            // lane needs them.
             const float ub = t + Mb;
-            const float db = fsqrt(fmaf(ub, ub, E2b)) - bound.w;
-            const bool near = db < fmaf(fabsf(dist), 1e-3f, dist + 1e-3f);
-            if (__ballot(marching && near)) {
-                evalSteps += marching ? 1u : 0u;
+            const float yb = fmaf(fabsf(dist), 1e-3f, dist + Rm);
+            const uint64_t nearM = __builtin_amdgcn_fcmpf(fmaf(ub, ub, E2b), yb * yb, kCmpLt);
+            if (marchM & nearM) {
+                evalSteps += (uint32_t)__popcll(marchM);
                 const uint32_t z = opaque_zero();
 #pragma unroll
                 for (int i = 0; i < kMaxSph / 4; ++i) {
@@ -477,12 +508,10 @@ __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uin
                     }
                 }
             }
-            if (marching) {
-                tHit = t;
-                d = dist;
-                t += dist;
-                ++steps;
-            }
+            tHit = sel_lane(marchM, t, tHit);
+            d = sel_lane(marchM, dist, d);
+            t = sel_lane(marchM, t + dist, t);
+            steps = sel_lane(marchM, steps + 1, steps);
         }
         // retire finished rays into the shade queue
         const bool fin = active && !(d > sc.eps && steps < sc.maxSteps && t < sc.tMax);
@@ -528,14 +557,11 @@ __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uin
         if (lane == 0) atomicAdd(stepCount, s);
     }
     if (work) {
-        unsigned long long a = evalSteps, b = evalShades;
+        unsigned long long b = evalShades;
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            a += __shfl_xor(a, o, 64);
-            b += __shfl_xor(b, o, 64);
-        }
+        for (int o = 32; o > 0; o >>= 1) b += __shfl_xor(b, o, 64);
         if (lane == 0) {
-            atomicAdd(work, a);
+            atomicAdd(work, evalSteps);
             atomicAdd(work + 1, b);
         }
     }
