@@ -244,10 +244,12 @@ CKD_DEV float sel_lane(uint64_t m, float a, float b)
     asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(r) : "v"(b), "v"(a), "s"(m));
     return r;
 }
-CKD_DEV int sel_lane(uint64_t m, int a, int b)
+// a + the lane's bit of m: one v_addc with m as the carry in
+CKD_DEV int add_lane(uint64_t m, int a)
 {
     int r;
-    asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(r) : "v"(b), "v"(a), "s"(m));
+    uint64_t carry;
+    asm("v_addc_co_u32_e64 %0, %1, %2, 0, %3" : "=v"(r), "=s"(carry) : "v"(a), "s"(m));
     return r;
 }
 
@@ -378,7 +380,11 @@ __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uin
     // ray state (per lane)
     bool active = false;
     uint32_t pix = 0;
-    float t = 0.f, d = 1.f, tHit = 0.f, dx = 0.f, dy = 0.f, dz = 0.f;
+    // The ray's last march step evaluated the field at tHit and found d; its
+    // distance total is t = tHit + d, which is how the step formed it, so t
+    // is recomputed from the two instead of kept.  A new ray has tHit = -1
+    // and d = 1, so that t = 0.
+    float d = 1.f, tHit = -1.f, dx = 0.f, dy = 0.f, dz = 0.f;
     int steps = 0;
     float M[kMaxSph], E2[kMaxSph]; // d.(o-c_i) and |w_i|^2 of the lane's ray
     float Mb = 0.f, E2b = 0.f;     // the same for the centre of the spheres' bounding sphere
@@ -422,9 +428,8 @@ __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uin
                         Mb = m;
                         E2b = wx * wx + wy * wy + wz * wz;
                     }
-                    t = 0.f;
                     d = 1.f;
-                    tHit = 0.f;
+                    tHit = -1.f;
                     steps = 0;
                     active = true;
                 }
@@ -476,6 +481,7 @@ __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uin
         const uint64_t activeM = __ballot(active);
 #pragma unroll
         for (int k = 0; k < kStepsPerRefill; ++k) {
+            const float t = tHit + d;
             const uint64_t marchM = activeM & __builtin_amdgcn_fcmpf(d, sc.eps, kCmpGt) &
                                     __builtin_amdgcn_sicmp(steps, sc.maxSteps, kCmpSlt) &
                                     __builtin_amdgcn_fcmpf(t, sc.tMax, kCmpLt);
@@ -510,16 +516,17 @@ __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uin
             }
             tHit = sel_lane(marchM, t, tHit);
             d = sel_lane(marchM, dist, d);
-            t = sel_lane(marchM, t + dist, t);
-            steps = sel_lane(marchM, steps + 1, steps);
+            steps = add_lane(marchM, steps);
         }
         // retire finished rays into the shade queue
+        const float t = tHit + d;
         const bool fin = active && !(d > sc.eps && steps < sc.maxSteps && t < sc.tMax);
         const uint64_t finMask = __ballot(fin);
         if (finMask) {
             if (fin) {
                 const uint32_t slot = qn + __popcll(finMask & ((1ull << lane) - 1));
-                q[slot] = ShadeEntry{(uint32_t)job, pix, tHit, t};
+                // a ray that took no step shades at its origin
+                q[slot] = ShadeEntry{(uint32_t)job, pix, steps ? tHit : 0.f, t};
                 mySteps += (unsigned long long)steps;
                 active = false;
             }
