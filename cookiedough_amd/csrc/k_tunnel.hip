@@ -34,7 +34,7 @@ constexpr int kTexQuadMaxLog2 = 11;
 
 // ckd_b07_count_div of every window count 0..255 (radius <= 127), evaluated
 // by the compiler: blocks copy it instead of dividing
-struct B07DivTable {
+struct alignas(16) B07DivTable {
     uint16_t d[256];
     constexpr B07DivTable() : d{}
     {
@@ -42,6 +42,17 @@ struct B07DivTable {
     }
 };
 __constant__ const B07DivTable kB07Div = B07DivTable();
+
+// the table into a block's LDS copy D (16-byte aligned) with one load per
+// thread: 8 bytes each from 64 threads, 4 from 128
+template <int kThreads> CKD_DEV void copy_div_table(uint16_t *D)
+{
+    static_assert(kThreads == 64 || kThreads == 128, "512 bytes over 64 or 128 threads");
+    if constexpr (kThreads == 64)
+        reinterpret_cast<uint2 *>(D)[threadIdx.x] = reinterpret_cast<const uint2 *>(kB07Div.d)[threadIdx.x];
+    else
+        reinterpret_cast<uint32_t *>(D)[threadIdx.x] = reinterpret_cast<const uint32_t *>(kB07Div.d)[threadIdx.x];
+}
 
 // the job's images and grid are global memory: pointers in that address
 // space make their accesses global_* instructions rather than flat ones,
@@ -97,7 +108,7 @@ __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict_
 {
     const uint32_t row = blockIdx.x;
     extern __shared__ uint4 lds[];
-    __shared__ uint16_t D[256];
+    __shared__ __align__(16) uint16_t D[256];
     __shared__ uint64_t waveTot[kT1Block / 64];
     const TunJob J = pick(jobs, nJobs, single, row, &TunJob::rowBase);
     const int W = (int)J.W;
@@ -113,7 +124,7 @@ __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict_
     // into its 16-byte slot, so that P[1] is 16-byte aligned.
     uint64_t *P = reinterpret_cast<uint64_t *>(lds + NB) + 1;
 
-    for (int i = threadIdx.x; i < 256; i += kT1Block) D[i] = kB07Div.d[i];
+    copy_div_table<kT1Block>(D);
     const unsigned GW = (unsigned)NB + 1;
     const uint32_t j = y & 7;
     const global_u32c *g0 = (const global_u32c *)(J.grid) + 2 * (size_t)(y >> 3) * GW;
@@ -236,9 +247,9 @@ __global__ __launch_bounds__(64) void k_tunnel_v(const TunJob *__restrict__ jobs
                                                  const TunJob single, int kT2Seg)
 {
     const uint32_t unit = blockIdx.x;
-    __shared__ uint16_t D[256];
+    __shared__ __align__(16) uint16_t D[256];
     const TunJob J = pick(jobs, nJobs, single, unit, &TunJob::vBase);
-    for (int i = threadIdx.x; i < 256; i += 64) D[i] = kB07Div.d[i];
+    copy_div_table<64>(D);
     __syncthreads();
     const uint32_t local = unit - J.vBase;
     const int x = (int)((local % J.vBlocksX) * 64 + threadIdx.x);
@@ -308,9 +319,9 @@ __global__ __launch_bounds__(64) void k_tunnel_v_ring(const TunJob *__restrict__
 {
     const uint32_t unit = blockIdx.x;
     extern __shared__ uint32_t ring[]; // (2 * maxEdge + 1) slots of 64 lanes
-    __shared__ uint16_t D[256];
+    __shared__ __align__(16) uint16_t D[256];
     const TunJob J = pick(jobs, nJobs, single, unit, &TunJob::vBase);
-    for (int i = threadIdx.x; i < 256; i += 64) D[i] = kB07Div.d[i];
+    copy_div_table<64>(D);
     __syncthreads();
     const uint32_t local = unit - J.vBase;
     const int x = (int)((local % J.vBlocksX) * 64 + threadIdx.x);
@@ -325,37 +336,69 @@ __global__ __launch_bounds__(64) void k_tunnel_v_ring(const TunJob *__restrict__
     auto Drow = [&](int r) { return dst + (size_t)r * W; };
     uint32_t *mine = ring + threadIdx.x;
 
-    // the window of row y0, rows clipped to the image, into F and the ring
+    // the window of row y0, rows clipped to the image, into F and the ring;
+    // U rows' loads at a time are in flight together (past the window's last
+    // row they read that row again, and are not used)
     uint64_t F = 0;
     {
         const int r0 = max(0, y0 - e), r1 = min(H - 1, y0 + e);
         int s = r0 % R;
-        for (int r = r0; r <= r1; ++r) {
-            const uint32_t v = T(r);
-            F += ckd_spread(v);
-            mine[s * 64] = v;
-            s = s + 1 == R ? 0 : s + 1;
+        for (int r = r0; r <= r1; r += U) {
+            const int n = min(U, r1 + 1 - r);
+            uint32_t v[U];
+#pragma unroll
+            for (int j = 0; j < U; ++j) v[j] = T(min(r + j, r1));
+            // keeps the compiler from moving each load into the branch that
+            // uses it, behind the previous one's wait
+#pragma unroll
+            for (int j = 0; j < U; ++j) asm volatile("" : "+v"(v[j]));
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                if (j < n) {
+                    F += ckd_spread(v[j]);
+                    mine[s * 64] = v[j];
+                    s = s + 1 == R ? 0 : s + 1;
+                }
+            }
         }
     }
     // slot of row y - e (taken mod R also where y - e < 0): the leaving row
     // at row y, and the slot the entering row y + e + 1 takes
     int sl = ((y0 - e) % R + R) % R;
     auto advance = [&]() { sl = sl + 1 == R ? 0 : sl + 1; };
-    auto edge_row = [&](int y) {
-        const int lo = max(y - e, 0), hi = min(y + e, H - 1);
-        global_u32 *d = Drow(y);
-        d[xc] = ckd_adds_u8x4(d[xc], ckd_b07_div_pack(F, D[hi - lo + 1]));
-        if (y - e >= 0) F -= ckd_spread(mine[sl * 64]);
-        if (y + e + 1 <= H - 1) {
-            const uint32_t a = T(y + e + 1);
-            F += ckd_spread(a);
-            mine[sl * 64] = a;
+    // rows [ya, yb) whose window the image clips: their own counts, and the
+    // rows entering and leaving only where they exist; U rows' loads at a
+    // time in flight together (loads past the group's last row or the
+    // image's read those rows again, and are not used)
+    auto edge_rows = [&](int ya, int yb) {
+        for (int g = ya; g < yb; g += U) {
+            const int n = min(U, yb - g);
+            uint32_t dg[U], ag[U];
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                dg[j] = Drow(min(g + j, yb - 1))[xc];
+                ag[j] = T(min(g + j + e + 1, H - 1));
+            }
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                if (j < n) {
+                    const int yy = g + j;
+                    const int lo = max(yy - e, 0), hi = min(yy + e, H - 1);
+                    Drow(yy)[xc] = ckd_adds_u8x4(dg[j], ckd_b07_div_pack(F, D[hi - lo + 1]));
+                    if (yy - e >= 0) F -= ckd_spread(mine[sl * 64]);
+                    if (yy + e + 1 <= H - 1) {
+                        F += ckd_spread(ag[j]);
+                        mine[sl * 64] = ag[j];
+                    }
+                    advance();
+                }
+            }
         }
-        advance();
     };
     const int iBeg = max(y0, e), iEnd = min(y1, H - 1 - e);
     int y = y0;
-    for (; y < min(iBeg, y1); ++y) edge_row(y);
+    edge_rows(y, min(iBeg, y1));
+    y = max(y, min(iBeg, y1));
     const uint32_t div = D[2 * e + 1];
     // groups of U rows, the next group's loads in flight while this one is
     // blended; with R >= U the group's U leaving rows are distinct slots and
@@ -419,7 +462,7 @@ __global__ __launch_bounds__(64) void k_tunnel_v_ring(const TunJob *__restrict__
         F += ckd_spread(a) - ckd_spread(leaving);
         advance();
     }
-    for (; y < y1; ++y) edge_row(y);
+    edge_rows(y, y1);
 }
 
 } // namespace
