@@ -74,14 +74,15 @@ CKD_DEV uint32_t ckd_bilerp32(uint32_t p0, uint32_t p1, uint32_t p2, uint32_t p3
     const uint32_t M = 0x00ff00ffu;
     const ckd_u16x2 f = ckd_splat16(fu), g = ckd_splat16(fv);
     const ckd_u16x2 nf = (unsigned short)256 - f, ng = (unsigned short)256 - g;
-    const ckd_u16x2 br = ckd_lerp2x(ckd_lerp2x(ckd_u16x2_of(p0 & M), ckd_u16x2_of(p1 & M), nf, f),
-                                    ckd_lerp2x(ckd_u16x2_of(p2 & M), ckd_u16x2_of(p3 & M), nf, f),
-                                    ng, g);
+    // the vertical blend before its >> 8: each 16-bit lane's quotient by 256
+    // is its high byte, so one byte permute packs B, G, R, A from bytes 1
+    // and 3 of br and ga
+    const ckd_u16x2 br = ckd_lerp2x(ckd_u16x2_of(p0 & M), ckd_u16x2_of(p1 & M), nf, f) * ng +
+                         ckd_lerp2x(ckd_u16x2_of(p2 & M), ckd_u16x2_of(p3 & M), nf, f) * g;
     const ckd_u16x2 ga =
-        ckd_lerp2x(ckd_lerp2x(ckd_u16x2_of(ckd_odd_bytes(p0)), ckd_u16x2_of(ckd_odd_bytes(p1)), nf, f),
-                   ckd_lerp2x(ckd_u16x2_of(ckd_odd_bytes(p2)), ckd_u16x2_of(ckd_odd_bytes(p3)), nf, f),
-                   ng, g);
-    return ckd_u32_of(br) | (ckd_u32_of(ga) << 8);
+        ckd_lerp2x(ckd_u16x2_of(ckd_odd_bytes(p0)), ckd_u16x2_of(ckd_odd_bytes(p1)), nf, f) * ng +
+        ckd_lerp2x(ckd_u16x2_of(ckd_odd_bytes(p2)), ckd_u16x2_of(ckd_odd_bytes(p3)), nf, f) * g;
+    return ckd_perm(ckd_u32_of(ga), ckd_u32_of(br), 0x07030501u);
 }
 
 // bsamp8 (code/bilinear.h:34-53), one channel.

@@ -200,9 +200,13 @@ __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict_
     uint4 *Q = reinterpret_cast<uint4 *>(P + 1); // Q[j] = (P[2j + 1], P[2j + 2])
     auto pair_lo = [](const uint4 &q) { return ((uint64_t)q.y << 32) | q.x; };
     auto pair_hi = [](const uint4 &q) { return ((uint64_t)q.w << 32) | q.z; };
+    // the run's pairs from one base at constant offsets
+    const int np = (c1 - c0) >> 1;
+    uint4 *Qc = Q + (c0 >> 1);
     uint64_t local = 0;
-    for (int i = c0; i + 1 < c1; i += 2) {
-        const uint4 q = Q[i >> 1];
+#pragma unroll 4
+    for (int k = 0; k < np; ++k) {
+        const uint4 q = Qc[k];
         local += pair_lo(q) + pair_hi(q);
     }
     if ((c1 - c0) & 1) local += P[c1];
@@ -218,10 +222,11 @@ __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict_
     uint64_t run = incl - local;
     for (int w = 0; w < wave; ++w) run += waveTot[w];
     if (threadIdx.x == 0) P[0] = 0;
-    for (int i = c0; i + 1 < c1; i += 2) {
-        const uint4 q = Q[i >> 1];
+#pragma unroll 4
+    for (int k = 0; k < np; ++k) {
+        const uint4 q = Qc[k];
         const uint64_t a = run + pair_lo(q), b = a + pair_hi(q);
-        Q[i >> 1] = make_uint4((uint32_t)a, (uint32_t)(a >> 32), (uint32_t)b, (uint32_t)(b >> 32));
+        Qc[k] = make_uint4((uint32_t)a, (uint32_t)(a >> 32), (uint32_t)b, (uint32_t)(b >> 32));
         run = b;
     }
     if ((c1 - c0) & 1) P[c1] += run;
@@ -230,7 +235,7 @@ __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict_
     const int e = (int)J.edge;
     const uint32_t divI = D[2 * e + 1];
     global_u32 *trow = (global_u32 *)(J.tmp + (size_t)y * W);
-    for (int x = threadIdx.x; x < W; x += kT1Block) {
+    auto blur_px = [&](int x) {
         // the wave's 64 pixels [xw, xw + 63] (lane 0 is active whenever
         // any lane is): a full window for all of them, or clipped ones
         const int xw = __builtin_amdgcn_readfirstlane(x);
@@ -240,7 +245,15 @@ __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict_
             const int lo = max(x - e, 0), hi = min(x + e, W - 1);
             __builtin_nontemporal_store(ckd_b07_div_pack(P[hi + 1] - P[lo], D[hi - lo + 1]), &trow[x]);
         }
+    };
+    // four pixels at a time while all four are in the row: their LDS reads
+    // and stores at constant offsets from one base
+    int x = threadIdx.x;
+    for (; x + 3 * kT1Block < W; x += 4 * kT1Block) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) blur_px(x + k * kT1Block);
     }
+    for (; x < W; x += kT1Block) blur_px(x);
 }
 
 __global__ __launch_bounds__(64) void k_tunnel_v(const TunJob *__restrict__ jobs, int nJobs,
