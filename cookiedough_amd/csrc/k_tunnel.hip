@@ -98,6 +98,31 @@ __global__ __launch_bounds__(256) void k_tex_quads(const uint32_t *__restrict__ 
     quads[i] = make_uint4(tex[v0 + u], tex[v0 + u1], tex[v1 + u], tex[v1 + u1]);
 }
 
+// Inclusive prefix sum of a 64-bit value over the wave's 64 lanes (mod
+// 2^64), with DPP moves rather than LDS permutes: shifts by 1, 2, 4 and 8
+// within each row of 16 lanes, then lane 15's sum into rows 1 and 3 and
+// lane 31's into rows 2 and 3.  A lane with no source lane, or in a row the
+// step leaves out, adds 0.
+template <int kCtrl, int kRowMask> CKD_DEV void dpp_add(uint32_t &lo, uint32_t &hi)
+{
+    const uint32_t dl = __builtin_amdgcn_update_dpp(0u, lo, kCtrl, kRowMask, 0xf, true);
+    const uint32_t dh = __builtin_amdgcn_update_dpp(0u, hi, kCtrl, kRowMask, 0xf, true);
+    const uint32_t s = lo + dl;
+    hi += dh + (s < lo ? 1u : 0u);
+    lo = s;
+}
+CKD_DEV uint64_t wave_incl_scan(uint64_t x)
+{
+    uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    dpp_add<0x111, 0xf>(lo, hi); // row_shr:1
+    dpp_add<0x112, 0xf>(lo, hi); // row_shr:2
+    dpp_add<0x114, 0xf>(lo, hi); // row_shr:4
+    dpp_add<0x118, 0xf>(lo, hi); // row_shr:8
+    dpp_add<0x142, 0xa>(lo, hi); // row_bcast:15
+    dpp_add<0x143, 0xc>(lo, hi); // row_bcast:31
+    return ((uint64_t)hi << 32) | lo;
+}
+
 // kQuads: the texture's texels come from its quads (one load per pixel)
 template <bool kQuads>
 __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict__ jobs, int nJobs,
@@ -211,12 +236,7 @@ __global__ __launch_bounds__(kT1Block) void k_tunnel_h(const TunJob *__restrict_
     }
     if ((c1 - c0) & 1) local += P[c1];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint64_t incl = local;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint64_t v = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += v;
-    }
+    const uint64_t incl = wave_incl_scan(local);
     if (lane == 63) waveTot[wave] = incl;
     __syncthreads();
     uint64_t run = incl - local;
