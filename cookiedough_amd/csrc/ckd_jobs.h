@@ -13,6 +13,7 @@ struct SdfJob {
     uint32_t chunksX;   // chunk columns
     uint32_t chunkBase; // first global chunk index of this request
     float cam[12];      // origin, forward*focal, right, up
+    float invH;         // 1 / H, rounded to nearest
 };
 
 // voxel request: 64-column blocks, one lane per column

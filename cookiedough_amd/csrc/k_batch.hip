@@ -137,6 +137,7 @@ void make_plan(const ckd_tuning &T, const ckd_request *requests, unsigned n, uin
             J.chunksX = (r.W + chunk - 1) / chunk;
             J.chunkBase = P.chunks;
             memcpy(J.cam, r.cam, sizeof(J.cam));
+            J.invH = 1.0f / (float)r.H;
             P.chunks += J.chunksX * ((r.H + chunk - 1) / chunk);
             sdf.push_back(J);
         } else if (r.effect == CKD_FX_VOXEL) {

@@ -226,7 +226,7 @@ __global__ __launch_bounds__(256) void k_sdf(uint32_t *__restrict__ dst, unsigne
 // and moves rays that run out of steps.
 
 struct ShadeEntry {
-    uint32_t job, pix;
+    uint32_t job, xy; // xy = x | y << 16 (W <= 4096, H <= 2160)
     float tHit, total;
 };
 
@@ -274,13 +274,25 @@ CKD_DEV float uniform_f(float v)
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
-// direction of pixel (x, y) through the camera basis
-CKD_DEV void ray_dir(const float *cam, uint32_t W, uint32_t H, uint32_t x, uint32_t y, float &dx,
-                     float &dy, float &dz)
+// a / b rounded to nearest, from rb = 1 / b rounded to nearest: Markstein's
+// correction of the quotient estimate a * rb (within an ulp) by the residual
+// a - b * (a * rb), which the FMA gives exactly.  For the integers |a| <=
+// 16384 and 1 <= b <= 16384 it equals the IEEE quotient bit for bit (checked
+// over every pair); here |a| <= 4096 and b <= 2160.
+CKD_DEV float div_rn(float a, float b, float rb)
+{
+    const float q = a * rb;
+    return fmaf(fmaf(-b, q, a), rb, q);
+}
+
+// direction of pixel (x, y) through the camera basis; invH = 1 / H rounded
+// to nearest
+CKD_DEV void ray_dir(const float *cam, uint32_t W, uint32_t H, float invH, uint32_t x, uint32_t y,
+                     float &dx, float &dy, float &dz)
 {
     const float fH = (float)H;
-    const float u = (2.f * (float)x - (float)W) / fH;
-    const float v = (fH - 2.f * (float)y) / fH;
+    const float u = div_rn(2.f * (float)x - (float)W, fH, invH);
+    const float v = div_rn(fH - 2.f * (float)y, fH, invH);
     dx = cam[3] + u * cam[6] + v * cam[9];
     dy = cam[4] + u * cam[7] + v * cam[10];
     dz = cam[5] + u * cam[8] + v * cam[11];
@@ -373,13 +385,14 @@ __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uin
 
     // chunk state (uniform)
     int job = -1;
-    uint32_t next = 0, end = 0, cx0 = 0, cy0 = 0, cw = 1;
+    uint32_t next = 0, end = 0, cx0 = 0, cy0 = 0, cw = 1, cwM = 1u << 20;
+    float invH = 1.f;
     float cam[12];
     uint32_t W = 0, H = 0;
 
     // ray state (per lane)
     bool active = false;
-    uint32_t pix = 0;
+    uint32_t xy = 0; // the ray's pixel, x | y << 16
     // The ray's last march step evaluated the field at tHit and found d; its
     // distance total is t = tHit + d, which is how the step formed it, so t
     // is recomputed from the two instead of kept.  A new ray has tHit = -1
@@ -406,9 +419,12 @@ __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uin
             if (!active) {
                 const uint32_t k = next + rank;
                 if (k < end) {
-                    const uint32_t x = cx0 + k % cw, y = cy0 + k / cw;
-                    pix = y * W + x;
-                    ray_dir(cam, W, H, x, y, dx, dy, dz);
+                    // k / cw as (k * ceil(2^20 / cw)) >> 20, exact for k
+                    // < 4096 and cw <= 64
+                    const uint32_t row = __umul24(k, cwM) >> 20;
+                    const uint32_t x = cx0 + k - __umul24(row, cw), y = cy0 + row;
+                    xy = x | (y << 16);
+                    ray_dir(cam, W, H, invH, x, y, dx, dy, dz);
                     const uint32_t z = opaque_zero();
 #pragma unroll
                     for (int i = 0; i < kMaxSph; ++i) {
@@ -446,12 +462,14 @@ __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uin
             const SdfJob &J = jobs[job];
             W = J.W;
             H = J.H;
+            invH = uniform_f(J.invH);
 #pragma unroll
             for (int k = 0; k < 12; ++k) cam[k] = uniform_f(J.cam[k]);
             const uint32_t local = c - J.chunkBase;
             cx0 = (local % J.chunksX) * kChunkSide;
             cy0 = (local / J.chunksX) * kChunkSide;
             cw = min((uint32_t)kChunkSide, W - cx0);
+            cwM = ((1u << 20) + cw - 1) / cw;
             const uint32_t chh = min((uint32_t)kChunkSide, H - cy0);
             next = 0;
             end = cw * chh;
@@ -526,7 +544,7 @@ __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uin
             if (fin) {
                 const uint32_t slot = qn + __popcll(finMask & ((1ull << lane) - 1));
                 // a ray that took no step shades at its origin
-                q[slot] = ShadeEntry{(uint32_t)job, pix, steps ? tHit : 0.f, t};
+                q[slot] = ShadeEntry{(uint32_t)job, xy, steps ? tHit : 0.f, t};
                 mySteps += (unsigned long long)steps;
                 active = false;
             }
@@ -536,12 +554,12 @@ __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uin
                 // shade 64 queued hits with the whole wave
                 const ShadeEntry e = q[qn - 64 + lane];
                 const SdfJob &J = jobs[e.job];
-                const uint32_t x = e.pix % J.W, y = e.pix / J.W;
+                const uint32_t x = e.xy & 0xffffu, y = e.xy >> 16;
                 float ex, ey, ez;
-                ray_dir(J.cam, J.W, J.H, x, y, ex, ey, ez);
+                ray_dir(J.cam, J.W, J.H, J.invH, x, y, ex, ey, ez);
                 // a global store (the job's pointer would make a flat one,
                 // whose completion the next LDS reads would wait for)
-                ((global_px *)J.dst)[e.pix] = shade_hit<kHalf, kMaxSph>(sc, J.cam, ex, ey, ez, e.tHit, e.total, bound, evalShades);
+                ((global_px *)J.dst)[__umul24(y, J.W) + x] = shade_hit<kHalf, kMaxSph>(sc, J.cam, ex, ey, ez, e.tHit, e.total, bound, evalShades);
                 qn -= 64;
                 __builtin_amdgcn_wave_barrier();
             }
@@ -552,10 +570,10 @@ __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uin
     if ((int)lane < qn) {
         const ShadeEntry e = q[lane];
         const SdfJob &J = jobs[e.job];
-        const uint32_t x = e.pix % J.W, y = e.pix / J.W;
+        const uint32_t x = e.xy & 0xffffu, y = e.xy >> 16;
         float ex, ey, ez;
-        ray_dir(J.cam, J.W, J.H, x, y, ex, ey, ez);
-        ((global_px *)J.dst)[e.pix] = shade_hit<kHalf, kMaxSph>(sc, J.cam, ex, ey, ez, e.tHit, e.total, bound, evalShades);
+        ray_dir(J.cam, J.W, J.H, J.invH, x, y, ex, ey, ez);
+        ((global_px *)J.dst)[__umul24(y, J.W) + x] = shade_hit<kHalf, kMaxSph>(sc, J.cam, ex, ey, ez, e.tHit, e.total, bound, evalShades);
     }
     if (stepCount) {
         unsigned long long s = mySteps;
