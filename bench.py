@@ -152,8 +152,10 @@ class Timer:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    # the first replays after the autotune and the graph capture run 5-12%
+    # slower while the clocks come back up: three untimed ones by default
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--tuning", default=None,
