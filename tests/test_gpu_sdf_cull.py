@@ -99,3 +99,43 @@ def test_steps_per_refill_change_nothing(ctx, n_spheres):
     for k, o in zip((2, 4, 8, 16), outs[1:]):
         assert np.array_equal(o, outs[0]), k
     assert len(set(counts)) == 1 and counts[0] > 0, counts
+
+
+@pytest.mark.parametrize("shading", [CKD_SHADE_FP16, CKD_SHADE_FP32])
+def test_launch_shape_changes_nothing(ctx, shading):
+    """16x16 or 32x32 chunks and 1 to 8 persistent blocks per CU change which
+    wave traces which pixel, not the pixel: frames and the step count of a
+    batch of ragged sizes (a chunk row narrower than the chunk, the largest
+    height 2160, the largest width 4096) are identical"""
+    sc = wl.sdf_scene(2, 8, 64)
+    path = wl.landscape_path(0)
+    sizes = [(8, 2160, 3.0), (4096, 9, 5.0), (1000, 563, 7.0), (333, 17, 11.0), (640, 360, 29.0),
+             (31, 33, 41.0)]
+    total = sum(W * H for W, H, _ in sizes)
+    out = ctx.empty(total)
+    offs, o = [], 0
+    for W, H, _ in sizes:
+        offs.append(o)
+        o += W * H
+    reqs = ctx.request_array([wl.make_request(wl.BatchItem(CKD_FX_SDF, W, H, t), out.ptr + 4 * offs[k], sc, path,
+                                              None) for k, (W, H, t) in enumerate(sizes)])
+    saved = ctx.get_tuning()
+    frames, counts = [], []
+    try:
+        for chunk, bpc in ((32, 3), (16, 1), (16, 8), (32, 8)):
+            t = ctx.get_tuning()
+            t.sdfChunk, t.sdfBlocksPerCU = chunk, bpc
+            ctx.set_tuning(t)
+            cnt = ctx.upload(np.zeros(1, np.uint64))
+            ctx.memset32(out, 0x12345678, total)
+            print("launch shape", chunk, bpc, shading, flush=True)
+            ctx.render_batch(reqs, sc, shading, None, 9, None, 0, cnt)
+            ctx.sync()
+            frames.append(out.numpy().copy())
+            counts.append(int(cnt.numpy()[0]))
+    finally:
+        ctx.set_tuning(saved)
+    assert not (frames[0] == 0x12345678).all()
+    for k, f in enumerate(frames[1:], 1):
+        assert np.array_equal(f, frames[0]), k
+    assert len(set(counts)) == 1 and counts[0] > 0, counts
