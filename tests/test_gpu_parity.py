@@ -415,10 +415,13 @@ def test_batch_matches_single_calls(ctx, lmaps):
 @pytest.mark.parametrize("shading", [CKD_SHADE_FP32, CKD_SHADE_FP16])
 def test_batch_sdf_vs_fp64(ctx, orc, nsph, steps, shading):
     """the persistent tracer (chunk queue, lane refill, deferred shading)
-    against the FP64 render, over requests of several sizes in one launch:
-    within 2/255 outside the rounding-decided pixels"""
+    against the FP64 render, over requests of several sizes in one launch,
+    among them the batch's limit shapes (height 2160 with a chunk row of 8
+    pixels, width 4096 with 9 rows): within 2/255 outside the
+    rounding-decided pixels"""
     sc = wl.sdf_scene(0, nsph, steps)
-    sizes = [(320, 180, 0.0), (200, 120, 3.3), (64, 40, 12.0), (333, 77, 45.5)]
+    sizes = [(320, 180, 0.0), (200, 120, 3.3), (64, 40, 12.0), (333, 77, 45.5), (8, 2160, 21.0),
+             (4096, 9, 33.0)]
     total = sum(w * h for w, h, _ in sizes)
     out = ctx.empty(total)
     cnt = ctx.upload(np.zeros(1, np.uint64))
