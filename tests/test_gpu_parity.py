@@ -247,9 +247,13 @@ def test_tunnel_frame_grid_steps(ctx, orc, kind):
 # starting from a window summed over its halo: radii up to the limit, segments
 # from the smallest to the largest, one that is not a multiple of four (the
 # pass's row group) and widths that are not a multiple of 64 (its column
-# group).
+# group).  The window's first rows and the clipped rows at the image's top
+# and bottom load in groups of 8: a 65-row window over 8-row segments (eight
+# full groups and a partial one), a 7-row window (one partial group) and a
+# frame whose rows are nearly all clipped.
 @pytest.mark.parametrize("W,H,r,seg", [(1000, 600, 80, 32), (1000, 600, 127, 32), (328, 200, 5, 50),
-                                       (328, 200, 99, 8), (1280, 720, 32, 512), (1280, 720, 16, 128)])
+                                       (328, 200, 99, 8), (1280, 720, 32, 512), (1280, 720, 16, 128),
+                                       (640, 300, 32, 8), (72, 100, 3, 16), (200, 40, 19, 16)])
 def test_tunnel_frame_vertical_segments(ctx, orc, W, H, r, seg):
     _say("tunnel frame", W, H, r, "seg", seg)
     saved = ctx.get_tuning()
