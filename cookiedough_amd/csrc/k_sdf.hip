@@ -228,6 +228,9 @@ __global__ __launch_bounds__(256) void k_sdf(uint32_t *__restrict__ dst, unsigne
 struct ShadeEntry {
     uint32_t job, xy; // xy = x | y << 16 (W <= 4096, H <= 2160)
     float tHit, total;
+    // the ray's direction as its set-up computed it: shading reads it rather
+    // than computing it again from the job's camera
+    float dx, dy, dz, pad;
 };
 
 constexpr int kQueueCap = 128;
@@ -544,7 +547,7 @@ __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uin
             if (fin) {
                 const uint32_t slot = qn + __popcll(finMask & ((1ull << lane) - 1));
                 // a ray that took no step shades at its origin
-                q[slot] = ShadeEntry{(uint32_t)job, xy, steps ? tHit : 0.f, t};
+                q[slot] = ShadeEntry{(uint32_t)job, xy, steps ? tHit : 0.f, t, dx, dy, dz, 0.f};
                 mySteps += (unsigned long long)steps;
                 active = false;
             }
@@ -555,8 +558,7 @@ __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uin
                 const ShadeEntry e = q[qn - 64 + lane];
                 const SdfJob &J = jobs[e.job];
                 const uint32_t x = e.xy & 0xffffu, y = e.xy >> 16;
-                float ex, ey, ez;
-                ray_dir(J.cam, J.W, J.H, J.invH, x, y, ex, ey, ez);
+                const float ex = e.dx, ey = e.dy, ez = e.dz;
                 // a global store (the job's pointer would make a flat one,
                 // whose completion the next LDS reads would wait for)
                 ((global_px *)J.dst)[__umul24(y, J.W) + x] = shade_hit<kHalf, kMaxSph>(sc, J.cam, ex, ey, ez, e.tHit, e.total, bound, evalShades);
@@ -571,8 +573,7 @@ __global__ __launch_bounds__(256, kMaxSph > 8 ? 4 : 5) void k_sdf_persistent(uin
         const ShadeEntry e = q[lane];
         const SdfJob &J = jobs[e.job];
         const uint32_t x = e.xy & 0xffffu, y = e.xy >> 16;
-        float ex, ey, ez;
-        ray_dir(J.cam, J.W, J.H, J.invH, x, y, ex, ey, ez);
+        const float ex = e.dx, ey = e.dy, ez = e.dz;
         ((global_px *)J.dst)[__umul24(y, J.W) + x] = shade_hit<kHalf, kMaxSph>(sc, J.cam, ex, ey, ez, e.tHit, e.total, bound, evalShades);
     }
     if (stepCount) {
